@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: 8-lambda path segments per second of the HIP transport path on one MI355X.
+
+Workload (single GPU job): a uniaxial, pleochroic sapphire (Sellmeier n_o ~ 1.768,
+n_e = n_o - 0.0081, seeded optic axis, 8 Gaussian absorption bands per axis with peaks
+0.01-2 cm^-1) cut as a 69-plane oval brilliant, lit by a 2048x1024 procedural HDR
+lat-long environment, rendered at 1920x1080 with 256 samples per pixel and at most 32
+bounces.  One step = one full-frame render (530.8 M paths) accumulated into a
+device-resident film; inputs are uploaded before the timed region.
+
+Prints one JSON line (rank 0).  ``value`` is whole-job segments per second over the K
+timed steps.  ``roofline`` reports the trace kernel's algorithmic FP32 rate (exact
+per-class event counters x the op-count model of gemray_studio_amd/op_counts.py) over its
+average launch time from HIP events on the library's stream.  ``cpu_baseline`` times the
+oracle (the CPU restatement of the reference tracer) on a bounded window of the same
+frame, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "8-λ path segments/sec on 1 MI355X; p99 job latency, batched vs plain"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def single_job_scene(width=1920, height=1080):
+    from gemray_studio_amd import backend as B
+    from gemray_studio_amd import scenes
+
+    material = scenes.pleochroic_sapphire(seed=2)
+    planes = scenes.oval_brilliant()
+    env = scenes.procedural_env(seed=2, width=2048, height=1024)
+    cam = B.camera_new(0.35, 0.45, 5.0, 28.0)
+    scene = B.SceneRef(camera=cam, width=width, height=height, planes=planes, material=material,
+                       max_bounces=32, environment=B.HdrMap(env))
+    return scene, (0.35, 0.45, 5.0, 28.0)
+
+
+def cpu_baseline(scene, cam_args, spp_total, budget_s):
+    """Oracle on a centred pixel window of the same frame; window grown until the run
+    takes about budget_s seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from gemray_studio_amd import materials as M
+
+    threads = len(os.sched_getaffinity(0))
+    cam = po.camera_new(*cam_args)
+    env = po.map_env(scene.environment.rgb)
+    mat = M.to_struct(scene.material)
+    w, h = scene.width, scene.height
+    spp = 16
+    rows = 2
+    total_segs = 0
+    total_paths = 0
+    elapsed = 0.0
+    y0 = h // 2
+    while elapsed < budget_s and y0 + rows <= h:
+        p0, p1 = y0 * w, (y0 + rows) * w
+        t0 = time.perf_counter()
+        _, _, _, segs = po.render(cam, w, h, scene.planes, mat, scene.max_bounces, env, 0, spp,
+                                  pixel_begin=p0, pixel_end=p1, threads=threads)
+        dt = time.perf_counter() - t0
+        total_segs += segs
+        total_paths += (p1 - p0) * spp
+        elapsed += dt
+        y0 += rows
+        rows = min(rows * 2, 64)
+    return {
+        "value": total_segs / elapsed,
+        "unit": "segments/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle (C++ restatement of trace_spectral_ray) on rows {h // 2}..{y0} of the "
+                  f"{w}x{h} frame at {spp} spp: {total_paths} paths, {total_segs} segments in "
+                  f"{elapsed:.1f} s on {threads} threads",
+        "paths_per_s": total_paths / elapsed,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=256, help="samples per pixel per step")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design note, not a run", file=sys.stderr)
+        sys.exit(2)
+
+    from gemray_studio_amd import backend as B
+    from gemray_studio_amd import op_counts
+
+    scene, cam_args = single_job_scene(args.width, args.height)
+    gpu = B.GpuBackend.acquire()
+    if not gpu.available:
+        print("bench.py: no gfx950 device", file=sys.stderr)
+        sys.exit(1)
+    print(f"bench: {gpu.adapter_label()}; {args.width}x{args.height} x {args.spp} spp/step, "
+          f"P={scene.planes.shape[0]}", file=sys.stderr, flush=True)
+    dscene = gpu.upload(scene)
+    film = gpu.film(scene.width, scene.height)
+    for i in range(args.warmup):
+        gpu.render(dscene, i * args.spp, args.spp, film)
+        print(f"bench: warmup {i} {gpu.last_stats()['total_ms']:.0f} ms", file=sys.stderr, flush=True)
+    segments = 0
+    paths = 0
+    kernel_ms = 0.0
+    launches = 0
+    events = None
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        gpu.render(dscene, (args.warmup + k) * args.spp, args.spp, film)
+        st = gpu.last_stats()
+        segments += st["segments"]
+        paths += st["paths"]
+        kernel_ms += st["trace_kernel_ms"]
+        launches += st["launches"]
+        if events is None:
+            events = dict(st["events"])
+        else:
+            for n, v in st["events"].items():
+                events[n] += v
+        print(f"bench: step {k} {st['total_ms']:.0f} ms, {st['segments']} segments", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    film.free()
+    dscene.free()
+
+    m = scene.material
+    flops = op_counts.algorithmic_flops(events, scene.planes.shape[0], paths, m.is_anisotropic,
+                                        bool(m.o_bands or m.e_bands), False,
+                                        n_bands=len(m.o_bands) + len(m.e_bands))
+    avg_launch_s = kernel_ms / 1e3 / launches
+    achieved_tf = flops / launches / avg_launch_s / 1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "trace_kernel_single_job_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": segments / elapsed,
+        "unit": "segments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "single job: pleochroic uniaxial sapphire, 69-plane oval brilliant, 2048x1024 "
+                        f"procedural HDR env, {args.width}x{args.height} x {args.spp} spp per step, depth 32",
+            "width": args.width, "height": args.height, "spp_per_step": args.spp,
+            "max_bounces": scene.max_bounces, "n_planes": int(scene.planes.shape[0]),
+            "paths": paths, "segments": segments,
+            "segments_per_path": segments / max(paths, 1),
+            "paths_per_s": paths / elapsed,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_tf,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tf / PEAK_FP32_TFLOPS,
+            "traffic": traffic,
+            "kernel": "trace_kernel",
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "launches": launches,
+            "algorithmic_flops_per_launch": flops / launches,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, cam_args, args.spp, args.cpu_budget)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

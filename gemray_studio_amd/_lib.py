@@ -1,0 +1,136 @@
+"""ctypes binding of libgemray_hip.so (include/gemray_hip.h).
+
+The library is the only compute path: if it is missing or fails to load, every entry
+point raises; there is no CPU fallback in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .materials import MaterialStruct
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgemray_hip.so")
+
+GR_OK = 0
+GR_DECLINE_NO_DEVICE = 1
+GR_DECLINE_ENV = 2
+GR_DECLINE_MATERIAL = 3
+GR_DECLINE_BUDGET = 4
+GR_ERR_INVALID = 5
+GR_ERR_DEVICE = 6
+GR_ERR_NOT_FOUND = 7
+GR_PENDING = 8
+STATUS_NAMES = {
+    0: "GR_OK", 1: "GR_DECLINE_NO_DEVICE", 2: "GR_DECLINE_ENV", 3: "GR_DECLINE_MATERIAL",
+    4: "GR_DECLINE_BUDGET", 5: "GR_ERR_INVALID", 6: "GR_ERR_DEVICE", 7: "GR_ERR_NOT_FOUND",
+    8: "GR_PENDING",
+}
+
+GR_ENV_STUDIO = 0
+GR_ENV_MAP = 1
+EV_NAMES = ["segment", "miss", "hit", "rotate", "absorb", "tir", "reflect", "refract",
+            "entry_aniso", "walkoff", "rr"]
+GR_EV_COUNT = len(EV_NAMES)
+
+
+class GrError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {STATUS_NAMES.get(status, status)}")
+        self.status = status
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("chunk_tuples", ctypes.c_uint64),
+                ("hbm_budget_bytes", ctypes.c_uint64)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("forward", ctypes.c_float * 3),
+                ("right", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("fov_tan", ctypes.c_float)]
+
+
+class Env(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("temp_k", ctypes.c_float), ("spot_mult", ctypes.c_float),
+                ("exposure", ctypes.c_float), ("light_yaw", ctypes.c_float),
+                ("light_pitch", ctypes.c_float), ("map_w", ctypes.c_int32),
+                ("map_h", ctypes.c_int32), ("map_rgb", ctypes.POINTER(ctypes.c_float))]
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [("camera", ctypes.POINTER(Camera)), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("planes", ctypes.POINTER(ctypes.c_float)),
+                ("n_planes", ctypes.c_uint32), ("finishes", ctypes.POINTER(ctypes.c_uint32)),
+                ("material", ctypes.POINTER(MaterialStruct)), ("max_bounces", ctypes.c_uint32),
+                ("env", ctypes.POINTER(Env))]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("events", ctypes.c_uint64 * GR_EV_COUNT), ("paths", ctypes.c_uint64),
+                ("launches", ctypes.c_uint32), ("trace_kernel_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("n_planes", ctypes.c_uint32)]
+
+
+# Every symbol include/gemray_hip.h declares.
+EXPORTS = [
+    "gr_open", "gr_close", "gr_adapter_label", "gr_set_chunk_tuples", "gr_accumulate",
+    "gr_trace_samples", "gr_last_stats", "gr_scene_upload", "gr_scene_free", "gr_film_create",
+    "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_camera_new",
+]
+
+_lib = None
+
+
+def load():
+    """Loads the HIP library; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m gemray_studio_amd.build_lib` "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    u32 = ctypes.c_uint32
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.gr_open.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Options)]
+    L.gr_open.restype = ctypes.c_int
+    L.gr_close.argtypes = [vp]
+    L.gr_close.restype = None
+    L.gr_adapter_label.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    L.gr_adapter_label.restype = ctypes.c_int
+    L.gr_set_chunk_tuples.argtypes = [vp, ctypes.c_uint64]
+    L.gr_set_chunk_tuples.restype = ctypes.c_int
+    L.gr_accumulate.argtypes = [vp, ctypes.POINTER(Scene), u32, u32, fp, ctypes.c_size_t]
+    L.gr_accumulate.restype = ctypes.c_int
+    L.gr_trace_samples.argtypes = [vp, ctypes.POINTER(Scene), u32, u32, u32, u32, fp,
+                                   ctypes.POINTER(ctypes.c_uint32)]
+    L.gr_trace_samples.restype = ctypes.c_int
+    L.gr_last_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.gr_last_stats.restype = ctypes.c_int
+    L.gr_scene_upload.argtypes = [vp, ctypes.POINTER(Scene), ctypes.POINTER(vp)]
+    L.gr_scene_upload.restype = ctypes.c_int
+    L.gr_scene_free.argtypes = [vp, vp]
+    L.gr_scene_free.restype = None
+    L.gr_film_create.argtypes = [vp, u32, u32, ctypes.POINTER(vp)]
+    L.gr_film_create.restype = ctypes.c_int
+    L.gr_film_clear.argtypes = [vp, vp]
+    L.gr_film_clear.restype = ctypes.c_int
+    L.gr_film_free.argtypes = [vp, vp]
+    L.gr_film_free.restype = None
+    L.gr_film_download.argtypes = [vp, vp, fp, ctypes.c_size_t]
+    L.gr_film_download.restype = ctypes.c_int
+    L.gr_render.argtypes = [vp, vp, u32, u32, vp]
+    L.gr_render.restype = ctypes.c_int
+    L.gr_camera_new.argtypes = [ctypes.c_float] * 4 + [ctypes.POINTER(Camera)]
+    L.gr_camera_new.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(status: int, what: str) -> None:
+    if status != GR_OK:
+        raise GrError(status, what)

@@ -1,0 +1,285 @@
+"""The render-backend boundary: gemray's ``GpuBackend`` over the HIP C ABI.
+
+Mirrors ``crates/gemray/src/renderer/gpu_backend.rs:85-238``:
+
+* ``GpuBackend.acquire()`` opens the gfx950 device or returns a backend that declines
+  every call (``gpu_backend.rs:114-126``);
+* ``GpuBackend.disabled()`` never touches a device (``:135``);
+* ``adapter_label()`` is ``None`` when no device is held (``:146-153``);
+* ``try_accumulate(scene, sample_offset, spp, accum)`` ADDS the samples into ``accum`` and
+  returns ``True``, or returns ``False`` and leaves ``accum`` untouched so the caller can
+  fall back to its CPU tracer (``:160-193``).  A length mismatch between ``accum`` and the
+  frame is a programmer error and raises, as the reference asserts (``frame.rs:673-677``).
+
+The package itself has no CPU tracer: a declined call is the caller's to handle.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from dataclasses import dataclass
+from typing import Optional, Sequence, Union
+
+import numpy as np
+
+from . import _lib
+from ._lib import Camera, Env, GrError, Options, Scene, Stats, check
+from .materials import GemMaterial, MaterialStruct, to_struct
+
+# LightingPreset::params (optics/raytracer/environment.rs:81-100): (temp_k, spot_mult).
+PRESETS = {
+    "Daylight": (6500.0, 1.0),
+    "Incandescent": (3200.0, 1.2),
+    "RingLights": (5000.0, 1.6),
+    "DarkSpotlight": (6000.0, 2.4),
+}
+
+
+@dataclass(frozen=True)
+class Studio:
+    """``EnvironmentSource::Studio`` (environment.rs:189-194)."""
+
+    preset: str = "Daylight"
+    exposure: float = 1.0
+    light_yaw: float = 0.0
+    light_pitch: float = 0.0
+
+
+@dataclass(frozen=True)
+class HdrMap:
+    """``EnvironmentSource::HdrMap`` -- an ``EnvironmentMap`` of shape (H, W, 3) f32."""
+
+    rgb: np.ndarray
+
+    @staticmethod
+    def uniform(l0: float) -> "HdrMap":
+        """``EnvironmentMap::uniform(1, 1, [l0; 3])`` (env_map.rs:163-168): the furnace."""
+        return HdrMap(np.full((1, 1, 3), l0, dtype=np.float32))
+
+
+Environment = Union[Studio, HdrMap]
+
+
+def camera_new(yaw: float, pitch: float, distance: float, fov_deg: float) -> Camera:
+    """``Camera::new`` (optics/raytracer/camera.rs:63-93), evaluated by the library."""
+    cam = Camera()
+    check(_lib.load().gr_camera_new(yaw, pitch, distance, fov_deg, ctypes.byref(cam)), "gr_camera_new")
+    return cam
+
+
+@dataclass
+class SceneRef:
+    """``GpuSceneRef`` (gpu_backend.rs:85-99)."""
+
+    camera: Camera
+    width: int
+    height: int
+    planes: np.ndarray  # (P, 4) f32: normal xyz, d
+    material: GemMaterial
+    max_bounces: int
+    environment: Environment
+    facet_finishes: Optional[Sequence[int]] = None
+
+
+class _Packed:
+    """A ``gr_scene`` plus the buffers its pointers reference (kept alive together)."""
+
+    def __init__(self, scene: SceneRef):
+        self.planes = np.ascontiguousarray(scene.planes, dtype=np.float32).reshape(-1, 4)
+        self.material = to_struct(scene.material)
+        self.camera = scene.camera
+        self.env = Env()
+        self.map = None
+        env = scene.environment
+        if isinstance(env, Studio):
+            t, s = PRESETS[env.preset]
+            self.env.kind = _lib.GR_ENV_STUDIO
+            self.env.temp_k, self.env.spot_mult = t, s
+            self.env.exposure, self.env.light_yaw, self.env.light_pitch = (
+                env.exposure, env.light_yaw, env.light_pitch)
+        elif isinstance(env, HdrMap):
+            self.map = np.ascontiguousarray(env.rgb, dtype=np.float32)
+            if self.map.ndim != 3 or self.map.shape[2] != 3:
+                raise ValueError("HdrMap.rgb must have shape (H, W, 3)")
+            self.env.kind = _lib.GR_ENV_MAP
+            self.env.map_h, self.env.map_w = self.map.shape[0], self.map.shape[1]
+            self.env.map_rgb = self.map.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        else:
+            raise TypeError(f"unknown environment {env!r}")
+        self.finishes = None
+        if scene.facet_finishes is not None:
+            self.finishes = np.ascontiguousarray(scene.facet_finishes, dtype=np.uint32)
+        self.scene = Scene()
+        self.scene.camera = ctypes.pointer(self.camera)
+        self.scene.width, self.scene.height = scene.width, scene.height
+        self.scene.planes = self.planes.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        self.scene.n_planes = self.planes.shape[0]
+        if self.finishes is not None:
+            self.scene.finishes = self.finishes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self.scene.material = ctypes.pointer(self.material)
+        self.scene.max_bounces = scene.max_bounces
+        self.scene.env = ctypes.pointer(self.env)
+
+
+class DeviceScene:
+    """A scene uploaded once to HBM (``gr_scene_upload``)."""
+
+    def __init__(self, backend: "GpuBackend", scene: SceneRef):
+        self._backend = backend
+        self.width, self.height = scene.width, scene.height
+        packed = _Packed(scene)
+        h = ctypes.c_void_p()
+        check(backend._L.gr_scene_upload(backend._ctx, ctypes.byref(packed.scene), ctypes.byref(h)),
+              "gr_scene_upload")
+        self.handle = h
+
+    def free(self):
+        if self.handle:
+            self._backend._L.gr_scene_free(self._backend._ctx, self.handle)
+            self.handle = None
+
+
+class Film:
+    """A device-resident XYZ film (``gr_film_create``); accumulates across renders."""
+
+    def __init__(self, backend: "GpuBackend", width: int, height: int):
+        self._backend = backend
+        self.width, self.height = width, height
+        h = ctypes.c_void_p()
+        check(backend._L.gr_film_create(backend._ctx, width, height, ctypes.byref(h)), "gr_film_create")
+        self.handle = h
+
+    def clear(self):
+        check(self._backend._L.gr_film_clear(self._backend._ctx, self.handle), "gr_film_clear")
+
+    def download(self) -> np.ndarray:
+        out = np.zeros((self.height * self.width, 3), dtype=np.float32)
+        check(self._backend._L.gr_film_download(
+            self._backend._ctx, self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.size),
+            "gr_film_download")
+        return out
+
+    def free(self):
+        if self.handle:
+            self._backend._L.gr_film_free(self._backend._ctx, self.handle)
+            self.handle = None
+
+
+class GpuBackend:
+    """``renderer::gpu_backend::GpuBackend`` over ``libgemray_hip.so``."""
+
+    def __init__(self, ctx=None):
+        self._ctx = ctx
+        self._L = _lib.load() if ctx is not None else None
+        self._mu = threading.Lock()
+
+    @classmethod
+    def acquire(cls, device: int = 0, chunk_tuples: int = 0) -> "GpuBackend":
+        L = _lib.load()
+        opts = Options(device, chunk_tuples, 0)
+        h = ctypes.c_void_p()
+        st = L.gr_open(ctypes.byref(h), ctypes.byref(opts))
+        if st != _lib.GR_OK:
+            return cls(None)
+        return cls(h)
+
+    @classmethod
+    def disabled(cls) -> "GpuBackend":
+        return cls(None)
+
+    @property
+    def available(self) -> bool:
+        return self._ctx is not None
+
+    def close(self):
+        if self._ctx is not None:
+            self._L.gr_close(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def adapter_label(self) -> Optional[str]:
+        if self._ctx is None:
+            return None
+        buf = ctypes.create_string_buffer(256)
+        self._L.gr_adapter_label(self._ctx, buf, 256)
+        return buf.value.decode()
+
+    def _require(self):
+        if self._ctx is None:
+            raise GrError(_lib.GR_DECLINE_NO_DEVICE, "no gfx950 device held")
+
+    def accumulate(self, scene: SceneRef, sample_offset: int, spp: int, accum: np.ndarray) -> None:
+        """Like ``try_accumulate`` but raises ``GrError`` instead of returning False."""
+        self._require()
+        if accum.dtype != np.float32 or not accum.flags["C_CONTIGUOUS"]:
+            raise TypeError("accum must be a C-contiguous float32 array")
+        n_pixels = scene.width * scene.height
+        if accum.size != 3 * n_pixels:
+            raise AssertionError("accumulation buffer must have one entry per pixel")
+        packed = _Packed(scene)
+        with self._mu:
+            st = self._L.gr_accumulate(self._ctx, ctypes.byref(packed.scene), sample_offset, spp,
+                                       accum.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n_pixels)
+        check(st, "gr_accumulate")
+
+    def try_accumulate(self, scene: SceneRef, sample_offset: int, spp: int, accum: np.ndarray) -> bool:
+        if self._ctx is None:
+            return False
+        try:
+            self.accumulate(scene, sample_offset, spp, accum)
+        except GrError:
+            return False
+        return True
+
+    def trace_samples(self, scene: SceneRef, pixel_begin: int, pixel_count: int, sample_offset: int,
+                      spp: int, with_info: bool = True):
+        """Per-tuple XYZ (and segments / termination / facet hash) for a pixel range."""
+        self._require()
+        n = pixel_count * spp
+        xyz = np.zeros((n, 3), dtype=np.float32)
+        info = np.zeros((n, 3), dtype=np.uint32) if with_info else None
+        packed = _Packed(scene)
+        with self._mu:
+            st = self._L.gr_trace_samples(
+                self._ctx, ctypes.byref(packed.scene), pixel_begin, pixel_count, sample_offset, spp,
+                xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                info.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if info is not None else None)
+        check(st, "gr_trace_samples")
+        return xyz, info
+
+    def upload(self, scene: SceneRef) -> DeviceScene:
+        self._require()
+        return DeviceScene(self, scene)
+
+    def film(self, width: int, height: int) -> Film:
+        self._require()
+        return Film(self, width, height)
+
+    def render(self, scene: DeviceScene, sample_offset: int, spp: int, film: Film) -> None:
+        self._require()
+        with self._mu:
+            st = self._L.gr_render(self._ctx, scene.handle, sample_offset, spp, film.handle)
+        check(st, "gr_render")
+
+    def set_chunk_tuples(self, tuples: int) -> None:
+        self._require()
+        check(self._L.gr_set_chunk_tuples(self._ctx, tuples), "gr_set_chunk_tuples")
+
+    def last_stats(self) -> dict:
+        self._require()
+        s = Stats()
+        check(self._L.gr_last_stats(self._ctx, ctypes.byref(s)), "gr_last_stats")
+        return {
+            "events": {name: int(s.events[i]) for i, name in enumerate(_lib.EV_NAMES)},
+            "segments": int(s.events[0]),
+            "paths": int(s.paths),
+            "launches": int(s.launches),
+            "trace_kernel_ms": float(s.trace_kernel_ms),
+            "total_ms": float(s.total_ms),
+            "n_planes": int(s.n_planes),
+        }

@@ -1,0 +1,261 @@
+// device_physics.h -- gfx950 device functions of the spectral transport estimator.
+//
+// The arithmetic follows gemray's CPU tracer (crates/gemray/src/optics/raytracer/*.rs and
+// optics/*.rs): every `mul_add` of the reference is an explicit fmaf here and nothing
+// else is fused (the library is compiled with -ffp-contract=off), dot products are
+// left-associative, cross products use glam's operand order, and normalize is
+// v * (1 / sqrt(dot(v, v))).  Division and sqrt are correctly rounded (HIP's default for
+// f32).  Mueller products skip the structural zeros of each matrix: that changes at most
+// the sign of a zero Stokes component, which no output depends on.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gr {
+
+constexpr float kPi = 3.14159265358979323846f;
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 mul(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ float len2(V3 a) { return dot(a, a); }
+__device__ __forceinline__ V3 normalize(V3 a) { return mul(a, 1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ V3 normalize_or_zero(V3 a) {
+    float rcp = 1.0f / sqrtf(dot(a, a));
+    if (isfinite(rcp) && rcp > 0.0f) return mul(a, rcp);
+    return v3(0.0f, 0.0f, 0.0f);
+}
+
+// f32::clamp keeps NaN; fmaxf/fminf are IEEE maxNum/minNum like f32::max/min.
+__device__ __forceinline__ float clampf(float x, float lo, float hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+__device__ __forceinline__ float rem_euclid(float x, float rhs) {
+    float r = fmodf(x, rhs);
+    return r < 0.0f ? r + fabsf(rhs) : r;
+}
+// x^n, n a small positive constant, rounded once: the product of the set-bit squares
+// is formed in f64 (at most 5 roundings of 2^-53 each), so the single f32 rounding at
+// the end is the correctly rounded power except within ~1e-15 of a rounding boundary.
+template <unsigned N>
+__device__ __forceinline__ float powi_cr(float x) {
+    double sq = (double)x;
+    double res = 1.0;
+    unsigned n = N;
+    while (n) {
+        if (n & 1u) res *= sq;
+        n >>= 1;
+        if (n) sq *= sq;
+    }
+    return (float)res;
+}
+
+// ---- Stokes vectors and the Mueller matrices of optics/polarization.rs:95-147 --------
+
+struct Stokes {
+    float i, q, u, v;
+};
+__device__ __forceinline__ Stokes scale(Stokes s, float k) {
+    return Stokes{s.i * k, s.q * k, s.u * k, s.v * k};
+}
+__device__ __forceinline__ float intensity(const Stokes &s) { return fmaxf(s.i, 0.0f); }
+__device__ __forceinline__ float degree_of_polarization(const Stokes &s) {
+    if (s.i <= 1e-7f) return 0.0f;
+    return clampf(sqrtf(fmaf(s.v, s.v, fmaf(s.u, s.u, s.q * s.q))) / s.i, 0.0f, 1.0f);
+}
+// f32::midpoint on x86_64 is computed in f64; for finite normal operands that equals
+// (a + b) * 0.5 in f32 (halving commutes with rounding).
+__device__ __forceinline__ float midpoint(float a, float b) { return (a + b) * 0.5f; }
+
+// Mueller for a diagonal-block matrix [[a,b,0,0],[b,a,0,0],[0,0,c,0],[0,0,0,c]].
+__device__ __forceinline__ Stokes apply_abc(Stokes s, float a, float b, float c) {
+    return Stokes{a * s.i + b * s.q, b * s.i + a * s.q, c * s.u, c * s.v};
+}
+// frame_rotation(psi) with c2 = cos 2psi, s2 = sin 2psi.
+__device__ __forceinline__ Stokes apply_rotation(Stokes s, float c2, float s2) {
+    return Stokes{s.i, c2 * s.q + s2 * s.u, (-s2) * s.q + c2 * s.u, s.v};
+}
+// tir_retardation(delta) with cd = cos delta, sd = sin delta.
+__device__ __forceinline__ Stokes apply_retardation(Stokes s, float cd, float sd) {
+    return Stokes{s.i, s.q, cd * s.u + sd * s.v, (-sd) * s.u + cd * s.v};
+}
+__device__ __forceinline__ Stokes apply_fresnel_reflection(Stokes s, float r_s, float r_p) {
+    float rs2 = r_s * r_s;
+    float rp2 = r_p * r_p;
+    return apply_abc(s, midpoint(rs2, rp2), 0.5f * (rs2 - rp2), r_s * r_p);
+}
+__device__ __forceinline__ Stokes apply_fresnel_transmission(Stokes s, float n1, float n2,
+                                                             float cos_i, float cos_t,
+                                                             float t_s, float t_p) {
+    float factor = (n2 * cos_t) / fmaxf(n1 * cos_i, 1e-6f);
+    float ts2 = t_s * t_s * factor;
+    float tp2 = t_p * t_p * factor;
+    return apply_abc(s, midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor);
+}
+
+// ---- exp_f32x8 lane (simd/exp_poly.rs:8-40): Cephes range reduction + degree-5 poly ---
+
+__device__ __forceinline__ float exp_poly(float x) {
+    const float LOG2E = 1.44269504088896340736f;
+    const float C1 = __uint_as_float(0x3F318000u);
+    const float C2 = -2.1219444e-4f;
+    x = clampf(x, -87.33654f, 88.02875f);
+    float n = floorf(fmaf(x, LOG2E, 0.5f));
+    x = fmaf(n, -C1, x);
+    x = fmaf(n, -C2, x);
+    float z = x * x;
+    float p = 1.9875691e-4f;
+    p = fmaf(p, x, 1.3981999e-3f);
+    p = fmaf(p, x, 8.333452e-3f);
+    p = fmaf(p, x, 4.1665796e-2f);
+    p = fmaf(p, x, 1.6666665e-1f);
+    p = fmaf(p, x, 5.0000003e-1f);
+    p = fmaf(p, z, x) + 1.0f;
+    return p * __uint_as_float((uint32_t)(((int32_t)n + 127) << 23));
+}
+
+// ---- CIE 1931 analytic fit, batched form (color/cie1931.rs:48-98) ---------------------
+
+__device__ __forceinline__ float cmf_lobe(float l, float mu, float lo, float hi) {
+    float sigma = l < mu ? lo : hi;
+    float t = (l - mu) / sigma;
+    return exp_poly(-0.5f * t * t);
+}
+__device__ __forceinline__ V3 cmf_x8_channel(float l) {
+    float x0 = cmf_lobe(l, 501.1f, 20.4f, 26.2f);
+    float x1 = cmf_lobe(l, 599.8f, 37.9f, 31.0f);
+    float x2 = cmf_lobe(l, 442.0f, 16.0f, 26.7f);
+    float y0 = cmf_lobe(l, 530.9f, 16.3f, 31.1f);
+    float y1 = cmf_lobe(l, 568.8f, 46.9f, 40.5f);
+    float z0 = cmf_lobe(l, 459.0f, 26.0f, 13.8f);
+    float z1 = cmf_lobe(l, 437.0f, 11.8f, 36.0f);
+    float x = fmaf(0.065f, -x0, fmaf(1.056f, x1, 0.362f * x2));
+    float y = fmaf(0.286f, y0, 0.821f * y1);
+    float z = fmaf(0.681f, z0, 1.217f * z1);
+    return v3(fmaxf(x, 0.0f), fmaxf(y, 0.0f), fmaxf(z, 0.0f));
+}
+
+// ---- Sampling streams (optics/raytracer/sampling.rs) -----------------------------------
+
+constexpr uint32_t FRESNEL_BRANCH_STREAM = 0x9E3779B1u;
+constexpr uint32_t RUSSIAN_ROULETTE_STREAM = 0x517CC1B7u;
+constexpr uint32_t BIREFRINGENT_SPLIT_STREAM = 0x2545F491u;
+constexpr uint32_t MODE_COUPLING_STREAM = 0xCC9E2D51u;
+constexpr uint32_t PIXEL_JITTER_X_ROTATION_STREAM = 0xA511E9B3u;
+constexpr uint32_t PIXEL_JITTER_Y_ROTATION_STREAM = 0x63D81B23u;
+constexpr uint32_t HERO_WAVELENGTH_ROTATION_STREAM = 0x1B873593u;
+
+__host__ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ float unit_draw(uint32_t h) { return (float)h / 4294967295.0f; }
+__device__ __forceinline__ float stream_draw(uint32_t seed, uint32_t bounce, uint32_t stream) {
+    return unit_draw(hash_u32(seed ^ hash_u32(bounce ^ stream)));
+}
+__device__ __forceinline__ float low_discrepancy_base2(uint32_t n) {
+    return (float)__builtin_bitreverse32(n) / 4294967296.0f;
+}
+__device__ __forceinline__ float radical_inverse(uint32_t n, uint32_t base) {
+    uint32_t remaining = n;
+    float val = 0.0f;
+    float inv_base = 1.0f / (float)base;
+    while (remaining > 0) {
+        uint32_t digit = remaining % base;
+        val = fmaf((float)digit, inv_base, val);
+        inv_base /= (float)base;
+        remaining /= base;
+    }
+    return val;
+}
+__device__ __forceinline__ float cranley_patterson(float x, float offset) {
+    float sum = x + offset;
+    return sum - floorf(sum);
+}
+
+// ---- Uniaxial birefringence (optics/birefringence.rs:27-131) ---------------------------
+
+__device__ __forceinline__ float effective_extraordinary_index(float n_o, float n_e,
+                                                               float theta) {
+    if (fabsf(n_o - n_e) < 1e-5f) return n_o;
+    float sin_t, cos_t;
+    sincosf(theta, &sin_t, &cos_t);
+    float os = n_o * sin_t;
+    float denom_sq = fmaf(n_e * cos_t, n_e * cos_t, os * os);
+    if (denom_sq <= 1e-8f) return n_o;
+    return (n_o * n_e) / sqrtf(denom_sq);
+}
+__device__ __forceinline__ float walk_off_angle(float n_o, float n_e, float theta) {
+    if (fabsf(n_o - n_e) < 1e-5f) return 0.0f;
+    float n_o2 = n_o * n_o;
+    float n_e2 = n_e * n_e;
+    float sin_t, cos_t;
+    sincosf(theta, &sin_t, &cos_t);
+    float tan_rho = ((n_o2 - n_e2) * sin_t * cos_t) /
+                    fmaxf(fmaf(n_e2 * cos_t, cos_t, n_o2 * sin_t * sin_t), 1e-6f);
+    return atanf(tan_rho);
+}
+__device__ __forceinline__ V3 extraordinary_poynting_dir(V3 w, V3 c, float n_o, float n_e) {
+    float cos_theta = clampf(dot(w, c), -1.0f, 1.0f);
+    float theta = acosf(fabsf(cos_theta));
+    float delta = walk_off_angle(n_o, n_e, theta);
+    if (fabsf(delta) < 1e-5f) return w;
+    V3 c_proj = normalize_or_zero(sub(c, mul(w, cos_theta)));
+    if (len2(c_proj) < 1e-6f) return w;
+    float sign = cos_theta >= 0.0f ? 1.0f : -1.0f;
+    float sd, cd;
+    sincosf(delta, &sd, &cd);
+    return normalize(sub(mul(w, cd), mul(mul(c_proj, sign), sd)));
+}
+__device__ __forceinline__ void stable_orthonormal_basis(V3 n, V3 *t_out, V3 *b_out) {
+    V3 a = fabsf(n.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    V3 t = normalize_or_zero(sub(a, mul(n, dot(n, a))));
+    *t_out = t;
+    *b_out = cross(n, t);
+}
+__device__ __forceinline__ V3 ordinary_eigen_polarization(V3 w, V3 c) {
+    V3 cr = cross(w, c);
+    if (len2(cr) > 1e-8f) return normalize(cr);
+    V3 t, b;
+    stable_orthonormal_basis(normalize_or_zero(w), &t, &b);
+    return t;
+}
+__device__ __forceinline__ V3 extraordinary_eigen_polarization(V3 w, V3 c) {
+    return normalize_or_zero(cross(w, ordinary_eigen_polarization(w, c)));
+}
+
+// polarization.rs:172-200.
+__device__ __forceinline__ V3 arbitrary_perpendicular(V3 n) {
+    V3 a = fabsf(n.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    return normalize_or_zero(sub(a, mul(n, dot(n, a))));
+}
+
+// refraction.rs:45-50.
+__device__ __forceinline__ float tir_phase_delta(float n1k, float cos_i, float sin_i) {
+    float tan_half = (cos_i * sqrtf(fmaxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f))) /
+                     fmaxf(n1k * sin_i * sin_i, 1e-6f);
+    return 2.0f * atanf(tan_half);
+}
+
+__device__ __forceinline__ void fresnel_rs_rp(float n1k, float n2k, float cos_i, float cos_t,
+                                              float *r_s, float *r_p) {
+    *r_s = fmaf(n2k, -cos_t, n1k * cos_i) / fmaf(n2k, cos_t, n1k * cos_i);
+    *r_p = fmaf(n1k, -cos_t, n2k * cos_i) / fmaf(n1k, cos_t, n2k * cos_i);
+}
+
+}  // namespace gr

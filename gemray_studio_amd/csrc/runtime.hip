@@ -1,0 +1,613 @@
+// runtime.hip -- the C ABI of include/gemray_hip.h: context, validation and decline rules,
+// host-side scene constants, chunked launches, film accumulation and statistics.
+//
+// Host-side arithmetic here (studio rig, white balance, absorption frame) follows the
+// reference's CPU code operation for operation: the reference computes these on the host
+// too (frame.rs:691 compute_illuminant_white_balance) or once per ray with values that do
+// not depend on the ray (optics/studio_rig.rs:50-80).  The library is built with
+// -ffp-contract=off so the host compiler does not fuse them either.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "../../include/gemray_hip.h"
+#include "transport.h"
+
+namespace {
+
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr uint64_t kDefaultChunkTuples = 1ull << 25;
+constexpr uint64_t kMaxChunkTuples = 1ull << 30;
+
+// ---- host restatements of reference host-side maths --------------------------------
+
+struct H3 {
+    float x, y, z;
+};
+inline float hdot(H3 a, H3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+inline H3 hmul(H3 a, float s) { return H3{a.x * s, a.y * s, a.z * s}; }
+inline H3 hsub(H3 a, H3 b) { return H3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline H3 hcross(H3 a, H3 b) {
+    return H3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+inline H3 hnormalize(H3 a) { return hmul(a, 1.0f / std::sqrt(hdot(a, a))); }
+inline H3 hnormalize_or_zero(H3 a) {
+    float rcp = 1.0f / std::sqrt(hdot(a, a));
+    if (std::isfinite(rcp) && rcp > 0.0f) return hmul(a, rcp);
+    return H3{0.0f, 0.0f, 0.0f};
+}
+inline float hclamp(float x, float lo, float hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+// x^n rounded once (the reference's powi, as pinned by its golden vectors).
+inline float hpowi(float x, int n) { return (float)std::pow((double)x, (double)n); }
+
+// blackbody_spectrum (optics/raytracer/environment.rs:12-21).
+float h_blackbody(float lambda_nm, float temp_k) {
+    float t_k = std::fmax(temp_k, 1000.0f);
+    const float h_c_k = 14388000.0f;
+    float exp_val = std::exp(std::fmin(h_c_k / (lambda_nm * t_k), 80.0f));
+    float exp_560 = std::exp(std::fmin(h_c_k / (560.0f * t_k), 80.0f));
+    float denom = std::fmax(exp_val - 1.0f, 1e-6f);
+    float denom_560 = std::fmax(exp_560 - 1.0f, 1e-6f);
+    return hclamp(hpowi(560.0f / lambda_nm, 5) * (denom_560 / denom), 0.01f, 20.0f);
+}
+inline float h_lobe(float x, float mu, float lo, float hi) {
+    float sigma = x < mu ? lo : hi;
+    float t = (x - mu) / sigma;
+    return std::exp(-0.5f * t * t);
+}
+// cie_1931_cmf, scalar form (color/cie1931.rs:29-42).
+H3 h_cmf(float l) {
+    float x = std::fmaf(0.065f, -h_lobe(l, 501.1f, 20.4f, 26.2f),
+                        std::fmaf(1.056f, h_lobe(l, 599.8f, 37.9f, 31.0f), 0.362f * h_lobe(l, 442.0f, 16.0f, 26.7f)));
+    float y = std::fmaf(0.286f, h_lobe(l, 530.9f, 16.3f, 31.1f), 0.821f * h_lobe(l, 568.8f, 46.9f, 40.5f));
+    float z = std::fmaf(0.681f, h_lobe(l, 459.0f, 26.0f, 13.8f), 1.217f * h_lobe(l, 437.0f, 11.8f, 36.0f));
+    return H3{std::fmax(x, 0.0f), std::fmax(y, 0.0f), std::fmax(z, 0.0f)};
+}
+inline H3 h_xyz_to_lms(H3 v) {
+    return H3{std::fmaf(0.8951f, v.x, std::fmaf(0.2664f, v.y, -0.1614f * v.z)),
+              std::fmaf(-0.7502f, v.x, std::fmaf(1.7135f, v.y, 0.0367f * v.z)),
+              std::fmaf(0.0389f, v.x, std::fmaf(-0.0685f, v.y, 1.0296f * v.z))};
+}
+// compute_illuminant_white_balance (optics/raytracer/color.rs:281-315).
+H3 h_white_balance(float temp_k) {
+    H3 w{0.0f, 0.0f, 0.0f};
+    for (int step = 0; step <= 400; ++step) {
+        float lambda = 380.0f + (float)step;
+        H3 c = h_cmf(lambda);
+        float b = h_blackbody(lambda, temp_k);
+        w = H3{w.x + c.x * b, w.y + c.y * b, w.z + c.z * b};
+    }
+    const float dx = 0.3127f, dy = 0.3290f;
+    float ty = std::fmax(w.y, 1e-6f);
+    H3 target{(dx / dy) * ty, ty, ((1.0f - dx - dy) / dy) * ty};
+    H3 src = h_xyz_to_lms(w), tgt = h_xyz_to_lms(target);
+    return H3{src.x > 1e-6f ? tgt.x / src.x : 1.0f, src.y > 1e-6f ? tgt.y / src.y : 1.0f,
+              src.z > 1e-6f ? tgt.z / src.z : 1.0f};
+}
+
+// Per-temperature white-balance cache (the reference caches one per preset,
+// color.rs:329-350).
+struct WbCache {
+    std::mutex mu;
+    float temps[16];
+    H3 vals[16];
+    int n = 0;
+    H3 get(float t) {
+        std::lock_guard<std::mutex> g(mu);
+        for (int i = 0; i < n; ++i)
+            if (temps[i] == t) return vals[i];
+        H3 v = h_white_balance(t);
+        if (n < 16) {
+            temps[n] = t;
+            vals[n] = v;
+            ++n;
+        }
+        return v;
+    }
+};
+WbCache g_wb;
+
+bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+double dispersion_f64(const gr_material *m, double lambda_nm) {
+    double l2 = (lambda_nm * 1e-3) * (lambda_nm * 1e-3);
+    if (m->disp_kind == GR_DISP_SELLMEIER1)
+        return std::sqrt(std::fmax(1.0 + m->disp[0] * l2 / (l2 - m->disp[1]), 1.0));
+    if (m->disp_kind == GR_DISP_SELLMEIER3) {
+        double n2 = 1.0;
+        for (int i = 0; i < 3; ++i) n2 += m->disp[i] * l2 / (l2 - m->disp[3 + i]);
+        return std::sqrt(std::fmax(n2, 1.0));
+    }
+    return m->disp[0] + m->disp[1] / l2 + m->disp[2] / (l2 * l2);
+}
+
+// validate_scene (apps/gemray-worker/src/validate/mod.rs:74-192) plus the GPU decline
+// rules of frame.rs:664-700 for the branches this build does not implement.
+gr_status validate_scene(const gr_scene *s) {
+    if (!s || !s->camera || !s->material || !s->env || !s->planes) return GR_ERR_INVALID;
+    if (s->width == 0 || s->height == 0) return GR_ERR_INVALID;
+    if ((uint64_t)s->width * s->height > GR_MAX_PIXELS) return GR_ERR_INVALID;
+    if (s->max_bounces == 0 || s->max_bounces > GR_MAX_BOUNCES) return GR_ERR_INVALID;
+    if (s->n_planes == 0 || s->n_planes > GR_MAX_PLANES) return GR_ERR_INVALID;
+    for (uint32_t i = 0; i < s->n_planes; ++i) {
+        const float *p = s->planes + 4 * i;
+        if (!finite3(p) || !std::isfinite(p[3])) return GR_ERR_INVALID;
+        double len = std::sqrt((double)p[0] * p[0] + (double)p[1] * p[1] + (double)p[2] * p[2]);
+        if (len <= 1e-6) return GR_ERR_INVALID;
+    }
+    const gr_camera *c = s->camera;
+    if (!finite3(c->origin) || !finite3(c->forward) || !finite3(c->right) || !finite3(c->up) ||
+        !std::isfinite(c->fov_tan))
+        return GR_ERR_INVALID;
+    const gr_material *m = s->material;
+    if (m->disp_kind < 0 || m->disp_kind > 2) return GR_ERR_INVALID;
+    if (!finite3(m->c_axis) || !std::isfinite(m->birefringence_delta)) return GR_ERR_INVALID;
+    if (!std::isfinite(m->absorption_path_scale) || m->absorption_path_scale <= 0.0f) return GR_ERR_INVALID;
+    if (m->n_o_bands < 0 || m->n_o_bands > GR_MAX_BANDS || m->n_e_bands < 0 || m->n_e_bands > GR_MAX_BANDS)
+        return GR_ERR_INVALID;
+    for (double l : {380.0, 589.3, 780.0}) {
+        double n = dispersion_f64(m, l);
+        if (!std::isfinite(n) || n < 1.0 || n > 6.0) return GR_ERR_INVALID;
+    }
+    const gr_env *e = s->env;
+    if (e->kind == GR_ENV_STUDIO) {
+        if (!std::isfinite(e->temp_k) || !std::isfinite(e->spot_mult) || !std::isfinite(e->exposure) ||
+            !std::isfinite(e->light_yaw) || !std::isfinite(e->light_pitch))
+            return GR_ERR_INVALID;
+    } else if (e->kind == GR_ENV_MAP) {
+        if (e->map_w <= 0 || e->map_h <= 0 || !e->map_rgb) return GR_ERR_INVALID;
+        if ((uint64_t)e->map_w * (uint64_t)e->map_h > (1ull << 28)) return GR_ERR_INVALID;
+    } else {
+        return GR_DECLINE_ENV;
+    }
+    // Branches not built here: decline so the caller falls back to its CPU tracer.
+    if (!std::isnan(m->biaxial_delta_beta_alpha)) return GR_DECLINE_MATERIAL;
+    if (m->scattering_sigma_s > 0.0f) return GR_DECLINE_MATERIAL;
+    if (m->edge_rounding_radius > 0.0f) return GR_DECLINE_MATERIAL;
+    if (s->finishes)
+        for (uint32_t i = 0; i < s->n_planes; ++i)
+            if (s->finishes[i] != 0) return GR_DECLINE_MATERIAL;
+    return GR_OK;
+}
+
+void fill_dev_scene(const gr_scene *s, gr::DevScene *d) {
+    std::memset(d, 0, sizeof(*d));
+    const gr_camera *c = s->camera;
+    for (int i = 0; i < 3; ++i) {
+        d->cam_o[i] = c->origin[i];
+        d->cam_f[i] = c->forward[i];
+        d->cam_r[i] = c->right[i];
+        d->cam_u[i] = c->up[i];
+    }
+    d->fov_tan = c->fov_tan;
+    d->width = s->width;
+    d->height = s->height;
+    d->n_planes = s->n_planes;
+    d->max_bounces = s->max_bounces;
+    const gr_material *m = s->material;
+    d->disp_kind = m->disp_kind;
+    for (int i = 0; i < 6; ++i) d->disp[i] = m->disp[i];
+    d->birefringence_delta = m->birefringence_delta;
+    d->anisotropic = (!m->crystal_cubic && std::fabs(m->birefringence_delta) > 1e-4f) ? 1 : 0;
+    d->has_absorption = (m->n_o_bands > 0 || m->n_e_bands > 0) ? 1 : 0;
+    for (int i = 0; i < 3; ++i) d->c_axis[i] = m->c_axis[i];
+    // AbsorptionTensor3::uniaxial frame (birefringence.rs:126-131, 202-209).
+    H3 cn = hnormalize_or_zero(H3{m->c_axis[0], m->c_axis[1], m->c_axis[2]});
+    H3 a = std::fabs(cn.x) > 0.9f ? H3{0.0f, 1.0f, 0.0f} : H3{1.0f, 0.0f, 0.0f};
+    H3 a1 = hnormalize_or_zero(hsub(a, hmul(cn, hdot(cn, a))));
+    H3 a2 = hcross(cn, a1);
+    H3 fr[3] = {a1, a2, cn};
+    for (int j = 0; j < 3; ++j) {
+        d->abs_frame[j][0] = fr[j].x;
+        d->abs_frame[j][1] = fr[j].y;
+        d->abs_frame[j][2] = fr[j].z;
+    }
+    d->n_o_bands = m->n_o_bands;
+    d->n_e_bands = m->n_e_bands;
+    std::memcpy(d->o_bands, m->o_bands, sizeof(d->o_bands));
+    std::memcpy(d->e_bands, m->e_bands, sizeof(d->e_bands));
+    d->absorption_path_scale = m->absorption_path_scale;
+    const gr_env *e = s->env;
+    d->env_kind = e->kind;
+    if (e->kind == GR_ENV_STUDIO) {
+        d->temp_k_clamped = std::fmax(e->temp_k, 1000.0f);
+        d->spot_mult = e->spot_mult;
+        d->exposure = e->exposure;
+        float exp_560 = std::exp(std::fmin(14388000.0f / (560.0f * d->temp_k_clamped), 80.0f));
+        d->denom_560 = std::fmax(exp_560 - 1.0f, 1e-6f);
+        // StudioRig::new (optics/studio_rig.rs:50-80).
+        float cos_lp = std::cos(e->light_pitch), sin_lp = std::sin(e->light_pitch);
+        float cos_ly = std::cos(e->light_yaw), sin_ly = std::sin(e->light_yaw);
+        H3 key = hnormalize(H3{cos_lp * sin_ly, sin_lp, cos_lp * cos_ly});
+        float fill_yaw = std::fmaf(kPiF, 0.78f, e->light_yaw);
+        float fill_pitch = hclamp(e->light_pitch * 0.65f, 0.15f, 1.2f);
+        H3 fill = hnormalize(H3{std::cos(fill_pitch) * std::sin(fill_yaw), std::sin(fill_pitch),
+                                std::cos(fill_pitch) * std::cos(fill_yaw)});
+        d->key[0] = key.x; d->key[1] = key.y; d->key[2] = key.z;
+        d->fill[0] = fill.x; d->fill[1] = fill.y; d->fill[2] = fill.z;
+        const float step = kPiF * 2.0f / 16.0f;
+        for (int i = 0; i < 16; ++i) {
+            float ang = std::fmaf((float)i, step, e->light_yaw);
+            H3 r = hnormalize(H3{std::cos(ang) * 0.75f, sin_lp * 0.8f, std::sin(ang) * 0.75f});
+            d->ring[i][0] = r.x; d->ring[i][1] = r.y; d->ring[i][2] = r.z;
+        }
+        H3 wb = g_wb.get(e->temp_k);
+        d->wb[0] = wb.x; d->wb[1] = wb.y; d->wb[2] = wb.z;
+    } else {
+        // HdrMap: no white balance (environment.rs:238-243), LMS round trip with scale 1.
+        d->wb[0] = d->wb[1] = d->wb[2] = 1.0f;
+        d->map_w = e->map_w;
+        d->map_h = e->map_h;
+    }
+}
+
+}  // namespace
+
+struct gr_scene_dev {
+    gr::DevScene sc;
+    float4 *planes = nullptr;
+    float *map = nullptr;
+};
+
+struct gr_film {
+    float *d = nullptr;
+    uint32_t width = 0, height = 0;
+};
+
+struct gr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    std::mutex mu;
+    uint64_t chunk_tuples = kDefaultChunkTuples;
+    float *xyz = nullptr;
+    uint64_t xyz_cap = 0;  // tuples
+    uint32_t *info = nullptr;
+    uint64_t info_cap = 0;
+    unsigned long long *stats = nullptr;
+    gr_stats last{};
+    char label[256] = {0};
+};
+
+namespace {
+
+#define GR_CHECK(call)                                     \
+    do {                                                   \
+        hipError_t err_ = (call);                          \
+        if (err_ != hipSuccess) {                          \
+            std::fprintf(stderr, "gemray_hip: %s failed: %s\n", #call, hipGetErrorString(err_)); \
+            return GR_ERR_DEVICE;                          \
+        }                                                  \
+    } while (0)
+
+gr_status ensure_xyz(gr_ctx *ctx, uint64_t tuples) {
+    if (tuples <= ctx->xyz_cap) return GR_OK;
+    if (ctx->xyz) GR_CHECK(hipFree(ctx->xyz));
+    ctx->xyz = nullptr;
+    ctx->xyz_cap = 0;
+    GR_CHECK(hipMalloc(&ctx->xyz, tuples * 3 * sizeof(float)));
+    ctx->xyz_cap = tuples;
+    return GR_OK;
+}
+gr_status ensure_info(gr_ctx *ctx, uint64_t tuples) {
+    if (tuples <= ctx->info_cap) return GR_OK;
+    if (ctx->info) GR_CHECK(hipFree(ctx->info));
+    ctx->info = nullptr;
+    ctx->info_cap = 0;
+    GR_CHECK(hipMalloc(&ctx->info, tuples * 3 * sizeof(uint32_t)));
+    ctx->info_cap = tuples;
+    return GR_OK;
+}
+
+gr_status upload_scene(gr_ctx *ctx, const gr_scene *s, gr_scene_dev *out) {
+    fill_dev_scene(s, &out->sc);
+    GR_CHECK(hipMalloc(&out->planes, s->n_planes * sizeof(float4)));
+    GR_CHECK(hipMemcpyAsync(out->planes, s->planes, s->n_planes * sizeof(float4), hipMemcpyHostToDevice,
+                            ctx->stream));
+    if (s->env->kind == GR_ENV_MAP) {
+        size_t bytes = (size_t)s->env->map_w * s->env->map_h * 3 * sizeof(float);
+        GR_CHECK(hipMalloc(&out->map, bytes));
+        GR_CHECK(hipMemcpyAsync(out->map, s->env->map_rgb, bytes, hipMemcpyHostToDevice, ctx->stream));
+        out->sc.map_rgb = out->map;
+    }
+    GR_CHECK(hipStreamSynchronize(ctx->stream));
+    return GR_OK;
+}
+
+void free_scene(gr_scene_dev *s) {
+    if (!s) return;
+    if (s->planes) (void)hipFree(s->planes);
+    if (s->map) (void)hipFree(s->map);
+    s->planes = nullptr;
+    s->map = nullptr;
+}
+
+// Traces pixels [pixel_begin, pixel_begin + n_pixels) x samples [sample_offset, +spp) in
+// launches of at most chunk_tuples tuples.  With film != NULL each pixel's ordered sum is
+// added into film; with xyz_host/info_host != NULL per-tuple outputs are copied out.
+gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, uint32_t n_pixels,
+                    uint32_t sample_offset, uint32_t spp, float *film, float *xyz_host,
+                    uint32_t *info_host) {
+    std::memset(&ctx->last, 0, sizeof(ctx->last));
+    ctx->last.n_planes = s->sc.n_planes;
+    if (spp == 0 || n_pixels == 0) return GR_OK;
+    uint64_t chunk_pixels = ctx->chunk_tuples / spp;
+    if (chunk_pixels == 0) chunk_pixels = 1;
+    if (chunk_pixels > n_pixels) chunk_pixels = n_pixels;
+    uint64_t cap = chunk_pixels * spp;
+    gr_status st = ensure_xyz(ctx, cap);
+    if (st != GR_OK) return st;
+    if (info_host) {
+        st = ensure_info(ctx, cap);
+        if (st != GR_OK) return st;
+    }
+    GR_CHECK(hipMemsetAsync(ctx->stats, 0, GR_EV_COUNT * sizeof(unsigned long long), ctx->stream));
+    GR_CHECK(hipEventRecord(ctx->ev_begin, ctx->stream));
+    double kernel_ms = 0.0;
+    for (uint64_t done = 0; done < n_pixels; done += chunk_pixels) {
+        uint32_t np = (uint32_t)((n_pixels - done) < chunk_pixels ? (n_pixels - done) : chunk_pixels);
+        uint32_t first = pixel_begin + (uint32_t)done;
+        uint32_t n_tuples = np * spp;
+        GR_CHECK(hipEventRecord(ctx->ev_k0, ctx->stream));
+        GR_CHECK(gr_launch_trace(s->sc, s->planes, first, sample_offset, spp, n_tuples, ctx->xyz,
+                                 info_host ? ctx->info : nullptr, ctx->stats, ctx->stream));
+        GR_CHECK(hipEventRecord(ctx->ev_k1, ctx->stream));
+        if (film) GR_CHECK(gr_launch_film_reduce(ctx->xyz, np, spp, first, film, ctx->stream));
+        if (xyz_host)
+            GR_CHECK(hipMemcpyAsync(xyz_host + 3 * done * spp, ctx->xyz, (size_t)n_tuples * 3 * sizeof(float),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        if (info_host)
+            GR_CHECK(hipMemcpyAsync(info_host + 3 * done * spp, ctx->info, (size_t)n_tuples * 3 * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        GR_CHECK(hipEventSynchronize(ctx->ev_k1));
+        float ms = 0.0f;
+        GR_CHECK(hipEventElapsedTime(&ms, ctx->ev_k0, ctx->ev_k1));
+        kernel_ms += ms;
+        ctx->last.launches++;
+    }
+    GR_CHECK(hipEventRecord(ctx->ev_end, ctx->stream));
+    GR_CHECK(hipStreamSynchronize(ctx->stream));
+    float total = 0.0f;
+    GR_CHECK(hipEventElapsedTime(&total, ctx->ev_begin, ctx->ev_end));
+    unsigned long long ev[GR_EV_COUNT];
+    GR_CHECK(hipMemcpy(ev, ctx->stats, sizeof(ev), hipMemcpyDeviceToHost));
+    for (int i = 0; i < GR_EV_COUNT; ++i) ctx->last.events[i] = ev[i];
+    ctx->last.paths = (uint64_t)n_pixels * spp;
+    ctx->last.trace_kernel_ms = kernel_ms;
+    ctx->last.total_ms = total;
+    return GR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gr_status gr_open(gr_ctx **out, const gr_options *opts) {
+    if (!out) return GR_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return GR_DECLINE_NO_DEVICE;
+    int dev = opts ? opts->device : 0;
+    if (dev < 0 || dev >= count) return GR_DECLINE_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return GR_DECLINE_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return GR_DECLINE_NO_DEVICE;
+    gr_ctx *ctx = new (std::nothrow) gr_ctx();
+    if (!ctx) return GR_ERR_DEVICE;
+    ctx->device = dev;
+    if (opts && opts->chunk_tuples) ctx->chunk_tuples = opts->chunk_tuples < kMaxChunkTuples ? opts->chunk_tuples : kMaxChunkTuples;
+    std::snprintf(ctx->label, sizeof(ctx->label), "%s (%s, HIP)", prop.name, prop.gcnArchName);
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev_begin) != hipSuccess || hipEventCreate(&ctx->ev_end) != hipSuccess ||
+        hipEventCreate(&ctx->ev_k0) != hipSuccess || hipEventCreate(&ctx->ev_k1) != hipSuccess ||
+        hipMalloc(&ctx->stats, GR_EV_COUNT * sizeof(unsigned long long)) != hipSuccess) {
+        gr_close(ctx);
+        return GR_ERR_DEVICE;
+    }
+    *out = ctx;
+    return GR_OK;
+}
+
+void gr_close(gr_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->xyz) (void)hipFree(ctx->xyz);
+    if (ctx->info) (void)hipFree(ctx->info);
+    if (ctx->stats) (void)hipFree(ctx->stats);
+    for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_k0, ctx->ev_k1})
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int gr_adapter_label(gr_ctx *ctx, char *buf, size_t len) {
+    if (!ctx) return -1;
+    int n = (int)std::strlen(ctx->label);
+    if (buf && len) std::snprintf(buf, len, "%s", ctx->label);
+    return n;
+}
+
+gr_status gr_set_chunk_tuples(gr_ctx *ctx, uint64_t tuples) {
+    if (!ctx || tuples == 0) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->chunk_tuples = tuples < kMaxChunkTuples ? tuples : kMaxChunkTuples;
+    return GR_OK;
+}
+
+gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, gr_camera *out) {
+    if (!out || !std::isfinite(yaw) || !std::isfinite(pitch) || !std::isfinite(distance) ||
+        !(distance > 0.0f) || !std::isfinite(fov_deg))
+        return GR_ERR_INVALID;
+    float cos_p = std::cos(pitch), sin_p = std::sin(pitch);
+    float cos_y = std::cos(yaw), sin_y = std::sin(yaw);
+    H3 origin{distance * cos_p * sin_y, distance * sin_p, distance * cos_p * cos_y};
+    H3 forward = hnormalize(H3{-origin.x, -origin.y, -origin.z});
+    H3 world_up = std::fabs(cos_p) < 1e-4f ? H3{0.0f, 0.0f, -1.0f} : H3{0.0f, 1.0f, 0.0f};
+    H3 right = hnormalize(hcross(forward, world_up));
+    H3 up = hnormalize(hcross(right, forward));
+    const float rads_per_deg = kPiF / 180.0f;
+    float fov_tan = std::tan(fov_deg * rads_per_deg * 0.5f);
+    H3 v[4] = {origin, forward, right, up};
+    float *dst[4] = {out->origin, out->forward, out->right, out->up};
+    for (int i = 0; i < 4; ++i) {
+        dst[i][0] = v[i].x;
+        dst[i][1] = v[i].y;
+        dst[i][2] = v[i].z;
+    }
+    out->fov_tan = fov_tan;
+    return GR_OK;
+}
+
+gr_status gr_last_stats(gr_ctx *ctx, gr_stats *out) {
+    if (!ctx || !out) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    *out = ctx->last;
+    return GR_OK;
+}
+
+gr_status gr_scene_upload(gr_ctx *ctx, const gr_scene *scene, gr_scene_dev **out) {
+    if (!ctx || !out) return GR_ERR_INVALID;
+    *out = nullptr;
+    gr_status st = validate_scene(scene);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess) return GR_ERR_DEVICE;
+    gr_scene_dev *s = new (std::nothrow) gr_scene_dev();
+    if (!s) return GR_ERR_DEVICE;
+    st = upload_scene(ctx, scene, s);
+    if (st != GR_OK) {
+        free_scene(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return GR_OK;
+}
+
+void gr_scene_free(gr_ctx *ctx, gr_scene_dev *scene) {
+    if (!ctx || !scene) return;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_scene(scene);
+    delete scene;
+}
+
+gr_status gr_film_create(gr_ctx *ctx, uint32_t width, uint32_t height, gr_film **out) {
+    if (!ctx || !out || width == 0 || height == 0 || (uint64_t)width * height > GR_MAX_PIXELS) return GR_ERR_INVALID;
+    *out = nullptr;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess) return GR_ERR_DEVICE;
+    gr_film *f = new (std::nothrow) gr_film();
+    if (!f) return GR_ERR_DEVICE;
+    f->width = width;
+    f->height = height;
+    size_t bytes = (size_t)width * height * 3 * sizeof(float);
+    if (hipMalloc(&f->d, bytes) != hipSuccess || hipMemsetAsync(f->d, 0, bytes, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        if (f->d) (void)hipFree(f->d);
+        delete f;
+        return GR_ERR_DEVICE;
+    }
+    *out = f;
+    return GR_OK;
+}
+
+gr_status gr_film_clear(gr_ctx *ctx, gr_film *film) {
+    if (!ctx || !film) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GR_CHECK(hipSetDevice(ctx->device));
+    GR_CHECK(hipMemsetAsync(film->d, 0, (size_t)film->width * film->height * 3 * sizeof(float), ctx->stream));
+    GR_CHECK(hipStreamSynchronize(ctx->stream));
+    return GR_OK;
+}
+
+void gr_film_free(gr_ctx *ctx, gr_film *film) {
+    if (!ctx || !film) return;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (film->d) (void)hipFree(film->d);
+    delete film;
+}
+
+gr_status gr_film_download(gr_ctx *ctx, gr_film *film, float *xyz_out, size_t n_floats) {
+    if (!ctx || !film || !xyz_out) return GR_ERR_INVALID;
+    size_t need = (size_t)film->width * film->height * 3;
+    if (n_floats != need) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GR_CHECK(hipSetDevice(ctx->device));
+    GR_CHECK(hipMemcpyAsync(xyz_out, film->d, need * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    GR_CHECK(hipStreamSynchronize(ctx->stream));
+    return GR_OK;
+}
+
+gr_status gr_render(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset, uint32_t spp,
+                    gr_film *film) {
+    if (!ctx || !scene || !film) return GR_ERR_INVALID;
+    if (film->width != scene->sc.width || film->height != scene->sc.height) return GR_ERR_INVALID;
+    if (spp > GR_MAX_SAMPLES_PER_CALL) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GR_CHECK(hipSetDevice(ctx->device));
+    return run_trace(ctx, scene, 0, film->width * film->height, sample_offset, spp, film->d, nullptr, nullptr);
+}
+
+gr_status gr_accumulate(gr_ctx *ctx, const gr_scene *scene, uint32_t sample_offset, uint32_t spp,
+                        float *accum_xyz, size_t accum_pixels) {
+    if (!ctx || !scene || !accum_xyz) return GR_ERR_INVALID;
+    gr_status st = validate_scene(scene);
+    if (st != GR_OK) return st;
+    size_t n_pixels = (size_t)scene->width * scene->height;
+    if (accum_pixels != n_pixels) return GR_ERR_INVALID;
+    if (spp > GR_MAX_SAMPLES_PER_CALL) return GR_ERR_INVALID;
+    if (spp == 0) return GR_OK;
+    gr_scene_dev *s = nullptr;
+    st = gr_scene_upload(ctx, scene, &s);
+    if (st != GR_OK) return st;
+    gr_film *f = nullptr;
+    st = gr_film_create(ctx, scene->width, scene->height, &f);
+    if (st == GR_OK) st = gr_render(ctx, s, sample_offset, spp, f);
+    float *host = nullptr;
+    if (st == GR_OK) {
+        host = new (std::nothrow) float[n_pixels * 3];
+        if (!host) st = GR_ERR_DEVICE;
+    }
+    if (st == GR_OK) st = gr_film_download(ctx, f, host, n_pixels * 3);
+    if (st == GR_OK)
+        for (size_t i = 0; i < n_pixels * 3; ++i) accum_xyz[i] += host[i];
+    delete[] host;
+    gr_film_free(ctx, f);
+    gr_scene_free(ctx, s);
+    return st;
+}
+
+gr_status gr_trace_samples(gr_ctx *ctx, const gr_scene *scene, uint32_t pixel_begin, uint32_t pixel_count,
+                           uint32_t sample_offset, uint32_t spp, float *xyz_out, uint32_t *info_out) {
+    if (!ctx || !scene || !xyz_out) return GR_ERR_INVALID;
+    gr_status st = validate_scene(scene);
+    if (st != GR_OK) return st;
+    uint64_t n_pixels = (uint64_t)scene->width * scene->height;
+    if ((uint64_t)pixel_begin + pixel_count > n_pixels) return GR_ERR_INVALID;
+    if (spp > GR_MAX_SAMPLES_PER_CALL) return GR_ERR_INVALID;
+    gr_scene_dev *s = nullptr;
+    st = gr_scene_upload(ctx, scene, &s);
+    if (st != GR_OK) return st;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (hipSetDevice(ctx->device) != hipSuccess) st = GR_ERR_DEVICE;
+        else st = run_trace(ctx, s, pixel_begin, pixel_count, sample_offset, spp, nullptr, xyz_out, info_out);
+    }
+    gr_scene_free(ctx, s);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// transport.h -- the by-value scene block the transport kernel consumes, and the
+// launchers the C ABI calls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gemray_hip.h"
+
+namespace gr {
+
+// Everything one trace launch needs besides the facet planes and the env-map texels.
+// Passed as a kernel argument (it lands in the kernarg segment and is read with scalar
+// loads).  Host-derived constants are computed in host C++ with the same operation order
+// as the reference's host/CPU code: the studio rig (optics/studio_rig.rs:50-80), the
+// white-balance LMS scale (optics/raytracer/color.rs:281-315), the 560 nm blackbody
+// denominator (environment.rs:16-18) and the absorption frame
+// (optics/birefringence.rs:126-131, 202-209).
+struct DevScene {
+    float cam_o[3], cam_f[3], cam_r[3], cam_u[3];
+    float fov_tan;
+    uint32_t width, height, n_planes, max_bounces;
+    // material
+    int32_t disp_kind;
+    float disp[6];
+    float birefringence_delta;
+    int32_t anisotropic;      // crystal not cubic and |delta n| > 1e-4
+    int32_t has_absorption;   // any absorption band present
+    float c_axis[3];
+    float abs_frame[3][3];    // a1, a2, normalized c
+    int32_t n_o_bands, n_e_bands;
+    float o_bands[GR_MAX_BANDS][3];
+    float e_bands[GR_MAX_BANDS][3];
+    float absorption_path_scale;
+    // environment
+    int32_t env_kind;         // GR_ENV_STUDIO / GR_ENV_MAP
+    float temp_k_clamped, spot_mult, exposure, denom_560;
+    float key[3], fill[3], ring[16][3];
+    float wb[3];
+    int32_t map_w, map_h;
+    const float *map_rgb;     // device pointer
+};
+
+}  // namespace gr
+
+hipError_t gr_launch_trace(const gr::DevScene &sc, const float4 *planes, uint32_t pixel_begin,
+                           uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, float *xyz,
+                           uint32_t *info, unsigned long long *stats, hipStream_t stream);
+hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t spp,
+                                 uint32_t pixel_begin, float *film, hipStream_t stream);

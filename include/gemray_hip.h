@@ -1,0 +1,180 @@
+/*
+ * gemray_hip.h -- C ABI of the MI355X (gfx950) spectral transport library.
+ *
+ * Drop-in for gemray's GPU render backend (crates/gemray/src/renderer/gpu_backend.rs,
+ * renderer/gpu/frame.rs): facet planes + material + camera + environment in, an XYZ film
+ * out.  Plain C types only; every pointer argument is borrowed for the duration of the
+ * call and nothing is retained after it returns unless the entry point says so.
+ *
+ * Determinism: every (pixel, absolute sample) result is a pure function of its indices,
+ * and each pixel's samples are summed in sample order, so results are bit-identical for
+ * any chunking, batching or co-scheduling (gpu_backend.rs:45-53).
+ *
+ * Errors: every call returns a gr_status.  Any status other than GR_OK leaves the
+ * caller's film untouched (frame.rs:194-228).  GR_DECLINE_* are the reference's
+ * "decline, use the CPU tracer" outcomes; GR_ERR_* are caller or device errors.
+ */
+#ifndef GEMRAY_HIP_H
+#define GEMRAY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GR_OK = 0,
+    GR_DECLINE_NO_DEVICE = 1, /* no usable gfx950 device (GpuFrameError::Acquire)      */
+    GR_DECLINE_ENV = 2,       /* environment kind not handled (UnsupportedEnvironment) */
+    GR_DECLINE_MATERIAL = 3,  /* material/finish branch not handled (UnsupportedMaterial) */
+    GR_DECLINE_BUDGET = 4,    /* the job does not fit the HBM budget                   */
+    GR_ERR_INVALID = 5,       /* argument outside the limits (validate/mod.rs:23-192)  */
+    GR_ERR_DEVICE = 6,        /* a HIP call failed                                     */
+    GR_ERR_NOT_FOUND = 7,     /* unknown job id                                        */
+    GR_PENDING = 8            /* gr_job_poll: the job has not finished                 */
+} gr_status;
+
+/* Limits (apps/gemray-worker/src/validate/mod.rs:23-57, renderer/buffers.rs:148). */
+#define GR_MAX_PIXELS (7680u * 4320u)
+#define GR_MAX_SAMPLES_PER_CALL 65536u
+#define GR_MAX_BOUNCES 64u
+#define GR_MAX_PLANES 1024u
+#define GR_MAX_BANDS 8
+
+typedef struct gr_ctx gr_ctx;
+
+typedef struct {
+    int32_t device;            /* HIP device ordinal                                    */
+    uint64_t chunk_tuples;     /* (pixel, sample) tuples per launch; 0 = library default */
+    uint64_t hbm_budget_bytes; /* admission budget for served jobs; 0 = 3/4 of free HBM */
+} gr_options;
+
+/* GpuCameraParams / Camera (renderer/buffers.rs:344-367, optics/raytracer/camera.rs:53-59). */
+typedef struct {
+    float origin[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float fov_tan;
+} gr_camera;
+
+enum { GR_DISP_SELLMEIER1 = 0, GR_DISP_SELLMEIER3 = 1, GR_DISP_CAUCHY = 2 };
+
+/* The estimator-visible fields of GemMaterial (optics/materials.rs:32-120), replacing the
+ * 432-byte GpuGemMaterial encoding (renderer/buffers.rs:237-316).  Bands are Gaussian
+ * absorption bands (center_nm, width_nm, peak) per eigenmode axis. */
+typedef struct {
+    int32_t disp_kind;        /* GR_DISP_*                                              */
+    float disp[6];            /* S1: b1,c1 | S3: b0,b1,b2,c0,c1,c2 | Cauchy: a,b,c       */
+    float birefringence_delta;
+    int32_t crystal_cubic;    /* 1 if CrystalSystem::Cubic                               */
+    float c_axis[3];
+    int32_t n_o_bands, n_e_bands;
+    float o_bands[GR_MAX_BANDS][3];
+    float e_bands[GR_MAX_BANDS][3];
+    float absorption_path_scale;
+    /* Branches this build declines with GR_DECLINE_MATERIAL when enabled: */
+    float biaxial_delta_beta_alpha; /* NaN = None (uniaxial or isotropic)             */
+    float scattering_sigma_s;       /* > 0: Henyey-Greenstein inclusions              */
+    float edge_rounding_radius;     /* > 0: edge-rounded shading normals              */
+} gr_material;
+
+/* EnvironmentSource (optics/raytracer/environment.rs:188-196). */
+enum { GR_ENV_STUDIO = 0, GR_ENV_MAP = 1 };
+typedef struct {
+    int32_t kind;
+    /* Studio: LightingPreset::params() (temp_k, spot_mult) plus exposure/yaw/pitch. */
+    float temp_k, spot_mult, exposure, light_yaw, light_pitch;
+    /* Map: an EnvironmentMap, map_w x map_h RGB f32 texels, row-major (a 1x1 map is the
+     * uniform furnace environment). */
+    int32_t map_w, map_h;
+    const float *map_rgb;
+} gr_env;
+
+/* GpuSceneRef (renderer/gpu_backend.rs:85-99). */
+typedef struct {
+    const gr_camera *camera;
+    uint32_t width, height;
+    const float *planes;       /* n_planes x {normal.x, normal.y, normal.z, d}           */
+    uint32_t n_planes;
+    const uint32_t *finishes;  /* NULL = all polished; else n_planes, 0 = polished       */
+    const gr_material *material;
+    uint32_t max_bounces;
+    const gr_env *env;
+} gr_scene;
+
+/* Per-class event counters of the last render (device-side, exact). */
+enum {
+    GR_EV_SEGMENT = 0, GR_EV_MISS, GR_EV_HIT, GR_EV_ROTATE, GR_EV_ABSORB, GR_EV_TIR,
+    GR_EV_REFLECT, GR_EV_REFRACT, GR_EV_ENTRY_ANISO, GR_EV_WALKOFF, GR_EV_RR, GR_EV_COUNT
+};
+typedef struct {
+    uint64_t events[GR_EV_COUNT]; /* events[GR_EV_SEGMENT] = 8-lambda path segments     */
+    uint64_t paths;
+    uint32_t launches;
+    double trace_kernel_ms;       /* sum of trace-kernel durations (HIP events)          */
+    double total_ms;              /* wall time of the call on its stream                 */
+    uint32_t n_planes;
+} gr_stats;
+
+/* GpuBackend::acquire / GpuFrameRenderer::new (gpu_backend.rs:114-126, frame.rs:574-594).
+ * Returns GR_DECLINE_NO_DEVICE (and *out = NULL) when no gfx950 device is usable. */
+gr_status gr_open(gr_ctx **out, const gr_options *opts);
+/* Drop of GpuBackend. */
+void gr_close(gr_ctx *ctx);
+/* GpuBackend::adapter_label (gpu_backend.rs:146-153).  Writes a NUL-terminated label;
+ * returns the label length. */
+int gr_adapter_label(gr_ctx *ctx, char *buf, size_t len);
+/* GpuFrameRenderer::set_chunk_budget_bytes (frame.rs:602), in tuples. */
+gr_status gr_set_chunk_tuples(gr_ctx *ctx, uint64_t tuples);
+
+/* GpuBackend::try_accumulate / GpuFrameRenderer::accumulate (gpu_backend.rs:160-193,
+ * frame.rs:632-793): ADDS each pixel's sum over samples [sample_offset,
+ * sample_offset + spp) into accum_xyz[3 * pixel + c].  accum_pixels must equal
+ * width * height (the reference asserts this, frame.rs:673-677).  Thread-safe: calls on
+ * one context are serialized. */
+gr_status gr_accumulate(gr_ctx *ctx, const gr_scene *scene, uint32_t sample_offset,
+                        uint32_t spp, float *accum_xyz, size_t accum_pixels);
+
+/* The per-(pixel, sample) estimator for a pixel range, for the parity tiers: the dispatch
+ * of transport_main with per-tuple outputs (spectral_transport.wgsl:672-1470,
+ * frame.rs:437-452).  xyz_out receives pixel_count * spp * 3 floats, tuple (p, s) at
+ * (p - pixel_begin) * spp + s.  info_out, when non-NULL, receives 3 u32 per tuple:
+ * segments, termination (0 escaped, 2 Russian roulette, 3 bounce cap) and an FNV-1a hash
+ * of the per-bounce (facet, event) sequence. */
+gr_status gr_trace_samples(gr_ctx *ctx, const gr_scene *scene, uint32_t pixel_begin,
+                           uint32_t pixel_count, uint32_t sample_offset, uint32_t spp,
+                           float *xyz_out, uint32_t *info_out);
+
+/* Camera::new (optics/raytracer/camera.rs:63-93): orbit camera at (yaw, pitch), distance
+ * from the origin, vertical field of view in degrees. */
+gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, gr_camera *out);
+
+/* Statistics of the last gr_accumulate / gr_trace_samples / gr_render call. */
+gr_status gr_last_stats(gr_ctx *ctx, gr_stats *out);
+
+/* ---- Device-resident scenes and films (the served / batched path) ------------------
+ * No reference counterpart as separate calls: GpuFrameRenderer re-uploads every input
+ * per chunk (frame.rs:375-426) and reads the film back per chunk (:799-814).  These keep
+ * scene inputs and the film in HBM so a caller can accumulate progressively
+ * (sample_offset resumes, gpu_backend.rs:45-53) and read the film once. */
+typedef struct gr_scene_dev gr_scene_dev;
+typedef struct gr_film gr_film;
+
+gr_status gr_scene_upload(gr_ctx *ctx, const gr_scene *scene, gr_scene_dev **out);
+void gr_scene_free(gr_ctx *ctx, gr_scene_dev *scene);
+gr_status gr_film_create(gr_ctx *ctx, uint32_t width, uint32_t height, gr_film **out);
+gr_status gr_film_clear(gr_ctx *ctx, gr_film *film);
+void gr_film_free(gr_ctx *ctx, gr_film *film);
+/* Copies the film (width * height * 3 floats) to host memory. */
+gr_status gr_film_download(gr_ctx *ctx, gr_film *film, float *xyz_out, size_t n_floats);
+/* Adds samples [sample_offset, sample_offset + spp) of every pixel into film. */
+gr_status gr_render(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset,
+                    uint32_t spp, gr_film *film);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

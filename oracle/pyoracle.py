@@ -1,0 +1,236 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+OR_EV_NAMES = [
+    "segment", "miss", "hit", "rotate", "absorb", "tir", "reflect", "refract",
+    "entry_aniso", "walkoff", "rr",
+]
+OR_EV_COUNT = len(OR_EV_NAMES)
+
+
+class OrEnv(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("temp_k", ctypes.c_float),
+        ("spot_mult", ctypes.c_float),
+        ("exposure", ctypes.c_float),
+        ("light_yaw", ctypes.c_float),
+        ("light_pitch", ctypes.c_float),
+        ("map_w", ctypes.c_int32),
+        ("map_h", ctypes.c_int32),
+        ("map_rgb", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+class OrCamera(ctypes.Structure):
+    _fields_ = [
+        ("origin", ctypes.c_float * 3),
+        ("forward", ctypes.c_float * 3),
+        ("right", ctypes.c_float * 3),
+        ("up", ctypes.c_float * 3),
+        ("fov_tan", ctypes.c_float),
+    ]
+
+
+class OrPathInfo(ctypes.Structure):
+    _fields_ = [
+        ("segments", ctypes.c_uint32),
+        ("termination", ctypes.c_uint32),
+        ("facet_hash", ctypes.c_uint32),
+    ]
+
+
+PATH_INFO_DTYPE = np.dtype([("segments", "<u4"), ("termination", "<u4"), ("facet_hash", "<u4")])
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _f3():
+    return ctypes.c_float * 3
+
+
+def _declare(L):
+    fp = ctypes.POINTER(ctypes.c_float)
+    u32 = ctypes.c_uint32
+    L.or_hash_u32.argtypes = [u32]
+    L.or_hash_u32.restype = u32
+    L.or_sample_draws.argtypes = [u32, u32, ctypes.POINTER(u32), fp]
+    L.or_radical_inverse.argtypes = [u32, u32]
+    L.or_radical_inverse.restype = ctypes.c_float
+    L.or_wrapped_hero_wavelengths.argtypes = [ctypes.c_float, fp]
+    L.or_exp_f32x8.argtypes = [fp, fp]
+    L.or_cie_1931_cmf.argtypes = [ctypes.c_float, fp]
+    L.or_cie_1931_cmf_x8.argtypes = [fp, fp]
+    L.or_dispersion_evaluate.argtypes = [ctypes.c_void_p, ctypes.c_float]
+    L.or_dispersion_evaluate.restype = ctypes.c_float
+    L.or_blackbody.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.or_blackbody.restype = ctypes.c_float
+    L.or_white_balance.argtypes = [ctypes.c_float, fp]
+    L.or_sample_studio_environment.argtypes = [fp] + [ctypes.c_float] * 6
+    L.or_sample_studio_environment.restype = ctypes.c_float
+    L.or_env_map_radiance.argtypes = [ctypes.POINTER(OrEnv), fp, ctypes.c_float]
+    L.or_env_map_radiance.restype = ctypes.c_float
+    L.or_analytic_furnace_target.argtypes = [ctypes.c_float, fp]
+    L.or_standard_round_brilliant.argtypes = [fp]
+    L.or_standard_round_brilliant.restype = ctypes.c_int
+    L.or_plane_new.argtypes = [fp, ctypes.c_float, fp]
+    L.or_camera_new.argtypes = [ctypes.c_float] * 4 + [ctypes.POINTER(OrCamera)]
+    L.or_generate_ray.argtypes = [ctypes.POINTER(OrCamera)] + [ctypes.c_float] * 6 + [fp, fp]
+    L.or_intersect.argtypes = [fp, ctypes.c_int, fp, fp, fp, fp, ctypes.POINTER(ctypes.c_int)]
+    L.or_intersect.restype = ctypes.c_int
+    L.or_effective_extraordinary_index.argtypes = [ctypes.c_float] * 3
+    L.or_effective_extraordinary_index.restype = ctypes.c_float
+    L.or_walk_off_angle.argtypes = [ctypes.c_float] * 3
+    L.or_walk_off_angle.restype = ctypes.c_float
+    L.or_extraordinary_poynting_dir.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float, fp]
+    L.or_tir_phase_delta.argtypes = [ctypes.c_float] * 3
+    L.or_tir_phase_delta.restype = ctypes.c_float
+    L.or_signed_frame_rotation_psi.argtypes = [fp, fp, fp]
+    L.or_signed_frame_rotation_psi.restype = ctypes.c_float
+    L.or_trace_spectral_ray.argtypes = [
+        fp, fp, fp, ctypes.c_int, ctypes.c_void_p, u32, ctypes.POINTER(OrEnv), u32,
+        ctypes.c_float, fp, ctypes.POINTER(OrPathInfo), ctypes.POINTER(ctypes.c_uint64),
+    ]
+    L.or_render.argtypes = [
+        ctypes.POINTER(OrCamera), u32, u32, fp, ctypes.c_int, ctypes.c_void_p, u32,
+        ctypes.POINTER(OrEnv), u32, u32, u32, u32, ctypes.c_int, fp, fp, ctypes.c_void_p,
+        ctypes.POINTER(ctypes.c_uint64),
+    ]
+    L.or_render.restype = ctypes.c_uint64
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def f32arr(v) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float32))
+
+
+def bits(v) -> int:
+    return int(np.float32(v).view(np.uint32))
+
+
+# Studio presets (optics/raytracer/environment.rs:81-100): (temp_k, spot_mult).
+PRESETS = {
+    "Daylight": (6500.0, 1.0),
+    "Incandescent": (3200.0, 1.2),
+    "RingLights": (5000.0, 1.6),
+    "DarkSpotlight": (6000.0, 2.4),
+}
+
+
+class Env:
+    """Keeps the map buffer alive alongside the OrEnv struct."""
+
+    def __init__(self, struct: OrEnv, keep=None):
+        self.struct = struct
+        self.keep = keep
+
+
+def studio_env(preset: str, exposure: float, light_yaw: float, light_pitch: float) -> Env:
+    t, s = PRESETS[preset]
+    e = OrEnv()
+    e.kind = 0
+    e.temp_k, e.spot_mult, e.exposure, e.light_yaw, e.light_pitch = t, s, exposure, light_yaw, light_pitch
+    return Env(e)
+
+
+def map_env(rgb: np.ndarray) -> Env:
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w, _ = rgb.shape
+    e = OrEnv()
+    e.kind = 1
+    e.map_w, e.map_h = w, h
+    e.map_rgb = fptr(rgb)
+    return Env(e, rgb)
+
+
+def uniform_env(l0: float) -> Env:
+    return map_env(np.full((1, 1, 3), l0, dtype=np.float32))
+
+
+def standard_round_brilliant() -> np.ndarray:
+    out = np.zeros((74, 4), dtype=np.float32)
+    n = lib().or_standard_round_brilliant(fptr(out))
+    assert n == 74
+    return out
+
+
+def camera_new(yaw, pitch, distance, fov_deg) -> OrCamera:
+    c = OrCamera()
+    lib().or_camera_new(yaw, pitch, distance, fov_deg, ctypes.byref(c))
+    return c
+
+
+def trace(origin, direction, planes, material_struct, max_bounces, env: Env, seed, hero_rand,
+          events: Optional[np.ndarray] = None):
+    o = f32arr(origin)
+    d = f32arr(direction)
+    p = np.ascontiguousarray(planes, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    info = OrPathInfo()
+    ev = None
+    if events is not None:
+        ev = events.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    lib().or_trace_spectral_ray(
+        fptr(o), fptr(d), fptr(p), p.shape[0], ctypes.byref(material_struct), max_bounces,
+        ctypes.byref(env.struct), seed, hero_rand, fptr(out), ctypes.byref(info), ev,
+    )
+    return out, info
+
+
+def render(camera: OrCamera, width, height, planes, material_struct, max_bounces, env: Env,
+           first_sample, spp, accum: Optional[np.ndarray] = None, pixel_begin=0,
+           pixel_end=None, threads=None, per_sample=False, events: Optional[np.ndarray] = None):
+    """render_core::trace_into restated; returns (accum, per_sample_xyz, per_sample_info,
+    total_segments)."""
+    if pixel_end is None:
+        pixel_end = width * height
+    if threads is None:
+        threads = os.cpu_count() or 1
+    p = np.ascontiguousarray(planes, dtype=np.float32)
+    if accum is None:
+        accum = np.zeros((height * width, 3), dtype=np.float32)
+    n = (pixel_end - pixel_begin) * spp
+    xyz = np.zeros((n, 3), dtype=np.float32) if per_sample else None
+    info = np.zeros(n, dtype=PATH_INFO_DTYPE) if per_sample else None
+    ev = None
+    if events is not None:
+        ev = events.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    segs = lib().or_render(
+        ctypes.byref(camera), width, height, fptr(p), p.shape[0], ctypes.byref(material_struct),
+        max_bounces, ctypes.byref(env.struct), first_sample, spp, pixel_begin, pixel_end, threads,
+        fptr(accum), fptr(xyz) if xyz is not None else None,
+        info.ctypes.data if info is not None else None, ev,
+    )
+    return accum, xyz, info, int(segs)

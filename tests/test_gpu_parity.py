@@ -1,0 +1,209 @@
+"""HIP transport path vs the oracle, through the C ABI (needs an MI355X).
+
+Tiers (after the reference's harness, crates/gemray/docs/gpu.md:68-95):
+  T1  per-path exactness: segment count, termination reason and the FNV hash of the
+      per-bounce (facet, event, eigenmode) sequence agree with the oracle for at least
+      SEQ_MATCH_MIN of the paths (a path can only diverge where a branch decision sits
+      within a few ULP of its threshold, since device and host libm differ by ULPs);
+  T2  per-path XYZ: for paths with identical sequences the XYZ agrees within XYZ_RTOL of
+      the path's largest component;
+  T0  determinism: re-runs and any chunking are bit-identical; the film equals the
+      per-tuple outputs summed in sample order;
+  furnace anchor: a colourless stone in a uniform environment converges to the
+      analytic target (estimator_check/furnace.rs:157-302);
+  T3  statistical image comparison against the oracle on disjoint sample sets
+      (estimator_check/image_comparison.rs:52-58, 221-238).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from gemray_studio_amd import backend as B
+from gemray_studio_amd import materials as M
+from gemray_studio_amd import scenes
+from scene_util import Case
+
+pytestmark = pytest.mark.gpu
+
+SEQ_MATCH_MIN = 0.995   # T1 threshold (fraction of paths with identical sequences)
+XYZ_RTOL = 2e-4         # T2: |gpu - oracle| <= XYZ_RTOL * max(|oracle xyz|) per path
+XYZ_RTOL_MEDIAN = 2e-6  # T2: median of that normalized error
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    b = B.GpuBackend.acquire()
+    assert b.available, "no gfx950 device: the HIP path is required for these tests"
+    yield b
+    b.close()
+
+
+@pytest.fixture(scope="module")
+def srb():
+    return po.standard_round_brilliant()
+
+
+def _cases(srb):
+    env_small = scenes.procedural_env(11, 256, 128)
+    pleo = scenes.pleochroic_sapphire(3)
+    return [
+        Case("diamond-ringlights", M.by_name("Diamond"), srb, B.Studio("RingLights", 1.0, 0.85, 0.95), 24, 24, 12),
+        Case("cz-daylight", M.by_name("Cubic Zirconia"), srb, B.Studio("Daylight", 1.0, 0.4, 0.35), 24, 24, 16),
+        Case("sapphire-ringlights", M.by_name("Sapphire"), srb, B.Studio("RingLights", 1.0, 0.85, 0.95), 24, 24, 12),
+        Case("ruby-incandescent", M.by_name("Ruby"), srb, B.Studio("Incandescent", 1.3, 0.0, 1.2), 24, 24, 16),
+        Case("moissanite-dark", M.by_name("Synthetic Moissanite"), srb, B.Studio("DarkSpotlight", 1.0, 2.1, 0.4), 24, 24, 16),
+        Case("zircon-daylight", M.by_name("Zircon"), srb, B.Studio("Daylight", 1.0, 3.14159265, 0.3), 24, 24, 32),
+        Case("tourmaline-map", M.by_name("Tourmaline"), srb, B.HdrMap(env_small), 24, 24, 16),
+        Case("spinel-map", M.by_name("Spinel"), scenes.round_brilliant(), B.HdrMap(env_small), 24, 24, 16),
+        Case("pleochroic-oval-map", pleo, scenes.oval_brilliant(), B.HdrMap(env_small), 24, 24, 32),
+        Case("furnace-uniform", M.furnace_material(), srb, B.HdrMap.uniform(2.5), 24, 24, 12),
+        Case("stress-stepcut", scenes.stress_material(), scenes.step_cut(), B.HdrMap(env_small), 16, 16, 64),
+    ]
+
+
+CASE_IDS = ["diamond-ringlights", "cz-daylight", "sapphire-ringlights", "ruby-incandescent",
+            "moissanite-dark", "zircon-daylight", "tourmaline-map", "spinel-map",
+            "pleochroic-oval-map", "furnace-uniform", "stress-stepcut"]
+
+
+@pytest.mark.parametrize("idx", range(len(CASE_IDS)), ids=CASE_IDS)
+def test_per_path_parity_with_oracle(gpu, srb, idx):
+    case = _cases(srb)[idx]
+    spp = 4
+    n_pix = case.width * case.height
+    print(f"[parity] {case.name} {case.width}x{case.height}x{spp} P={case.planes.shape[0]}", flush=True)
+    xyz_g, info_g = gpu.trace_samples(case.product_scene(), 0, n_pix, 0, spp)
+    stats = gpu.last_stats()
+    ev = np.zeros(po.OR_EV_COUNT, dtype=np.uint64)
+    _, xyz_o, info_o, segs_o = case.oracle_render(0, spp, per_sample=True, events=ev)
+    seg_o, term_o, hash_o = info_o["segments"], info_o["termination"], info_o["facet_hash"]
+    same = (info_g[:, 0] == seg_o) & (info_g[:, 1] == term_o) & (info_g[:, 2] == hash_o)
+    frac = same.mean()
+    # T1
+    assert frac >= SEQ_MATCH_MIN, f"{case.name}: only {frac:.4f} of paths match"
+    # T2 on matched paths
+    scale = np.maximum(np.abs(xyz_o).max(axis=1), 1e-30)
+    err = (np.abs(xyz_g - xyz_o).max(axis=1) / scale)[same]
+    print(f"[parity] {case.name}: match={frac:.5f} max_rel={err.max():.3e} median_rel={np.median(err):.3e}"
+          f" segments gpu={stats['segments']} oracle={segs_o}", flush=True)
+    assert np.all(np.isfinite(xyz_g))
+    assert err.max() <= XYZ_RTOL, f"{case.name}: worst matched-path rel err {err.max():.3e}"
+    assert np.median(err) <= XYZ_RTOL_MEDIAN
+    # Segment accounting: device counter == sum of per-path segments; oracle agrees on
+    # matched paths.
+    assert stats["segments"] == int(info_g[:, 0].sum())
+    assert int(info_g[same, 0].sum()) == int(seg_o[same].sum())
+    if frac == 1.0:
+        gpu_ev = [stats["events"][n] for n in B._lib.EV_NAMES]
+        assert gpu_ev == [int(v) for v in ev]
+
+
+def test_film_is_ordered_sum_of_tuples_and_chunk_invariant(gpu, srb):
+    case = _cases(srb)[2]  # sapphire: anisotropic + absorbing
+    spp = 8
+    n_pix = case.width * case.height
+    scene = case.product_scene()
+    xyz, _ = gpu.trace_samples(scene, 0, n_pix, 5, spp, with_info=False)
+    expect = np.zeros((n_pix, 3), dtype=np.float32)
+    per = xyz.reshape(n_pix, spp, 3)
+    for s in range(spp):  # sequential f32 sum in sample order (frame.rs:806-812)
+        expect = (expect + per[:, s, :]).astype(np.float32)
+    accum = np.zeros((n_pix, 3), dtype=np.float32)
+    assert gpu.try_accumulate(scene, 5, spp, accum)
+    assert np.array_equal(accum.view(np.uint32), expect.view(np.uint32))
+    # Re-run and forced tiny chunks are bit-identical (frame.rs:983-1042).
+    accum2 = np.zeros_like(accum)
+    gpu.set_chunk_tuples(13 * spp)
+    try:
+        assert gpu.try_accumulate(scene, 5, spp, accum2)
+    finally:
+        gpu.set_chunk_tuples(1 << 25)
+    assert np.array_equal(accum.view(np.uint32), accum2.view(np.uint32))
+
+
+def test_accumulate_adds_into_existing_film_and_splits_are_additive(gpu, srb):
+    case = _cases(srb)[0]
+    scene = case.product_scene()
+    n_pix = case.width * case.height
+    base = np.full((n_pix, 3), 0.25, dtype=np.float32)
+    whole = base.copy()
+    assert gpu.try_accumulate(scene, 0, 8, whole)
+    a = np.zeros((n_pix, 3), np.float32)
+    b = np.zeros((n_pix, 3), np.float32)
+    assert gpu.try_accumulate(scene, 0, 4, a)
+    assert gpu.try_accumulate(scene, 4, 4, b)
+    split = a + b + 0.25
+    rel = np.abs(whole - split) / np.maximum(np.maximum(np.abs(whole), np.abs(split)), 1e-6)
+    assert rel.max() < 1e-3  # render_core/mod.rs:374-399
+
+
+def test_device_resident_film_progressive(gpu, srb):
+    case = _cases(srb)[3]
+    scene = case.product_scene()
+    n_pix = case.width * case.height
+    ds = gpu.upload(scene)
+    film = gpu.film(case.width, case.height)
+    try:
+        gpu.render(ds, 0, 6, film)
+        gpu.render(ds, 6, 6, film)
+        got = film.download()
+    finally:
+        film.free()
+        ds.free()
+    a = np.zeros((n_pix, 3), np.float32)
+    assert gpu.try_accumulate(scene, 0, 6, a)
+    assert gpu.try_accumulate(scene, 6, 6, a)
+    assert np.array_equal(got.view(np.uint32), a.view(np.uint32))
+
+
+def test_furnace_anchor(gpu, srb):
+    """Colourless n=1.5 stone, uniform l0 = 2.5, 12 bounces, 32x32 x 600 spp
+    (estimator_check/furnace.rs:157-239): relative error <= 0.02 against the analytic
+    target for both engines and |z| <= 4 between them."""
+    case = Case("furnace", M.furnace_material(), srb, B.HdrMap.uniform(2.5), 32, 32, 12)
+    spp = 600
+    n_pix = case.width * case.height
+    xyz_g, _ = gpu.trace_samples(case.product_scene(), 0, n_pix, 0, spp, with_info=False)
+    _, xyz_o, _, _ = case.oracle_render(0, spp, per_sample=True)
+    target = np.zeros(3, np.float32)
+    po.lib().or_analytic_furnace_target(2.5, po.fptr(target))
+    for name, x in (("gpu", xyz_g), ("oracle", xyz_o)):
+        rel = np.abs(x.astype(np.float64).mean(axis=0) - target) / np.abs(target)
+        assert rel.max() <= 0.02, (name, rel)
+    mg, mo = xyz_g.astype(np.float64).mean(0), xyz_o.astype(np.float64).mean(0)
+    se = np.sqrt(xyz_g.astype(np.float64).var(0) / len(xyz_g) + xyz_o.astype(np.float64).var(0) / len(xyz_o))
+    z = (mg - mo) / se
+    assert np.all(np.abs(z) <= 4.0), z
+
+
+def test_statistical_image_comparison(gpu, srb):
+    """T3 on disjoint samples: 32x32 px, 256 spp per engine; per-pixel Welch z; pass if
+    |z| > 3 pixels <= max(5 x binomial expectation, 10) (image_comparison.rs:52-58)."""
+    pleo = scenes.pleochroic_sapphire(7)
+    case = Case("t3", pleo, scenes.oval_brilliant(), B.HdrMap(scenes.procedural_env(5, 256, 128)), 32, 32, 32)
+    spp = 256
+    n_pix = case.width * case.height
+    xyz_g, _ = gpu.trace_samples(case.product_scene(), 0, n_pix, 0, spp, with_info=False)
+    _, xyz_o, _, _ = case.oracle_render(spp, spp, per_sample=True)  # disjoint sample range
+    g = xyz_g.astype(np.float64).reshape(n_pix, spp, 3)[:, :, 1]
+    o = xyz_o.astype(np.float64).reshape(n_pix, spp, 3)[:, :, 1]
+    se = np.sqrt(g.var(1, ddof=1) / spp + o.var(1, ddof=1) / spp)
+    z = np.where(se > 0, (g.mean(1) - o.mean(1)) / np.maximum(se, 1e-30), 0.0)
+    n_bad = int((np.abs(z) > 3.0).sum())
+    expected = n_pix * 0.0027
+    assert n_bad <= max(5 * expected, 10), n_bad
+
+
+def test_declines_leave_film_untouched(gpu, srb):
+    accum = np.full((8 * 8, 3), 3.0, np.float32)
+    base = dict(camera=B.camera_new(0.35, 0.28, 5.0, 18.0), width=8, height=8, planes=srb,
+                max_bounces=8, environment=B.Studio())
+    frosted = [0] * 74
+    frosted[40] = 1
+    assert not gpu.try_accumulate(B.SceneRef(material=M.by_name("Diamond"), facet_finishes=frosted, **base), 0, 2, accum)
+    assert not gpu.try_accumulate(B.SceneRef(material=M.by_name("Alexandrite"), **base), 0, 2, accum)
+    assert np.all(accum == 3.0)
+    assert gpu.try_accumulate(B.SceneRef(material=M.by_name("Diamond"), **base), 0, 2, accum)
+    assert not np.all(accum == 3.0)
+    with pytest.raises(AssertionError):
+        gpu.try_accumulate(B.SceneRef(material=M.by_name("Diamond"), **base), 0, 2, np.zeros((5, 3), np.float32))
