@@ -242,7 +242,7 @@ class GpuBackend:
         self._require()
         n = pixel_count * spp
         xyz = np.zeros((n, 3), dtype=np.float32)
-        info = np.zeros((n, 3), dtype=np.uint32) if with_info else None
+        info = np.zeros((n, 4), dtype=np.uint32) if with_info else None
         packed = _Packed(scene)
         with self._mu:
             st = self._L.gr_trace_samples(

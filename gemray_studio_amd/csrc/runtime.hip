@@ -302,7 +302,7 @@ gr_status ensure_info(gr_ctx *ctx, uint64_t tuples) {
     if (ctx->info) GR_CHECK(hipFree(ctx->info));
     ctx->info = nullptr;
     ctx->info_cap = 0;
-    GR_CHECK(hipMalloc(&ctx->info, tuples * 3 * sizeof(uint32_t)));
+    GR_CHECK(hipMalloc(&ctx->info, tuples * 4 * sizeof(uint32_t)));
     ctx->info_cap = tuples;
     return GR_OK;
 }
@@ -365,7 +365,7 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
             GR_CHECK(hipMemcpyAsync(xyz_host + 3 * done * spp, ctx->xyz, (size_t)n_tuples * 3 * sizeof(float),
                                     hipMemcpyDeviceToHost, ctx->stream));
         if (info_host)
-            GR_CHECK(hipMemcpyAsync(info_host + 3 * done * spp, ctx->info, (size_t)n_tuples * 3 * sizeof(uint32_t),
+            GR_CHECK(hipMemcpyAsync(info_host + 4 * done * spp, ctx->info, (size_t)n_tuples * 4 * sizeof(uint32_t),
                                     hipMemcpyDeviceToHost, ctx->stream));
         GR_CHECK(hipEventSynchronize(ctx->ev_k1));
         float ms = 0.0f;

@@ -141,9 +141,10 @@ gr_status gr_accumulate(gr_ctx *ctx, const gr_scene *scene, uint32_t sample_offs
 /* The per-(pixel, sample) estimator for a pixel range, for the parity tiers: the dispatch
  * of transport_main with per-tuple outputs (spectral_transport.wgsl:672-1470,
  * frame.rs:437-452).  xyz_out receives pixel_count * spp * 3 floats, tuple (p, s) at
- * (p - pixel_begin) * spp + s.  info_out, when non-NULL, receives 3 u32 per tuple:
- * segments, termination (0 escaped, 2 Russian roulette, 3 bounce cap) and an FNV-1a hash
- * of the per-bounce (facet, event) sequence. */
+ * (p - pixel_begin) * spp + s.  info_out, when non-NULL, receives 4 u32 per tuple:
+ * segments, termination (0 escaped, 2 Russian roulette, 3 bounce cap), an FNV-1a hash of
+ * the per-bounce (facet, event, eigenmode) sequence, and an FNV-1a hash of the per-bounce
+ * channel fates (channels past their critical angle, channels with zero path pdf). */
 gr_status gr_trace_samples(gr_ctx *ctx, const gr_scene *scene, uint32_t pixel_begin,
                            uint32_t pixel_count, uint32_t sample_offset, uint32_t spp,
                            float *xyz_out, uint32_t *info_out);

@@ -914,6 +914,7 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
 
     uint32_t term_bounce = max_bounces, term_reason = OR_TERM_CAP;
     uint32_t fh = 2166136261u;
+    uint32_t fate_h = 2166136261u;
 
     for (uint32_t bounce = 0; bounce < max_bounces; ++bounce) {
         if (ev) ev[OR_EV_SEGMENT]++;
@@ -1028,6 +1029,14 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
             is_extra = stream_draw(rng_seed, bounce, MODE_COUPLING_STREAM) < 0.5f;
         fnv(fh, (uint32_t)hit.facet);
         fnv(fh, event | (is_extra ? 16u : 0u));
+        // Per-channel fates of this bounce: channels past their own critical angle, and
+        // channels whose path pdf is zero (chromatically terminated).
+        uint32_t fate = 0;
+        for (int k = 0; k < 8; ++k) {
+            if (g.sin2_t_ch[k] > 1.0f) fate |= 1u << k;
+            if (path_pdf[k] == 0.0f) fate |= 1u << (8 + k);
+        }
+        fnv(fate_h, fate);
 
         // apply_russian_roulette (transport.rs:204-220).
         if (bounce > 4) {
@@ -1066,6 +1075,7 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
         info->segments = term_reason == OR_TERM_CAP ? max_bounces : term_bounce + 1;
         info->termination = term_reason;
         info->facet_hash = fh;
+        info->fate_hash = fate_h;
     }
 }
 

@@ -75,6 +75,7 @@ typedef struct {
     uint32_t segments;     /* bounce+1 on escape / RR, max_bounces on HitCap  */
     uint32_t termination;  /* OR_TERM_*                                        */
     uint32_t facet_hash;   /* FNV-1a over (facet index, event) per bounce      */
+    uint32_t fate_hash;    /* FNV-1a over per-bounce channel-fate masks        */
 } or_path_info;
 
 uint32_t or_hash_u32(uint32_t x);

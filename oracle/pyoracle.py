@@ -51,10 +51,12 @@ class OrPathInfo(ctypes.Structure):
         ("segments", ctypes.c_uint32),
         ("termination", ctypes.c_uint32),
         ("facet_hash", ctypes.c_uint32),
+        ("fate_hash", ctypes.c_uint32),
     ]
 
 
-PATH_INFO_DTYPE = np.dtype([("segments", "<u4"), ("termination", "<u4"), ("facet_hash", "<u4")])
+PATH_INFO_DTYPE = np.dtype([("segments", "<u4"), ("termination", "<u4"), ("facet_hash", "<u4"),
+                            ("fate_hash", "<u4")])
 
 _lib = None
 
@@ -73,8 +75,26 @@ def lib():
     return _lib
 
 
-def _f3():
-    return ctypes.c_float * 3
+class use_library:
+    """Context manager: route pyoracle calls to another oracle build (e.g. liboracle_cr.so,
+    the restatement with correctly rounded transcendentals)."""
+
+    def __init__(self, name: str):
+        self.path = os.path.join(HERE, name)
+
+    def __enter__(self):
+        global _lib
+        lib()
+        self.saved = _lib
+        alt = ctypes.CDLL(self.path)
+        _declare(alt)
+        _lib = alt
+        return alt
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
 
 
 def _declare(L):
