@@ -30,6 +30,8 @@ STATUS_NAMES = {
 
 GR_ENV_STUDIO = 0
 GR_ENV_MAP = 1
+GR_TRACE_PLAIN = 0
+GR_TRACE_WAVEFRONT = 1
 EV_NAMES = ["segment", "miss", "hit", "rotate", "absorb", "tir", "reflect", "refract",
             "entry_aniso", "walkoff", "rr"]
 GR_EV_COUNT = len(EV_NAMES)
@@ -70,7 +72,8 @@ class Scene(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("events", ctypes.c_uint64 * GR_EV_COUNT), ("paths", ctypes.c_uint64),
                 ("launches", ctypes.c_uint32), ("trace_kernel_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double), ("n_planes", ctypes.c_uint32)]
+                ("total_ms", ctypes.c_double), ("n_planes", ctypes.c_uint32),
+                ("finish_kernel_ms", ctypes.c_double)]
 
 
 # Every symbol include/gemray_hip.h declares.
@@ -78,6 +81,7 @@ EXPORTS = [
     "gr_open", "gr_close", "gr_adapter_label", "gr_set_chunk_tuples", "gr_accumulate",
     "gr_trace_samples", "gr_last_stats", "gr_scene_upload", "gr_scene_free", "gr_film_create",
     "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_camera_new",
+    "gr_set_trace_mode",
 ]
 
 _lib = None
@@ -127,6 +131,8 @@ def load():
     L.gr_render.restype = ctypes.c_int
     L.gr_camera_new.argtypes = [ctypes.c_float] * 4 + [ctypes.POINTER(Camera)]
     L.gr_camera_new.restype = ctypes.c_int
+    L.gr_set_trace_mode.argtypes = [vp, ctypes.c_int]
+    L.gr_set_trace_mode.restype = ctypes.c_int
     _lib = L
     return L
 

@@ -266,6 +266,12 @@ class GpuBackend:
             st = self._L.gr_render(self._ctx, scene.handle, sample_offset, spp, film.handle)
         check(st, "gr_render")
 
+    def set_trace_mode(self, mode: str) -> None:
+        """"plain" (one lane per tuple) or "wavefront" (persistent lanes + finish pass)."""
+        self._require()
+        code = {"plain": _lib.GR_TRACE_PLAIN, "wavefront": _lib.GR_TRACE_WAVEFRONT}[mode]
+        check(self._L.gr_set_trace_mode(self._ctx, code), "gr_set_trace_mode")
+
     def set_chunk_tuples(self, tuples: int) -> None:
         self._require()
         check(self._L.gr_set_chunk_tuples(self._ctx, tuples), "gr_set_chunk_tuples")
@@ -280,6 +286,7 @@ class GpuBackend:
             "paths": int(s.paths),
             "launches": int(s.launches),
             "trace_kernel_ms": float(s.trace_kernel_ms),
+            "finish_kernel_ms": float(s.finish_kernel_ms),
             "total_ms": float(s.total_ms),
             "n_planes": int(s.n_planes),
         }

@@ -200,6 +200,15 @@ __device__ __forceinline__ float effective_extraordinary_index(float n_o, float 
     if (denom_sq <= 1e-8f) return n_o;
     return (n_o * n_e) / sqrtf(denom_sq);
 }
+// The same index with sin/cos of theta supplied by the caller (shared across channels).
+__device__ __forceinline__ float effective_extraordinary_index_sc(float n_o, float n_e,
+                                                                  float sin_t, float cos_t) {
+    if (fabsf(n_o - n_e) < 1e-5f) return n_o;
+    float os = n_o * sin_t;
+    float denom_sq = fmaf(n_e * cos_t, n_e * cos_t, os * os);
+    if (denom_sq <= 1e-8f) return n_o;
+    return (n_o * n_e) / sqrtf(denom_sq);
+}
 __device__ __forceinline__ float walk_off_angle(float n_o, float n_e, float theta) {
     if (fabsf(n_o - n_e) < 1e-5f) return 0.0f;
     float n_o2 = n_o * n_o;
@@ -245,11 +254,17 @@ __device__ __forceinline__ V3 arbitrary_perpendicular(V3 n) {
     return normalize_or_zero(sub(a, mul(n, dot(n, a))));
 }
 
-// refraction.rs:45-50.
-__device__ __forceinline__ float tir_phase_delta(float n1k, float cos_i, float sin_i) {
-    float tan_half = (cos_i * sqrtf(fmaxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f))) /
-                     fmaxf(n1k * sin_i * sin_i, 1e-6f);
-    return 2.0f * atanf(tan_half);
+// refraction.rs:45-50: delta = 2 atan(x) with x the tangent of the half retardation.
+// The retardation matrix (polarization.rs:141-147) needs only cos delta and sin delta,
+// which are the rational functions (1 - x^2) / (1 + x^2) and 2x / (1 + x^2).
+__device__ __forceinline__ void tir_retardation_cs(float n1k, float cos_i, float sin_i,
+                                                   float *cos_d, float *sin_d) {
+    float x = (cos_i * sqrtf(fmaxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f))) /
+              fmaxf(n1k * sin_i * sin_i, 1e-6f);
+    float x2 = x * x;
+    float inv = 1.0f / (1.0f + x2);
+    *cos_d = (1.0f - x2) * inv;
+    *sin_d = (2.0f * x) * inv;
 }
 
 __device__ __forceinline__ void fresnel_rs_rp(float n1k, float n2k, float cos_i, float cos_t,

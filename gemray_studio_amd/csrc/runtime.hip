@@ -13,6 +13,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "../../include/gemray_hip.h"
 #include "transport.h"
@@ -265,13 +266,19 @@ struct gr_film {
 struct gr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_k2 = nullptr;
     std::mutex mu;
     uint64_t chunk_tuples = kDefaultChunkTuples;
+    int trace_mode = GR_TRACE_WAVEFRONT;
     float *xyz = nullptr;
     uint64_t xyz_cap = 0;  // tuples
     uint32_t *info = nullptr;
     uint64_t info_cap = 0;
+    float *rec = nullptr;  // escape records, kRecFields x rec_cap
+    uint64_t rec_cap = 0;
+    float2 *ri35 = nullptr;  // radical inverses (3, 5) of the call's sample range
+    uint32_t ri35_cap = 0, ri35_offset = 0, ri35_n = 0;
+    std::vector<float2> ri35_host;
     unsigned long long *stats = nullptr;
     gr_stats last{};
     char label[256] = {0};
@@ -304,6 +311,51 @@ gr_status ensure_info(gr_ctx *ctx, uint64_t tuples) {
     ctx->info_cap = 0;
     GR_CHECK(hipMalloc(&ctx->info, tuples * 4 * sizeof(uint32_t)));
     ctx->info_cap = tuples;
+    return GR_OK;
+}
+
+gr_status ensure_rec(gr_ctx *ctx, uint64_t tuples) {
+    if (tuples <= ctx->rec_cap) return GR_OK;
+    if (ctx->rec) GR_CHECK(hipFree(ctx->rec));
+    ctx->rec = nullptr;
+    ctx->rec_cap = 0;
+    GR_CHECK(hipMalloc(&ctx->rec, tuples * 20 * sizeof(float)));
+    ctx->rec_cap = tuples;
+    return GR_OK;
+}
+
+// radical_inverse_base (optics/raytracer/sampling.rs:170-181), the same operations as the
+// device's, evaluated once per call for its sample range.
+float h_radical_inverse(uint32_t n, uint32_t base) {
+    uint32_t remaining = n;
+    float val = 0.0f;
+    float inv_base = 1.0f / (float)base;
+    while (remaining > 0) {
+        uint32_t digit = remaining % base;
+        val = std::fmaf((float)digit, inv_base, val);
+        inv_base /= (float)base;
+        remaining /= base;
+    }
+    return val;
+}
+
+gr_status ensure_ri35(gr_ctx *ctx, uint32_t sample_offset, uint32_t spp) {
+    if (ctx->ri35 && ctx->ri35_offset == sample_offset && ctx->ri35_n == spp) return GR_OK;
+    if (spp > ctx->ri35_cap) {
+        if (ctx->ri35) GR_CHECK(hipFree(ctx->ri35));
+        ctx->ri35 = nullptr;
+        ctx->ri35_cap = 0;
+        GR_CHECK(hipMalloc(&ctx->ri35, (size_t)spp * sizeof(float2)));
+        ctx->ri35_cap = spp;
+    }
+    ctx->ri35_host.resize(spp);
+    for (uint32_t i = 0; i < spp; ++i) {
+        uint32_t s = sample_offset + i;
+        ctx->ri35_host[i] = make_float2(h_radical_inverse(s, 3), h_radical_inverse(s, 5));
+    }
+    GR_CHECK(hipMemcpy(ctx->ri35, ctx->ri35_host.data(), (size_t)spp * sizeof(float2), hipMemcpyHostToDevice));
+    ctx->ri35_offset = sample_offset;
+    ctx->ri35_n = spp;
     return GR_OK;
 }
 
@@ -343,23 +395,44 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     if (chunk_pixels == 0) chunk_pixels = 1;
     if (chunk_pixels > n_pixels) chunk_pixels = n_pixels;
     uint64_t cap = chunk_pixels * spp;
+    const bool wavefront = ctx->trace_mode == GR_TRACE_WAVEFRONT;
     gr_status st = ensure_xyz(ctx, cap);
     if (st != GR_OK) return st;
     if (info_host) {
         st = ensure_info(ctx, cap);
         if (st != GR_OK) return st;
     }
+    if (wavefront) {
+        st = ensure_rec(ctx, cap);
+        if (st != GR_OK) return st;
+    }
+    st = ensure_ri35(ctx, sample_offset, spp);
+    if (st != GR_OK) return st;
     GR_CHECK(hipMemsetAsync(ctx->stats, 0, GR_EV_COUNT * sizeof(unsigned long long), ctx->stream));
     GR_CHECK(hipEventRecord(ctx->ev_begin, ctx->stream));
-    double kernel_ms = 0.0;
+    double kernel_ms = 0.0, finish_ms = 0.0;
     for (uint64_t done = 0; done < n_pixels; done += chunk_pixels) {
         uint32_t np = (uint32_t)((n_pixels - done) < chunk_pixels ? (n_pixels - done) : chunk_pixels);
         uint32_t first = pixel_begin + (uint32_t)done;
         uint32_t n_tuples = np * spp;
+        gr::TraceArgs args{};
+        args.planes = s->planes;
+        args.ri35 = ctx->ri35;
+        args.pixel_begin = first;
+        args.sample_offset = sample_offset;
+        args.spp = spp;
+        args.n_tuples = n_tuples;
+        args.xyz = ctx->xyz;
+        args.info = info_host ? ctx->info : nullptr;
+        args.stats = ctx->stats;
+        args.wavefront = wavefront;
+        args.rec = ctx->rec;
+        args.rec_cap = (uint32_t)ctx->rec_cap;
+        args.trace_done = wavefront ? ctx->ev_k1 : nullptr;
         GR_CHECK(hipEventRecord(ctx->ev_k0, ctx->stream));
-        GR_CHECK(gr_launch_trace(s->sc, s->planes, first, sample_offset, spp, n_tuples, ctx->xyz,
-                                 info_host ? ctx->info : nullptr, ctx->stats, ctx->stream));
-        GR_CHECK(hipEventRecord(ctx->ev_k1, ctx->stream));
+        GR_CHECK(gr_launch_trace(s->sc, args, ctx->stream));
+        if (!wavefront) GR_CHECK(hipEventRecord(ctx->ev_k1, ctx->stream));
+        GR_CHECK(hipEventRecord(ctx->ev_k2, ctx->stream));
         if (film) GR_CHECK(gr_launch_film_reduce(ctx->xyz, np, spp, first, film, ctx->stream));
         if (xyz_host)
             GR_CHECK(hipMemcpyAsync(xyz_host + 3 * done * spp, ctx->xyz, (size_t)n_tuples * 3 * sizeof(float),
@@ -367,10 +440,12 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
         if (info_host)
             GR_CHECK(hipMemcpyAsync(info_host + 4 * done * spp, ctx->info, (size_t)n_tuples * 4 * sizeof(uint32_t),
                                     hipMemcpyDeviceToHost, ctx->stream));
-        GR_CHECK(hipEventSynchronize(ctx->ev_k1));
-        float ms = 0.0f;
+        GR_CHECK(hipEventSynchronize(ctx->ev_k2));
+        float ms = 0.0f, fms = 0.0f;
         GR_CHECK(hipEventElapsedTime(&ms, ctx->ev_k0, ctx->ev_k1));
+        GR_CHECK(hipEventElapsedTime(&fms, ctx->ev_k1, ctx->ev_k2));
         kernel_ms += ms;
+        finish_ms += fms;
         ctx->last.launches++;
     }
     GR_CHECK(hipEventRecord(ctx->ev_end, ctx->stream));
@@ -382,6 +457,7 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     for (int i = 0; i < GR_EV_COUNT; ++i) ctx->last.events[i] = ev[i];
     ctx->last.paths = (uint64_t)n_pixels * spp;
     ctx->last.trace_kernel_ms = kernel_ms;
+    ctx->last.finish_kernel_ms = finish_ms;
     ctx->last.total_ms = total;
     return GR_OK;
 }
@@ -408,6 +484,7 @@ gr_status gr_open(gr_ctx **out, const gr_options *opts) {
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev_begin) != hipSuccess || hipEventCreate(&ctx->ev_end) != hipSuccess ||
         hipEventCreate(&ctx->ev_k0) != hipSuccess || hipEventCreate(&ctx->ev_k1) != hipSuccess ||
+        hipEventCreate(&ctx->ev_k2) != hipSuccess ||
         hipMalloc(&ctx->stats, GR_EV_COUNT * sizeof(unsigned long long)) != hipSuccess) {
         gr_close(ctx);
         return GR_ERR_DEVICE;
@@ -422,8 +499,10 @@ void gr_close(gr_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->xyz) (void)hipFree(ctx->xyz);
     if (ctx->info) (void)hipFree(ctx->info);
+    if (ctx->rec) (void)hipFree(ctx->rec);
+    if (ctx->ri35) (void)hipFree(ctx->ri35);
     if (ctx->stats) (void)hipFree(ctx->stats);
-    for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_k0, ctx->ev_k1})
+    for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_k0, ctx->ev_k1, ctx->ev_k2})
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -434,6 +513,13 @@ int gr_adapter_label(gr_ctx *ctx, char *buf, size_t len) {
     int n = (int)std::strlen(ctx->label);
     if (buf && len) std::snprintf(buf, len, "%s", ctx->label);
     return n;
+}
+
+gr_status gr_set_trace_mode(gr_ctx *ctx, int mode) {
+    if (!ctx || (mode != GR_TRACE_PLAIN && mode != GR_TRACE_WAVEFRONT)) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->trace_mode = mode;
+    return GR_OK;
 }
 
 gr_status gr_set_chunk_tuples(gr_ctx *ctx, uint64_t tuples) {

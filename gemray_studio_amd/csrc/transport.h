@@ -41,10 +41,23 @@ struct DevScene {
     const float *map_rgb;     // device pointer
 };
 
+// Arguments of one trace launch over tuples [0, n_tuples): tuple t is pixel
+// pixel_begin + t / spp, sample sample_offset + t % spp.
+struct TraceArgs {
+    const float4 *planes;        // device, n_planes
+    const float2 *ri35;          // device, spp entries: radical inverses (base 3, base 5)
+    uint32_t pixel_begin, sample_offset, spp, n_tuples;
+    float *xyz;                  // device, 3 floats per tuple
+    uint32_t *info;              // device, 4 u32 per tuple, or null
+    unsigned long long *stats;   // device, GR_EV_COUNT counters (added into)
+    bool wavefront;              // persistent wavefront kernel + finish pass
+    float *rec;                  // device escape records (wavefront mode), 20 x rec_cap
+    uint32_t rec_cap;
+    hipEvent_t trace_done;       // recorded between the trace and finish kernels, or null
+};
+
 }  // namespace gr
 
-hipError_t gr_launch_trace(const gr::DevScene &sc, const float4 *planes, uint32_t pixel_begin,
-                           uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, float *xyz,
-                           uint32_t *info, unsigned long long *stats, hipStream_t stream);
+hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &args, hipStream_t stream);
 hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t spp,
                                  uint32_t pixel_begin, float *film, hipStream_t stream);

@@ -1,12 +1,20 @@
-// transport.hip -- the gfx950 spectral transport megakernel and its film reduction.
+// transport.hip -- the gfx950 spectral transport kernels.
 //
-// One lane traces one (pixel, absolute sample) tuple: the full estimator of
-// gemray's transport_main (crates/gemray/src/renderer/shaders/spectral_transport.wgsl:672)
-// whose CPU statement is trace_spectral_ray (optics/raytracer/transport.rs:617-872).
-// Facet planes are staged in LDS once per workgroup; the material, camera and
-// environment constants arrive as one by-value kernel argument (scalar loads); per-path
-// state (8 Stokes vectors, 8 path pdfs, 8 radiances, the wavelength comb and its
-// dispersion/absorption caches) lives in VGPRs.  No MFMA: there is no dense contraction.
+// The estimator is gemray's transport_main
+// (crates/gemray/src/renderer/shaders/spectral_transport.wgsl:672), whose CPU statement is
+// trace_spectral_ray (optics/raytracer/transport.rs:617-872).  It is split into
+//   init_path    -- draws, camera ray and per-path state for one (pixel, sample) tuple;
+//   bounce_step  -- one bounce-loop iteration: facet-plane clip plus the 8-channel event;
+//   finish_path  -- environment radiance at the escape, spectral MIS, CIE XYZ, white balance.
+// Two trace kernels drive them:
+//   trace_kernel        -- one lane per tuple, bounce loop then finish (the plain path);
+//   trace_wave_kernel   -- persistent wavefronts: a lane whose path ends takes the next
+//                          tuple of its wave's range (ballot + popcount compaction), and
+//                          the finished path is written as an escape record that
+//                          finish_kernel turns into XYZ in a separate, fully coherent pass.
+// Facet planes are staged in LDS once per workgroup; material, camera and environment
+// constants arrive as one by-value kernel argument; per-path state lives in VGPRs.
+// No MFMA: there is no dense contraction.
 #include "transport.h"
 
 #include "device_physics.h"
@@ -14,7 +22,7 @@
 namespace gr {
 
 // ---------------------------------------------------------------------------------
-// Environment radiance for an escaping direction (transport.rs:67-90).
+// Environment radiance (transport.rs:67-90).
 // ---------------------------------------------------------------------------------
 
 // blackbody_spectrum (optics/raytracer/environment.rs:12-21) with the lambda-independent
@@ -72,496 +80,544 @@ __device__ __forceinline__ V3 map_rgb(V3 dir, const DevScene &sc) {
     return v3(out[0], out[1], out[2]);
 }
 
-// ---------------------------------------------------------------------------------
-// The estimator for one tuple.
-// ---------------------------------------------------------------------------------
-
 // Column j of the absorption tensor's principal frame (a1, a2, c).
 __device__ __forceinline__ V3 afr(const DevScene &sc, int j) {
     return v3(sc.abs_frame[j][0], sc.abs_frame[j][1], sc.abs_frame[j][2]);
 }
-
-struct PathOut {
-    V3 xyz;
-    uint32_t segments, termination, facet_hash, fate_hash;
-};
 
 __device__ __forceinline__ void fnv(uint32_t &h, uint32_t v) {
     h ^= v;
     h *= 16777619u;
 }
 
-template <bool kAniso, bool kAbsorb, int kEnv>
-__device__ __forceinline__ PathOut trace_path(V3 ray_o, V3 ray_d, uint32_t rng_seed, float hero_rand,
-                              const DevScene &sc, const float4 *__restrict__ planes,
-                              uint32_t *ev) {
-    // wrapped_hero_wavelengths (transport.rs:45-53), evaluated on demand instead of held
-    // in 8 registers.  offset = k * 50 + (lambda_hero - 380) lies in [0, 750], so
-    // rem_euclid(offset, 400) is offset or offset - 400, and that subtraction is exact
-    // (Sterbenz): the same bits as the reference's fmod.
-    const float hero_off = fmaf(hero_rand, 780.0f - 380.0f, 380.0f) - 380.0f;
-    auto lam = [hero_off](int k) -> float {
-        float offset = fmaf((float)k, (780.0f - 380.0f) / 8.0f, hero_off);
-        return 380.0f + (offset >= 400.0f ? offset - 400.0f : offset);
-    };
-    // Per-ray caches (refraction.rs:127-174): ordinary index per channel and absorption
-    // coefficients per channel and eigen-axis.  Both are pure functions of the comb, so
-    // they are filled lazily: the indices at the first facet hit, the absorption at the
-    // first interior segment (a path that misses the stone never pays for either).
-    float n_o[8], ao[8], ae[8];
-    bool have_n = false, have_abs = false;
-    const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
-    const float dn = sc.birefringence_delta;
+// wrapped_hero_wavelengths (transport.rs:45-53) for channel k, from
+// hero_off = lambda_hero - 380.  offset = k * 50 + hero_off lies in [0, 750], so
+// rem_euclid(offset, 400) is offset or offset - 400, and that subtraction is exact
+// (Sterbenz): the same bits as the reference's fmod.
+__device__ __forceinline__ float comb_lambda(float hero_off, int k) {
+    float offset = fmaf((float)k, (780.0f - 380.0f) / 8.0f, hero_off);
+    return 380.0f + (offset >= 400.0f ? offset - 400.0f : offset);
+}
 
+// ---------------------------------------------------------------------------------
+// Per-path state.
+// ---------------------------------------------------------------------------------
+
+struct PathState {
+    V3 ray_o, ray_d, cur_k, prev_pn;
     Stokes st[8];
     float pdf[8];
+    float n_o[8], ao[8], ae[8];  // lazily filled caches (refraction.rs:127-174)
+    float hero_off;
+    uint32_t seed, bounce;
+    uint32_t term_bounce, term_reason, fh, fate_h;
+    bool inside, is_extra, have_prev, have_n, have_abs, escaped;
+};
+
+// pixel_rotations + sample_draws (sampling.rs:232-271) with the sample-only radical
+// inverses (bases 3 and 5) taken from a host-built table, then Camera::generate_ray
+// (camera.rs:96-114) and the initial state of trace_spectral_ray_inner
+// (transport.rs:651-722).
+__device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_t sample_num,
+                                          float2 ri35, const DevScene &sc) {
+    const uint32_t x = pixel % sc.width, y = pixel / sc.width;
+    float rot_x = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_X_ROTATION_STREAM));
+    float rot_y = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_Y_ROTATION_STREAM));
+    float rot_h = low_discrepancy_base2(hash_u32(pixel ^ HERO_WAVELENGTH_ROTATION_STREAM));
+    ps.seed = hash_u32((pixel * 0x9e3779b9u) ^ (sample_num * 0x85ebca6bu));
+    float jx = cranley_patterson(low_discrepancy_base2(sample_num), rot_x) - 0.5f;
+    float jy = cranley_patterson(ri35.x, rot_y) - 0.5f;
+    float hero = cranley_patterson(ri35.y, rot_h);
+    float w = (float)sc.width, h = (float)sc.height;
+    float aspect = w / h;
+    float u = (((float)x + jx) / w - 0.5f) * 2.0f * aspect * sc.fov_tan;
+    float v = (0.5f - ((float)y + jy) / h) * 2.0f * sc.fov_tan;
+    ps.ray_d = normalize(add(add(v3(sc.cam_f[0], sc.cam_f[1], sc.cam_f[2]),
+                                 mul(v3(sc.cam_r[0], sc.cam_r[1], sc.cam_r[2]), u)),
+                             mul(v3(sc.cam_u[0], sc.cam_u[1], sc.cam_u[2]), v)));
+    ps.ray_o = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
+    ps.cur_k = ps.ray_d;
+    ps.prev_pn = v3(0.0f, 0.0f, 0.0f);
+    ps.hero_off = fmaf(hero, 780.0f - 380.0f, 380.0f) - 380.0f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        st[k] = Stokes{1.0f, 0.0f, 0.0f, 0.0f};
-        pdf[k] = 1.0f;
+        ps.st[k] = Stokes{1.0f, 0.0f, 0.0f, 0.0f};
+        ps.pdf[k] = 1.0f;
     }
-    V3 cur_k = ray_d;
-    bool inside = false, is_extra = false, have_prev = false, escaped = false;
-    V3 prev_pn = v3(0.0f, 0.0f, 0.0f);
-    uint32_t term_bounce = sc.max_bounces, term_reason = 3u;
-    uint32_t fh = 2166136261u, fate_h = 2166136261u;
-    const int n_planes = (int)sc.n_planes;
+    ps.bounce = 0u;
+    ps.term_bounce = sc.max_bounces;
+    ps.term_reason = 3u;
+    ps.fh = 2166136261u;
+    ps.fate_h = 2166136261u;
+    ps.inside = ps.is_extra = ps.have_prev = ps.have_n = ps.have_abs = ps.escaped = false;
+}
 
-    for (uint32_t bounce = 0; bounce < sc.max_bounces; ++bounce) {
-        ev[GR_EV_SEGMENT]++;
-        // ---- convex facet-plane clip (simd/slab.rs:94-152, intersect.rs:93-132) ----
-        // The reference divides t = -side / denom for every plane and keeps the first
-        // strict maximum (entering) / minimum (exiting).  Here a = -side * rcp(denom)
-        // (v_rcp_f32, 1 ulp) screens each plane: |a - t| <= |a| * 2^-19, so a plane whose
-        // exact t could beat the running extreme always passes the screen, and only those
-        // planes take the correctly rounded division and the exact comparison (a 2^-120
-        // absolute slack covers denormal products).  The selected planes and t values are
-        // therefore exactly the reference's.
-        float t_near = -1e30f, t_far = 1e30f;
-        int near_idx = -1, far_idx = -1;
-        bool outside = false;
-        for (int i = 0; i < n_planes; ++i) {
-            float4 p = planes[i];
-            float denom = (p.x * ray_d.x + p.y * ray_d.y) + p.z * ray_d.z;
-            float side = p.w + ((p.x * ray_o.x + p.y * ray_o.y) + p.z * ray_o.z);
-            float a = (-side) * __builtin_amdgcn_rcpf(denom);
-            if (fabsf(denom) > 1e-7f) {
-                if (denom < 0.0f) {
-                    if (fmaf(fabsf(a), 0x1p-19f, a) + 0x1p-120f > t_near) {
-                        float t = (-side) / denom;
-                        if (t > t_near) {
-                            t_near = t;
-                            near_idx = i;
-                        }
-                    }
-                } else if (fmaf(-fabsf(a), 0x1p-19f, a) - 0x1p-120f < t_far) {
+// One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
+// has ended (escape, Russian roulette, or the bounce cap).
+template <bool kAniso, bool kAbsorb>
+__device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc,
+                                            const float4 *__restrict__ planes, uint32_t *ev) {
+    const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
+    const float dn = sc.birefringence_delta;
+    const uint32_t bounce = ps.bounce;
+    ev[GR_EV_SEGMENT]++;
+
+    // ---- convex facet-plane clip (simd/slab.rs:94-152, intersect.rs:93-132) ----
+    // The reference divides t = -side / denom for every plane and keeps the first strict
+    // maximum (entering) / minimum (exiting).  Here a = -side * rcp(denom) (v_rcp_f32,
+    // 1 ulp) screens each plane: |a - t| <= |a| * 2^-19, so a plane whose exact t could
+    // beat the running extreme always passes the screen, and only those planes take the
+    // correctly rounded division and the exact comparison (a 2^-120 absolute slack covers
+    // denormal products).  The selected planes and t values are exactly the reference's.
+    float t_near = -1e30f, t_far = 1e30f;
+    int near_idx = -1, far_idx = -1;
+    bool outside = false;
+    const V3 ro = ps.ray_o, rd = ps.ray_d;
+    for (int i = 0; i < (int)sc.n_planes; ++i) {
+        float4 p = planes[i];
+        float denom = (p.x * rd.x + p.y * rd.y) + p.z * rd.z;
+        float side = p.w + ((p.x * ro.x + p.y * ro.y) + p.z * ro.z);
+        float a = (-side) * __builtin_amdgcn_rcpf(denom);
+        if (fabsf(denom) > 1e-7f) {
+            if (denom < 0.0f) {
+                if (fmaf(fabsf(a), 0x1p-19f, a) + 0x1p-120f > t_near) {
                     float t = (-side) / denom;
-                    if (t < t_far) {
-                        t_far = t;
-                        far_idx = i;
+                    if (t > t_near) {
+                        t_near = t;
+                        near_idx = i;
                     }
                 }
-            } else if (side > 0.0f) {
-                outside = true;
-                break;
+            } else if (fmaf(-fabsf(a), 0x1p-19f, a) - 0x1p-120f < t_far) {
+                float t = (-side) / denom;
+                if (t < t_far) {
+                    t_far = t;
+                    far_idx = i;
+                }
             }
-        }
-        bool hit = false;
-        float hit_t = 0.0f;
-        int facet = 0;
-        if (!outside && !(t_near > t_far)) {
-            if (t_near > 1e-4f) {
-                hit = true;
-                hit_t = t_near;
-                facet = near_idx;
-            } else if (t_far > 1e-4f) {
-                hit = true;
-                hit_t = t_far;
-                facet = far_idx;
-            }
-        }
-        if (!hit) {
-            // Escape: the environment is sampled after the loop (the path's only radiance
-            // contribution; transport.rs:307-318).
-            ev[GR_EV_MISS]++;
-            escaped = true;
-            fnv(fh, 0xFFFFu);
-            term_bounce = bounce;
-            term_reason = 0u;
+        } else if (side > 0.0f) {
+            outside = true;
             break;
         }
-        ev[GR_EV_HIT]++;
-        if (!have_n) {
-            // DispersionModel::evaluate (optics/dispersion.rs:12-37) for the comb.
+    }
+    bool hit = false;
+    float hit_t = 0.0f;
+    int facet = 0;
+    if (!outside && !(t_near > t_far)) {
+        if (t_near > 1e-4f) {
+            hit = true;
+            hit_t = t_near;
+            facet = near_idx;
+        } else if (t_far > 1e-4f) {
+            hit = true;
+            hit_t = t_far;
+            facet = far_idx;
+        }
+    }
+    if (!hit) {
+        // Escape: the environment is sampled by finish_path (the path's only radiance
+        // contribution; transport.rs:307-318).
+        ev[GR_EV_MISS]++;
+        ps.escaped = true;
+        fnv(ps.fh, 0xFFFFu);
+        ps.term_bounce = bounce;
+        ps.term_reason = 0u;
+        return true;
+    }
+    ev[GR_EV_HIT]++;
+    if (!ps.have_n) {
+        // DispersionModel::evaluate (optics/dispersion.rs:12-37) for the comb.
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                float l_um = lam(k) * 1e-3f;
-                float l2 = l_um * l_um;
-                float n;
-                if (sc.disp_kind == 0) {
-                    float n2 = 1.0f + (sc.disp[0] * l2) / (l2 - sc.disp[1]);
-                    n = sqrtf(fmaxf(n2, 1.0f));
-                } else if (sc.disp_kind == 1) {
-                    float n2 = 1.0f;
-                    n2 += (sc.disp[0] * l2) / (l2 - sc.disp[3]);
-                    n2 += (sc.disp[1] * l2) / (l2 - sc.disp[4]);
-                    n2 += (sc.disp[2] * l2) / (l2 - sc.disp[5]);
-                    n = sqrtf(fmaxf(n2, 1.0f));
-                } else {
-                    float l4 = l2 * l2;
-                    n = sc.disp[0] + (sc.disp[1] / l2) + (sc.disp[2] / l4);
-                }
-                n_o[k] = n;
+        for (int k = 0; k < 8; ++k) {
+            float l_um = comb_lambda(ps.hero_off, k) * 1e-3f;
+            float l2 = l_um * l_um;
+            float n;
+            if (sc.disp_kind == 0) {
+                float n2 = 1.0f + (sc.disp[0] * l2) / (l2 - sc.disp[1]);
+                n = sqrtf(fmaxf(n2, 1.0f));
+            } else if (sc.disp_kind == 1) {
+                float n2 = 1.0f;
+                n2 += (sc.disp[0] * l2) / (l2 - sc.disp[3]);
+                n2 += (sc.disp[1] * l2) / (l2 - sc.disp[4]);
+                n2 += (sc.disp[2] * l2) / (l2 - sc.disp[5]);
+                n = sqrtf(fmaxf(n2, 1.0f));
+            } else {
+                float l4 = l2 * l2;
+                n = sc.disp[0] + (sc.disp[1] / l2) + (sc.disp[2] / l4);
             }
-            have_n = true;
+            ps.n_o[k] = n;
         }
-        // An unbounded side (no plane on it) reports facet 0 with a zero normal, as the
-        // reference's HitRecord does for a negative index (intersect.rs:105-126).
-        V3 normal = v3(0.0f, 0.0f, 0.0f);
-        if (facet >= 0) {
-            float4 hp = planes[facet];
-            normal = v3(hp.x, hp.y, hp.z);
-        } else {
-            facet = 0;
-        }
-        V3 hit_point = add(ray_o, mul(ray_d, hit_t));
+        ps.have_n = true;
+    }
+    // An unbounded side (no plane on it) reports facet 0 with a zero normal, as the
+    // reference's HitRecord does for a negative index (intersect.rs:105-126).
+    V3 normal = v3(0.0f, 0.0f, 0.0f);
+    if (facet >= 0) {
+        float4 hp = planes[facet];
+        normal = v3(hp.x, hp.y, hp.z);
+    } else {
+        facet = 0;
+    }
+    V3 hit_point = add(ro, mul(rd, hit_t));
+    V3 cur_k = ps.cur_k;
+    const bool inside = ps.inside;
 
-        // ---- Stokes frame rotation into the new plane of incidence
-        //      (raytracer/absorption.rs:44-89) ----
-        V3 cpn = normalize_or_zero(cross(cur_k, normal));
-        if (have_prev && len2(cpn) > 1e-6f && len2(prev_pn) > 1e-6f) {
-            ev[GR_EV_ROTATE]++;
-            float cos_psi = clampf(dot(prev_pn, cpn), -1.0f, 1.0f);
-            float sin_psi = dot(cross(prev_pn, cpn), normalize_or_zero(cur_k));
-            float psi = atan2f(sin_psi, cos_psi);
-            float s2, c2;
-            sincosf(2.0f * psi, &s2, &c2);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) st[k] = apply_rotation(st[k], c2, s2);
+    // ---- Stokes frame rotation into the new plane of incidence
+    //      (raytracer/absorption.rs:44-89) ----
+    V3 cpn = normalize_or_zero(cross(cur_k, normal));
+    if (ps.have_prev && len2(cpn) > 1e-6f && len2(ps.prev_pn) > 1e-6f) {
+        ev[GR_EV_ROTATE]++;
+        float cos_psi = clampf(dot(ps.prev_pn, cpn), -1.0f, 1.0f);
+        float sin_psi = dot(cross(ps.prev_pn, cpn), normalize_or_zero(cur_k));
+        // frame_rotation(atan2(sin_psi, cos_psi)) needs cos 2psi and sin 2psi only: the
+        // double-angle forms of the (unnormalized) pair replace atan2 + sincos.
+        float cc = cos_psi * cos_psi, ss = sin_psi * sin_psi;
+        float r2 = cc + ss;
+        float c2 = 1.0f, s2 = 0.0f;
+        if (r2 > 0.0f) {
+            float inv = 1.0f / r2;
+            c2 = (cc - ss) * inv;
+            s2 = (2.0f * cos_psi * sin_psi) * inv;
         }
-        prev_pn = cpn;
-        have_prev = true;
-
-        // ---- interior segment: pleochroic Beer-Lambert (raytracer/absorption.rs:115-216,
-        //      birefringence.rs:252-286, polarization.rs:172-192) ----
-        if (inside) {
-            normal = neg(normal);
-            ev[GR_EV_ABSORB]++;
-            if (kAbsorb) {
-                if (!have_abs) {
-                    // spectral_absorption per eigen-axis (raytracer/absorption.rs:28-30,
-                    // optics/absorption.rs:68-71), summed from -0.0 like Iterator::sum.
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        float so = -0.0f, se = -0.0f;
-                        const float lk = lam(k);
-                        for (int b = 0; b < sc.n_o_bands; ++b) {
-                            float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
-                            so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
-                        }
-                        for (int b = 0; b < sc.n_e_bands; ++b) {
-                            float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
-                            se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
-                        }
-                        ao[k] = so;
-                        ae[k] = se;
-                    }
-                    have_abs = true;
-                }
-                V3 ea = ordinary_eigen_polarization(cur_k, c_axis);
-                V3 eb = extraordinary_eigen_polarization(cur_k, c_axis);
-                // Channel-invariant parts of electric_field_direction and of the
-                // quadratic forms: the absorption frame (a1, a2, c), the s/p axes, and
-                // the eigenmode projections.
-                V3 kk = normalize_or_zero(cur_k);
-                V3 s_raw = sub(cpn, mul(kk, dot(kk, cpn)));
-                V3 s_hat = len2(s_raw) > 1e-8f ? normalize(s_raw) : arbitrary_perpendicular(kk);
-                V3 p_hat = cross(kk, s_hat);
-                V3 ea_l = v3(dot(afr(sc, 0), ea), dot(afr(sc, 1), ea), dot(afr(sc, 2), ea));
-                V3 eb_l = v3(dot(afr(sc, 0), eb), dot(afr(sc, 1), eb), dot(afr(sc, 2), eb));
-                float spl = hit_t * sc.absorption_path_scale;
+        for (int k = 0; k < 8; ++k) ps.st[k] = apply_rotation(ps.st[k], c2, s2);
+    }
+    ps.prev_pn = cpn;
+    ps.have_prev = true;
+
+    // ---- interior segment: pleochroic Beer-Lambert (raytracer/absorption.rs:115-216,
+    //      birefringence.rs:252-286, polarization.rs:172-192) ----
+    if (inside) {
+        normal = neg(normal);
+        ev[GR_EV_ABSORB]++;
+        if (kAbsorb) {
+            if (!ps.have_abs) {
+                // spectral_absorption per eigen-axis (raytracer/absorption.rs:28-30,
+                // optics/absorption.rs:68-71), summed from -0.0 like Iterator::sum.
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    float psi = 0.5f * atan2f(st[k].u, st[k].q);
-                    float sp, cp;
-                    sincosf(psi, &sp, &cp);
-                    V3 e = add(mul(s_hat, cp), mul(p_hat, sp));
-                    V3 e_hat = len2(e) > 1e-8f ? normalize(e) : s_hat;
-                    V3 el = v3(dot(afr(sc, 0), e_hat), dot(afr(sc, 1), e_hat), dot(afr(sc, 2), e_hat));
-                    float qf_e = (ao[k] * (el.x * el.x) + ao[k] * (el.y * el.y)) + ae[k] * (el.z * el.z);
-                    float qf_a = (ao[k] * (ea_l.x * ea_l.x) + ao[k] * (ea_l.y * ea_l.y)) + ae[k] * (ea_l.z * ea_l.z);
-                    float qf_b = (ao[k] * (eb_l.x * eb_l.x) + ao[k] * (eb_l.y * eb_l.y)) + ae[k] * (eb_l.z * eb_l.z);
-                    float p = clampf(degree_of_polarization(st[k]), 0.0f, 1.0f);
-                    float unpol = midpoint(qf_a, qf_b);
-                    float alpha = fmaf(p, qf_e - unpol, unpol);
-                    st[k] = scale(st[k], exp_poly(-alpha * spl));
+                    float so = -0.0f, se = -0.0f;
+                    const float lk = comb_lambda(ps.hero_off, k);
+                    for (int b = 0; b < sc.n_o_bands; ++b) {
+                        float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
+                        so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
+                    }
+                    for (int b = 0; b < sc.n_e_bands; ++b) {
+                        float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
+                        se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
+                    }
+                    ps.ao[k] = so;
+                    ps.ae[k] = se;
                 }
+                ps.have_abs = true;
+            }
+            V3 ea = ordinary_eigen_polarization(cur_k, c_axis);
+            V3 eb = extraordinary_eigen_polarization(cur_k, c_axis);
+            // Channel-invariant parts of electric_field_direction and of the quadratic
+            // forms: the s/p axes and the projections on the absorption frame (a1, a2, c).
+            V3 kk = normalize_or_zero(cur_k);
+            V3 s_raw = sub(cpn, mul(kk, dot(kk, cpn)));
+            V3 s_hat = len2(s_raw) > 1e-8f ? normalize(s_raw) : arbitrary_perpendicular(kk);
+            V3 p_hat = cross(kk, s_hat);
+            V3 ea_l = v3(dot(afr(sc, 0), ea), dot(afr(sc, 1), ea), dot(afr(sc, 2), ea));
+            V3 eb_l = v3(dot(afr(sc, 0), eb), dot(afr(sc, 1), eb), dot(afr(sc, 2), eb));
+            // The field direction e = cos(psi) s + sin(psi) p is a unit vector, so its
+            // principal-frame components are cos(psi) (a_i.s) + sin(psi) (a_i.p).
+            V3 s_l = v3(dot(afr(sc, 0), s_hat), dot(afr(sc, 1), s_hat), dot(afr(sc, 2), s_hat));
+            V3 p_l = v3(dot(afr(sc, 0), p_hat), dot(afr(sc, 1), p_hat), dot(afr(sc, 2), p_hat));
+            float spl = hit_t * sc.absorption_path_scale;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                // psi = atan2(u, q) / 2 (polarization.rs:58-60) through the half-angle
+                // identities, in the cancellation-free branch for the sign of q.
+                float cp = 1.0f, sp = 0.0f;
+                float m = fmaxf(fabsf(ps.st[k].q), fabsf(ps.st[k].u));
+                if (m > 0.0f) {
+                    float inv_m = 1.0f / m;
+                    float qn = ps.st[k].q * inv_m, un = ps.st[k].u * inv_m;
+                    float r = sqrtf(qn * qn + un * un);
+                    if (qn >= 0.0f) {
+                        cp = sqrtf((r + qn) / (2.0f * r));
+                        sp = un / (2.0f * r * cp);
+                    } else {
+                        sp = copysignf(sqrtf((r - qn) / (2.0f * r)), un);
+                        cp = fabsf(un) / (2.0f * r * fabsf(sp));
+                    }
+                }
+                V3 el = v3(fmaf(cp, s_l.x, sp * p_l.x), fmaf(cp, s_l.y, sp * p_l.y), fmaf(cp, s_l.z, sp * p_l.z));
+                const float ao = ps.ao[k], ae = ps.ae[k];
+                float qf_e = (ao * (el.x * el.x) + ao * (el.y * el.y)) + ae * (el.z * el.z);
+                float qf_a = (ao * (ea_l.x * ea_l.x) + ao * (ea_l.y * ea_l.y)) + ae * (ea_l.z * ea_l.z);
+                float qf_b = (ao * (eb_l.x * eb_l.x) + ao * (eb_l.y * eb_l.y)) + ae * (eb_l.z * eb_l.z);
+                float p = clampf(degree_of_polarization(ps.st[k]), 0.0f, 1.0f);
+                float unpol = midpoint(qf_a, qf_b);
+                float alpha = fmaf(p, qf_e - unpol, unpol);
+                ps.st[k] = scale(ps.st[k], exp_poly(-alpha * spl));
             }
         }
+    }
 
-        // ---- per-bounce refraction geometry (refraction.rs:235-547) ----
-        float cos_i = clampf(dot(neg(cur_k), normal), 0.0f, 1.0f);
-        float sin_i = sqrtf(fmaxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
-        float one_m_c2 = fmaf(cos_i, -cos_i, 1.0f);
-        float n_med[8];
-        if (kAniso) {
-            float theta_c;
-            if (!inside) {
-                // theta_c_for_bounce's two-step fixed point (refraction.rs:248-269).
-                float n_seed = n_o[0];
-                float n_e_seed = n_seed + dn;
-                float n_guess = n_seed;
-                theta_c = 0.0f;
-                for (int it = 0; it < 2; ++it) {
-                    float eta_g = 1.0f / n_guess;
-                    float sin2_g = eta_g * eta_g * one_m_c2;
-                    if (sin2_g > 1.0f) break;
-                    float cos_t_g = sqrtf(fmaxf(1.0f - sin2_g, 0.0f));
-                    V3 wave = normalize(add(mul(cur_k, eta_g), mul(normal, fmaf(eta_g, cos_i, -cos_t_g))));
-                    theta_c = acosf(fabsf(clampf(dot(wave, c_axis), -1.0f, 1.0f)));
-                    n_guess = effective_extraordinary_index(n_seed, n_e_seed, theta_c);
-                }
-            } else {
-                theta_c = acosf(fabsf(clampf(dot(cur_k, c_axis), -1.0f, 1.0f)));
+    // ---- per-bounce refraction geometry (refraction.rs:235-547) ----
+    float cos_i = clampf(dot(neg(cur_k), normal), 0.0f, 1.0f);
+    float sin_i = sqrtf(fmaxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
+    float one_m_c2 = fmaf(cos_i, -cos_i, 1.0f);
+    float n_med[8];
+    if (kAniso) {
+        float theta_c;
+        if (!inside) {
+            // theta_c_for_bounce's two-step fixed point (refraction.rs:248-269).
+            float n_seed = ps.n_o[0];
+            float n_e_seed = n_seed + dn;
+            float n_guess = n_seed;
+            theta_c = 0.0f;
+            for (int it = 0; it < 2; ++it) {
+                float eta_g = 1.0f / n_guess;
+                float sin2_g = eta_g * eta_g * one_m_c2;
+                if (sin2_g > 1.0f) break;
+                float cos_t_g = sqrtf(fmaxf(1.0f - sin2_g, 0.0f));
+                V3 wave = normalize(add(mul(cur_k, eta_g), mul(normal, fmaf(eta_g, cos_i, -cos_t_g))));
+                theta_c = acosf(fabsf(clampf(dot(wave, c_axis), -1.0f, 1.0f)));
+                n_guess = effective_extraordinary_index(n_seed, n_e_seed, theta_c);
             }
-            bool use_o = inside && !is_extra;
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                n_med[k] = use_o ? n_o[k] : effective_extraordinary_index(n_o[k], n_o[k] + dn, theta_c);
         } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) n_med[k] = n_o[k];
+            theta_c = acosf(fabsf(clampf(dot(cur_k, c_axis), -1.0f, 1.0f)));
         }
-        float n1_ch[8], n2_ch[8], s2t_ch[8];
+        bool use_o = inside && !ps.is_extra;
+        // theta_c is shared by the 8 channels: one sincos serves all of them.
+        float sin_t, cos_t;
+        sincosf(theta_c, &sin_t, &cos_t);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            n_med[k] = use_o ? ps.n_o[k] : effective_extraordinary_index_sc(ps.n_o[k], ps.n_o[k] + dn, sin_t, cos_t);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) n_med[k] = ps.n_o[k];
+    }
+    float s2t_ch[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float n1k = inside ? n_med[k] : 1.0f;
+        float n2k = inside ? 1.0f : n_med[k];
+        float etak = n1k / n2k;
+        s2t_ch[k] = etak * etak * one_m_c2;
+    }
+    const float n1 = inside ? n_med[0] : 1.0f, n2 = inside ? 1.0f : n_med[0];
+    const float sin2_t = s2t_ch[0];
+    const float n_o_hero = ps.n_o[0], n_e_hero = ps.n_o[0] + dn;
+
+    // ---- TIR / partial-reflect / refract dispatch (transport.rs:128-193) ----
+    V3 new_k, new_s;
+    bool new_inside = inside, has_update = false, update = false;
+    uint32_t event;
+    // Hero event: TIR when sin2_t > 1, else a Fresnel-weighted reflect/refract draw
+    // (transport.rs:158-176, refraction.rs:1036-1086).
+    const bool tir = sin2_t > 1.0f;
+    float r_unpol = 1.0f;
+    bool reflect = tir;
+    if (!tir) {
+        float cos_t = sqrtf(1.0f - sin2_t);
+        float r_s, r_p;
+        fresnel_rs_rp(n1, n2, cos_i, cos_t, &r_s, &r_p);
+        r_unpol = clampf(0.5f * fmaf(r_p, r_p, r_s * r_s), 1e-4f, 1.0f - 1e-4f);
+        reflect = stream_draw(ps.seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol;
+    }
+    if (reflect) {
+        // apply_tir_bounce (refraction.rs:599-637) and apply_partial_reflect_bounce
+        // (:650-690) share their per-channel work: retardation for channels past their
+        // own critical angle, Fresnel reflection (and its pdf factor) otherwise.  The
+        // partial reflection then divides by the hero's r_unpol; TIR scales by 1.0, which
+        // is exact.
+        if (tir) {
+            ev[GR_EV_TIR]++;
+            event = 1u;
+        } else {
+            ev[GR_EV_REFLECT]++;
+            event = 2u;
+        }
+        const float inv_r = tir ? 1.0f : 1.0f / r_unpol;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float n1k = inside ? n_med[k] : 1.0f, n2k = inside ? 1.0f : n_med[k];
+            if (s2t_ch[k] > 1.0f) {
+                float sd, cd;
+                tir_retardation_cs(n1k, cos_i, sin_i, &cd, &sd);
+                ps.st[k] = apply_retardation(ps.st[k], cd, sd);
+            } else {
+                float cos_t_k = sqrtf(fmaxf(1.0f - s2t_ch[k], 0.0f));
+                float rs, rp;
+                fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
+                ps.pdf[k] *= clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
+                ps.st[k] = apply_fresnel_reflection(ps.st[k], rs, rp);
+            }
+            ps.st[k] = scale(ps.st[k], inv_r);
+        }
+        new_k = sub(cur_k, mul(normal, 2.0f * dot(cur_k, normal)));
+        new_s = new_k;
+        if (kAniso && inside && ps.is_extra) {
+            ev[GR_EV_WALKOFF]++;
+            new_s = extraordinary_poynting_dir(new_k, c_axis, n_o_hero, n_e_hero);
+        }
+    } else {
+        // apply_refract_bounce + apply_refract_channel (refraction.rs:748-1011).
+        ev[GR_EV_REFRACT]++;
+        event = 3u;
+        const float DIRECTION_MATCH_COS_TOL = 1.0f - 1e-6f;
+        bool entering = kAniso && !inside;
+        bool use_extra = ps.is_extra;
+        if (entering) {
+            ev[GR_EV_ENTRY_ANISO]++;
+            use_extra = stream_draw(ps.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
+        }
+        float n2_dir = (entering && !use_extra) ? n_o_hero : n2;
+        float eta_dir = n1 / n2_dir;
+        float sin2_dir = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
+        float cos_t_dir = sqrtf(fmaxf(1.0f - sin2_dir, 0.0f));
+        V3 refr = normalize(add(mul(cur_k, eta_dir), mul(normal, fmaf(eta_dir, cos_i, -cos_t_dir))));
+        V3 final_dir = refr;
+        if (entering && use_extra) {
+            ev[GR_EV_WALKOFF]++;
+            final_dir = extraordinary_poynting_dir(refr, c_axis, n_o_hero, n_e_hero);
+        }
+        float inv_t = 1.0f / (1.0f - r_unpol);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             float n1k = inside ? n_med[k] : 1.0f;
-            float n2k = inside ? 1.0f : n_med[k];
-            float etak = n1k / n2k;
-            n1_ch[k] = n1k;
-            n2_ch[k] = n2k;
-            s2t_ch[k] = etak * etak * one_m_c2;
-        }
-        const float n1 = n1_ch[0], n2 = n2_ch[0];
-        const float sin2_t = s2t_ch[0];
-        const float n_o_hero = n_o[0], n_e_hero = n_o[0] + dn;
-
-        // ---- TIR / partial-reflect / refract dispatch (transport.rs:128-193) ----
-        V3 new_k, new_s;
-        bool new_inside = inside, has_update = false, update = false;
-        uint32_t event;
-        // Hero event: TIR when sin2_t > 1, else a Fresnel-weighted reflect/refract draw
-        // (transport.rs:158-176, refraction.rs:1036-1086).
-        const bool tir = sin2_t > 1.0f;
-        float r_unpol = 1.0f;
-        bool reflect = tir;
-        if (!tir) {
-            float cos_t = sqrtf(1.0f - sin2_t);
-            float r_s, r_p;
-            fresnel_rs_rp(n1, n2, cos_i, cos_t, &r_s, &r_p);
-            r_unpol = clampf(0.5f * fmaf(r_p, r_p, r_s * r_s), 1e-4f, 1.0f - 1e-4f);
-            reflect = stream_draw(rng_seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol;
-        }
-        if (reflect) {
-            // apply_tir_bounce (refraction.rs:599-637) and apply_partial_reflect_bounce
-            // (:650-690) share their per-channel work: retardation for channels past their
-            // own critical angle, Fresnel reflection (and its pdf factor) otherwise.  The
-            // partial reflection then divides by the hero's r_unpol; TIR scales by 1.0,
-            // which is exact.
-            if (tir) {
-                ev[GR_EV_TIR]++;
-                event = 1u;
+            float n2k = (entering && !use_extra) ? ps.n_o[k] : (inside ? 1.0f : n_med[k]);
+            float q = n1k / n2k;
+            float s2k = q * q * one_m_c2;
+            bool alive = !(s2k > 1.0f);
+            float cos_t_k = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+            if (alive) {
+                V3 dk = normalize(add(mul(cur_k, q), mul(normal, fmaf(q, cos_i, -cos_t_k))));
+                if (entering && use_extra)
+                    dk = extraordinary_poynting_dir(dk, c_axis, ps.n_o[k], ps.n_o[k] + dn);
+                alive = dot(dk, final_dir) >= DIRECTION_MATCH_COS_TOL;
+            }
+            if (alive) {
+                float num = 2.0f * n1k * cos_i;
+                float t_s = num / fmaf(n2k, cos_t_k, n1k * cos_i);
+                float t_p = num / fmaf(n1k, cos_t_k, n2k * cos_i);
+                ps.st[k] = scale(apply_fresnel_transmission(ps.st[k], n1k, n2k, cos_i, cos_t_k, t_s, t_p), inv_t);
+                float rs, rp;
+                fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
+                ps.pdf[k] *= 1.0f - clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
             } else {
-                ev[GR_EV_REFLECT]++;
-                event = 2u;
-            }
-            const float inv_r = tir ? 1.0f : 1.0f / r_unpol;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (s2t_ch[k] > 1.0f) {
-                    float sd, cd;
-                    sincosf(tir_phase_delta(n1_ch[k], cos_i, sin_i), &sd, &cd);
-                    st[k] = apply_retardation(st[k], cd, sd);
-                } else {
-                    float cos_t_k = sqrtf(fmaxf(1.0f - s2t_ch[k], 0.0f));
-                    float rs, rp;
-                    fresnel_rs_rp(n1_ch[k], n2_ch[k], cos_i, cos_t_k, &rs, &rp);
-                    pdf[k] *= clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
-                    st[k] = apply_fresnel_reflection(st[k], rs, rp);
-                }
-                st[k] = scale(st[k], inv_r);
-            }
-            new_k = sub(cur_k, mul(normal, 2.0f * dot(cur_k, normal)));
-            new_s = new_k;
-            if (kAniso && inside && is_extra) {
-                ev[GR_EV_WALKOFF]++;
-                new_s = extraordinary_poynting_dir(new_k, c_axis, n_o_hero, n_e_hero);
-            }
-        } else {
-            {
-                // apply_refract_bounce + apply_refract_channel (refraction.rs:748-1011).
-                ev[GR_EV_REFRACT]++;
-                event = 3u;
-                const float DIRECTION_MATCH_COS_TOL = 1.0f - 1e-6f;
-                bool entering = kAniso && !inside;
-                bool use_extra = is_extra;
-                if (entering) {
-                    ev[GR_EV_ENTRY_ANISO]++;
-                    use_extra = stream_draw(rng_seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
-                }
-                float n2_dir = (entering && !use_extra) ? n_o_hero : n2;
-                float eta_dir = n1 / n2_dir;
-                float sin2_dir = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
-                float cos_t_dir = sqrtf(fmaxf(1.0f - sin2_dir, 0.0f));
-                V3 refr = normalize(add(mul(cur_k, eta_dir), mul(normal, fmaf(eta_dir, cos_i, -cos_t_dir))));
-                V3 final_dir = refr;
-                if (entering && use_extra) {
-                    ev[GR_EV_WALKOFF]++;
-                    final_dir = extraordinary_poynting_dir(refr, c_axis, n_o_hero, n_e_hero);
-                }
-                float inv_t = 1.0f / (1.0f - r_unpol);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    float n1k = n1_ch[k];
-                    float n2k = (entering && !use_extra) ? n_o[k] : n2_ch[k];
-                    float q = n1k / n2k;
-                    float s2k = q * q * one_m_c2;
-                    bool alive = !(s2k > 1.0f);
-                    float cos_t_k = sqrtf(fmaxf(1.0f - s2k, 0.0f));
-                    if (alive) {
-                        V3 dk = normalize(add(mul(cur_k, q), mul(normal, fmaf(q, cos_i, -cos_t_k))));
-                        if (entering && use_extra)
-                            dk = extraordinary_poynting_dir(dk, c_axis, n_o[k], n_o[k] + dn);
-                        alive = dot(dk, final_dir) >= DIRECTION_MATCH_COS_TOL;
-                    }
-                    if (alive) {
-                        float num = 2.0f * n1k * cos_i;
-                        float t_s = num / fmaf(n2k, cos_t_k, n1k * cos_i);
-                        float t_p = num / fmaf(n1k, cos_t_k, n2k * cos_i);
-                        st[k] = scale(apply_fresnel_transmission(st[k], n1k, n2k, cos_i, cos_t_k, t_s, t_p), inv_t);
-                        float rs, rp;
-                        fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
-                        pdf[k] *= 1.0f - clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
-                    } else {
-                        st[k] = scale(st[k], 0.0f);
-                        pdf[k] = 0.0f;
-                    }
-                }
-                new_k = refr;
-                new_s = final_dir;
-                new_inside = !inside;
-                has_update = entering;
-                update = use_extra;
+                ps.st[k] = scale(ps.st[k], 0.0f);
+                ps.pdf[k] = 0.0f;
             }
         }
-        bool internal_reflection = inside && !has_update && new_inside == inside;
-        ray_d = new_s;
-        ray_o = add(hit_point, mul(new_s, 1e-4f));
-        cur_k = new_k;
-        inside = new_inside;
-        if (has_update) is_extra = update;
-        if (kAniso && internal_reflection)
-            is_extra = stream_draw(rng_seed, bounce, MODE_COUPLING_STREAM) < 0.5f;
-        fnv(fh, (uint32_t)facet);
-        fnv(fh, event | (is_extra ? 16u : 0u));
-        {
-            // Per-channel fates: past the channel's critical angle / pdf zero.
-            uint32_t fate = 0u;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                fate |= (s2t_ch[k] > 1.0f ? 1u : 0u) << k;
-                fate |= (pdf[k] == 0.0f ? 1u : 0u) << (8 + k);
-            }
-            fnv(fate_h, fate);
-        }
-
-        // ---- Russian roulette (transport.rs:204-220) ----
-        if (bounce > 4) {
-            ev[GR_EV_RR]++;
-            float max_i = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) max_i = fmaxf(max_i, intensity(st[k]));
-            float q = clampf(max_i, 0.05f, 1.0f);
-            if (stream_draw(rng_seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) {
-                term_bounce = bounce;
-                term_reason = 2u;
-                break;
-            }
-            float inv_q = 1.0f / q;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) st[k] = scale(st[k], inv_q);
-        }
+        new_k = refr;
+        new_s = final_dir;
+        new_inside = !inside;
+        has_update = entering;
+        update = use_extra;
     }
-
-    PathOut out;
-    // A path that never escaped carries zero radiance: the reference's integration and
-    // white balance then produce exactly +0 in every component (every product is +-0 and
-    // every sum has a +0 operand), so that work is skipped.
-    out.xyz = v3(0.0f, 0.0f, 0.0f);
-    if (escaped) {
-        // ---- environment radiance of the escape direction (transport.rs:67-90) ----
-        float rad[8];
-        if (kEnv == 0) {
-            // sample_studio_environment_with_rig (environment.rs:280-321): the
-            // direction-only terms are formed once, the fma chain runs per channel.
-            V3 d = normalize(ray_d);
-            float bg = fmaxf(fmaf(0.012f, fmaf(d.y, 0.5f, 0.5f), 0.015f), 0.005f) * sc.exposure;
-            float terms[18];
-            bool act[18];
-            float kd = fmaxf(dot(d, v3(sc.key[0], sc.key[1], sc.key[2])), 0.0f);
-            act[0] = kd > 0.0f;
-            terms[0] = powi_cr<28>(kd) * 12.0f * sc.spot_mult * sc.exposure;
-            float fd = fmaxf(dot(d, v3(sc.fill[0], sc.fill[1], sc.fill[2])), 0.0f);
-            act[1] = fd > 0.0f;
-            terms[1] = powi_cr<18>(fd) * 4.5f * sc.exposure;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                float rd = fmaxf(dot(d, v3(sc.ring[r][0], sc.ring[r][1], sc.ring[r][2])), 0.0f);
-                act[2 + r] = rd > 0.96f;
-                float spark = (rd - 0.96f) / 0.04f;
-                terms[2 + r] = powi_cr<6>(spark) * 22.0f * sc.spot_mult * sc.exposure;
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                float spec = blackbody(lam(k), sc);
-                float r = bg * spec;
-#pragma unroll
-                for (int j = 0; j < 18; ++j)
-                    if (act[j]) r = fmaf(terms[j], spec, r);
-                rad[k] = fmaf(intensity(st[k]), r, 0.0f);
-            }
-        } else {
-            V3 rgb = map_rgb(ray_d, sc);
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                rad[k] = fmaf(intensity(st[k]), rgb_to_spectral(rgb, lam(k)), 0.0f);
-        }
-        // ---- spectral MIS + CIE XYZ (color.rs:121-164) + von Kries (color.rs:215-251) --
-        float sum_pdf = -0.0f;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) sum_pdf = sum_pdf + pdf[k];
-        float mis = sum_pdf <= 1e-12f ? 1.0f : (8.0f * pdf[0]) / sum_pdf;
-        const float norm_factor = (400.0f / 8.0f) / 106.856f;
-        V3 xyz = v3(0.0f, 0.0f, 0.0f);
+    bool internal_reflection = inside && !has_update && new_inside == inside;
+    ps.ray_d = new_s;
+    ps.ray_o = add(hit_point, mul(new_s, 1e-4f));
+    ps.cur_k = new_k;
+    ps.inside = new_inside;
+    if (has_update) ps.is_extra = update;
+    if (kAniso && internal_reflection)
+        ps.is_extra = stream_draw(ps.seed, bounce, MODE_COUPLING_STREAM) < 0.5f;
+    fnv(ps.fh, (uint32_t)facet);
+    fnv(ps.fh, event | (ps.is_extra ? 16u : 0u));
+    {
+        // Per-channel fates: past the channel's critical angle / pdf zero.
+        uint32_t fate = 0u;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            V3 c = cmf_x8_channel(lam(k));
-            float w = rad[k] * mis * norm_factor;
-            xyz = add(xyz, mul(c, w));
+            fate |= (s2t_ch[k] > 1.0f ? 1u : 0u) << k;
+            fate |= (ps.pdf[k] == 0.0f ? 1u : 0u) << (8 + k);
         }
-        V3 lms = v3(fmaf(0.8951f, xyz.x, fmaf(0.2664f, xyz.y, -0.1614f * xyz.z)),
-                    fmaf(-0.7502f, xyz.x, fmaf(1.7135f, xyz.y, 0.0367f * xyz.z)),
-                    fmaf(0.0389f, xyz.x, fmaf(-0.0685f, xyz.y, 1.0296f * xyz.z)));
-        lms = v3(lms.x * sc.wb[0], lms.y * sc.wb[1], lms.z * sc.wb[2]);
-        out.xyz = v3(fmaf(0.986993f, lms.x, fmaf(-0.147054f, lms.y, 0.159963f * lms.z)),
-                     fmaf(0.432305f, lms.x, fmaf(0.518360f, lms.y, 0.049291f * lms.z)),
-                     fmaf(-0.008529f, lms.x, fmaf(0.040043f, lms.y, 0.968487f * lms.z)));
+        fnv(ps.fate_h, fate);
     }
-    out.segments = term_reason == 3u ? sc.max_bounces : term_bounce + 1u;
-    out.termination = term_reason;
-    out.facet_hash = fh;
-    out.fate_hash = fate_h;
-    return out;
+
+    // ---- Russian roulette (transport.rs:204-220) ----
+    if (bounce > 4) {
+        ev[GR_EV_RR]++;
+        float max_i = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) max_i = fmaxf(max_i, intensity(ps.st[k]));
+        float q = clampf(max_i, 0.05f, 1.0f);
+        if (stream_draw(ps.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) {
+            ps.term_bounce = bounce;
+            ps.term_reason = 2u;
+            return true;
+        }
+        float inv_q = 1.0f / q;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ps.st[k] = scale(ps.st[k], inv_q);
+    }
+    ps.bounce = bounce + 1u;
+    return ps.bounce >= sc.max_bounces;
+}
+
+// Environment radiance of the escape direction, spectral MIS, CIE XYZ and white balance
+// (transport.rs:67-90, color.rs:121-164, 215-251).  A path that never escaped carries
+// zero radiance: the reference's integration and white balance then produce exactly +0
+// in every component (every product is +-0 and every sum has a +0 operand), so that work
+// is skipped by the callers.
+template <int kEnv>
+__device__ __forceinline__ V3 finish_escaped(V3 dir, const float *inten, const float *pdf,
+                                             float hero_off, const DevScene &sc) {
+    float rad[8];
+    if (kEnv == 0) {
+        // sample_studio_environment_with_rig (environment.rs:280-321): the direction-only
+        // terms are formed once, the fma chain runs per channel.
+        V3 d = normalize(dir);
+        float bg = fmaxf(fmaf(0.012f, fmaf(d.y, 0.5f, 0.5f), 0.015f), 0.005f) * sc.exposure;
+        float terms[18];
+        bool act[18];
+        float kd = fmaxf(dot(d, v3(sc.key[0], sc.key[1], sc.key[2])), 0.0f);
+        act[0] = kd > 0.0f;
+        terms[0] = powi_cr<28>(kd) * 12.0f * sc.spot_mult * sc.exposure;
+        float fd = fmaxf(dot(d, v3(sc.fill[0], sc.fill[1], sc.fill[2])), 0.0f);
+        act[1] = fd > 0.0f;
+        terms[1] = powi_cr<18>(fd) * 4.5f * sc.exposure;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float rd = fmaxf(dot(d, v3(sc.ring[r][0], sc.ring[r][1], sc.ring[r][2])), 0.0f);
+            act[2 + r] = rd > 0.96f;
+            float spark = (rd - 0.96f) / 0.04f;
+            terms[2 + r] = powi_cr<6>(spark) * 22.0f * sc.spot_mult * sc.exposure;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float spec = blackbody(comb_lambda(hero_off, k), sc);
+            float r = bg * spec;
+#pragma unroll
+            for (int j = 0; j < 18; ++j)
+                if (act[j]) r = fmaf(terms[j], spec, r);
+            rad[k] = fmaf(inten[k], r, 0.0f);
+        }
+    } else {
+        V3 rgb = map_rgb(dir, sc);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) rad[k] = fmaf(inten[k], rgb_to_spectral(rgb, comb_lambda(hero_off, k)), 0.0f);
+    }
+    float sum_pdf = -0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum_pdf = sum_pdf + pdf[k];
+    float mis = sum_pdf <= 1e-12f ? 1.0f : (8.0f * pdf[0]) / sum_pdf;
+    const float norm_factor = (400.0f / 8.0f) / 106.856f;
+    V3 xyz = v3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        V3 c = cmf_x8_channel(comb_lambda(hero_off, k));
+        float w = rad[k] * mis * norm_factor;
+        xyz = add(xyz, mul(c, w));
+    }
+    V3 lms = v3(fmaf(0.8951f, xyz.x, fmaf(0.2664f, xyz.y, -0.1614f * xyz.z)),
+                fmaf(-0.7502f, xyz.x, fmaf(1.7135f, xyz.y, 0.0367f * xyz.z)),
+                fmaf(0.0389f, xyz.x, fmaf(-0.0685f, xyz.y, 1.0296f * xyz.z)));
+    lms = v3(lms.x * sc.wb[0], lms.y * sc.wb[1], lms.z * sc.wb[2]);
+    return v3(fmaf(0.986993f, lms.x, fmaf(-0.147054f, lms.y, 0.159963f * lms.z)),
+              fmaf(0.432305f, lms.x, fmaf(0.518360f, lms.y, 0.049291f * lms.z)),
+              fmaf(-0.008529f, lms.x, fmaf(0.040043f, lms.y, 0.968487f * lms.z)));
+}
+
+__device__ __forceinline__ uint32_t path_segments(const PathState &ps, const DevScene &sc) {
+    return ps.term_reason == 3u ? sc.max_bounces : ps.term_bounce + 1u;
 }
 
 // ---------------------------------------------------------------------------------
@@ -570,7 +626,7 @@ __device__ __forceinline__ PathOut trace_path(V3 ray_o, V3 ray_d, uint32_t rng_s
 
 constexpr int kBlock = 256;
 
-// Occupancy target of the trace kernel (waves per SIMD); 0 leaves the register budget to
+// Occupancy target of the trace kernels (waves per SIMD); 0 leaves the register budget to
 // the compiler.  A tuning parameter chosen by measurement: on the single-job workload
 // 3 waves/SIMD (168 VGPRs) ran 1.36x faster than the compiler's 2 and 1.03x faster than 4.
 #ifndef GR_TRACE_WAVES_PER_EU
@@ -582,20 +638,42 @@ constexpr int kBlock = 256;
 #define GR_TRACE_OCCUPANCY
 #endif
 
-// One lane per (pixel, sample) tuple t of the launch: pixel = pixel_begin + t / spp,
-// sample = sample_offset + t % spp.  Writes the tuple's XYZ (and, with kInfo, its
-// segment count, termination and facet-sequence hash); accumulates per-class event
-// counters into stats (one atomic per counter per workgroup).
+// Tuples per wavefront of trace_wave_kernel: 64 lanes x this many rounds.
+#ifndef GR_WAVE_ROUNDS
+#define GR_WAVE_ROUNDS 8
+#endif
+constexpr uint32_t kWaveTuples = 64u * GR_WAVE_ROUNDS;
+
+__device__ __forceinline__ void stage_planes(float4 *lds, const float4 *__restrict__ g, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) lds[i] = g[i];
+}
+
+// Per-class counters: wave-level sum, one LDS atomic per wave, one global atomic per
+// counter per workgroup.
+__device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block_ev,
+                                             unsigned long long *stats) {
+#pragma unroll
+    for (int i = 0; i < GR_EV_COUNT; ++i) {
+        uint32_t v = ev[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&block_ev[i], v);
+    }
+    __syncthreads();
+    if (threadIdx.x < GR_EV_COUNT && block_ev[threadIdx.x])
+        atomicAdd(&stats[threadIdx.x], (unsigned long long)block_ev[threadIdx.x]);
+}
+
+// The plain path: one lane per (pixel, sample) tuple t of the launch, pixel =
+// pixel_begin + t / spp, sample = sample_offset + t % spp.  Writes the tuple's XYZ (and,
+// with kInfo, segments, termination and the two path hashes).
 template <bool kAniso, bool kAbsorb, int kEnv, bool kInfo>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(const DevScene sc, const float4 *__restrict__ planes_g,
-                                                       uint32_t pixel_begin, uint32_t sample_offset,
-                                                       uint32_t spp, uint32_t n_tuples,
-                                                       float *__restrict__ xyz_out,
-                                                       uint32_t *__restrict__ info_out,
-                                                       unsigned long long *__restrict__ stats) {
+__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
+    const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
+    uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples,
+    float *__restrict__ xyz_out, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    for (uint32_t i = threadIdx.x; i < sc.n_planes; i += kBlock) planes[i] = planes_g[i];
+    stage_planes(planes, planes_g, sc.n_planes);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -607,48 +685,129 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(const 
     if (t < n_tuples) {
         const uint32_t local_pixel = t / spp;
         const uint32_t s = t - local_pixel * spp;
-        const uint32_t pixel = pixel_begin + local_pixel;
-        const uint32_t sample_num = sample_offset + s;
-        const uint32_t x = pixel % sc.width, y = pixel / sc.width;
-        // pixel_rotations + sample_draws (sampling.rs:232-271).
-        float rot_x = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_X_ROTATION_STREAM));
-        float rot_y = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_Y_ROTATION_STREAM));
-        float rot_h = low_discrepancy_base2(hash_u32(pixel ^ HERO_WAVELENGTH_ROTATION_STREAM));
-        uint32_t seed = hash_u32((pixel * 0x9e3779b9u) ^ (sample_num * 0x85ebca6bu));
-        float jx = cranley_patterson(low_discrepancy_base2(sample_num), rot_x) - 0.5f;
-        float jy = cranley_patterson(radical_inverse(sample_num, 3), rot_y) - 0.5f;
-        float hero = cranley_patterson(radical_inverse(sample_num, 5), rot_h);
-        // Camera::generate_ray (camera.rs:96-114).
-        float w = (float)sc.width, h = (float)sc.height;
-        float aspect = w / h;
-        float u = (((float)x + jx) / w - 0.5f) * 2.0f * aspect * sc.fov_tan;
-        float v = (0.5f - ((float)y + jy) / h) * 2.0f * sc.fov_tan;
-        V3 dir = normalize(add(add(v3(sc.cam_f[0], sc.cam_f[1], sc.cam_f[2]),
-                                   mul(v3(sc.cam_r[0], sc.cam_r[1], sc.cam_r[2]), u)),
-                               mul(v3(sc.cam_u[0], sc.cam_u[1], sc.cam_u[2]), v)));
-        V3 org = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
-        PathOut r = trace_path<kAniso, kAbsorb, kEnv>(org, dir, seed, hero, sc, planes, ev);
-        xyz_out[3 * (size_t)t + 0] = r.xyz.x;
-        xyz_out[3 * (size_t)t + 1] = r.xyz.y;
-        xyz_out[3 * (size_t)t + 2] = r.xyz.z;
+        PathState ps;
+        init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], sc);
+        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, ev)) {
+        }
+        V3 xyz = v3(0.0f, 0.0f, 0.0f);
+        if (ps.escaped) {
+            float inten[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) inten[k] = intensity(ps.st[k]);
+            xyz = finish_escaped<kEnv>(ps.ray_d, inten, ps.pdf, ps.hero_off, sc);
+        }
+        xyz_out[3 * (size_t)t + 0] = xyz.x;
+        xyz_out[3 * (size_t)t + 1] = xyz.y;
+        xyz_out[3 * (size_t)t + 2] = xyz.z;
         if (kInfo) {
-            info_out[4 * (size_t)t + 0] = r.segments;
-            info_out[4 * (size_t)t + 1] = r.termination;
-            info_out[4 * (size_t)t + 2] = r.facet_hash;
-            info_out[4 * (size_t)t + 3] = r.fate_hash;
+            info_out[4 * (size_t)t + 0] = path_segments(ps, sc);
+            info_out[4 * (size_t)t + 1] = ps.term_reason;
+            info_out[4 * (size_t)t + 2] = ps.fh;
+            info_out[4 * (size_t)t + 3] = ps.fate_h;
         }
     }
-    // Per-class counters: wave-level sum, one LDS atomic per wave, one global atomic per
-    // counter per workgroup.
-#pragma unroll
-    for (int i = 0; i < GR_EV_COUNT; ++i) {
-        uint32_t v = ev[i];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&block_ev[i], v);
-    }
+    flush_events(ev, block_ev, stats);
+}
+
+// Escape records: field-major (record field f of tuple t at rec[f * cap + t]).
+// Fields: 0-2 escape direction, 3-10 channel intensities, 11-18 channel pdfs,
+// 19 hero offset (negative: the path never escaped).
+constexpr int kRecFields = 20;
+
+// Persistent wavefronts.  Wave w owns tuples [w * kWaveTuples, (w+1) * kWaveTuples) of
+// the launch.  Every lane starts on one tuple; whenever lanes finish their paths, the
+// wave hands them the next unstarted tuples of its range in lane order (ballot +
+// popcount), so lanes stay busy until the range is exhausted instead of idling behind the
+// wave's longest path.  Each finished path writes its escape record; every loop
+// iteration advances every busy lane by one bounce, and every path ends within
+// max_bounces iterations, so the loop ends after at most kWaveTuples * max_bounces
+// iterations.
+template <bool kAniso, bool kAbsorb, bool kInfo>
+__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
+    const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
+    uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, uint32_t rec_cap,
+    float *__restrict__ rec, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
+    __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ uint32_t block_ev[GR_EV_COUNT];
+    stage_planes(planes, planes_g, sc.n_planes);
+    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
-    if (threadIdx.x < GR_EV_COUNT && block_ev[threadIdx.x])
-        atomicAdd(&stats[threadIdx.x], (unsigned long long)block_ev[threadIdx.x]);
+
+    uint32_t ev[GR_EV_COUNT];
+#pragma unroll
+    for (int i = 0; i < GR_EV_COUNT; ++i) ev[i] = 0u;
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t begin = wave * kWaveTuples;
+    const uint32_t end = min(begin + kWaveTuples, n_tuples);
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+
+    PathState ps;
+    uint32_t t = begin + lane;
+    bool busy = t < end;
+    uint32_t next = begin + 64u;  // wave-uniform
+    if (busy) {
+        const uint32_t lp = t / spp, s = t - lp * spp;
+        init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], sc);
+    }
+    while (__ballot(busy) != 0ull) {
+        if (busy && bounce_step<kAniso, kAbsorb>(ps, sc, planes, ev)) {
+            // Path ended: escape record (and per-tuple info).
+            float *r = rec + t;
+            r[0 * (size_t)rec_cap] = ps.ray_d.x;
+            r[1 * (size_t)rec_cap] = ps.ray_d.y;
+            r[2 * (size_t)rec_cap] = ps.ray_d.z;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                r[(3 + k) * (size_t)rec_cap] = intensity(ps.st[k]);
+                r[(11 + k) * (size_t)rec_cap] = ps.pdf[k];
+            }
+            r[19 * (size_t)rec_cap] = ps.escaped ? ps.hero_off : -1.0f;
+            if (kInfo) {
+                info_out[4 * (size_t)t + 0] = path_segments(ps, sc);
+                info_out[4 * (size_t)t + 1] = ps.term_reason;
+                info_out[4 * (size_t)t + 2] = ps.fh;
+                info_out[4 * (size_t)t + 3] = ps.fate_h;
+            }
+            busy = false;
+        }
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t slot = next + (uint32_t)__popcll(idle & lanes_below);
+        next += (uint32_t)__popcll(idle);
+        if (!busy && slot < end) {
+            t = slot;
+            const uint32_t lp = t / spp, s = t - lp * spp;
+            init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], sc);
+            busy = true;
+        }
+    }
+    flush_events(ev, block_ev, stats);
+}
+
+// Escape records -> per-tuple XYZ (the finish of trace_spectral_ray_inner).
+template <int kEnv>
+__global__ __launch_bounds__(kBlock) void finish_kernel(const DevScene sc, uint32_t n_tuples, uint32_t rec_cap,
+                                                        const float *__restrict__ rec,
+                                                        float *__restrict__ xyz_out) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_tuples) return;
+    const float *r = rec + t;
+    const float hero_off = r[19 * (size_t)rec_cap];
+    V3 xyz = v3(0.0f, 0.0f, 0.0f);
+    if (hero_off >= 0.0f) {
+        float inten[8], pdf[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            inten[k] = r[(3 + k) * (size_t)rec_cap];
+            pdf[k] = r[(11 + k) * (size_t)rec_cap];
+        }
+        V3 dir = v3(r[0], r[(size_t)rec_cap], r[2 * (size_t)rec_cap]);
+        xyz = finish_escaped<kEnv>(dir, inten, pdf, hero_off, sc);
+    }
+    xyz_out[3 * (size_t)t + 0] = xyz.x;
+    xyz_out[3 * (size_t)t + 1] = xyz.y;
+    xyz_out[3 * (size_t)t + 2] = xyz.z;
 }
 
 // Each pixel's spp tuples summed in sample order and ADDED into the film
@@ -679,34 +838,62 @@ __global__ __launch_bounds__(kBlock) void film_reduce_kernel(const float *__rest
 // ---------------------------------------------------------------------------------
 
 namespace {
+
 template <bool A, bool B, int E>
-hipError_t launch_trace_t(const gr::DevScene &sc, const float4 *planes, uint32_t pixel_begin,
-                          uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, float *xyz,
-                          uint32_t *info, unsigned long long *stats, hipStream_t stream) {
-    dim3 grid((n_tuples + gr::kBlock - 1) / gr::kBlock);
-    if (info)
-        hipLaunchKernelGGL((gr::trace_kernel<A, B, E, true>), grid, dim3(gr::kBlock), 0, stream, sc, planes,
-                           pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats);
+hipError_t launch_plain(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
+    dim3 grid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
+    if (a.info)
+        hipLaunchKernelGGL((gr::trace_kernel<A, B, E, true>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
+                           a.ri35, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats);
     else
-        hipLaunchKernelGGL((gr::trace_kernel<A, B, E, false>), grid, dim3(gr::kBlock), 0, stream, sc, planes,
-                           pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats);
+        hipLaunchKernelGGL((gr::trace_kernel<A, B, E, false>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
+                           a.ri35, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats);
     return hipGetLastError();
 }
+
+template <bool A, bool B, int E>
+hipError_t launch_wave(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
+    const uint32_t waves = (a.n_tuples + gr::kWaveTuples - 1) / gr::kWaveTuples;
+    dim3 grid((waves + gr::kBlock / 64 - 1) / (gr::kBlock / 64));
+    if (a.info)
+        hipLaunchKernelGGL((gr::trace_wave_kernel<A, B, true>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
+                           a.ri35, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec, a.info,
+                           a.stats);
+    else
+        hipLaunchKernelGGL((gr::trace_wave_kernel<A, B, false>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
+                           a.ri35, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec, a.info,
+                           a.stats);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 fgrid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
+    hipEvent_t mark = a.trace_done;
+    if (mark) {
+        e = hipEventRecord(mark, stream);
+        if (e != hipSuccess) return e;
+    }
+    if (E == 0)
+        hipLaunchKernelGGL((gr::finish_kernel<0>), fgrid, dim3(gr::kBlock), 0, stream, sc, a.n_tuples, a.rec_cap,
+                           a.rec, a.xyz);
+    else
+        hipLaunchKernelGGL((gr::finish_kernel<1>), fgrid, dim3(gr::kBlock), 0, stream, sc, a.n_tuples, a.rec_cap,
+                           a.rec, a.xyz);
+    return hipGetLastError();
+}
+
+template <bool A, bool B, int E>
+hipError_t launch_one(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
+    return a.wavefront ? launch_wave<A, B, E>(sc, a, stream) : launch_plain<A, B, E>(sc, a, stream);
+}
+
 }  // namespace
 
-hipError_t gr_launch_trace(const gr::DevScene &sc, const float4 *planes, uint32_t pixel_begin,
-                           uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, float *xyz,
-                           uint32_t *info, unsigned long long *stats, hipStream_t stream) {
-    const bool a = sc.anisotropic != 0, b = sc.has_absorption != 0;
+hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
+    const bool an = sc.anisotropic != 0, ab = sc.has_absorption != 0;
     const int e = sc.env_kind;
-    if (a && b && e == 0) return launch_trace_t<true, true, 0>(sc, planes, pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats, stream);
-    if (a && b && e == 1) return launch_trace_t<true, true, 1>(sc, planes, pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats, stream);
-    if (a && !b && e == 0) return launch_trace_t<true, false, 0>(sc, planes, pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats, stream);
-    if (a && !b && e == 1) return launch_trace_t<true, false, 1>(sc, planes, pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats, stream);
-    if (!a && b && e == 0) return launch_trace_t<false, true, 0>(sc, planes, pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats, stream);
-    if (!a && b && e == 1) return launch_trace_t<false, true, 1>(sc, planes, pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats, stream);
-    if (!a && !b && e == 0) return launch_trace_t<false, false, 0>(sc, planes, pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats, stream);
-    return launch_trace_t<false, false, 1>(sc, planes, pixel_begin, sample_offset, spp, n_tuples, xyz, info, stats, stream);
+    if (an && ab) return e == 0 ? launch_one<true, true, 0>(sc, a, stream) : launch_one<true, true, 1>(sc, a, stream);
+    if (an && !ab) return e == 0 ? launch_one<true, false, 0>(sc, a, stream) : launch_one<true, false, 1>(sc, a, stream);
+    if (!an && ab) return e == 0 ? launch_one<false, true, 0>(sc, a, stream) : launch_one<false, true, 1>(sc, a, stream);
+    return e == 0 ? launch_one<false, false, 0>(sc, a, stream) : launch_one<false, false, 1>(sc, a, stream);
 }
 
 hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t spp,

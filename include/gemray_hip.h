@@ -117,7 +117,14 @@ typedef struct {
     double trace_kernel_ms;       /* sum of trace-kernel durations (HIP events)          */
     double total_ms;              /* wall time of the call on its stream                 */
     uint32_t n_planes;
+    double finish_kernel_ms;      /* wavefront mode: sum of finish-kernel durations      */
 } gr_stats;
+
+/* Trace kernels.  Both evaluate identical per-tuple results; they differ in scheduling.
+ * GR_TRACE_PLAIN: one lane per (pixel, sample) tuple, as transport_main dispatches
+ * (frame.rs:437-452).  GR_TRACE_WAVEFRONT (default): persistent wavefronts that refill
+ * finished lanes with the next tuples of their range, plus a separate finish pass. */
+enum { GR_TRACE_PLAIN = 0, GR_TRACE_WAVEFRONT = 1 };
 
 /* GpuBackend::acquire / GpuFrameRenderer::new (gpu_backend.rs:114-126, frame.rs:574-594).
  * Returns GR_DECLINE_NO_DEVICE (and *out = NULL) when no gfx950 device is usable. */
@@ -129,6 +136,9 @@ void gr_close(gr_ctx *ctx);
 int gr_adapter_label(gr_ctx *ctx, char *buf, size_t len);
 /* GpuFrameRenderer::set_chunk_budget_bytes (frame.rs:602), in tuples. */
 gr_status gr_set_chunk_tuples(gr_ctx *ctx, uint64_t tuples);
+/* Selects the trace kernel (GR_TRACE_PLAIN / GR_TRACE_WAVEFRONT).  No reference
+ * counterpart: the reference has one dispatch shape (frame.rs:437-452). */
+gr_status gr_set_trace_mode(gr_ctx *ctx, int mode);
 
 /* GpuBackend::try_accumulate / GpuFrameRenderer::accumulate (gpu_backend.rs:160-193,
  * frame.rs:632-793): ADDS each pixel's sum over samples [sample_offset,

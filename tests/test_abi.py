@@ -37,7 +37,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.MaterialStruct) == 4 * 66
     assert ctypes.sizeof(_lib.Camera) == 4 * 13
     assert ctypes.sizeof(_lib.Env) == 4 * 8 + 8
-    assert ctypes.sizeof(_lib.Stats) == 8 * 11 + 8 + 8 + 8 + 8 + 8
+    assert ctypes.sizeof(_lib.Stats) == 8 * 11 + 8 + 8 + 8 + 8 + 8 + 8
 
 
 @pytest.mark.parametrize("args", [(0.35, 0.28, 5.0, 18.0), (0.4, 0.3, 3.0, 30.0),

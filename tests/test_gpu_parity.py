@@ -223,3 +223,25 @@ def test_declines_leave_film_untouched(gpu, srb):
     assert not np.all(accum == 3.0)
     with pytest.raises(AssertionError):
         gpu.try_accumulate(B.SceneRef(material=M.by_name("Diamond"), **base), 0, 2, np.zeros((5, 3), np.float32))
+
+
+@pytest.mark.parametrize("idx", [0, 2, 8, 10], ids=["diamond", "sapphire", "pleochroic-oval", "stress"])
+def test_plain_and_wavefront_kernels_agree_bit_for_bit(gpu, srb, idx):
+    """The persistent-wavefront kernel only reschedules tuples: every tuple's XYZ, segment
+    count, termination and path hashes equal the plain kernel's, and so does the film."""
+    case = _cases(srb)[idx]
+    scene = case.product_scene()
+    n_pix = case.width * case.height
+    out = {}
+    try:
+        for mode in ("plain", "wavefront"):
+            gpu.set_trace_mode(mode)
+            xyz, info = gpu.trace_samples(scene, 0, n_pix, 3, 6)
+            film = np.zeros((n_pix, 3), np.float32)
+            assert gpu.try_accumulate(scene, 3, 6, film)
+            out[mode] = (xyz, info, film, gpu.last_stats()["events"])
+    finally:
+        gpu.set_trace_mode("wavefront")
+    for a, b in zip(out["plain"][:3], out["wavefront"][:3]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert out["plain"][3] == out["wavefront"][3]
