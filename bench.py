@@ -60,27 +60,36 @@ def cpu_baseline(scene, cam_args, spp_total, budget_s):
     rows = 2
     total_segs = 0
     total_paths = 0
+    total_rows = 0
     elapsed = 0.0
     y0 = h // 2
-    while elapsed < budget_s and y0 + rows <= h:
-        p0, p1 = y0 * w, (y0 + rows) * w
+    sample_offset = 0
+    # Row windows from the frame centre downwards, wrapping to the top; a full pass moves
+    # on to the next 16 samples, so every pixel is weighted like the GPU frame.
+    while elapsed < budget_s:
+        n = min(rows, (h if y0 >= h // 2 else h // 2) - y0)
+        p0, p1 = y0 * w, (y0 + n) * w
         t0 = time.perf_counter()
-        _, _, _, segs = po.render(cam, w, h, scene.planes, mat, scene.max_bounces, env, 0, spp,
-                                  pixel_begin=p0, pixel_end=p1, threads=threads)
-        dt = time.perf_counter() - t0
+        _, _, _, segs = po.render(cam, w, h, scene.planes, mat, scene.max_bounces, env,
+                                  sample_offset, spp, pixel_begin=p0, pixel_end=p1, threads=threads)
+        elapsed += time.perf_counter() - t0
         total_segs += segs
         total_paths += (p1 - p0) * spp
-        elapsed += dt
-        y0 += rows
+        total_rows += n
+        y0 += n
+        if y0 == h:
+            y0 = 0
+        if y0 == h // 2:
+            sample_offset += spp
         rows = min(rows * 2, 64)
     return {
         "value": total_segs / elapsed,
         "unit": "segments/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle (C++ restatement of trace_spectral_ray) on rows {h // 2}..{y0} of the "
-                  f"{w}x{h} frame at {spp} spp: {total_paths} paths, {total_segs} segments in "
-                  f"{elapsed:.1f} s on {threads} threads",
+        "sample": f"oracle (C++ restatement of trace_spectral_ray) on {total_rows} rows of the "
+                  f"{w}x{h} frame (from row {h // 2}, wrapping) at {spp} spp per row pass: "
+                  f"{total_paths} paths, {total_segs} segments in {elapsed:.1f} s on {threads} threads",
         "paths_per_s": total_paths / elapsed,
     }
 
@@ -95,6 +104,7 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--trace-mode", choices=["plain", "wavefront"], default="plain")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -111,6 +121,7 @@ def main():
         sys.exit(1)
     print(f"bench: {gpu.adapter_label()}; {args.width}x{args.height} x {args.spp} spp/step, "
           f"P={scene.planes.shape[0]}", file=sys.stderr, flush=True)
+    gpu.set_trace_mode(args.trace_mode)
     dscene = gpu.upload(scene)
     film = gpu.film(scene.width, scene.height)
     for i in range(args.warmup):
@@ -118,7 +129,8 @@ def main():
         print(f"bench: warmup {i} {gpu.last_stats()['total_ms']:.0f} ms", file=sys.stderr, flush=True)
     segments = 0
     paths = 0
-    kernel_ms = 0.0
+    trace_ms = 0.0
+    finish_ms = 0.0
     launches = 0
     events = None
     t0 = time.perf_counter()
@@ -127,7 +139,8 @@ def main():
         st = gpu.last_stats()
         segments += st["segments"]
         paths += st["paths"]
-        kernel_ms += st["trace_kernel_ms"]
+        trace_ms += st["trace_kernel_ms"]
+        finish_ms += st["finish_kernel_ms"]
         launches += st["launches"]
         if events is None:
             events = dict(st["events"])
@@ -143,10 +156,13 @@ def main():
     flops = op_counts.algorithmic_flops(events, scene.planes.shape[0], paths, m.is_anisotropic,
                                         bool(m.o_bands or m.e_bands), False,
                                         n_bands=len(m.o_bands) + len(m.e_bands))
-    avg_launch_s = kernel_ms / 1e3 / launches
+    # One launch = one chunk's trace kernel plus, in wavefront mode, its finish kernel
+    # (the escape radiance pass); the op-count model covers both, so both are charged.
+    avg_launch_s = (trace_ms + finish_ms) / 1e3 / launches
     achieved_tf = flops / launches / avg_launch_s / 1e12
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "trace_kernel_single_job_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"{'trace_kernel' if args.trace_mode == 'plain' else 'trace_wave_kernel'}"
+                                           "_single_job_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
     out = {
@@ -166,6 +182,7 @@ def main():
             "workload": "single job: pleochroic uniaxial sapphire, 69-plane oval brilliant, 2048x1024 "
                         f"procedural HDR env, {args.width}x{args.height} x {args.spp} spp per step, depth 32",
             "width": args.width, "height": args.height, "spp_per_step": args.spp,
+            "trace_mode": args.trace_mode,
             "max_bounces": scene.max_bounces, "n_planes": int(scene.planes.shape[0]),
             "paths": paths, "segments": segments,
             "segments_per_path": segments / max(paths, 1),
@@ -178,8 +195,10 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved_tf / PEAK_FP32_TFLOPS,
             "traffic": traffic,
-            "kernel": "trace_kernel",
+            "kernel": "trace_kernel" if args.trace_mode == "plain" else "trace_wave_kernel + finish_kernel",
             "avg_launch_ms": avg_launch_s * 1e3,
+            "avg_trace_kernel_ms": trace_ms / launches,
+            "avg_finish_kernel_ms": finish_ms / launches,
             "launches": launches,
             "algorithmic_flops_per_launch": flops / launches,
         },

@@ -269,7 +269,7 @@ struct gr_ctx {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_k2 = nullptr;
     std::mutex mu;
     uint64_t chunk_tuples = kDefaultChunkTuples;
-    int trace_mode = GR_TRACE_WAVEFRONT;
+    int trace_mode = GR_TRACE_PLAIN;
     float *xyz = nullptr;
     uint64_t xyz_cap = 0;  // tuples
     uint32_t *info = nullptr;

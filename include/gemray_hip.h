@@ -121,8 +121,8 @@ typedef struct {
 } gr_stats;
 
 /* Trace kernels.  Both evaluate identical per-tuple results; they differ in scheduling.
- * GR_TRACE_PLAIN: one lane per (pixel, sample) tuple, as transport_main dispatches
- * (frame.rs:437-452).  GR_TRACE_WAVEFRONT (default): persistent wavefronts that refill
+ * GR_TRACE_PLAIN (default): one lane per (pixel, sample) tuple, as transport_main
+ * dispatches (frame.rs:437-452).  GR_TRACE_WAVEFRONT: persistent wavefronts that refill
  * finished lanes with the next tuples of their range, plus a separate finish pass. */
 enum { GR_TRACE_PLAIN = 0, GR_TRACE_WAVEFRONT = 1 };
 

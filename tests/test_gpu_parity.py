@@ -241,7 +241,7 @@ def test_plain_and_wavefront_kernels_agree_bit_for_bit(gpu, srb, idx):
             assert gpu.try_accumulate(scene, 3, 6, film)
             out[mode] = (xyz, info, film, gpu.last_stats()["events"])
     finally:
-        gpu.set_trace_mode("wavefront")
+        gpu.set_trace_mode("plain")
     for a, b in zip(out["plain"][:3], out["wavefront"][:3]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert out["plain"][3] == out["wavefront"][3]
