@@ -28,10 +28,10 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
+def build(verbose: bool = False, out: str = LIB, defines=(), extra_flags=()) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = out + ".tmp"
-    cmd = [HIPCC, *FLAGS, *[f"-D{d}" for d in defines], "-o", tmp, *srcs]
+    cmd = [HIPCC, *FLAGS, *extra_flags, *[f"-D{d}" for d in defines], "-o", tmp, *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
