@@ -44,6 +44,120 @@ def single_job_scene(width=1920, height=1080):
     return scene, (0.35, 0.45, 5.0, 28.0)
 
 
+ISOTROPIC = ["Diamond", "Cubic Zirconia", "Spinel"]
+UNIAXIAL = ["Sapphire", "Ruby", "Emerald", "Zircon", "Quartz", "Tourmaline", "Synthetic Moissanite"]
+PRESETS = ["Daylight", "Incandescent", "RingLights", "DarkSpotlight"]
+
+
+def batched_jobs(seed=3, n_jobs=64):
+    """Batched workload: n_jobs seeded convex stones with 17-181 planes (cut angles jittered
+    +-3 deg), an isotropic/uniaxial material mix (uniaxial ones on a seeded optic axis),
+    square frames of log-uniform 128^2..1024^2 pixels, log-uniform 64..1024 spp, studio
+    lighting with a random preset, depth 16."""
+    from dataclasses import replace
+
+    from gemray_studio_amd import backend as B
+    from gemray_studio_amd import materials as M
+    from gemray_studio_amd import scenes
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_jobs):
+        n_planes = int(rng.integers(17, 182))
+        planes = scenes.random_convex_stone(seed * 1000 + i, n_planes, 3.0)
+        if rng.random() < 0.5:
+            mat = M.by_name(ISOTROPIC[int(rng.integers(len(ISOTROPIC)))])
+        else:
+            mat = M.by_name(UNIAXIAL[int(rng.integers(len(UNIAXIAL)))])
+            mat = replace(mat, c_axis=scenes.seeded_unit_vector(seed * 1000 + i))
+        side = int(round(float(np.exp(rng.uniform(np.log(128), np.log(1024))))))
+        spp = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1024))))))
+        cam = B.camera_new(float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.2, 0.6)), 5.0,
+                           float(rng.uniform(16.0, 28.0)))
+        env = B.Studio(PRESETS[int(rng.integers(len(PRESETS)))], 1.0, float(rng.uniform(0, 2 * np.pi)),
+                       float(rng.uniform(0.2, 1.0)))
+        jobs.append((B.SceneRef(cam, side, side, planes, mat, 16, env), 0, spp))
+    return jobs
+
+
+def served_jobs(env_map, seed=4, n_jobs=40):
+    """Served workload: the batched workload's per-job distribution with the single-job
+    workload's material and environment family (a seeded pleochroic sapphire, the 2048x1024
+    procedural HDR map, depth 32), plus 10 % of jobs at 1920x1080 x 512 spp on the 69-plane
+    oval brilliant."""
+    from gemray_studio_amd import backend as B
+    from gemray_studio_amd import scenes
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_jobs):
+        mat = scenes.pleochroic_sapphire(seed * 1000 + i)
+        cam = B.camera_new(float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.2, 0.6)), 5.0,
+                           float(rng.uniform(16.0, 28.0)))
+        if rng.random() < 0.1:
+            jobs.append((B.SceneRef(cam, 1920, 1080, scenes.oval_brilliant(), mat, 32, B.HdrMap(env_map)), 0, 512))
+            continue
+        planes = scenes.random_convex_stone(seed * 1000 + i, int(rng.integers(17, 182)), 3.0)
+        side = int(round(float(np.exp(rng.uniform(np.log(128), np.log(1024))))))
+        spp = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1024))))))
+        jobs.append((B.SceneRef(cam, side, side, planes, mat, 32, B.HdrMap(env_map)), 0, spp))
+    return jobs
+
+
+def run_plain(gpu, jobs):
+    """The plain per-job path: one gr_accumulate per job, in order, into a host film
+    (scene and environment uploaded per call, film read back per call)."""
+    service, segs = [], 0
+    t0 = time.perf_counter()
+    for sc, first, spp in jobs:
+        film = np.zeros((sc.width * sc.height, 3), np.float32)
+        t = time.perf_counter()
+        gpu.accumulate(sc, first, spp, film)
+        service.append(time.perf_counter() - t)
+        segs += gpu.last_stats()["segments"]
+    return time.perf_counter() - t0, service, segs
+
+
+def run_batched(gpu, jobs, arrivals=None):
+    """Submits every job (at its arrival time, if given) to the job loop and waits for all;
+    returns wall seconds from the first submission, per-job latencies and segments."""
+    from gemray_studio_amd import _lib
+
+    gpu.metrics_reset()
+    ids = []
+    t0 = time.perf_counter()
+    for k, (sc, first, spp) in enumerate(jobs):
+        if arrivals is not None:
+            delay = t0 + arrivals[k] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+        ids.append(gpu.submit(sc, first, spp))
+    for jid in ids:
+        if gpu.wait(jid, 600000) != _lib.GR_OK:
+            raise RuntimeError(f"job {jid} did not complete")
+    wall = time.perf_counter() - t0
+    infos = [gpu.job_info(jid) for jid in ids]
+    for jid in ids:
+        gpu.release(jid)
+    m = gpu.metrics()
+    return wall, [i["latency_ms"] / 1e3 for i in infos], sum(i["events"]["segment"] for i in infos), m
+
+
+def fifo_latencies(arrivals, service):
+    """Submit-to-completion latencies of a single FIFO server with these service times."""
+    free_at, out = 0.0, []
+    for a, s in zip(arrivals, service):
+        start = max(a, free_at)
+        free_at = start + s
+        out.append(free_at - a)
+    return out
+
+
+def pct(v, q):
+    v = sorted(v)
+    return v[min(len(v), max(1, int(np.ceil(q * len(v))))) - 1]
+
+
 def cpu_baseline(scene, cam_args, spp_total, budget_s):
     """Oracle on a centred pixel window of the same frame; window grown until the run
     takes about budget_s seconds."""
@@ -94,6 +208,56 @@ def cpu_baseline(scene, cam_args, spp_total, budget_s):
     }
 
 
+def job_workloads(gpu, env_map, args):
+    """Batched (all jobs submitted at once) and served (Poisson arrivals) throughput and
+    latency of the job loop, each beside the plain per-job path on the same jobs."""
+    jobs = batched_jobs(n_jobs=args.batched_jobs)
+    paths = sum(sc.width * sc.height * spp for sc, _, spp in jobs)
+    print(f"bench: batched workload, {len(jobs)} jobs, {paths} paths", file=sys.stderr, flush=True)
+    plain_s, plain_service, plain_segs = run_plain(gpu, jobs)
+    print(f"bench: plain per-job path {plain_s:.2f} s", file=sys.stderr, flush=True)
+    b_wall, b_lat, b_segs, b_m = run_batched(gpu, jobs)
+    print(f"bench: job loop {b_wall:.2f} s", file=sys.stderr, flush=True)
+    if b_segs != plain_segs:
+        raise RuntimeError("batched and plain paths traced different work")
+    batched = {
+        "workload": f"{len(jobs)} seeded convex stones (17-181 planes, +-3 deg), isotropic/uniaxial mix, "
+                    "128^2-1024^2 px, 64-1024 spp (log-uniform), studio presets, depth 16, all submitted at t=0",
+        "jobs": len(jobs), "paths": paths, "segments": b_segs,
+        "plain_s": plain_s, "plain_segments_per_s": plain_segs / plain_s,
+        "batched_s": b_wall, "batched_segments_per_s": b_segs / b_wall,
+        "speedup": plain_s / b_wall,
+        "batched_busy_s": b_m["busy_ms"] / 1e3, "ticks": b_m["ticks"],
+        "plain_latency_p50_ms": 1e3 * pct(np.cumsum(plain_service), 0.5),
+        "plain_latency_p99_ms": 1e3 * pct(np.cumsum(plain_service), 0.99),
+        "batched_latency_p50_ms": b_m["latency_p50_ms"], "batched_latency_p99_ms": b_m["latency_p99_ms"],
+    }
+    sjobs = served_jobs(env_map, n_jobs=args.served_jobs)
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.arrival_rate, size=len(sjobs)))
+    arrivals -= arrivals[0]
+    s_plain_s, s_service, s_plain_segs = run_plain(gpu, sjobs)
+    plain_lat = fifo_latencies(arrivals, s_service)
+    print(f"bench: served workload, plain per-job path {s_plain_s:.2f} s", file=sys.stderr, flush=True)
+    s_wall, s_lat, s_segs, s_m = run_batched(gpu, sjobs, arrivals=arrivals)
+    print(f"bench: served workload, job loop {s_wall:.2f} s", file=sys.stderr, flush=True)
+    served = {
+        "workload": f"{len(sjobs)} jobs, Poisson arrivals at {args.arrival_rate:g}/s; batched-workload "
+                    "per-job distribution with the single-job material/env family (seeded pleochroic "
+                    "sapphire, 2048x1024 HDR map, depth 32), 10% at 1920x1080 x 512 spp",
+        "jobs": len(sjobs), "arrival_rate_per_s": args.arrival_rate,
+        "offered_load": float(np.sum(s_service) / (arrivals[-1] + 1.0 / args.arrival_rate)),
+        "segments": s_segs,
+        "served_segments_per_s": s_segs / s_wall, "served_wall_s": s_wall,
+        "latency_p50_ms": 1e3 * pct(s_lat, 0.5), "latency_p99_ms": 1e3 * pct(s_lat, 0.99),
+        "plain_fifo_latency_p50_ms": 1e3 * pct(plain_lat, 0.5),
+        "plain_fifo_latency_p99_ms": 1e3 * pct(plain_lat, 0.99),
+        "plain_segments_per_s": s_plain_segs / s_plain_s,
+        "plain_fifo_wall_s": float(arrivals[-1] + plain_lat[-1]),
+    }
+    return batched, served
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +269,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--trace-mode", choices=["plain", "wavefront"], default="plain")
+    ap.add_argument("--no-jobs", action="store_true", help="skip the batched and served workloads")
+    ap.add_argument("--batched-jobs", type=int, default=64)
+    ap.add_argument("--served-jobs", type=int, default=40)
+    ap.add_argument("--arrival-rate", type=float, default=20.0, help="served jobs per second (Poisson)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -203,6 +371,8 @@ def main():
             "algorithmic_flops_per_launch": flops / launches,
         },
     }
+    if not args.no_jobs:
+        out["batched"], out["served"] = job_workloads(gpu, scene.environment.rgb, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, cam_args, args.spp, args.cpu_budget)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -22,11 +22,13 @@ GR_ERR_INVALID = 5
 GR_ERR_DEVICE = 6
 GR_ERR_NOT_FOUND = 7
 GR_PENDING = 8
+GR_CANCELLED = 9
 STATUS_NAMES = {
     0: "GR_OK", 1: "GR_DECLINE_NO_DEVICE", 2: "GR_DECLINE_ENV", 3: "GR_DECLINE_MATERIAL",
     4: "GR_DECLINE_BUDGET", 5: "GR_ERR_INVALID", 6: "GR_ERR_DEVICE", 7: "GR_ERR_NOT_FOUND",
-    8: "GR_PENDING",
+    8: "GR_PENDING", 9: "GR_CANCELLED",
 }
+JOB_STATES = ["queued", "running", "done", "cancelled", "failed"]
 
 GR_ENV_STUDIO = 0
 GR_ENV_MAP = 1
@@ -76,12 +78,37 @@ class Stats(ctypes.Structure):
                 ("finish_kernel_ms", ctypes.c_double)]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [("scene", ctypes.POINTER(Scene)), ("sample_offset", ctypes.c_uint32),
+                ("spp", ctypes.c_uint32)]
+
+
+class JobStats(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("sample_offset", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("footprint_bytes", ctypes.c_uint64), ("paths", ctypes.c_uint64),
+                ("events", ctypes.c_uint64 * GR_EV_COUNT), ("queue_ms", ctypes.c_double),
+                ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
+class ServeMetrics(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "submitted", "admitted", "completed", "cancelled", "failed", "declined", "queued", "running",
+        "paths", "segments", "ticks", "launches")] + [
+        (n, ctypes.c_double) for n in (
+            "busy_ms", "wall_ms", "latency_p50_ms", "latency_p99_ms", "latency_mean_ms",
+            "latency_max_ms")] + [
+        (n, ctypes.c_uint64) for n in ("hbm_in_use", "hbm_budget", "tick_tuples")]
+
+
 # Every symbol include/gemray_hip.h declares.
 EXPORTS = [
     "gr_open", "gr_close", "gr_adapter_label", "gr_set_chunk_tuples", "gr_accumulate",
     "gr_trace_samples", "gr_last_stats", "gr_scene_upload", "gr_scene_free", "gr_film_create",
     "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_camera_new",
-    "gr_set_trace_mode",
+    "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
+    "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
+    "gr_serve_set_tick_tuples",
 ]
 
 _lib = None
@@ -133,6 +160,21 @@ def load():
     L.gr_camera_new.restype = ctypes.c_int
     L.gr_set_trace_mode.argtypes = [vp, ctypes.c_int]
     L.gr_set_trace_mode.restype = ctypes.c_int
+    u64 = ctypes.c_uint64
+    L.gr_job_submit.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(u64)]
+    L.gr_job_poll.argtypes = [vp, u64]
+    L.gr_job_wait.argtypes = [vp, u64, ctypes.c_int64]
+    L.gr_job_cancel.argtypes = [vp, u64]
+    L.gr_job_film.argtypes = [vp, u64, fp, ctypes.c_size_t]
+    L.gr_job_info.argtypes = [vp, u64, ctypes.POINTER(JobStats)]
+    L.gr_job_release.argtypes = [vp, u64]
+    L.gr_metrics.argtypes = [vp, ctypes.POINTER(ServeMetrics)]
+    L.gr_metrics_reset.argtypes = [vp]
+    L.gr_serve_set_tick_tuples.argtypes = [vp, u64]
+    for name in ("gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel", "gr_job_film",
+                 "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
+                 "gr_serve_set_tick_tuples"):
+        getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
 

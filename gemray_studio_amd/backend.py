@@ -23,7 +23,7 @@ from typing import Optional, Sequence, Union
 import numpy as np
 
 from . import _lib
-from ._lib import Camera, Env, GrError, Options, Scene, Stats, check
+from ._lib import Camera, Env, GrError, JobDesc, JobStats, Options, Scene, ServeMetrics, Stats, check
 from .materials import GemMaterial, MaterialStruct, to_struct
 
 # LightingPreset::params (optics/raytracer/environment.rs:81-100): (temp_k, spot_mult).
@@ -174,9 +174,9 @@ class GpuBackend:
         self._mu = threading.Lock()
 
     @classmethod
-    def acquire(cls, device: int = 0, chunk_tuples: int = 0) -> "GpuBackend":
+    def acquire(cls, device: int = 0, chunk_tuples: int = 0, hbm_budget_bytes: int = 0) -> "GpuBackend":
         L = _lib.load()
-        opts = Options(device, chunk_tuples, 0)
+        opts = Options(device, chunk_tuples, hbm_budget_bytes)
         h = ctypes.c_void_p()
         st = L.gr_open(ctypes.byref(h), ctypes.byref(opts))
         if st != _lib.GR_OK:
@@ -265,6 +265,69 @@ class GpuBackend:
         with self._mu:
             st = self._L.gr_render(self._ctx, scene.handle, sample_offset, spp, film.handle)
         check(st, "gr_render")
+
+    # ---- batched / served jobs (gr_job_*) -------------------------------------------
+
+    def submit(self, scene: SceneRef, sample_offset: int, spp: int) -> int:
+        """Queues a render job (a RenderRequest: scene, first_sample, samples); returns its id.
+        Raises GrError on a decline (GR_DECLINE_*) or invalid input."""
+        self._require()
+        packed = _Packed(scene)
+        desc = JobDesc(ctypes.pointer(packed.scene), sample_offset, spp)
+        jid = ctypes.c_uint64()
+        check(self._L.gr_job_submit(self._ctx, ctypes.byref(desc), ctypes.byref(jid)), "gr_job_submit")
+        return int(jid.value)
+
+    def poll(self, job: int) -> int:
+        """GR_OK (done), GR_PENDING, GR_CANCELLED, GR_ERR_NOT_FOUND or the job's error."""
+        self._require()
+        return int(self._L.gr_job_poll(self._ctx, job))
+
+    def wait(self, job: int, timeout_ms: int = -1) -> int:
+        self._require()
+        return int(self._L.gr_job_wait(self._ctx, job, timeout_ms))
+
+    def cancel(self, job: int) -> None:
+        self._require()
+        check(self._L.gr_job_cancel(self._ctx, job), "gr_job_cancel")
+
+    def job_film(self, job: int, accum: np.ndarray) -> None:
+        """ADDS the finished job's film into accum (float32, width*height*3)."""
+        self._require()
+        if accum.dtype != np.float32 or not accum.flags["C_CONTIGUOUS"]:
+            raise TypeError("accum must be a C-contiguous float32 array")
+        check(self._L.gr_job_film(self._ctx, job, accum.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                  accum.size // 3), "gr_job_film")
+
+    def job_info(self, job: int) -> dict:
+        self._require()
+        s = JobStats()
+        check(self._L.gr_job_info(self._ctx, job, ctypes.byref(s)), "gr_job_info")
+        return {
+            "state": _lib.JOB_STATES[s.state], "width": s.width, "height": s.height,
+            "sample_offset": s.sample_offset, "spp": s.spp, "footprint_bytes": int(s.footprint_bytes),
+            "paths": int(s.paths), "events": {n: int(s.events[i]) for i, n in enumerate(_lib.EV_NAMES)},
+            "queue_ms": s.queue_ms, "run_ms": s.run_ms, "latency_ms": s.latency_ms,
+        }
+
+    def release(self, job: int) -> None:
+        self._require()
+        check(self._L.gr_job_release(self._ctx, job), "gr_job_release")
+
+    def metrics(self) -> dict:
+        self._require()
+        m = ServeMetrics()
+        check(self._L.gr_metrics(self._ctx, ctypes.byref(m)), "gr_metrics")
+        return {name: (float(getattr(m, name)) if isinstance(getattr(m, name), float) else int(getattr(m, name)))
+                for name, _ in ServeMetrics._fields_}
+
+    def metrics_reset(self) -> None:
+        self._require()
+        check(self._L.gr_metrics_reset(self._ctx), "gr_metrics_reset")
+
+    def set_tick_tuples(self, tuples: int) -> None:
+        self._require()
+        check(self._L.gr_serve_set_tick_tuples(self._ctx, tuples), "gr_serve_set_tick_tuples")
 
     def set_trace_mode(self, mode: str) -> None:
         """"plain" (one lane per tuple) or "wavefront" (persistent lanes + finish pass)."""

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/gemray_hip.h"
+#include "runtime_internal.h"
 #include "transport.h"
 
 namespace {
@@ -130,8 +131,10 @@ double dispersion_f64(const gr_material *m, double lambda_nm) {
     return m->disp[0] + m->disp[1] / l2 + m->disp[2] / (l2 * l2);
 }
 
-// validate_scene (apps/gemray-worker/src/validate/mod.rs:74-192) plus the GPU decline
-// rules of frame.rs:664-700 for the branches this build does not implement.
+}  // namespace
+
+namespace gr {
+
 gr_status validate_scene(const gr_scene *s) {
     if (!s || !s->camera || !s->material || !s->env || !s->planes) return GR_ERR_INVALID;
     if (s->width == 0 || s->height == 0) return GR_ERR_INVALID;
@@ -179,7 +182,7 @@ gr_status validate_scene(const gr_scene *s) {
     return GR_OK;
 }
 
-void fill_dev_scene(const gr_scene *s, gr::DevScene *d) {
+void fill_dev_scene(const gr_scene *s, DevScene *d) {
     std::memset(d, 0, sizeof(*d));
     const gr_camera *c = s->camera;
     for (int i = 0; i < 3; ++i) {
@@ -250,7 +253,26 @@ void fill_dev_scene(const gr_scene *s, gr::DevScene *d) {
     }
 }
 
-}  // namespace
+// radical_inverse_base (optics/raytracer/sampling.rs:170-181), the same operations as the
+// device's, evaluated once per call for its sample range.
+float h_radical_inverse(uint32_t n, uint32_t base) {
+    uint32_t remaining = n;
+    float val = 0.0f;
+    float inv_base = 1.0f / (float)base;
+    while (remaining > 0) {
+        uint32_t digit = remaining % base;
+        val = std::fmaf((float)digit, inv_base, val);
+        inv_base /= (float)base;
+        remaining /= base;
+    }
+    return val;
+}
+
+}  // namespace gr
+
+using gr::fill_dev_scene;
+using gr::h_radical_inverse;
+using gr::validate_scene;
 
 struct gr_scene_dev {
     gr::DevScene sc;
@@ -261,27 +283,6 @@ struct gr_scene_dev {
 struct gr_film {
     float *d = nullptr;
     uint32_t width = 0, height = 0;
-};
-
-struct gr_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_k2 = nullptr;
-    std::mutex mu;
-    uint64_t chunk_tuples = kDefaultChunkTuples;
-    int trace_mode = GR_TRACE_PLAIN;
-    float *xyz = nullptr;
-    uint64_t xyz_cap = 0;  // tuples
-    uint32_t *info = nullptr;
-    uint64_t info_cap = 0;
-    float *rec = nullptr;  // escape records, kRecFields x rec_cap
-    uint64_t rec_cap = 0;
-    float2 *ri35 = nullptr;  // radical inverses (3, 5) of the call's sample range
-    uint32_t ri35_cap = 0, ri35_offset = 0, ri35_n = 0;
-    std::vector<float2> ri35_host;
-    unsigned long long *stats = nullptr;
-    gr_stats last{};
-    char label[256] = {0};
 };
 
 namespace {
@@ -322,21 +323,6 @@ gr_status ensure_rec(gr_ctx *ctx, uint64_t tuples) {
     GR_CHECK(hipMalloc(&ctx->rec, tuples * 20 * sizeof(float)));
     ctx->rec_cap = tuples;
     return GR_OK;
-}
-
-// radical_inverse_base (optics/raytracer/sampling.rs:170-181), the same operations as the
-// device's, evaluated once per call for its sample range.
-float h_radical_inverse(uint32_t n, uint32_t base) {
-    uint32_t remaining = n;
-    float val = 0.0f;
-    float inv_base = 1.0f / (float)base;
-    while (remaining > 0) {
-        uint32_t digit = remaining % base;
-        val = std::fmaf((float)digit, inv_base, val);
-        inv_base /= (float)base;
-        remaining /= base;
-    }
-    return val;
 }
 
 gr_status ensure_ri35(gr_ctx *ctx, uint32_t sample_offset, uint32_t spp) {
@@ -479,6 +465,8 @@ gr_status gr_open(gr_ctx **out, const gr_options *opts) {
     gr_ctx *ctx = new (std::nothrow) gr_ctx();
     if (!ctx) return GR_ERR_DEVICE;
     ctx->device = dev;
+    ctx->chunk_tuples = kDefaultChunkTuples;
+    if (opts) ctx->hbm_budget = opts->hbm_budget_bytes;
     if (opts && opts->chunk_tuples) ctx->chunk_tuples = opts->chunk_tuples < kMaxChunkTuples ? opts->chunk_tuples : kMaxChunkTuples;
     std::snprintf(ctx->label, sizeof(ctx->label), "%s (%s, HIP)", prop.name, prop.gcnArchName);
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -495,6 +483,7 @@ gr_status gr_open(gr_ctx **out, const gr_options *opts) {
 
 void gr_close(gr_ctx *ctx) {
     if (!ctx) return;
+    gr::serve_shutdown(ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->xyz) (void)hipFree(ctx->xyz);

@@ -1,0 +1,52 @@
+// runtime_internal.h -- the context object and the host helpers shared by the C ABI's
+// translation units (runtime.hip: plain calls; serve.hip: the batched job loop).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/gemray_hip.h"
+#include "transport.h"
+
+namespace gr {
+
+struct ServeState;
+
+// validate_scene (apps/gemray-worker/src/validate/mod.rs:74-192) plus the decline rules of
+// frame.rs:664-700 for the branches this build does not implement.
+gr_status validate_scene(const gr_scene *s);
+// Host-derived scene constants (studio rig, white balance, absorption frame).
+void fill_dev_scene(const gr_scene *s, DevScene *d);
+// radical_inverse_base (optics/raytracer/sampling.rs:170-181), host evaluation.
+float h_radical_inverse(uint32_t n, uint32_t base);
+// Stops the job loop of ctx (if started), cancels what is left and frees its memory.
+void serve_shutdown(gr_ctx *ctx);
+
+}  // namespace gr
+
+struct gr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_k2 = nullptr;
+    std::mutex mu;
+    uint64_t chunk_tuples = 0;
+    int trace_mode = GR_TRACE_PLAIN;
+    float *xyz = nullptr;
+    uint64_t xyz_cap = 0;  // tuples
+    uint32_t *info = nullptr;
+    uint64_t info_cap = 0;
+    float *rec = nullptr;  // escape records, 20 floats x rec_cap
+    uint64_t rec_cap = 0;
+    float2 *ri35 = nullptr;  // radical inverses (3, 5) of the call's sample range
+    uint32_t ri35_cap = 0, ri35_offset = 0, ri35_n = 0;
+    std::vector<float2> ri35_host;
+    unsigned long long *stats = nullptr;
+    gr_stats last{};
+    char label[256] = {0};
+    uint64_t hbm_budget = 0;  // admission budget of the job loop; 0 = 3/4 of free HBM
+    std::mutex serve_mu;      // guards creation of serve
+    gr::ServeState *serve = nullptr;
+};

@@ -1,0 +1,673 @@
+// serve.hip -- the device-resident job loop behind gr_job_* (batched and served rendering).
+//
+// The reference renders one request at a time: gemray-worker's render path hands each
+// RenderRequest{request_id, scene, first_sample, samples} (messages/render.rs:42-70) to
+// GpuFrameRenderer::accumulate, which re-uploads the scene per chunk and reads the film
+// back per chunk (frame.rs:375-426, 799-814).  Here a job is admitted once: its scene,
+// planes, environment map, sample table and film live in HBM until it completes, and one
+// scheduler thread per context runs "ticks".  A tick gives every admitted job a piece of
+// its frame (a contiguous pixel range, all of the job's samples), traces the pieces of
+// each material class in one batched launch, adds each pixel's ordered sample sum into
+// its job's film, and retires the jobs whose last piece it traced.  Jobs that arrive while
+// a tick runs are admitted at the next one as HBM frees up (continuous batching).
+//
+// Determinism: a piece is whole pixels with all of their samples, so every pixel's film
+// value is (0 + sum over samples in order), the same value gr_accumulate produces for the
+// same (first_sample, samples); how jobs are mixed, sliced into pieces or scheduled never
+// changes a bit of any job's film.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gemray_hip.h"
+#include "runtime_internal.h"
+#include "transport.h"
+
+namespace gr {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint64_t kDefaultTickTuples = 1ull << 24;
+constexpr uint64_t kMinTickTuples = GR_MAX_SAMPLES_PER_CALL;  // one pixel always fits
+constexpr uint64_t kMaxTickTuples = 1ull << 28;
+constexpr uint32_t kBlockTuples = 256;  // workgroup size of the batched kernels
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Job {
+    uint64_t id = 0;
+    int state = GR_JOB_QUEUED;
+    gr_status error = GR_OK;
+    // Host copies of the request (inputs are borrowed only during gr_job_submit).
+    std::vector<float> planes;
+    std::vector<float> map;
+    DevScene host_sc{};
+    int cls = 0;
+    uint32_t width = 0, height = 0, sample_offset = 0, spp = 0;
+    uint64_t n_pixels = 0;
+    uint64_t footprint = 0;
+    // Device copies (from admission to completion).
+    DevScene *d_scene = nullptr;
+    float4 *d_planes = nullptr;
+    float *d_map = nullptr;
+    float2 *d_ri35 = nullptr;
+    float *d_film = nullptr;
+    unsigned long long *d_stats = nullptr;
+    // Progress (owned by the scheduler thread).
+    uint64_t next_pixel = 0;
+    // Results.
+    std::vector<float> film;
+    unsigned long long events[GR_EV_COUNT] = {0};
+    uint64_t paths_done = 0;
+    Clock::time_point t_submit, t_admit, t_done;
+    bool cancel_req = false;
+};
+
+struct Piece {
+    Job *job;
+    uint64_t pixel_begin, n_pixels;
+};
+
+}  // namespace
+
+struct ServeState {
+    gr_ctx *ctx = nullptr;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    uint64_t next_id = 1;
+    std::deque<std::shared_ptr<Job>> queue;    // submitted, not admitted
+    std::vector<std::shared_ptr<Job>> active;  // admitted, not finished
+    std::unordered_map<uint64_t, std::shared_ptr<Job>> jobs;
+    uint64_t budget = 0, in_use = 0;
+    uint64_t tick_tuples = kDefaultTickTuples;
+    // Device resources of the loop.
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float *staging = nullptr;
+    uint64_t staging_cap = 0;
+    BatchSlice *d_slices = nullptr;
+    uint32_t slices_cap = 0;
+    size_t rr = 0;  // round-robin start of the next tick
+    // Metrics since the last reset.
+    gr_serve_metrics m{};
+    std::vector<double> latencies;
+    bool have_first = false;
+    Clock::time_point t_first, t_last;
+};
+
+namespace {
+
+#define SV_CHECK(call)                                                                           \
+    do {                                                                                         \
+        hipError_t err_ = (call);                                                                \
+        if (err_ != hipSuccess) {                                                                \
+            std::fprintf(stderr, "gemray_hip serve: %s failed: %s\n", #call, hipGetErrorString(err_)); \
+            return GR_ERR_DEVICE;                                                                \
+        }                                                                                        \
+    } while (0)
+
+uint64_t job_footprint(const Job &j) {
+    uint64_t b = j.n_pixels * 3 * sizeof(float);             // film
+    b += j.planes.size() * sizeof(float);                    // planes
+    b += j.map.size() * sizeof(float);                       // environment map
+    b += (uint64_t)j.spp * sizeof(float2);                   // sample table
+    b += sizeof(DevScene) + GR_EV_COUNT * sizeof(unsigned long long);
+    return b;
+}
+
+void free_job_device(Job &j) {
+    for (void *p : {(void *)j.d_scene, (void *)j.d_planes, (void *)j.d_map, (void *)j.d_ri35, (void *)j.d_film,
+                    (void *)j.d_stats})
+        if (p) (void)hipFree(p);
+    j.d_scene = nullptr;
+    j.d_planes = nullptr;
+    j.d_map = nullptr;
+    j.d_ri35 = nullptr;
+    j.d_film = nullptr;
+    j.d_stats = nullptr;
+}
+
+gr_status upload_job(ServeState &S, Job &j) {
+    SV_CHECK(hipMalloc(&j.d_scene, sizeof(DevScene)));
+    SV_CHECK(hipMalloc(&j.d_planes, j.planes.size() * sizeof(float)));
+    if (!j.map.empty()) SV_CHECK(hipMalloc(&j.d_map, j.map.size() * sizeof(float)));
+    SV_CHECK(hipMalloc(&j.d_ri35, (size_t)j.spp * sizeof(float2)));
+    SV_CHECK(hipMalloc(&j.d_film, j.n_pixels * 3 * sizeof(float)));
+    SV_CHECK(hipMalloc(&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long)));
+    DevScene sc = j.host_sc;
+    sc.map_rgb = j.d_map;
+    std::vector<float2> ri(j.spp);
+    for (uint32_t i = 0; i < j.spp; ++i)
+        ri[i] = make_float2(h_radical_inverse(j.sample_offset + i, 3), h_radical_inverse(j.sample_offset + i, 5));
+    SV_CHECK(hipMemcpyAsync(j.d_scene, &sc, sizeof(DevScene), hipMemcpyHostToDevice, S.stream));
+    SV_CHECK(hipMemcpyAsync(j.d_planes, j.planes.data(), j.planes.size() * sizeof(float), hipMemcpyHostToDevice,
+                            S.stream));
+    if (!j.map.empty())
+        SV_CHECK(hipMemcpyAsync(j.d_map, j.map.data(), j.map.size() * sizeof(float), hipMemcpyHostToDevice,
+                                S.stream));
+    SV_CHECK(hipMemcpyAsync(j.d_ri35, ri.data(), ri.size() * sizeof(float2), hipMemcpyHostToDevice, S.stream));
+    SV_CHECK(hipMemsetAsync(j.d_film, 0, j.n_pixels * 3 * sizeof(float), S.stream));
+    SV_CHECK(hipMemsetAsync(j.d_stats, 0, GR_EV_COUNT * sizeof(unsigned long long), S.stream));
+    SV_CHECK(hipStreamSynchronize(S.stream));
+    // The host copy of the environment map is no longer needed.
+    std::vector<float>().swap(j.map);
+    return GR_OK;
+}
+
+gr_status finish_job(ServeState &S, Job &j) {
+    j.film.resize(j.n_pixels * 3);
+    SV_CHECK(hipMemcpyAsync(j.film.data(), j.d_film, j.n_pixels * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                            S.stream));
+    SV_CHECK(hipMemcpyAsync(j.events, j.d_stats, sizeof(j.events), hipMemcpyDeviceToHost, S.stream));
+    SV_CHECK(hipStreamSynchronize(S.stream));
+    return GR_OK;
+}
+
+gr_status ensure_staging(ServeState &S, uint64_t tuples, uint32_t slices) {
+    if (tuples > S.staging_cap) {
+        if (S.staging) SV_CHECK(hipFree(S.staging));
+        S.staging = nullptr;
+        S.staging_cap = 0;
+        SV_CHECK(hipMalloc(&S.staging, tuples * 3 * sizeof(float)));
+        S.staging_cap = tuples;
+    }
+    if (slices > S.slices_cap) {
+        if (S.d_slices) SV_CHECK(hipFree(S.d_slices));
+        S.d_slices = nullptr;
+        S.slices_cap = 0;
+        uint32_t cap = std::max<uint32_t>(slices, 64);
+        SV_CHECK(hipMalloc(&S.d_slices, cap * sizeof(BatchSlice)));
+        S.slices_cap = cap;
+    }
+    return GR_OK;
+}
+
+// Pieces of one tick: every active job gets an equal share of the tick's tuples (at least
+// one pixel), then the unused budget extends the pieces in round-robin order.  A job has at
+// most one piece per tick, so no two reduce threads of a tick touch the same film pixel.
+std::vector<Piece> plan_tick(ServeState &S, const std::vector<Job *> &jobs, uint64_t budget) {
+    std::vector<Piece> pieces;
+    const size_t n = jobs.size();
+    if (n == 0) return pieces;
+    const uint64_t share = std::max<uint64_t>(budget / n, 1);
+    uint64_t used = 0;
+    std::vector<uint64_t> take(n, 0);
+    for (size_t k = 0; k < n; ++k) {
+        const size_t i = (S.rr + k) % n;
+        Job &j = *jobs[i];
+        uint64_t left = j.n_pixels - j.next_pixel;
+        uint64_t px = std::min<uint64_t>(left, std::max<uint64_t>(share / j.spp, 1));
+        px = std::min<uint64_t>(px, (budget - used) / j.spp);
+        take[i] = px;
+        used += px * j.spp;
+    }
+    for (size_t k = 0; k < n && used < budget; ++k) {
+        const size_t i = (S.rr + k) % n;
+        Job &j = *jobs[i];
+        uint64_t left = j.n_pixels - j.next_pixel - take[i];
+        uint64_t more = std::min<uint64_t>(left, (budget - used) / j.spp);
+        take[i] += more;
+        used += more * j.spp;
+    }
+    S.rr = (S.rr + 1) % n;
+    for (size_t i = 0; i < n; ++i) {
+        if (take[i] == 0) continue;
+        pieces.push_back(Piece{jobs[i], jobs[i]->next_pixel, take[i]});
+        jobs[i]->next_pixel += take[i];
+    }
+    return pieces;
+}
+
+// Traces one tick's pieces and adds them into their films; returns the tick's GPU time.
+gr_status run_tick(ServeState &S, std::vector<Piece> &pieces, float *gpu_ms, uint32_t *launches) {
+    std::stable_sort(pieces.begin(), pieces.end(),
+                     [](const Piece &a, const Piece &b) { return a.job->cls < b.job->cls; });
+    uint64_t tuples = 0;
+    for (const Piece &p : pieces) tuples += p.n_pixels * p.job->spp;
+    gr_status st = ensure_staging(S, std::max<uint64_t>(tuples, 1), (uint32_t)pieces.size());
+    if (st != GR_OK) return st;
+    std::vector<BatchSlice> sl(pieces.size());
+    uint64_t off = 0;
+    uint32_t reduce_blocks = 0;
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        const Piece &p = pieces[i];
+        Job &j = *p.job;
+        BatchSlice &b = sl[i];
+        b.scene = j.d_scene;
+        b.planes = j.d_planes;
+        b.ri35 = j.d_ri35;
+        b.xyz = S.staging + 3 * off;
+        b.film = j.d_film;
+        b.stats = j.d_stats;
+        b.pixel_begin = (uint32_t)p.pixel_begin;
+        b.n_pixels = (uint32_t)p.n_pixels;
+        b.sample_offset = j.sample_offset;
+        b.spp = j.spp;
+        b.reduce_block_begin = reduce_blocks;
+        reduce_blocks += (uint32_t)((p.n_pixels + kBlockTuples - 1) / kBlockTuples);
+        off += p.n_pixels * j.spp;
+    }
+    // Class launches: slices of one class are contiguous; block numbering restarts per class.
+    struct ClassRange {
+        int cls;
+        uint32_t first, count, blocks;
+    };
+    std::vector<ClassRange> classes;
+    for (size_t i = 0; i < sl.size(); ++i) {
+        const int c = pieces[i].job->cls;
+        if (classes.empty() || classes.back().cls != c) classes.push_back(ClassRange{c, (uint32_t)i, 0, 0});
+        ClassRange &r = classes.back();
+        sl[i].trace_block_begin = r.blocks;
+        r.blocks += (uint32_t)(((uint64_t)sl[i].n_pixels * sl[i].spp + kBlockTuples - 1) / kBlockTuples);
+        r.count++;
+    }
+    SV_CHECK(hipMemcpyAsync(S.d_slices, sl.data(), sl.size() * sizeof(BatchSlice), hipMemcpyHostToDevice, S.stream));
+    SV_CHECK(hipEventRecord(S.ev0, S.stream));
+    for (const ClassRange &r : classes) {
+        SV_CHECK(gr_launch_trace_batch(S.d_slices + r.first, r.count, r.blocks, r.cls, S.stream));
+        ++*launches;
+    }
+    SV_CHECK(gr_launch_film_reduce_batch(S.d_slices, (uint32_t)sl.size(), reduce_blocks, S.stream));
+    SV_CHECK(hipEventRecord(S.ev1, S.stream));
+    SV_CHECK(hipEventSynchronize(S.ev1));
+    SV_CHECK(hipEventElapsedTime(gpu_ms, S.ev0, S.ev1));
+    return GR_OK;
+}
+
+void percentiles(std::vector<double> v, gr_serve_metrics *m) {
+    if (v.empty()) return;
+    std::sort(v.begin(), v.end());
+    auto at = [&](double q) {
+        // Nearest-rank percentile.
+        size_t r = (size_t)std::ceil(q * (double)v.size());
+        if (r < 1) r = 1;
+        return v[std::min(r, v.size()) - 1];
+    };
+    m->latency_p50_ms = at(0.50);
+    m->latency_p99_ms = at(0.99);
+    m->latency_max_ms = v.back();
+    double s = 0.0;
+    for (double x : v) s += x;
+    m->latency_mean_ms = s / (double)v.size();
+}
+
+void retire(ServeState &S, Job &j, int state, gr_status err) {
+    free_job_device(j);
+    S.in_use -= j.footprint;
+    j.state = state;
+    j.error = err;
+    j.t_done = Clock::now();
+    if (state == GR_JOB_DONE) {
+        double lat = ms_between(j.t_submit, j.t_done);
+        S.latencies.push_back(lat);
+        S.m.completed++;
+        S.m.paths += j.paths_done;
+        S.m.segments += j.events[GR_EV_SEGMENT];
+        S.t_last = j.t_done;
+    } else if (state == GR_JOB_CANCELLED) {
+        S.m.cancelled++;
+    } else {
+        S.m.failed++;
+    }
+}
+
+void loop(ServeState *Sp) {
+    ServeState &S = *Sp;
+    (void)hipSetDevice(S.ctx->device);
+    for (;;) {
+        std::vector<std::shared_ptr<Job>> admit;
+        {
+            std::unique_lock<std::mutex> lk(S.mu);
+            S.cv_work.wait(lk, [&] { return S.stop || !S.queue.empty() || !S.active.empty(); });
+            if (S.stop) return;
+            // Cancelled jobs leave first.
+            for (auto it = S.queue.begin(); it != S.queue.end();) {
+                if ((*it)->cancel_req) {
+                    (*it)->state = GR_JOB_CANCELLED;
+                    (*it)->t_done = Clock::now();
+                    S.m.cancelled++;
+                    it = S.queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            for (auto it = S.active.begin(); it != S.active.end();) {
+                if ((*it)->cancel_req) {
+                    retire(S, **it, GR_JOB_CANCELLED, GR_OK);
+                    it = S.active.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            // Admission by HBM footprint, in submission order.
+            while (!S.queue.empty() && S.in_use + S.queue.front()->footprint <= S.budget) {
+                S.in_use += S.queue.front()->footprint;
+                admit.push_back(S.queue.front());
+                S.queue.pop_front();
+            }
+            S.cv_done.notify_all();
+        }
+        for (auto &j : admit) {
+            gr_status st = upload_job(S, *j);
+            std::lock_guard<std::mutex> g(S.mu);
+            if (st != GR_OK) {
+                retire(S, *j, GR_JOB_FAILED, st);
+                S.cv_done.notify_all();
+                continue;
+            }
+            j->state = GR_JOB_RUNNING;
+            j->t_admit = Clock::now();
+            if (!S.have_first) {
+                S.have_first = true;
+                S.t_first = j->t_admit;
+            }
+            S.m.admitted++;
+            S.active.push_back(j);
+        }
+        std::vector<Job *> running;
+        uint64_t tick_tuples = 0;
+        {
+            std::lock_guard<std::mutex> g(S.mu);
+            for (auto &j : S.active) running.push_back(j.get());
+            tick_tuples = S.tick_tuples;
+        }
+        if (running.empty()) continue;
+        std::vector<Piece> pieces = plan_tick(S, running, tick_tuples);
+        float gpu_ms = 0.0f;
+        uint32_t launches = 0;
+        gr_status st = run_tick(S, pieces, &gpu_ms, &launches);
+        // Films and counters of finished jobs are read back before the lock is taken; until
+        // retire() marks them done no API call reads either.
+        std::vector<Job *> finished;
+        for (Job *j : running)
+            if (j->next_pixel == j->n_pixels) finished.push_back(j);
+        std::vector<gr_status> fin(finished.size(), st);
+        if (st == GR_OK)
+            for (size_t i = 0; i < finished.size(); ++i) fin[i] = finish_job(S, *finished[i]);
+        {
+            std::lock_guard<std::mutex> g(S.mu);
+            for (const Piece &p : pieces) p.job->paths_done += p.n_pixels * p.job->spp;
+            S.m.ticks++;
+            S.m.launches += launches + 1;
+            S.m.busy_ms += gpu_ms;
+            if (st != GR_OK) {
+                // A device error fails every running job.
+                for (auto &j : S.active) retire(S, *j, GR_JOB_FAILED, st);
+                S.active.clear();
+            } else {
+                for (size_t i = 0; i < finished.size(); ++i)
+                    retire(S, *finished[i], fin[i] == GR_OK ? GR_JOB_DONE : GR_JOB_FAILED, fin[i]);
+                S.active.erase(std::remove_if(S.active.begin(), S.active.end(),
+                                              [](const std::shared_ptr<Job> &j) { return j->state != GR_JOB_RUNNING; }),
+                               S.active.end());
+            }
+            S.cv_done.notify_all();
+        }
+    }
+}
+
+// Creates the loop on first use.
+gr_status get_serve(gr_ctx *ctx, ServeState **out) {
+    std::lock_guard<std::mutex> g(ctx->serve_mu);
+    if (!ctx->serve) {
+        if (hipSetDevice(ctx->device) != hipSuccess) return GR_ERR_DEVICE;
+        auto *S = new (std::nothrow) ServeState();
+        if (!S) return GR_ERR_DEVICE;
+        S->ctx = ctx;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            delete S;
+            return GR_ERR_DEVICE;
+        }
+        S->budget = ctx->hbm_budget ? ctx->hbm_budget : free_b / 4 * 3;
+        if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess) {
+            if (S->ev0) (void)hipEventDestroy(S->ev0);
+            if (S->ev1) (void)hipEventDestroy(S->ev1);
+            if (S->stream) (void)hipStreamDestroy(S->stream);
+            delete S;
+            return GR_ERR_DEVICE;
+        }
+        S->th = std::thread(loop, S);
+        ctx->serve = S;
+    }
+    *out = ctx->serve;
+    return GR_OK;
+}
+
+std::shared_ptr<Job> find_job(ServeState &S, uint64_t id) {
+    auto it = S.jobs.find(id);
+    return it == S.jobs.end() ? nullptr : it->second;
+}
+
+gr_status job_status(const Job &j) {
+    switch (j.state) {
+        case GR_JOB_DONE: return GR_OK;
+        case GR_JOB_CANCELLED: return GR_CANCELLED;
+        case GR_JOB_FAILED: return j.error == GR_OK ? GR_ERR_DEVICE : j.error;
+        default: return GR_PENDING;
+    }
+}
+
+}  // namespace
+
+void serve_shutdown(gr_ctx *ctx) {
+    ServeState *S = nullptr;
+    {
+        std::lock_guard<std::mutex> g(ctx->serve_mu);
+        S = ctx->serve;
+        ctx->serve = nullptr;
+    }
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->th.joinable()) S->th.join();
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(S->stream);
+    for (auto &j : S->active) free_job_device(*j);
+    if (S->staging) (void)hipFree(S->staging);
+    if (S->d_slices) (void)hipFree(S->d_slices);
+    (void)hipEventDestroy(S->ev0);
+    (void)hipEventDestroy(S->ev1);
+    (void)hipStreamDestroy(S->stream);
+    delete S;
+}
+
+}  // namespace gr
+
+using gr::ServeState;
+
+extern "C" {
+
+gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) {
+    if (!ctx || !desc || !job_id) return GR_ERR_INVALID;
+    *job_id = 0;
+    const gr_scene *s = desc->scene;
+    gr_status st = gr::validate_scene(s);
+    if (st != GR_OK) return st;
+    if (desc->spp == 0 || desc->spp > GR_MAX_SAMPLES_PER_CALL) return GR_ERR_INVALID;
+    if ((uint64_t)desc->sample_offset + desc->spp > 0xFFFFFFFFull) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    auto j = std::make_shared<gr::Job>();
+    gr::fill_dev_scene(s, &j->host_sc);
+    j->cls = gr::scene_class(j->host_sc);
+    j->planes.assign(s->planes, s->planes + 4 * (size_t)s->n_planes);
+    if (s->env->kind == GR_ENV_MAP)
+        j->map.assign(s->env->map_rgb, s->env->map_rgb + (size_t)s->env->map_w * s->env->map_h * 3);
+    j->width = s->width;
+    j->height = s->height;
+    j->n_pixels = (uint64_t)s->width * s->height;
+    j->sample_offset = desc->sample_offset;
+    j->spp = desc->spp;
+    j->footprint = gr::job_footprint(*j);
+    j->t_submit = gr::Clock::now();
+    std::lock_guard<std::mutex> g(S->mu);
+    if (j->footprint > S->budget) {
+        S->m.declined++;
+        return GR_DECLINE_BUDGET;
+    }
+    j->id = S->next_id++;
+    S->jobs[j->id] = j;
+    S->queue.push_back(j);
+    S->m.submitted++;
+    *job_id = j->id;
+    S->cv_work.notify_one();
+    return GR_OK;
+}
+
+gr_status gr_job_poll(gr_ctx *ctx, uint64_t job_id) {
+    if (!ctx) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto j = gr::find_job(*S, job_id);
+    return j ? gr::job_status(*j) : GR_ERR_NOT_FOUND;
+}
+
+gr_status gr_job_wait(gr_ctx *ctx, uint64_t job_id, int64_t timeout_ms) {
+    if (!ctx) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto j = gr::find_job(*S, job_id);
+    if (!j) return GR_ERR_NOT_FOUND;
+    auto settled = [&] { return gr::job_status(*j) != GR_PENDING; };
+    if (timeout_ms < 0) S->cv_done.wait(lk, settled);
+    else S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), settled);
+    return gr::job_status(*j);
+}
+
+gr_status gr_job_cancel(gr_ctx *ctx, uint64_t job_id) {
+    if (!ctx) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto j = gr::find_job(*S, job_id);
+    if (!j) return GR_ERR_NOT_FOUND;
+    if (j->state == GR_JOB_QUEUED || j->state == GR_JOB_RUNNING) {
+        j->cancel_req = true;
+        S->cv_work.notify_one();
+    }
+    return GR_OK;
+}
+
+gr_status gr_job_film(gr_ctx *ctx, uint64_t job_id, float *accum_xyz, size_t accum_pixels) {
+    if (!ctx || !accum_xyz) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto j = gr::find_job(*S, job_id);
+    if (!j) return GR_ERR_NOT_FOUND;
+    st = gr::job_status(*j);
+    if (st != GR_OK) return st;
+    if (accum_pixels != j->n_pixels) return GR_ERR_INVALID;
+    for (size_t i = 0; i < j->film.size(); ++i) accum_xyz[i] += j->film[i];
+    return GR_OK;
+}
+
+gr_status gr_job_info(gr_ctx *ctx, uint64_t job_id, gr_job_stats *out) {
+    if (!ctx || !out) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto j = gr::find_job(*S, job_id);
+    if (!j) return GR_ERR_NOT_FOUND;
+    std::memset(out, 0, sizeof(*out));
+    out->state = (uint32_t)j->state;
+    out->width = j->width;
+    out->height = j->height;
+    out->sample_offset = j->sample_offset;
+    out->spp = j->spp;
+    out->footprint_bytes = j->footprint;
+    out->paths = j->paths_done;
+    if (j->state == GR_JOB_DONE)
+        for (int i = 0; i < GR_EV_COUNT; ++i) out->events[i] = j->events[i];
+    if (j->state == GR_JOB_DONE || j->state == GR_JOB_FAILED || j->state == GR_JOB_CANCELLED) {
+        out->latency_ms = gr::ms_between(j->t_submit, j->t_done);
+        if (j->state == GR_JOB_DONE) {
+            out->queue_ms = gr::ms_between(j->t_submit, j->t_admit);
+            out->run_ms = gr::ms_between(j->t_admit, j->t_done);
+        }
+    }
+    return GR_OK;
+}
+
+gr_status gr_job_release(gr_ctx *ctx, uint64_t job_id) {
+    if (!ctx) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto j = gr::find_job(*S, job_id);
+    if (!j) return GR_ERR_NOT_FOUND;
+    if (gr::job_status(*j) == GR_PENDING) return GR_PENDING;
+    S->jobs.erase(job_id);
+    return GR_OK;
+}
+
+gr_status gr_metrics(gr_ctx *ctx, gr_serve_metrics *out) {
+    if (!ctx || !out) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    *out = S->m;
+    gr::percentiles(S->latencies, out);
+    out->wall_ms = S->have_first && S->m.completed ? gr::ms_between(S->t_first, S->t_last) : 0.0;
+    out->hbm_in_use = S->in_use;
+    out->hbm_budget = S->budget;
+    out->queued = S->queue.size();
+    out->running = S->active.size();
+    out->tick_tuples = S->tick_tuples;
+    return GR_OK;
+}
+
+gr_status gr_metrics_reset(gr_ctx *ctx) {
+    if (!ctx) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    S->m = gr_serve_metrics{};
+    S->latencies.clear();
+    S->have_first = false;
+    return GR_OK;
+}
+
+gr_status gr_serve_set_tick_tuples(gr_ctx *ctx, uint64_t tuples) {
+    if (!ctx || tuples < gr::kMinTickTuples || tuples > gr::kMaxTickTuples) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    S->tick_tuples = tuples;
+    return GR_OK;
+}
+
+}  // extern "C"

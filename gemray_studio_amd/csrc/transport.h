@@ -56,8 +56,38 @@ struct TraceArgs {
     hipEvent_t trace_done;       // recorded between the trace and finish kernels, or null
 };
 
+// One piece of one job in a batched launch: pixels [pixel_begin, pixel_begin + n_pixels)
+// x samples [sample_offset, sample_offset + spp) of the job's frame.  Tuple t of the
+// slice is pixel pixel_begin + t / spp, sample sample_offset + t % spp, exactly as in a
+// plain launch.  Slices of one material class are traced by one launch; the trace kernel
+// finds its slice by binary search of trace_block_begin over the launch's slices, the
+// film reduce by reduce_block_begin over all slices of the tick.
+struct BatchSlice {
+    const DevScene *scene;       // device
+    const float4 *planes;        // device, scene->n_planes
+    const float2 *ri35;          // device, radical inverses of samples [sample_offset, +spp)
+    float *xyz;                  // device staging, 3 floats per tuple of this slice
+    float *film;                 // device, the job's film (3 floats per pixel)
+    unsigned long long *stats;   // device, the job's GR_EV_COUNT counters
+    uint32_t pixel_begin, n_pixels, sample_offset, spp;
+    uint32_t trace_block_begin;  // first workgroup of this slice in its class launch
+    uint32_t reduce_block_begin; // first workgroup of this slice in the film reduce
+};
+
+// Material/environment class of a scene: bit 0 anisotropic, bit 1 absorbing, bit 2 HDR map.
+inline int scene_class(const DevScene &sc) {
+    return (sc.anisotropic ? 1 : 0) | (sc.has_absorption ? 2 : 0) | (sc.env_kind == GR_ENV_MAP ? 4 : 0);
+}
+
 }  // namespace gr
 
 hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &args, hipStream_t stream);
+// Traces slices [0, n_slices) of one class (all with scene_class == cls) in one launch of
+// n_blocks workgroups.
+hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, int cls,
+                                 hipStream_t stream);
+// Adds each slice's per-pixel ordered sample sums into its job's film.
+hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
+                                       hipStream_t stream);
 hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t spp,
                                  uint32_t pixel_begin, float *film, hipStream_t stream);

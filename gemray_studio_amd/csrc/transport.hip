@@ -709,6 +709,63 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     flush_events(ev, block_ev, stats);
 }
 
+// Index of the slice that owns workgroup b: the last slice whose first block is <= b
+// (slices are in block order and the first starts at block 0).
+__device__ __forceinline__ uint32_t find_slice(const BatchSlice *__restrict__ slices, uint32_t n, uint32_t b,
+                                               bool reduce) {
+    uint32_t lo = 0u, hi = n;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t first = reduce ? slices[mid].reduce_block_begin : slices[mid].trace_block_begin;
+        if (first <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The batched path: one launch traces the pieces of many jobs of one material class (the
+// reference's per-class pipeline specialisation, frame.rs:514-524).  Every workgroup
+// belongs to one slice, stages that job's planes and reads that job's scene block; the
+// per-lane work is the plain kernel's, so each tuple's result is bit-identical to a plain
+// launch of the same job.
+template <bool kAniso, bool kAbsorb, int kEnv>
+__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
+    const BatchSlice *__restrict__ slices, uint32_t n_slices) {
+    __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ uint32_t block_ev[GR_EV_COUNT];
+    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, false)];
+    const DevScene &sc = *J.scene;
+    stage_planes(planes, J.planes, sc.n_planes);
+    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
+    __syncthreads();
+
+    uint32_t ev[GR_EV_COUNT];
+#pragma unroll
+    for (int i = 0; i < GR_EV_COUNT; ++i) ev[i] = 0u;
+
+    const uint32_t spp = J.spp;
+    const uint32_t t = (blockIdx.x - J.trace_block_begin) * kBlock + threadIdx.x;
+    if (t < J.n_pixels * spp) {
+        const uint32_t local_pixel = t / spp;
+        const uint32_t s = t - local_pixel * spp;
+        PathState ps;
+        init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, J.ri35[s], sc);
+        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, ev)) {
+        }
+        V3 xyz = v3(0.0f, 0.0f, 0.0f);
+        if (ps.escaped) {
+            float inten[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) inten[k] = intensity(ps.st[k]);
+            xyz = finish_escaped<kEnv>(ps.ray_d, inten, ps.pdf, ps.hero_off, sc);
+        }
+        J.xyz[3 * (size_t)t + 0] = xyz.x;
+        J.xyz[3 * (size_t)t + 1] = xyz.y;
+        J.xyz[3 * (size_t)t + 2] = xyz.z;
+    }
+    flush_events(ev, block_ev, J.stats);
+}
+
 // Escape records: field-major (record field f of tuple t at rec[f * cap + t]).
 // Fields: 0-2 escape direction, 3-10 channel intensities, 11-18 channel pdfs,
 // 19 hero offset (negative: the path never escaped).
@@ -831,6 +888,27 @@ __global__ __launch_bounds__(kBlock) void film_reduce_kernel(const float *__rest
     dst[2] = dst[2] + sz;
 }
 
+// film_reduce_kernel over the slices of a batched tick: each slice's pixels sum their
+// spp staged tuples in sample order and add the sum into the slice's job film.
+__global__ __launch_bounds__(kBlock) void film_reduce_batch_kernel(const BatchSlice *__restrict__ slices,
+                                                                   uint32_t n_slices) {
+    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, true)];
+    const uint32_t p = (blockIdx.x - J.reduce_block_begin) * kBlock + threadIdx.x;
+    if (p >= J.n_pixels) return;
+    const uint32_t spp = J.spp;
+    const float *src = J.xyz + 3 * (size_t)p * spp;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (uint32_t s = 0; s < spp; ++s) {
+        sx = sx + src[3 * s + 0];
+        sy = sy + src[3 * s + 1];
+        sz = sz + src[3 * s + 2];
+    }
+    float *dst = J.film + 3 * (size_t)(J.pixel_begin + p);
+    dst[0] = dst[0] + sx;
+    dst[1] = dst[1] + sy;
+    dst[2] = dst[2] + sz;
+}
+
 }  // namespace gr
 
 // ---------------------------------------------------------------------------------
@@ -894,6 +972,38 @@ hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &a, hipSt
     if (an && !ab) return e == 0 ? launch_one<true, false, 0>(sc, a, stream) : launch_one<true, false, 1>(sc, a, stream);
     if (!an && ab) return e == 0 ? launch_one<false, true, 0>(sc, a, stream) : launch_one<false, true, 1>(sc, a, stream);
     return e == 0 ? launch_one<false, false, 0>(sc, a, stream) : launch_one<false, false, 1>(sc, a, stream);
+}
+
+namespace {
+template <int C>
+hipError_t launch_batch_class(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, hipStream_t stream) {
+    hipLaunchKernelGGL((gr::trace_batch_kernel<(C & 1) != 0, (C & 2) != 0, (C & 4) ? 1 : 0>), dim3(n_blocks),
+                       dim3(gr::kBlock), 0, stream, slices, n_slices);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, int cls,
+                                 hipStream_t stream) {
+    if (n_slices == 0 || n_blocks == 0) return hipSuccess;
+    switch (cls) {
+        case 0: return launch_batch_class<0>(slices, n_slices, n_blocks, stream);
+        case 1: return launch_batch_class<1>(slices, n_slices, n_blocks, stream);
+        case 2: return launch_batch_class<2>(slices, n_slices, n_blocks, stream);
+        case 3: return launch_batch_class<3>(slices, n_slices, n_blocks, stream);
+        case 4: return launch_batch_class<4>(slices, n_slices, n_blocks, stream);
+        case 5: return launch_batch_class<5>(slices, n_slices, n_blocks, stream);
+        case 6: return launch_batch_class<6>(slices, n_slices, n_blocks, stream);
+        case 7: return launch_batch_class<7>(slices, n_slices, n_blocks, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
+                                       hipStream_t stream) {
+    if (n_slices == 0 || n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(gr::film_reduce_batch_kernel, dim3(n_blocks), dim3(gr::kBlock), 0, stream, slices, n_slices);
+    return hipGetLastError();
 }
 
 hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t spp,

@@ -33,7 +33,8 @@ typedef enum {
     GR_ERR_INVALID = 5,       /* argument outside the limits (validate/mod.rs:23-192)  */
     GR_ERR_DEVICE = 6,        /* a HIP call failed                                     */
     GR_ERR_NOT_FOUND = 7,     /* unknown job id                                        */
-    GR_PENDING = 8            /* gr_job_poll: the job has not finished                 */
+    GR_PENDING = 8,           /* gr_job_poll: the job has not finished                 */
+    GR_CANCELLED = 9          /* gr_job_poll: the job was cancelled                    */
 } gr_status;
 
 /* Limits (apps/gemray-worker/src/validate/mod.rs:23-57, renderer/buffers.rs:148). */
@@ -184,6 +185,72 @@ gr_status gr_film_download(gr_ctx *ctx, gr_film *film, float *xyz_out, size_t n_
 /* Adds samples [sample_offset, sample_offset + spp) of every pixel into film. */
 gr_status gr_render(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset,
                     uint32_t spp, gr_film *film);
+
+/* ---- Batched and served rendering: the device-resident job loop ---------------------
+ * The reference serves one RenderRequest{request_id, scene, first_sample, samples} at a
+ * time through GpuFrameRenderer::accumulate (apps/gemray-worker render path;
+ * messages/render.rs:42-70, frame.rs:632-793).  Here requests become jobs: a job is
+ * admitted when its HBM footprint (film, planes, environment map, sample table) fits the
+ * context's budget (gr_options.hbm_budget_bytes; 0 = 3/4 of the HBM free when the loop
+ * starts), its inputs then stay in HBM, and a scheduler thread traces all admitted jobs
+ * together: every tick gives each job a piece of its frame, traces the pieces of each
+ * material class in one launch, and adds them into the jobs' device films.  Jobs
+ * submitted while others run join at the next tick.  Submission never blocks on the GPU.
+ *
+ * Determinism: a job's film is bit-identical to gr_accumulate of the same scene and sample
+ * range into a zeroed film, alone or in any mix of jobs. */
+enum { GR_JOB_QUEUED = 0, GR_JOB_RUNNING = 1, GR_JOB_DONE = 2, GR_JOB_CANCELLED = 3, GR_JOB_FAILED = 4 };
+
+typedef struct {
+    const gr_scene *scene;  /* copied during gr_job_submit                               */
+    uint32_t sample_offset; /* first absolute sample (RenderRequest.first_sample)        */
+    uint32_t spp;           /* samples per pixel (RenderRequest.samples)                 */
+} gr_job_desc;
+
+typedef struct {
+    uint32_t state;                /* GR_JOB_*                                            */
+    uint32_t width, height, sample_offset, spp;
+    uint64_t footprint_bytes;      /* HBM held from admission to completion               */
+    uint64_t paths;                /* paths traced so far                                 */
+    uint64_t events[GR_EV_COUNT];  /* per-class counters (when done)                      */
+    double queue_ms;               /* submit -> admission                                 */
+    double run_ms;                 /* admission -> completion                             */
+    double latency_ms;             /* submit -> completion                                */
+} gr_job_stats;
+
+typedef struct {
+    uint64_t submitted, admitted, completed, cancelled, failed, declined;
+    uint64_t queued, running;      /* now                                                 */
+    uint64_t paths, segments;      /* of completed jobs                                   */
+    uint64_t ticks, launches;
+    double busy_ms;                /* GPU time of the ticks (HIP events)                  */
+    double wall_ms;                /* first admission -> last completion                  */
+    double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;
+    uint64_t hbm_in_use, hbm_budget, tick_tuples;
+} gr_serve_metrics;
+
+/* Queues a job; *job_id identifies it until gr_job_release.  GR_DECLINE_* / GR_ERR_INVALID
+ * as gr_accumulate; GR_DECLINE_BUDGET when the job alone exceeds the HBM budget. */
+gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id);
+/* GR_OK (done), GR_PENDING (queued or running), GR_CANCELLED, GR_ERR_NOT_FOUND, or the
+ * error that failed the job. */
+gr_status gr_job_poll(gr_ctx *ctx, uint64_t job_id);
+/* Blocks until the job settles or timeout_ms passes (< 0: no limit); returns as poll. */
+gr_status gr_job_wait(gr_ctx *ctx, uint64_t job_id, int64_t timeout_ms);
+/* Requests cancellation (the reference's cancel epochs, tracer.rs:64-117); the job stops
+ * at the next tick boundary and frees its HBM. */
+gr_status gr_job_cancel(gr_ctx *ctx, uint64_t job_id);
+/* ADDS a finished job's film into accum_xyz (accum_pixels = width * height), like
+ * gr_accumulate.  GR_PENDING if the job has not finished. */
+gr_status gr_job_film(gr_ctx *ctx, uint64_t job_id, float *accum_xyz, size_t accum_pixels);
+gr_status gr_job_info(gr_ctx *ctx, uint64_t job_id, gr_job_stats *out);
+/* Forgets a settled job (GR_PENDING if it has not settled). */
+gr_status gr_job_release(gr_ctx *ctx, uint64_t job_id);
+/* Loop metrics since the last reset: throughput, p50/p99 submit-to-completion latency. */
+gr_status gr_metrics(gr_ctx *ctx, gr_serve_metrics *out);
+gr_status gr_metrics_reset(gr_ctx *ctx);
+/* Tuples traced per tick (default 2^24; at least GR_MAX_SAMPLES_PER_CALL). */
+gr_status gr_serve_set_tick_tuples(gr_ctx *ctx, uint64_t tuples);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,35 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.Stats) == 8 * 11 + 8 + 8 + 8 + 8 + 8 + 8
 
 
+# ctypes mirror of each C struct of the header, checked field by field against the C
+# compiler's own layout of include/gemray_hip.h.
+MIRRORS = {
+    "gr_options": _lib.Options, "gr_camera": _lib.Camera, "gr_material": _lib.MaterialStruct,
+    "gr_env": _lib.Env, "gr_scene": _lib.Scene, "gr_stats": _lib.Stats,
+    "gr_job_desc": _lib.JobDesc, "gr_job_stats": _lib.JobStats, "gr_serve_metrics": _lib.ServeMetrics,
+}
+
+
+def test_ctypes_mirrors_match_the_c_layout(tmp_path):
+    import subprocess
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {']
+    for cname, py in MIRRORS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in MIRRORS.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
 @pytest.mark.parametrize("args", [(0.35, 0.28, 5.0, 18.0), (0.4, 0.3, 3.0, 30.0),
                                   (2.1, -0.7, 7.5, 12.0), (0.0, 1.5707963, 4.0, 25.0)])
 def test_camera_new_matches_oracle_bit_for_bit(args):
