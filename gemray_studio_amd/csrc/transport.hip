@@ -109,7 +109,7 @@ struct PathState {
     float pdf[8];
     float n_o[8], ao[8], ae[8];  // lazily filled caches (refraction.rs:127-174)
     float hero_off;
-    uint32_t seed, bounce;
+    uint32_t seed, bounce, last_facet;
     uint32_t term_bounce, term_reason, fh, fate_h;
     bool inside, is_extra, have_prev, have_n, have_abs, escaped;
 };
@@ -145,6 +145,7 @@ __device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_
         ps.pdf[k] = 1.0f;
     }
     ps.bounce = 0u;
+    ps.last_facet = 0u;
     ps.term_bounce = sc.max_bounces;
     ps.term_reason = 3u;
     ps.fh = 2166136261u;
@@ -152,52 +153,80 @@ __device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_
     ps.inside = ps.is_extra = ps.have_prev = ps.have_n = ps.have_abs = ps.escaped = false;
 }
 
+// The convex facet-plane scan (simd/slab.rs:94-152, intersect.rs:93-132).  The reference
+// divides t = -side / denom for every plane and keeps the first strict maximum (entering)
+// and minimum (exiting).  Here a = -side * rcp(denom) (v_rcp_f32, 1 ulp) screens each
+// plane: |a - t| <= |a| * 2^-19, so a plane whose exact t could beat the running extreme
+// always passes the screen, and only those planes take the correctly rounded division and
+// the exact comparison (a 2^-120 absolute slack covers denormal products).  The selected
+// planes and t values are exactly the reference's.  The screen is branch-free; a lane
+// takes the one branch only when a plane may improve one of its extremes.  A parallel
+// plane with the origin outside makes the reference stop scanning (the ray misses whatever
+// the other planes give); here the flag is kept and the scan runs on, to the same outcome.
+// kCam: the origin is the camera's, whose per-plane sides are staged in cam_side.
+template <bool kCam>
+__device__ __forceinline__ void clip_scan(const float4 *__restrict__ planes, const float *__restrict__ cam_side,
+                                          uint32_t n, V3 ro, V3 rd, float &t_near, int &near_idx, float &t_far,
+                                          int &far_idx, bool &outside) {
+    for (int i = 0; i < (int)n; ++i) {
+        const float4 p = planes[i];
+        const float denom = (p.x * rd.x + p.y * rd.y) + p.z * rd.z;
+        const float side = kCam ? cam_side[i] : p.w + ((p.x * ro.x + p.y * ro.y) + p.z * ro.z);
+        const float a = (-side) * __builtin_amdgcn_rcpf(denom);
+        const float slack = fmaf(fabsf(a), 0x1p-19f, 0x1p-120f);
+        const bool steep = fabsf(denom) > 1e-7f;
+        outside = outside || (!steep && side > 0.0f);
+        const bool cand_near = steep && denom < 0.0f && a + slack > t_near;
+        const bool cand_far = steep && denom > 0.0f && a - slack < t_far;
+        if (cand_near || cand_far) {
+            const float t = (-side) / denom;
+            if (cand_near && t > t_near) {
+                t_near = t;
+                near_idx = i;
+            }
+            if (cand_far && t < t_far) {
+                t_far = t;
+                far_idx = i;
+            }
+        }
+    }
+}
+
+// True when the scan is certain to miss because of plane p alone: the origin is outside p
+// and the ray moves away from it, so p's exact t = -side / denom is negative, the far
+// extreme is at most that, and neither extreme can exceed the 1e-4 hit threshold.
+__device__ __forceinline__ bool exterior_miss(float4 p, V3 ro, V3 rd) {
+    const float denom = (p.x * rd.x + p.y * rd.y) + p.z * rd.z;
+    const float side = p.w + ((p.x * ro.x + p.y * ro.y) + p.z * ro.z);
+    return fabsf(denom) > 1e-7f && denom > 0.0f && side > 0.0f;
+}
+
 // One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
 // has ended (escape, Russian roulette, or the bounce cap).
 template <bool kAniso, bool kAbsorb>
-__device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc,
-                                            const float4 *__restrict__ planes, uint32_t *ev) {
+__device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, const float4 *__restrict__ planes,
+                                            const float *__restrict__ cam_side, uint32_t *ev) {
     const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
     const float dn = sc.birefringence_delta;
     const uint32_t bounce = ps.bounce;
     ev[GR_EV_SEGMENT]++;
 
     // ---- convex facet-plane clip (simd/slab.rs:94-152, intersect.rs:93-132) ----
-    // The reference divides t = -side / denom for every plane and keeps the first strict
-    // maximum (entering) / minimum (exiting).  Here a = -side * rcp(denom) (v_rcp_f32,
-    // 1 ulp) screens each plane: |a - t| <= |a| * 2^-19, so a plane whose exact t could
-    // beat the running extreme always passes the screen, and only those planes take the
-    // correctly rounded division and the exact comparison (a 2^-120 absolute slack covers
-    // denormal products).  The selected planes and t values are exactly the reference's.
     float t_near = -1e30f, t_far = 1e30f;
     int near_idx = -1, far_idx = -1;
     bool outside = false;
     const V3 ro = ps.ray_o, rd = ps.ray_d;
-    for (int i = 0; i < (int)sc.n_planes; ++i) {
-        float4 p = planes[i];
-        float denom = (p.x * rd.x + p.y * rd.y) + p.z * rd.z;
-        float side = p.w + ((p.x * ro.x + p.y * ro.y) + p.z * ro.z);
-        float a = (-side) * __builtin_amdgcn_rcpf(denom);
-        if (fabsf(denom) > 1e-7f) {
-            if (denom < 0.0f) {
-                if (fmaf(fabsf(a), 0x1p-19f, a) + 0x1p-120f > t_near) {
-                    float t = (-side) / denom;
-                    if (t > t_near) {
-                        t_near = t;
-                        near_idx = i;
-                    }
-                }
-            } else if (fmaf(-fabsf(a), 0x1p-19f, a) - 0x1p-120f < t_far) {
-                float t = (-side) / denom;
-                if (t < t_far) {
-                    t_far = t;
-                    far_idx = i;
-                }
-            }
-        } else if (side > 0.0f) {
-            outside = true;
-            break;
-        }
+    if (bounce == 0u) {
+        // Camera rays share their origin: the origin's signed distances are staged per
+        // workgroup (same operations, same values).
+        clip_scan<true>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+    } else if (!ps.inside && exterior_miss(planes[ps.last_facet], ro, rd)) {
+        // The ray left the stone's surface outwards (a refraction out, or a reflection off
+        // the outside): it starts outside the facet it just met and moves away from it, so
+        // that facet alone makes the scan's far extreme negative and the scan a miss.
+        outside = true;
+    } else {
+        clip_scan<false>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
     }
     bool hit = false;
     float hit_t = 0.0f;
@@ -519,6 +548,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc,
     if (has_update) ps.is_extra = update;
     if (kAniso && internal_reflection)
         ps.is_extra = stream_draw(ps.seed, bounce, MODE_COUPLING_STREAM) < 0.5f;
+    ps.last_facet = (uint32_t)facet;
     fnv(ps.fh, (uint32_t)facet);
     fnv(ps.fh, event | (ps.is_extra ? 16u : 0u));
     {
@@ -644,8 +674,16 @@ constexpr int kBlock = 256;
 #endif
 constexpr uint32_t kWaveTuples = 64u * GR_WAVE_ROUNDS;
 
-__device__ __forceinline__ void stage_planes(float4 *lds, const float4 *__restrict__ g, uint32_t n) {
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) lds[i] = g[i];
+// Facet planes into LDS, with the camera origin's signed distance to each
+// (p.w + n . origin, the clip's operations).
+__device__ __forceinline__ void stage_planes(float4 *lds, float *cam_side, const float4 *__restrict__ g,
+                                             const DevScene &sc) {
+    const V3 o = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
+    for (uint32_t i = threadIdx.x; i < sc.n_planes; i += kBlock) {
+        const float4 p = g[i];
+        lds[i] = p;
+        cam_side[i] = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
+    }
 }
 
 // Per-class counters: wave-level sum, one LDS atomic per wave, one global atomic per
@@ -672,8 +710,9 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples,
     float *__restrict__ xyz_out, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ float cam_side[GR_MAX_PLANES];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes(planes, planes_g, sc.n_planes);
+    stage_planes(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -687,7 +726,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
         const uint32_t s = t - local_pixel * spp;
         PathState ps;
         init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], sc);
-        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, ev)) {
+        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
         }
         V3 xyz = v3(0.0f, 0.0f, 0.0f);
         if (ps.escaped) {
@@ -732,10 +771,11 @@ template <bool kAniso, bool kAbsorb, int kEnv>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     const BatchSlice *__restrict__ slices, uint32_t n_slices) {
     __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ float cam_side[GR_MAX_PLANES];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, false)];
     const DevScene &sc = *J.scene;
-    stage_planes(planes, J.planes, sc.n_planes);
+    stage_planes(planes, cam_side, J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -750,7 +790,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         const uint32_t s = t - local_pixel * spp;
         PathState ps;
         init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, J.ri35[s], sc);
-        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, ev)) {
+        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
         }
         V3 xyz = v3(0.0f, 0.0f, 0.0f);
         if (ps.escaped) {
@@ -785,8 +825,9 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, uint32_t rec_cap,
     float *__restrict__ rec, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ float cam_side[GR_MAX_PLANES];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes(planes, planes_g, sc.n_planes);
+    stage_planes(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -809,7 +850,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
         init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], sc);
     }
     while (__ballot(busy) != 0ull) {
-        if (busy && bounce_step<kAniso, kAbsorb>(ps, sc, planes, ev)) {
+        if (busy && bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
             // Path ended: escape record (and per-tuple info).
             float *r = rec + t;
             r[0 * (size_t)rec_cap] = ps.ray_d.x;

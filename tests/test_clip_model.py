@@ -1,0 +1,180 @@
+"""The trace kernel's clip shortcuts, modelled in float32 numpy and checked against the
+oracle's intersect_polyhedron restatement (CPU only).
+
+The kernel (gemray_studio_amd/csrc/transport.hip, clip_scan / exterior_miss) departs from
+the reference's scan in two ways that must not change any result:
+
+* the screen: a = -side * rcp(denom) with a 1-ulp reciprocal, and a plane takes the exact
+  division only if a +- (|a| 2^-19 + 2^-120) could beat the running extreme;
+* the exterior shortcut: a ray that starts outside the facet it just met and moves away
+  from it is a miss without the scan.
+
+The model below reproduces the kernel's float32 operations (the same association order;
+the reciprocal is perturbed by -1, 0 or +1 ulp per plane, covering v_rcp_f32's error) and
+the tests require the oracle's exact hit, t bits and facet on randomized rays.
+"""
+import ctypes
+
+import numpy as np
+
+import pyoracle as po
+from gemray_studio_amd import scenes
+
+L = po.lib()
+F = np.float32
+
+
+def oracle_intersect(planes, o, d):
+    t = np.zeros(1, F)
+    n = np.zeros(3, F)
+    f = ctypes.c_int()
+    hit = L.or_intersect(po.fptr(planes), len(planes), po.fptr(po.f32arr(o)), po.fptr(po.f32arr(d)),
+                         po.fptr(t), po.fptr(n), ctypes.byref(f))
+    return bool(hit), t[0], f.value
+
+
+def dot_rows(p, v):
+    """(p.x*v.x + p.y*v.y) + p.z*v.z in float32, rows of p against rows of v."""
+    return ((p[..., 0] * v[..., 0] + p[..., 1] * v[..., 1]) + p[..., 2] * v[..., 2]).astype(F)
+
+
+def fma32(x, y, z):
+    # x*y is exact in float64 for float32 operands; one rounding to float32 after the add
+    # (the double-rounding corner cases only move the slack by an ulp of itself).
+    return (x.astype(np.float64) * y.astype(np.float64) + np.float64(z)).astype(F)
+
+
+def model_scan(planes, ro, rd, rng):
+    """clip_scan<false> for a batch of rays (ro, rd: (R, 3) float32)."""
+    R = ro.shape[0]
+    t_near = np.full(R, F(-1e30))
+    t_far = np.full(R, F(1e30))
+    near_idx = np.full(R, -1)
+    far_idx = np.full(R, -1)
+    outside = np.zeros(R, bool)
+    divisions = 0
+    for i, p in enumerate(planes):
+        denom = dot_rows(np.broadcast_to(p, (R, 4)), rd)
+        side = (p[3] + dot_rows(np.broadcast_to(p, (R, 4)), ro)).astype(F)
+        with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+            rcp = (F(1.0) / denom).astype(F)
+            step = rng.integers(-1, 2, size=R)
+            rcp = np.where(step > 0, np.nextafter(rcp, F(np.inf)),
+                           np.where(step < 0, np.nextafter(rcp, F(-np.inf)), rcp)).astype(F)
+            a = ((-side) * rcp).astype(F)
+            slack = fma32(np.abs(a), F(2.0 ** -19), F(2.0 ** -120))
+            steep = np.abs(denom) > F(1e-7)
+            outside |= (~steep) & (side > 0)
+            cand_near = steep & (denom < 0) & ((a + slack).astype(F) > t_near)
+            cand_far = steep & (denom > 0) & ((a - slack).astype(F) < t_far)
+            t = ((-side) / denom).astype(F)
+        divisions += int(np.count_nonzero(cand_near | cand_far))
+        upd_n = cand_near & (t > t_near)
+        upd_f = cand_far & (t < t_far)
+        t_near = np.where(upd_n, t, t_near)
+        near_idx = np.where(upd_n, i, near_idx)
+        t_far = np.where(upd_f, t, t_far)
+        far_idx = np.where(upd_f, i, far_idx)
+    hit = (~outside) & ~(t_near > t_far) & ((t_near > F(1e-4)) | (t_far > F(1e-4)))
+    use_near = t_near > F(1e-4)
+    t_hit = np.where(use_near, t_near, t_far)
+    facet = np.where(use_near, near_idx, far_idx)
+    facet = np.where(facet < 0, 0, facet)
+    return hit, t_hit, facet, divisions
+
+
+def exterior_miss(p, ro, rd):
+    denom = dot_rows(np.broadcast_to(p, ro.shape[:1] + (4,)), rd)
+    side = (p[..., 3] + dot_rows(np.broadcast_to(p, ro.shape[:1] + (4,)), ro)).astype(F)
+    return (np.abs(denom) > F(1e-7)) & (denom > 0) & (side > 0)
+
+
+def random_rays(rng, n, spread=2.5):
+    o = rng.normal(size=(n, 3)).astype(F)
+    o = (o / np.linalg.norm(o, axis=1, keepdims=True) * spread).astype(F)
+    # Aim at jittered points near the stone so that hits, grazes and misses all occur.
+    target = (rng.normal(size=(n, 3)) * 0.6).astype(F)
+    d = (target - o).astype(F)
+    d = (d / np.sqrt((d * d).sum(axis=1, keepdims=True))).astype(F)
+    return o, d
+
+
+STONES = [
+    ("round-brilliant-73", scenes.round_brilliant()),
+    ("oval-69", scenes.oval_brilliant()),
+    ("random-23", scenes.random_convex_stone(11, 23)),
+    ("random-181", scenes.random_convex_stone(12, 181)),
+    ("step-233", scenes.step_cut()),
+]
+
+
+def with_near_duplicates(planes, rng, n=8):
+    """Appends copies of n random planes with d moved by 0, 1 or 2 ulp: their t values tie
+    or differ by an ulp or two, which only the exact division and the first-index rule
+    can order."""
+    extra = planes[rng.integers(0, len(planes), size=n)].copy()
+    for k in range(n):
+        for _ in range(int(rng.integers(0, 3))):
+            extra[k, 3] = np.nextafter(extra[k, 3], F(np.inf) if rng.random() < 0.5 else F(-np.inf))
+    out = np.concatenate([planes, extra]).astype(F)
+    return np.ascontiguousarray(out[rng.permutation(len(out))])
+
+
+def test_screened_scan_selects_the_oracle_planes():
+    rng = np.random.default_rng(1)
+    total, divs, planes_seen = 0, 0, 0
+    for name, planes in STONES:
+        planes = with_near_duplicates(np.ascontiguousarray(planes, F), rng)
+        # Exterior rays, then interior rays from points inside the stone.
+        o, d = random_rays(rng, 1500)
+        inner = (rng.normal(size=(1500, 3)) * 0.15).astype(F)
+        o = np.concatenate([o, inner]).astype(F)
+        d = np.concatenate([d, random_rays(rng, 1500)[1]]).astype(F)
+        hit, t, facet, nd = model_scan(planes, o, d, rng)
+        divs += nd
+        planes_seen += len(planes) * len(o)
+        for k in range(len(o)):
+            h_ref, t_ref, f_ref = oracle_intersect(planes, o[k], d[k])
+            assert hit[k] == h_ref, (name, k)
+            if h_ref:
+                assert t[k].view(np.uint32) == t_ref.view(np.uint32), (name, k)
+                assert facet[k] == f_ref, (name, k)
+        total += len(o)
+    assert total == 15000
+    # The screen is worth having: most planes skip the division.
+    assert divs < 0.25 * planes_seen
+
+
+def test_exterior_shortcut_only_fires_on_misses():
+    rng = np.random.default_rng(2)
+    fired = 0
+    checked = 0
+    for name, planes in STONES:
+        planes = np.ascontiguousarray(planes, F)
+        # Surface points: the oracle's hits of rays from inside.
+        inner = (rng.normal(size=(800, 3)) * 0.15).astype(F)
+        d_in = random_rays(rng, 800)[1]
+        for k in range(len(inner)):
+            h, t, f = oracle_intersect(planes, inner[k], d_in[k])
+            if not h:
+                continue
+            p = planes[f]
+            hit_point = (inner[k] + d_in[k] * t).astype(F)
+            n = p[:3]
+            # Outgoing directions: refraction out (any direction on the outer side) and
+            # grazing ones, plus reflections off the outside.
+            for _ in range(3):
+                v = rng.normal(size=3).astype(F)
+                v = (v / np.sqrt(np.dot(v, v))).astype(F)
+                if rng.random() < 0.3:
+                    v = (v - n * np.dot(v, n) + n * F(rng.uniform(1e-6, 1e-3))).astype(F)
+                    v = (v / np.sqrt(np.dot(v, v))).astype(F)
+                elif np.dot(v, n) < 0:
+                    v = (-v).astype(F)
+                ro = (hit_point + v * F(1e-4)).astype(F)
+                if exterior_miss(p[None, :], ro[None, :], v[None, :])[0]:
+                    fired += 1
+                    assert not oracle_intersect(planes, ro, v)[0], (name, k)
+                checked += 1
+    assert checked > 3000
+    assert fired > 0.6 * checked
