@@ -278,6 +278,7 @@ struct gr_scene_dev {
     gr::DevScene sc;
     float4 *planes = nullptr;
     float *map = nullptr;
+    uint32_t *cert = nullptr;  // per-pixel camera-ray miss certificates
 };
 
 struct gr_film {
@@ -356,6 +357,8 @@ gr_status upload_scene(gr_ctx *ctx, const gr_scene *s, gr_scene_dev *out) {
         GR_CHECK(hipMemcpyAsync(out->map, s->env->map_rgb, bytes, hipMemcpyHostToDevice, ctx->stream));
         out->sc.map_rgb = out->map;
     }
+    GR_CHECK(hipMalloc(&out->cert, (size_t)s->width * s->height * sizeof(uint32_t)));
+    GR_CHECK(gr_launch_cert(out->sc, out->planes, out->cert, ctx->stream));
     GR_CHECK(hipStreamSynchronize(ctx->stream));
     return GR_OK;
 }
@@ -364,8 +367,10 @@ void free_scene(gr_scene_dev *s) {
     if (!s) return;
     if (s->planes) (void)hipFree(s->planes);
     if (s->map) (void)hipFree(s->map);
+    if (s->cert) (void)hipFree(s->cert);
     s->planes = nullptr;
     s->map = nullptr;
+    s->cert = nullptr;
 }
 
 // Traces pixels [pixel_begin, pixel_begin + n_pixels) x samples [sample_offset, +spp) in
@@ -404,6 +409,7 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
         gr::TraceArgs args{};
         args.planes = s->planes;
         args.ri35 = ctx->ri35;
+        args.cert = s->cert;
         args.pixel_begin = first;
         args.sample_offset = sample_offset;
         args.spp = spp;

@@ -65,6 +65,7 @@ struct Job {
     float4 *d_planes = nullptr;
     float *d_map = nullptr;
     float2 *d_ri35 = nullptr;
+    uint32_t *d_cert = nullptr;
     float *d_film = nullptr;
     unsigned long long *d_stats = nullptr;
     // Progress (owned by the scheduler thread).
@@ -127,18 +128,20 @@ uint64_t job_footprint(const Job &j) {
     b += j.planes.size() * sizeof(float);                    // planes
     b += j.map.size() * sizeof(float);                       // environment map
     b += (uint64_t)j.spp * sizeof(float2);                   // sample table
+    b += j.n_pixels * sizeof(uint32_t);                      // miss certificates
     b += sizeof(DevScene) + GR_EV_COUNT * sizeof(unsigned long long);
     return b;
 }
 
 void free_job_device(Job &j) {
-    for (void *p : {(void *)j.d_scene, (void *)j.d_planes, (void *)j.d_map, (void *)j.d_ri35, (void *)j.d_film,
-                    (void *)j.d_stats})
+    for (void *p : {(void *)j.d_scene, (void *)j.d_planes, (void *)j.d_map, (void *)j.d_ri35, (void *)j.d_cert,
+                    (void *)j.d_film, (void *)j.d_stats})
         if (p) (void)hipFree(p);
     j.d_scene = nullptr;
     j.d_planes = nullptr;
     j.d_map = nullptr;
     j.d_ri35 = nullptr;
+    j.d_cert = nullptr;
     j.d_film = nullptr;
     j.d_stats = nullptr;
 }
@@ -148,6 +151,7 @@ gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMalloc(&j.d_planes, j.planes.size() * sizeof(float)));
     if (!j.map.empty()) SV_CHECK(hipMalloc(&j.d_map, j.map.size() * sizeof(float)));
     SV_CHECK(hipMalloc(&j.d_ri35, (size_t)j.spp * sizeof(float2)));
+    SV_CHECK(hipMalloc(&j.d_cert, j.n_pixels * sizeof(uint32_t)));
     SV_CHECK(hipMalloc(&j.d_film, j.n_pixels * 3 * sizeof(float)));
     SV_CHECK(hipMalloc(&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long)));
     DevScene sc = j.host_sc;
@@ -164,6 +168,7 @@ gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMemcpyAsync(j.d_ri35, ri.data(), ri.size() * sizeof(float2), hipMemcpyHostToDevice, S.stream));
     SV_CHECK(hipMemsetAsync(j.d_film, 0, j.n_pixels * 3 * sizeof(float), S.stream));
     SV_CHECK(hipMemsetAsync(j.d_stats, 0, GR_EV_COUNT * sizeof(unsigned long long), S.stream));
+    SV_CHECK(gr_launch_cert(sc, j.d_planes, j.d_cert, S.stream));
     SV_CHECK(hipStreamSynchronize(S.stream));
     // The host copy of the environment map is no longer needed.
     std::vector<float>().swap(j.map);
@@ -252,6 +257,7 @@ gr_status run_tick(ServeState &S, std::vector<Piece> &pieces, float *gpu_ms, uin
         b.scene = j.d_scene;
         b.planes = j.d_planes;
         b.ri35 = j.d_ri35;
+        b.cert = j.d_cert;
         b.xyz = S.staging + 3 * off;
         b.film = j.d_film;
         b.stats = j.d_stats;

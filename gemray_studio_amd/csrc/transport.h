@@ -46,6 +46,7 @@ struct DevScene {
 struct TraceArgs {
     const float4 *planes;        // device, n_planes
     const float2 *ri35;          // device, spp entries: radical inverses (base 3, base 5)
+    const uint32_t *cert;        // device, per-pixel camera-ray miss certificates of the frame
     uint32_t pixel_begin, sample_offset, spp, n_tuples;
     float *xyz;                  // device, 3 floats per tuple
     uint32_t *info;              // device, 4 u32 per tuple, or null
@@ -66,6 +67,7 @@ struct BatchSlice {
     const DevScene *scene;       // device
     const float4 *planes;        // device, scene->n_planes
     const float2 *ri35;          // device, radical inverses of samples [sample_offset, +spp)
+    const uint32_t *cert;        // device, the job's per-pixel miss certificates
     float *xyz;                  // device staging, 3 floats per tuple of this slice
     float *film;                 // device, the job's film (3 floats per pixel)
     unsigned long long *stats;   // device, the job's GR_EV_COUNT counters
@@ -82,6 +84,8 @@ inline int scene_class(const DevScene &sc) {
 }  // namespace gr
 
 hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &args, hipStream_t stream);
+// Fills cert[0, width * height) with the frame's camera-ray miss certificates.
+hipError_t gr_launch_cert(const gr::DevScene &sc, const float4 *planes, uint32_t *cert, hipStream_t stream);
 // Traces slices [0, n_slices) of one class (all with scene_class == cls) in one launch of
 // n_blocks workgroups.
 hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, int cls,

@@ -109,7 +109,7 @@ struct PathState {
     float pdf[8];
     float n_o[8], ao[8], ae[8];  // lazily filled caches (refraction.rs:127-174)
     float hero_off;
-    uint32_t seed, bounce, last_facet;
+    uint32_t seed, bounce, last_facet, cert;
     uint32_t term_bounce, term_reason, fh, fate_h;
     bool inside, is_extra, have_prev, have_n, have_abs, escaped;
 };
@@ -119,7 +119,7 @@ struct PathState {
 // (camera.rs:96-114) and the initial state of trace_spectral_ray_inner
 // (transport.rs:651-722).
 __device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_t sample_num,
-                                          float2 ri35, const DevScene &sc) {
+                                          float2 ri35, uint32_t cert, const DevScene &sc) {
     const uint32_t x = pixel % sc.width, y = pixel / sc.width;
     float rot_x = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_X_ROTATION_STREAM));
     float rot_y = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_Y_ROTATION_STREAM));
@@ -146,6 +146,7 @@ __device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_
     }
     ps.bounce = 0u;
     ps.last_facet = 0u;
+    ps.cert = cert;
     ps.term_bounce = sc.max_bounces;
     ps.term_reason = 3u;
     ps.fh = 2166136261u;
@@ -201,6 +202,24 @@ __device__ __forceinline__ bool exterior_miss(float4 p, V3 ro, V3 rd) {
     return fabsf(denom) > 1e-7f && denom > 0.0f && side > 0.0f;
 }
 
+// Miss certificate of a camera ray: planes i (entering) and j (exiting) whose exact t
+// values, formed with the scan's own operations, satisfy t_i > t_j.  The scan's entering
+// extreme is the maximum over entering planes, so at least t_i, and its exiting extreme at
+// most t_j: the scan reports a miss, which these two planes prove without the others.
+// cert packs i | j << 16 (from the pixel's central ray, see cert_kernel); ~0 = none.
+__device__ __forceinline__ bool camera_miss_certified(const float4 *__restrict__ planes,
+                                                      const float *__restrict__ cam_side, uint32_t cert, V3 rd) {
+    if (cert == 0xFFFFFFFFu) return false;
+    const uint32_t i = cert & 0xFFFFu, j = cert >> 16;
+    const float4 pi = planes[i], pj = planes[j];
+    const float di = (pi.x * rd.x + pi.y * rd.y) + pi.z * rd.z;
+    const float dj = (pj.x * rd.x + pj.y * rd.y) + pj.z * rd.z;
+    if (!(fabsf(di) > 1e-7f && di < 0.0f && fabsf(dj) > 1e-7f && dj > 0.0f)) return false;
+    const float ti = (-cam_side[i]) / di;
+    const float tj = (-cam_side[j]) / dj;
+    return ti > tj;
+}
+
 // One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
 // has ended (escape, Russian roulette, or the bounce cap).
 template <bool kAniso, bool kAbsorb>
@@ -218,8 +237,12 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     const V3 ro = ps.ray_o, rd = ps.ray_d;
     if (bounce == 0u) {
         // Camera rays share their origin: the origin's signed distances are staged per
-        // workgroup (same operations, same values).
-        clip_scan<true>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+        // workgroup (same operations, same values).  Most background rays are proved to
+        // miss by their pixel's certificate.
+        if (camera_miss_certified(planes, cam_side, ps.cert, rd))
+            outside = true;
+        else
+            clip_scan<true>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
     } else if (!ps.inside && exterior_miss(planes[ps.last_facet], ro, rd)) {
         // The ray left the stone's surface outwards (a refraction out, or a reflection off
         // the outside): it starts outside the facet it just met and moves away from it, so
@@ -707,8 +730,9 @@ __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block
 template <bool kAniso, bool kAbsorb, int kEnv, bool kInfo>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
-    uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples,
-    float *__restrict__ xyz_out, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
+    const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp,
+    uint32_t n_tuples, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
+    unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ float cam_side[GR_MAX_PLANES];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
@@ -725,7 +749,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
         const uint32_t local_pixel = t / spp;
         const uint32_t s = t - local_pixel * spp;
         PathState ps;
-        init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], sc);
+        init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
         while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
         }
         V3 xyz = v3(0.0f, 0.0f, 0.0f);
@@ -789,7 +813,8 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         const uint32_t local_pixel = t / spp;
         const uint32_t s = t - local_pixel * spp;
         PathState ps;
-        init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, J.ri35[s], sc);
+        init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, J.ri35[s],
+                  J.cert[J.pixel_begin + local_pixel], sc);
         while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
         }
         V3 xyz = v3(0.0f, 0.0f, 0.0f);
@@ -822,7 +847,7 @@ constexpr int kRecFields = 20;
 template <bool kAniso, bool kAbsorb, bool kInfo>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
-    uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, uint32_t rec_cap,
+    const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, uint32_t rec_cap,
     float *__restrict__ rec, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ float cam_side[GR_MAX_PLANES];
@@ -847,7 +872,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     uint32_t next = begin + 64u;  // wave-uniform
     if (busy) {
         const uint32_t lp = t / spp, s = t - lp * spp;
-        init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], sc);
+        init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], cert[pixel_begin + lp], sc);
     }
     while (__ballot(busy) != 0ull) {
         if (busy && bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
@@ -876,7 +901,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
         if (!busy && slot < end) {
             t = slot;
             const uint32_t lp = t / spp, s = t - lp * spp;
-            init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], sc);
+            init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], cert[pixel_begin + lp], sc);
             busy = true;
         }
     }
@@ -906,6 +931,35 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(const DevScene sc, uint3
     xyz_out[3 * (size_t)t + 0] = xyz.x;
     xyz_out[3 * (size_t)t + 1] = xyz.y;
     xyz_out[3 * (size_t)t + 2] = xyz.z;
+}
+
+// Per-pixel camera-ray miss certificates: the scan of the ray through the pixel's centre
+// (jitter 0); when it misses because its entering extreme exceeds its exiting one, the two
+// extreme planes are the pixel's certificate, which every sample of the pixel then checks
+// exactly for its own ray (camera_miss_certified).
+__global__ __launch_bounds__(kBlock) void cert_kernel(const DevScene sc, const float4 *__restrict__ planes_g,
+                                                      uint32_t n_pixels, uint32_t *__restrict__ cert) {
+    __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ float cam_side[GR_MAX_PLANES];
+    stage_planes(planes, cam_side, planes_g, sc);
+    __syncthreads();
+    const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;
+    if (pixel >= n_pixels) return;
+    const uint32_t x = pixel % sc.width, y = pixel / sc.width;
+    const float w = (float)sc.width, h = (float)sc.height;
+    const float aspect = w / h;
+    const float u = ((float)x / w - 0.5f) * 2.0f * aspect * sc.fov_tan;
+    const float v = (0.5f - (float)y / h) * 2.0f * sc.fov_tan;
+    const V3 rd = normalize(add(add(v3(sc.cam_f[0], sc.cam_f[1], sc.cam_f[2]),
+                                    mul(v3(sc.cam_r[0], sc.cam_r[1], sc.cam_r[2]), u)),
+                                mul(v3(sc.cam_u[0], sc.cam_u[1], sc.cam_u[2]), v)));
+    const V3 ro = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
+    float t_near = -1e30f, t_far = 1e30f;
+    int near_idx = -1, far_idx = -1;
+    bool outside = false;
+    clip_scan<true>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+    const bool proof = !outside && t_near > t_far && near_idx >= 0 && far_idx >= 0;
+    cert[pixel] = proof ? ((uint32_t)near_idx | ((uint32_t)far_idx << 16)) : 0xFFFFFFFFu;
 }
 
 // Each pixel's spp tuples summed in sample order and ADDED into the film
@@ -963,10 +1017,12 @@ hipError_t launch_plain(const gr::DevScene &sc, const gr::TraceArgs &a, hipStrea
     dim3 grid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
     if (a.info)
         hipLaunchKernelGGL((gr::trace_kernel<A, B, E, true>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
-                           a.ri35, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats);
+                           a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info,
+                           a.stats);
     else
         hipLaunchKernelGGL((gr::trace_kernel<A, B, E, false>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
-                           a.ri35, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats);
+                           a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info,
+                           a.stats);
     return hipGetLastError();
 }
 
@@ -976,12 +1032,12 @@ hipError_t launch_wave(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream
     dim3 grid((waves + gr::kBlock / 64 - 1) / (gr::kBlock / 64));
     if (a.info)
         hipLaunchKernelGGL((gr::trace_wave_kernel<A, B, true>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
-                           a.ri35, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec, a.info,
-                           a.stats);
+                           a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec,
+                           a.info, a.stats);
     else
         hipLaunchKernelGGL((gr::trace_wave_kernel<A, B, false>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
-                           a.ri35, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec, a.info,
-                           a.stats);
+                           a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec,
+                           a.info, a.stats);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     dim3 fgrid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
@@ -1044,6 +1100,13 @@ hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_
                                        hipStream_t stream) {
     if (n_slices == 0 || n_blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(gr::film_reduce_batch_kernel, dim3(n_blocks), dim3(gr::kBlock), 0, stream, slices, n_slices);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_cert(const gr::DevScene &sc, const float4 *planes, uint32_t *cert, hipStream_t stream) {
+    const uint32_t n = sc.width * sc.height;
+    hipLaunchKernelGGL(gr::cert_kernel, dim3((n + gr::kBlock - 1) / gr::kBlock), dim3(gr::kBlock), 0, stream, sc,
+                       planes, n, cert);
     return hipGetLastError();
 }
 

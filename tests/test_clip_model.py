@@ -178,3 +178,59 @@ def test_exterior_shortcut_only_fires_on_misses():
                 checked += 1
     assert checked > 3000
     assert fired > 0.6 * checked
+
+
+def certificate_of(planes, ro, rd, rng):
+    """cert_kernel: the extreme planes of a missing central ray, or None."""
+    t_near, near_idx, t_far, far_idx = F(-1e30), -1, F(1e30), -1
+    outside = False
+    for i, p in enumerate(planes):
+        denom = dot_rows(p[None, :], rd[None, :])[0]
+        side = F(p[3] + dot_rows(p[None, :], ro[None, :])[0])
+        if not abs(denom) > F(1e-7):
+            outside |= bool(side > 0)
+            continue
+        with np.errstate(divide="ignore"):
+            t = F(-side / denom)
+        if denom < 0 and t > t_near:
+            t_near, near_idx = t, i
+        if denom > 0 and t < t_far:
+            t_far, far_idx = t, i
+    if outside or not t_near > t_far or near_idx < 0 or far_idx < 0:
+        return None
+    return near_idx, far_idx
+
+
+def certified(planes, cert, ro, rd):
+    """camera_miss_certified for one ray."""
+    i, j = cert
+    di = dot_rows(planes[i][None, :], rd[None, :])[0]
+    dj = dot_rows(planes[j][None, :], rd[None, :])[0]
+    if not (abs(di) > F(1e-7) and di < 0 and abs(dj) > F(1e-7) and dj > 0):
+        return False
+    si = F(planes[i][3] + dot_rows(planes[i][None, :], ro[None, :])[0])
+    sj = F(planes[j][3] + dot_rows(planes[j][None, :], ro[None, :])[0])
+    return bool(F(-si / di) > F(-sj / dj))
+
+
+def test_camera_miss_certificates_only_prove_misses():
+    rng = np.random.default_rng(3)
+    proved = checked = 0
+    for name, planes in STONES:
+        planes = np.ascontiguousarray(planes, F)
+        o, d = random_rays(rng, 200)
+        for k in range(len(o)):
+            cert = certificate_of(planes, o[k], d[k], rng)
+            if cert is None:
+                continue
+            # The pixel's other samples: the central direction jittered by up to a few
+            # milliradians, some of them onto the stone.
+            for _ in range(10):
+                dj = (d[k] + rng.normal(size=3).astype(F) * F(10.0 ** rng.uniform(-4, -1.5))).astype(F)
+                dj = (dj / np.sqrt(np.dot(dj, dj))).astype(F)
+                checked += 1
+                if certified(planes, cert, o[k], dj):
+                    proved += 1
+                    assert not oracle_intersect(planes, o[k], dj)[0], (name, k)
+    assert checked > 1000
+    assert proved > 0.5 * checked
