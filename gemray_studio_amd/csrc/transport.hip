@@ -36,8 +36,8 @@ __device__ __forceinline__ float blackbody(float lambda_nm, const DevScene &sc) 
 
 // rgb_to_spectral_radiance (renderer/env_map_spectrum.rs:60-82).
 __device__ __forceinline__ float asym_gauss(float x, float mu, float lo, float hi) {
-    float sigma = x < mu ? lo : hi;
-    float t = (x - mu) / sigma;
+    const bool below = x < mu;
+    float t = div_by_const(x - mu, below ? lo : hi, below ? 1.0f / lo : 1.0f / hi);
     return expf(-0.5f * t * t);
 }
 __device__ __forceinline__ float rgb_to_spectral(V3 rgb, float l) {
@@ -99,6 +99,21 @@ __device__ __forceinline__ float comb_lambda(float hero_off, int k) {
     return 380.0f + (offset >= 400.0f ? offset - 400.0f : offset);
 }
 
+// Per-lane event counters, two 16-bit counts per register: a lane counts at most
+// kWaveTuples paths x GR_MAX_BOUNCES segments (32768) before flush_events.
+constexpr int kEvWords = (GR_EV_COUNT + 1) / 2;
+__device__ __forceinline__ void ev_inc(uint32_t *ev, int i) { ev[i >> 1] += 1u << (16 * (i & 1)); }
+
+// The per-path dispersion and absorption caches (refraction.rs:127-174): 24 floats per
+// lane in LDS, channel-major so that a wave's accesses hit distinct banks.  They are
+// filled lazily and read a few times per bounce; keeping them out of VGPRs leaves
+// registers to the Stokes state.
+constexpr int kBlock = 256;
+#define LC_NO(k) lcache[(k) * kBlock + threadIdx.x]
+#define LC_AO(k) lcache[(8 + (k)) * kBlock + threadIdx.x]
+#define LC_AE(k) lcache[(16 + (k)) * kBlock + threadIdx.x]
+constexpr int kCacheFloats = 24 * kBlock;
+
 // ---------------------------------------------------------------------------------
 // Per-path state.
 // ---------------------------------------------------------------------------------
@@ -107,7 +122,6 @@ struct PathState {
     V3 ray_o, ray_d, cur_k, prev_pn;
     Stokes st[8];
     float pdf[8];
-    float n_o[8], ao[8], ae[8];  // lazily filled caches (refraction.rs:127-174)
     float hero_off;
     uint32_t seed, bounce, last_facet, cert;
     uint32_t term_bounce, term_reason, fh, fate_h;
@@ -224,11 +238,12 @@ __device__ __forceinline__ bool camera_miss_certified(const float4 *__restrict__
 // has ended (escape, Russian roulette, or the bounce cap).
 template <bool kAniso, bool kAbsorb>
 __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, const float4 *__restrict__ planes,
-                                            const float *__restrict__ cam_side, uint32_t *ev) {
+                                            const float *__restrict__ cam_side, float *__restrict__ lcache,
+                                            uint32_t *ev) {
     const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
     const float dn = sc.birefringence_delta;
     const uint32_t bounce = ps.bounce;
-    ev[GR_EV_SEGMENT]++;
+    ev_inc(ev, GR_EV_SEGMENT);
 
     // ---- convex facet-plane clip (simd/slab.rs:94-152, intersect.rs:93-132) ----
     float t_near = -1e30f, t_far = 1e30f;
@@ -268,14 +283,14 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     if (!hit) {
         // Escape: the environment is sampled by finish_path (the path's only radiance
         // contribution; transport.rs:307-318).
-        ev[GR_EV_MISS]++;
+        ev_inc(ev, GR_EV_MISS);
         ps.escaped = true;
         fnv(ps.fh, 0xFFFFu);
         ps.term_bounce = bounce;
         ps.term_reason = 0u;
         return true;
     }
-    ev[GR_EV_HIT]++;
+    ev_inc(ev, GR_EV_HIT);
     if (!ps.have_n) {
         // DispersionModel::evaluate (optics/dispersion.rs:12-37) for the comb.
 #pragma unroll
@@ -296,7 +311,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 float l4 = l2 * l2;
                 n = sc.disp[0] + (sc.disp[1] / l2) + (sc.disp[2] / l4);
             }
-            ps.n_o[k] = n;
+            LC_NO(k) = n;
         }
         ps.have_n = true;
     }
@@ -317,7 +332,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     //      (raytracer/absorption.rs:44-89) ----
     V3 cpn = normalize_or_zero(cross(cur_k, normal));
     if (ps.have_prev && len2(cpn) > 1e-6f && len2(ps.prev_pn) > 1e-6f) {
-        ev[GR_EV_ROTATE]++;
+        ev_inc(ev, GR_EV_ROTATE);
         float cos_psi = clampf(dot(ps.prev_pn, cpn), -1.0f, 1.0f);
         float sin_psi = dot(cross(ps.prev_pn, cpn), normalize_or_zero(cur_k));
         // frame_rotation(atan2(sin_psi, cos_psi)) needs cos 2psi and sin 2psi only: the
@@ -340,7 +355,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     //      birefringence.rs:252-286, polarization.rs:172-192) ----
     if (inside) {
         normal = neg(normal);
-        ev[GR_EV_ABSORB]++;
+        ev_inc(ev, GR_EV_ABSORB);
         if (kAbsorb) {
             if (!ps.have_abs) {
                 // spectral_absorption per eigen-axis (raytracer/absorption.rs:28-30,
@@ -357,8 +372,8 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                         float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
                         se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
                     }
-                    ps.ao[k] = so;
-                    ps.ae[k] = se;
+                    LC_AO(k) = so;
+                    LC_AE(k) = se;
                 }
                 ps.have_abs = true;
             }
@@ -396,7 +411,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                     }
                 }
                 V3 el = v3(fmaf(cp, s_l.x, sp * p_l.x), fmaf(cp, s_l.y, sp * p_l.y), fmaf(cp, s_l.z, sp * p_l.z));
-                const float ao = ps.ao[k], ae = ps.ae[k];
+                const float ao = LC_AO(k), ae = LC_AE(k);
                 float qf_e = (ao * (el.x * el.x) + ao * (el.y * el.y)) + ae * (el.z * el.z);
                 float qf_a = (ao * (ea_l.x * ea_l.x) + ao * (ea_l.y * ea_l.y)) + ae * (ea_l.z * ea_l.z);
                 float qf_b = (ao * (eb_l.x * eb_l.x) + ao * (eb_l.y * eb_l.y)) + ae * (eb_l.z * eb_l.z);
@@ -417,7 +432,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         float theta_c;
         if (!inside) {
             // theta_c_for_bounce's two-step fixed point (refraction.rs:248-269).
-            float n_seed = ps.n_o[0];
+            float n_seed = LC_NO(0);
             float n_e_seed = n_seed + dn;
             float n_guess = n_seed;
             theta_c = 0.0f;
@@ -439,10 +454,10 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         sincosf(theta_c, &sin_t, &cos_t);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            n_med[k] = use_o ? ps.n_o[k] : effective_extraordinary_index_sc(ps.n_o[k], ps.n_o[k] + dn, sin_t, cos_t);
+            n_med[k] = use_o ? LC_NO(k) : effective_extraordinary_index_sc(LC_NO(k), LC_NO(k) + dn, sin_t, cos_t);
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) n_med[k] = ps.n_o[k];
+        for (int k = 0; k < 8; ++k) n_med[k] = LC_NO(k);
     }
     float s2t_ch[8];
 #pragma unroll
@@ -454,7 +469,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     }
     const float n1 = inside ? n_med[0] : 1.0f, n2 = inside ? 1.0f : n_med[0];
     const float sin2_t = s2t_ch[0];
-    const float n_o_hero = ps.n_o[0], n_e_hero = ps.n_o[0] + dn;
+    const float n_o_hero = LC_NO(0), n_e_hero = LC_NO(0) + dn;
 
     // ---- TIR / partial-reflect / refract dispatch (transport.rs:128-193) ----
     V3 new_k, new_s;
@@ -479,10 +494,10 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         // partial reflection then divides by the hero's r_unpol; TIR scales by 1.0, which
         // is exact.
         if (tir) {
-            ev[GR_EV_TIR]++;
+            ev_inc(ev, GR_EV_TIR);
             event = 1u;
         } else {
-            ev[GR_EV_REFLECT]++;
+            ev_inc(ev, GR_EV_REFLECT);
             event = 2u;
         }
         const float inv_r = tir ? 1.0f : 1.0f / r_unpol;
@@ -505,18 +520,18 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         new_k = sub(cur_k, mul(normal, 2.0f * dot(cur_k, normal)));
         new_s = new_k;
         if (kAniso && inside && ps.is_extra) {
-            ev[GR_EV_WALKOFF]++;
+            ev_inc(ev, GR_EV_WALKOFF);
             new_s = extraordinary_poynting_dir(new_k, c_axis, n_o_hero, n_e_hero);
         }
     } else {
         // apply_refract_bounce + apply_refract_channel (refraction.rs:748-1011).
-        ev[GR_EV_REFRACT]++;
+        ev_inc(ev, GR_EV_REFRACT);
         event = 3u;
         const float DIRECTION_MATCH_COS_TOL = 1.0f - 1e-6f;
         bool entering = kAniso && !inside;
         bool use_extra = ps.is_extra;
         if (entering) {
-            ev[GR_EV_ENTRY_ANISO]++;
+            ev_inc(ev, GR_EV_ENTRY_ANISO);
             use_extra = stream_draw(ps.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
         }
         float n2_dir = (entering && !use_extra) ? n_o_hero : n2;
@@ -526,14 +541,14 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         V3 refr = normalize(add(mul(cur_k, eta_dir), mul(normal, fmaf(eta_dir, cos_i, -cos_t_dir))));
         V3 final_dir = refr;
         if (entering && use_extra) {
-            ev[GR_EV_WALKOFF]++;
+            ev_inc(ev, GR_EV_WALKOFF);
             final_dir = extraordinary_poynting_dir(refr, c_axis, n_o_hero, n_e_hero);
         }
         float inv_t = 1.0f / (1.0f - r_unpol);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             float n1k = inside ? n_med[k] : 1.0f;
-            float n2k = (entering && !use_extra) ? ps.n_o[k] : (inside ? 1.0f : n_med[k]);
+            float n2k = (entering && !use_extra) ? LC_NO(k) : (inside ? 1.0f : n_med[k]);
             float q = n1k / n2k;
             float s2k = q * q * one_m_c2;
             bool alive = !(s2k > 1.0f);
@@ -541,7 +556,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             if (alive) {
                 V3 dk = normalize(add(mul(cur_k, q), mul(normal, fmaf(q, cos_i, -cos_t_k))));
                 if (entering && use_extra)
-                    dk = extraordinary_poynting_dir(dk, c_axis, ps.n_o[k], ps.n_o[k] + dn);
+                    dk = extraordinary_poynting_dir(dk, c_axis, LC_NO(k), LC_NO(k) + dn);
                 alive = dot(dk, final_dir) >= DIRECTION_MATCH_COS_TOL;
             }
             if (alive) {
@@ -587,7 +602,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
 
     // ---- Russian roulette (transport.rs:204-220) ----
     if (bounce > 4) {
-        ev[GR_EV_RR]++;
+        ev_inc(ev, GR_EV_RR);
         float max_i = 0.0f;
 #pragma unroll
         for (int k = 0; k < 8; ++k) max_i = fmaxf(max_i, intensity(ps.st[k]));
@@ -631,7 +646,8 @@ __device__ __forceinline__ V3 finish_escaped(V3 dir, const float *inten, const f
         for (int r = 0; r < 16; ++r) {
             float rd = fmaxf(dot(d, v3(sc.ring[r][0], sc.ring[r][1], sc.ring[r][2])), 0.0f);
             act[2 + r] = rd > 0.96f;
-            float spark = (rd - 0.96f) / 0.04f;
+            // Used only when rd > 0.96, where div_by_const is the correctly rounded quotient.
+            float spark = div_by_const(rd - 0.96f, 0.04f, 1.0f / 0.04f);
             terms[2 + r] = powi_cr<6>(spark) * 22.0f * sc.spot_mult * sc.exposure;
         }
 #pragma unroll
@@ -677,8 +693,6 @@ __device__ __forceinline__ uint32_t path_segments(const PathState &ps, const Dev
 // Kernels.
 // ---------------------------------------------------------------------------------
 
-constexpr int kBlock = 256;
-
 // Occupancy target of the trace kernels (waves per SIMD); 0 leaves the register budget to
 // the compiler.  A tuning parameter chosen by measurement: on the single-job workload
 // 3 waves/SIMD (168 VGPRs) ran 1.36x faster than the compiler's 2 and 1.03x faster than 4.
@@ -715,7 +729,7 @@ __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block
                                              unsigned long long *stats) {
 #pragma unroll
     for (int i = 0; i < GR_EV_COUNT; ++i) {
-        uint32_t v = ev[i];
+        uint32_t v = (ev[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if ((threadIdx.x & 63) == 0 && v) atomicAdd(&block_ev[i], v);
     }
@@ -735,14 +749,15 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ float cam_side[GR_MAX_PLANES];
+    __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
-    uint32_t ev[GR_EV_COUNT];
+    uint32_t ev[kEvWords];
 #pragma unroll
-    for (int i = 0; i < GR_EV_COUNT; ++i) ev[i] = 0u;
+    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
 
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t < n_tuples) {
@@ -750,7 +765,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
         const uint32_t s = t - local_pixel * spp;
         PathState ps;
         init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
-        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
+        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
         }
         V3 xyz = v3(0.0f, 0.0f, 0.0f);
         if (ps.escaped) {
@@ -796,6 +811,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     const BatchSlice *__restrict__ slices, uint32_t n_slices) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ float cam_side[GR_MAX_PLANES];
+    __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, false)];
     const DevScene &sc = *J.scene;
@@ -803,9 +819,9 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
-    uint32_t ev[GR_EV_COUNT];
+    uint32_t ev[kEvWords];
 #pragma unroll
-    for (int i = 0; i < GR_EV_COUNT; ++i) ev[i] = 0u;
+    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
 
     const uint32_t spp = J.spp;
     const uint32_t t = (blockIdx.x - J.trace_block_begin) * kBlock + threadIdx.x;
@@ -815,7 +831,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         PathState ps;
         init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, J.ri35[s],
                   J.cert[J.pixel_begin + local_pixel], sc);
-        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
+        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
         }
         V3 xyz = v3(0.0f, 0.0f, 0.0f);
         if (ps.escaped) {
@@ -851,14 +867,15 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     float *__restrict__ rec, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ float cam_side[GR_MAX_PLANES];
+    __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
-    uint32_t ev[GR_EV_COUNT];
+    uint32_t ev[kEvWords];
 #pragma unroll
-    for (int i = 0; i < GR_EV_COUNT; ++i) ev[i] = 0u;
+    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
@@ -875,7 +892,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
         init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], cert[pixel_begin + lp], sc);
     }
     while (__ballot(busy) != 0ull) {
-        if (busy && bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, ev)) {
+        if (busy && bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
             // Path ended: escape record (and per-tuple info).
             float *r = rec + t;
             r[0 * (size_t)rec_cap] = ps.ray_d.x;
