@@ -329,6 +329,17 @@ class GpuBackend:
         self._require()
         check(self._L.gr_serve_set_tick_tuples(self._ctx, tuples), "gr_serve_set_tick_tuples")
 
+    def probe(self, fn: int, inputs: np.ndarray) -> np.ndarray:
+        """Evaluates device function fn (_lib.GR_PROBE_*) on rows of up to 9 float32 inputs;
+        returns (n, 3) float32 outputs (a test probe of the kernel's own functions)."""
+        self._require()
+        x = np.zeros((len(inputs), 9), np.float32)
+        x[:, :inputs.shape[1]] = inputs
+        out = np.zeros((len(inputs), 3), np.float32)
+        check(self._L.gr_probe(self._ctx, fn, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(x),
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "gr_probe")
+        return out
+
     def set_trace_mode(self, mode: str) -> None:
         """"plain" (one lane per tuple) or "wavefront" (persistent lanes + finish pass)."""
         self._require()

@@ -201,17 +201,8 @@ __device__ __forceinline__ float cranley_patterson(float x, float offset) {
 
 // ---- Uniaxial birefringence (optics/birefringence.rs:27-131) ---------------------------
 
-__device__ __forceinline__ float effective_extraordinary_index(float n_o, float n_e,
-                                                               float theta) {
-    if (fabsf(n_o - n_e) < 1e-5f) return n_o;
-    float sin_t, cos_t;
-    sincosf(theta, &sin_t, &cos_t);
-    float os = n_o * sin_t;
-    float denom_sq = fmaf(n_e * cos_t, n_e * cos_t, os * os);
-    if (denom_sq <= 1e-8f) return n_o;
-    return (n_o * n_e) / sqrtf(denom_sq);
-}
-// The same index with sin/cos of theta supplied by the caller (shared across channels).
+// effective_extraordinary_index (birefringence.rs:27-40) with sin and cos of theta supplied
+// by the caller (shared across channels).
 __device__ __forceinline__ float effective_extraordinary_index_sc(float n_o, float n_e,
                                                                   float sin_t, float cos_t) {
     if (fabsf(n_o - n_e) < 1e-5f) return n_o;
@@ -220,26 +211,93 @@ __device__ __forceinline__ float effective_extraordinary_index_sc(float n_o, flo
     if (denom_sq <= 1e-8f) return n_o;
     return (n_o * n_e) / sqrtf(denom_sq);
 }
-__device__ __forceinline__ float walk_off_angle(float n_o, float n_e, float theta) {
-    if (fabsf(n_o - n_e) < 1e-5f) return 0.0f;
-    float n_o2 = n_o * n_o;
-    float n_e2 = n_e * n_e;
-    float sin_t, cos_t;
-    sincosf(theta, &sin_t, &cos_t);
-    float tan_rho = ((n_o2 - n_e2) * sin_t * cos_t) /
-                    fmaxf(fmaf(n_e2 * cos_t, cos_t, n_o2 * sin_t * sin_t), 1e-6f);
-    return atanf(tan_rho);
+// effective_extraordinary_index(n_o, n_e, acos(|cos theta|)) (refraction.rs:248-269,
+// birefringence.rs:27-40) from the cosine: sin theta = sqrt(1 - cos^2).
+__device__ __forceinline__ void theta_sc_from_cos(float cos_theta, float *sin_t, float *cos_t) {
+    const float ct = fabsf(clampf(cos_theta, -1.0f, 1.0f));
+    *cos_t = ct;
+    *sin_t = sqrtf(fmaxf(fmaf(ct, -ct, 1.0f), 0.0f));
 }
+__device__ __forceinline__ float neff_from_cos(float n_o, float n_e, float cos_theta) {
+    float st, ct;
+    theta_sc_from_cos(cos_theta, &st, &ct);
+    return effective_extraordinary_index_sc(n_o, n_e, st, ct);
+}
+
+// MuellerMatrix::frame_rotation(signed_frame_rotation_psi(prev, cur, axis))
+// (raytracer/absorption.rs:44-54, polarization.rs:95-103) needs cos 2psi and sin 2psi
+// only: the double-angle forms of the (unnormalized) pair (cos psi, sin psi) replace
+// atan2 + sincos.
+__device__ __forceinline__ void frame_rotation_cs(V3 prev, V3 cur, V3 axis, float *c2, float *s2) {
+    const float cos_psi = clampf(dot(prev, cur), -1.0f, 1.0f);
+    const float sin_psi = dot(cross(prev, cur), normalize_or_zero(axis));
+    const float cc = cos_psi * cos_psi, ss = sin_psi * sin_psi;
+    const float r2 = cc + ss;
+    *c2 = 1.0f;
+    *s2 = 0.0f;
+    if (r2 > 0.0f) {
+        const float inv = 1.0f / r2;
+        *c2 = (cc - ss) * inv;
+        *s2 = (2.0f * cos_psi * sin_psi) * inv;
+    }
+}
+
+// The polarization quadratic form e . A . e of pleochroic absorption
+// (raytracer/absorption.rs:115-216, birefringence.rs:252-270, polarization.rs:172-192) for
+// the field e = cos psi s + sin psi p, psi = atan2(U, Q) / 2, in the absorption tensor's
+// principal frame where A = diag(ao, ao, ae).  With S_i, P_i the principal-frame
+// components of s and p, and cos^2 psi = (1 + cos 2psi) / 2, sin^2 psi = (1 - cos 2psi) / 2,
+// 2 cos psi sin psi = sin 2psi:
+//   e.A.e = sum_i w_i (S_i^2 + P_i^2) / 2 + cos 2psi sum_i w_i (S_i^2 - P_i^2) / 2
+//           + sin 2psi sum_i w_i S_i P_i,   w = (ao, ao, ae),
+// and cos 2psi = Q / r, sin 2psi = U / r with r = sqrt(Q^2 + U^2) (psi = 0 when r = 0).
+// The six channel-invariant sums are formed once per interior segment.
+struct QfInv {
+    float h_xy, h_z, d_xy, d_z, x_xy, x_z;
+};
+__device__ __forceinline__ QfInv qf_invariants(V3 s_l, V3 p_l) {
+    QfInv v;
+    const float sx = s_l.x * s_l.x, sy = s_l.y * s_l.y, sz = s_l.z * s_l.z;
+    const float px = p_l.x * p_l.x, py = p_l.y * p_l.y, pz = p_l.z * p_l.z;
+    v.h_xy = ((sx + px) + (sy + py)) * 0.5f;
+    v.h_z = (sz + pz) * 0.5f;
+    v.d_xy = ((sx - px) + (sy - py)) * 0.5f;
+    v.d_z = (sz - pz) * 0.5f;
+    v.x_xy = fmaf(s_l.x, p_l.x, s_l.y * p_l.y);
+    v.x_z = s_l.z * p_l.z;
+    return v;
+}
+__device__ __forceinline__ float qf_channel(const QfInv &v, float q, float u, float ao, float ae) {
+    const float h = fmaf(ao, v.h_xy, ae * v.h_z);
+    const float r2 = fmaf(q, q, u * u);
+    if (!(r2 > 0.0f)) return h + fmaf(ao, v.d_xy, ae * v.d_z);
+    const float inv_r = 1.0f / sqrtf(r2);
+    const float c2 = q * inv_r, s2 = u * inv_r;
+    const float d = fmaf(ao, v.d_xy, ae * v.d_z);
+    const float x = fmaf(ao, v.x_xy, ae * v.x_z);
+    return fmaf(c2, d, fmaf(s2, x, h));
+}
+
+// extraordinary_poynting_dir (birefringence.rs:74-100) without the angle round trips.  The
+// reference takes theta = acos(|cos theta|), the walk-off delta = atan(tan rho) from sin and
+// cos of theta (walk_off_angle, :42-55) and then sin and cos of delta; here
+// sin theta = sqrt(1 - cos^2) and cos delta = 1 / sqrt(1 + tan^2 rho),
+// sin delta = tan rho cos delta.  |delta| < 1e-5 is tested as |tan rho| < 1e-5 (atan is
+// monotonic and atan(x) = x to single precision there).
 __device__ __forceinline__ V3 extraordinary_poynting_dir(V3 w, V3 c, float n_o, float n_e) {
-    float cos_theta = clampf(dot(w, c), -1.0f, 1.0f);
-    float theta = acosf(fabsf(cos_theta));
-    float delta = walk_off_angle(n_o, n_e, theta);
-    if (fabsf(delta) < 1e-5f) return w;
+    const float cos_theta = clampf(dot(w, c), -1.0f, 1.0f);
+    if (fabsf(n_o - n_e) < 1e-5f) return w;
+    const float ct = fabsf(cos_theta);
+    const float st = sqrtf(fmaxf(fmaf(ct, -ct, 1.0f), 0.0f));
+    const float n_o2 = n_o * n_o;
+    const float n_e2 = n_e * n_e;
+    const float tan_rho = ((n_o2 - n_e2) * st * ct) / fmaxf(fmaf(n_e2 * ct, ct, n_o2 * st * st), 1e-6f);
+    if (fabsf(tan_rho) < 1e-5f) return w;
     V3 c_proj = normalize_or_zero(sub(c, mul(w, cos_theta)));
     if (len2(c_proj) < 1e-6f) return w;
-    float sign = cos_theta >= 0.0f ? 1.0f : -1.0f;
-    float sd, cd;
-    sincosf(delta, &sd, &cd);
+    const float sign = cos_theta >= 0.0f ? 1.0f : -1.0f;
+    const float cd = 1.0f / sqrtf(fmaf(tan_rho, tan_rho, 1.0f));
+    const float sd = tan_rho * cd;
     return normalize(sub(mul(w, cd), mul(mul(c_proj, sign), sd)));
 }
 __device__ __forceinline__ void stable_orthonormal_basis(V3 n, V3 *t_out, V3 *b_out) {

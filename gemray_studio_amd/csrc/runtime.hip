@@ -548,6 +548,23 @@ gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, g
     return GR_OK;
 }
 
+gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out) {
+    if (!ctx || !in || !out || fn < 0 || fn >= GR_PROBE_COUNT || n == 0 || n > (1u << 24)) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GR_CHECK(hipSetDevice(ctx->device));
+    float *d_in = nullptr, *d_out = nullptr;
+    gr_status st = GR_OK;
+    if (hipMalloc(&d_in, n * 9 * sizeof(float)) != hipSuccess || hipMalloc(&d_out, n * 3 * sizeof(float)) != hipSuccess ||
+        hipMemcpyAsync(d_in, in, n * 9 * sizeof(float), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        gr_launch_probe(fn, d_in, d_out, (uint32_t)n, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(out, d_out, n * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        st = GR_ERR_DEVICE;
+    if (d_in) (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
+    return st;
+}
+
 gr_status gr_last_stats(gr_ctx *ctx, gr_stats *out) {
     if (!ctx || !out) return GR_ERR_INVALID;
     std::lock_guard<std::mutex> g(ctx->mu);

@@ -333,18 +333,8 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     V3 cpn = normalize_or_zero(cross(cur_k, normal));
     if (ps.have_prev && len2(cpn) > 1e-6f && len2(ps.prev_pn) > 1e-6f) {
         ev_inc(ev, GR_EV_ROTATE);
-        float cos_psi = clampf(dot(ps.prev_pn, cpn), -1.0f, 1.0f);
-        float sin_psi = dot(cross(ps.prev_pn, cpn), normalize_or_zero(cur_k));
-        // frame_rotation(atan2(sin_psi, cos_psi)) needs cos 2psi and sin 2psi only: the
-        // double-angle forms of the (unnormalized) pair replace atan2 + sincos.
-        float cc = cos_psi * cos_psi, ss = sin_psi * sin_psi;
-        float r2 = cc + ss;
-        float c2 = 1.0f, s2 = 0.0f;
-        if (r2 > 0.0f) {
-            float inv = 1.0f / r2;
-            c2 = (cc - ss) * inv;
-            s2 = (2.0f * cos_psi * sin_psi) * inv;
-        }
+        float c2, s2;
+        frame_rotation_cs(ps.prev_pn, cpn, cur_k, &c2, &s2);
 #pragma unroll
         for (int k = 0; k < 8; ++k) ps.st[k] = apply_rotation(ps.st[k], c2, s2);
     }
@@ -387,36 +377,20 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             V3 p_hat = cross(kk, s_hat);
             V3 ea_l = v3(dot(afr(sc, 0), ea), dot(afr(sc, 1), ea), dot(afr(sc, 2), ea));
             V3 eb_l = v3(dot(afr(sc, 0), eb), dot(afr(sc, 1), eb), dot(afr(sc, 2), eb));
-            // The field direction e = cos(psi) s + sin(psi) p is a unit vector, so its
-            // principal-frame components are cos(psi) (a_i.s) + sin(psi) (a_i.p).
             V3 s_l = v3(dot(afr(sc, 0), s_hat), dot(afr(sc, 1), s_hat), dot(afr(sc, 2), s_hat));
             V3 p_l = v3(dot(afr(sc, 0), p_hat), dot(afr(sc, 1), p_hat), dot(afr(sc, 2), p_hat));
+            const QfInv qi = qf_invariants(s_l, p_l);
+            // The unpolarized term midpoint(ea.A.ea, eb.A.eb) with its channel-invariant
+            // parts gathered: ao (|ea_xy|^2 + |eb_xy|^2) / 2 + ae (ea_z^2 + eb_z^2) / 2.
+            const float u_xy = ((ea_l.x * ea_l.x + ea_l.y * ea_l.y) + (eb_l.x * eb_l.x + eb_l.y * eb_l.y)) * 0.5f;
+            const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
             float spl = hit_t * sc.absorption_path_scale;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                // psi = atan2(u, q) / 2 (polarization.rs:58-60) through the half-angle
-                // identities, in the cancellation-free branch for the sign of q.
-                float cp = 1.0f, sp = 0.0f;
-                float m = fmaxf(fabsf(ps.st[k].q), fabsf(ps.st[k].u));
-                if (m > 0.0f) {
-                    float inv_m = 1.0f / m;
-                    float qn = ps.st[k].q * inv_m, un = ps.st[k].u * inv_m;
-                    float r = sqrtf(qn * qn + un * un);
-                    if (qn >= 0.0f) {
-                        cp = sqrtf((r + qn) / (2.0f * r));
-                        sp = un / (2.0f * r * cp);
-                    } else {
-                        sp = copysignf(sqrtf((r - qn) / (2.0f * r)), un);
-                        cp = fabsf(un) / (2.0f * r * fabsf(sp));
-                    }
-                }
-                V3 el = v3(fmaf(cp, s_l.x, sp * p_l.x), fmaf(cp, s_l.y, sp * p_l.y), fmaf(cp, s_l.z, sp * p_l.z));
                 const float ao = LC_AO(k), ae = LC_AE(k);
-                float qf_e = (ao * (el.x * el.x) + ao * (el.y * el.y)) + ae * (el.z * el.z);
-                float qf_a = (ao * (ea_l.x * ea_l.x) + ao * (ea_l.y * ea_l.y)) + ae * (ea_l.z * ea_l.z);
-                float qf_b = (ao * (eb_l.x * eb_l.x) + ao * (eb_l.y * eb_l.y)) + ae * (eb_l.z * eb_l.z);
+                const float qf_e = qf_channel(qi, ps.st[k].q, ps.st[k].u, ao, ae);
+                const float unpol = fmaf(ao, u_xy, ae * u_z);
                 float p = clampf(degree_of_polarization(ps.st[k]), 0.0f, 1.0f);
-                float unpol = midpoint(qf_a, qf_b);
                 float alpha = fmaf(p, qf_e - unpol, unpol);
                 ps.st[k] = scale(ps.st[k], exp_poly(-alpha * spl));
             }
@@ -429,32 +403,39 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     float one_m_c2 = fmaf(cos_i, -cos_i, 1.0f);
     float n_med[8];
     if (kAniso) {
-        float theta_c;
-        if (!inside) {
-            // theta_c_for_bounce's two-step fixed point (refraction.rs:248-269).
-            float n_seed = LC_NO(0);
-            float n_e_seed = n_seed + dn;
-            float n_guess = n_seed;
-            theta_c = 0.0f;
-            for (int it = 0; it < 2; ++it) {
-                float eta_g = 1.0f / n_guess;
-                float sin2_g = eta_g * eta_g * one_m_c2;
-                if (sin2_g > 1.0f) break;
-                float cos_t_g = sqrtf(fmaxf(1.0f - sin2_g, 0.0f));
-                V3 wave = normalize(add(mul(cur_k, eta_g), mul(normal, fmaf(eta_g, cos_i, -cos_t_g))));
-                theta_c = acosf(fabsf(clampf(dot(wave, c_axis), -1.0f, 1.0f)));
-                n_guess = effective_extraordinary_index(n_seed, n_e_seed, theta_c);
-            }
-        } else {
-            theta_c = acosf(fabsf(clampf(dot(cur_k, c_axis), -1.0f, 1.0f)));
-        }
-        bool use_o = inside && !ps.is_extra;
-        // theta_c is shared by the 8 channels: one sincos serves all of them.
-        float sin_t, cos_t;
-        sincosf(theta_c, &sin_t, &cos_t);
+        const bool use_o = inside && !ps.is_extra;
+        if (use_o) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            n_med[k] = use_o ? LC_NO(k) : effective_extraordinary_index_sc(LC_NO(k), LC_NO(k) + dn, sin_t, cos_t);
+            for (int k = 0; k < 8; ++k) n_med[k] = LC_NO(k);
+        } else {
+            // theta_c, the angle between the wave normal and the optic axis, enters only
+            // through its sine and cosine.  The reference forms theta_c = acos(|cos|) and
+            // takes sin/cos of it (refraction.rs:248-269, birefringence.rs:27-40); here they
+            // come from the cosine directly, sin = sqrt(1 - cos^2), shared by the 8 channels.
+            float ct = 1.0f, st = 0.0f;  // theta_c = 0
+            if (!inside) {
+                // theta_c_for_bounce's two-step fixed point.
+                const float n_seed = LC_NO(0);
+                const float n_e_seed = n_seed + dn;
+                float n_guess = n_seed;
+                for (int it = 0; it < 2; ++it) {
+                    float eta_g = 1.0f / n_guess;
+                    float sin2_g = eta_g * eta_g * one_m_c2;
+                    if (sin2_g > 1.0f) break;
+                    float cos_t_g = sqrtf(fmaxf(1.0f - sin2_g, 0.0f));
+                    V3 wave = normalize(add(mul(cur_k, eta_g), mul(normal, fmaf(eta_g, cos_i, -cos_t_g))));
+                    ct = fabsf(clampf(dot(wave, c_axis), -1.0f, 1.0f));
+                    st = sqrtf(fmaxf(fmaf(ct, -ct, 1.0f), 0.0f));
+                    n_guess = effective_extraordinary_index_sc(n_seed, n_e_seed, st, ct);
+                }
+            } else {
+                ct = fabsf(clampf(dot(cur_k, c_axis), -1.0f, 1.0f));
+                st = sqrtf(fmaxf(fmaf(ct, -ct, 1.0f), 0.0f));
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                n_med[k] = effective_extraordinary_index_sc(LC_NO(k), LC_NO(k) + dn, st, ct);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) n_med[k] = LC_NO(k);
@@ -979,6 +960,43 @@ __global__ __launch_bounds__(kBlock) void cert_kernel(const DevScene sc, const f
     cert[pixel] = proof ? ((uint32_t)near_idx | ((uint32_t)far_idx << 16)) : 0xFFFFFFFFu;
 }
 
+// Per-function probes: one device function of the transport path per lane, on
+// caller-given inputs (9 floats in, 3 out per item), for the per-function ULP tests.
+__global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__restrict__ in,
+                                                       float *__restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float *a = in + 9 * (size_t)i;
+    float *o = out + 3 * (size_t)i;
+    o[0] = o[1] = o[2] = 0.0f;
+    switch (fn) {
+        case GR_PROBE_NEFF:
+            o[0] = neff_from_cos(a[0], a[1], a[2]);
+            break;
+        case GR_PROBE_POYNTING: {
+            const V3 d = extraordinary_poynting_dir(v3(a[0], a[1], a[2]), v3(a[3], a[4], a[5]), a[6], a[7]);
+            o[0] = d.x;
+            o[1] = d.y;
+            o[2] = d.z;
+            break;
+        }
+        case GR_PROBE_TIR_RETARDATION:
+            tir_retardation_cs(a[0], a[1], a[2], &o[0], &o[1]);
+            break;
+        case GR_PROBE_FRAME_ROTATION:
+            frame_rotation_cs(v3(a[0], a[1], a[2]), v3(a[3], a[4], a[5]), v3(a[6], a[7], a[8]), &o[0], &o[1]);
+            break;
+        case GR_PROBE_FIELD_QFORM: {
+            // a = (q, u, S.x, S.y, S.z, P.x, P.y, P.z, ao); ae = 1 - ao.
+            const QfInv qi = qf_invariants(v3(a[2], a[3], a[4]), v3(a[5], a[6], a[7]));
+            o[0] = qf_channel(qi, a[0], a[1], a[8], 1.0f - a[8]);
+            break;
+        }
+        default:
+            break;
+    }
+}
+
 // Each pixel's spp tuples summed in sample order and ADDED into the film
 // (frame.rs:799-814: sum = 0; sum += xyz[s] for s in order; accum[pixel] += sum).
 __global__ __launch_bounds__(kBlock) void film_reduce_kernel(const float *__restrict__ xyz,
@@ -1117,6 +1135,12 @@ hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_
                                        hipStream_t stream) {
     if (n_slices == 0 || n_blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(gr::film_reduce_batch_kernel, dim3(n_blocks), dim3(gr::kBlock), 0, stream, slices, n_slices);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_probe(int fn, const float *in, float *out, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(gr::probe_kernel, dim3((n + gr::kBlock - 1) / gr::kBlock), dim3(gr::kBlock), 0, stream, fn,
+                       in, out, n);
     return hipGetLastError();
 }
 

@@ -164,6 +164,18 @@ gr_status gr_trace_samples(gr_ctx *ctx, const gr_scene *scene, uint32_t pixel_be
  * from the origin, vertical field of view in degrees. */
 gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, gr_camera *out);
 
+/* Per-function probe (no reference counterpart; for the per-function ULP tests): evaluates
+ * one device function of the transport kernel on n items, 9 input floats and 3 output
+ * floats per item, and copies the outputs to out (3 * n floats).
+ *   GR_PROBE_NEFF             in (n_o, n_e, cos theta)          out (n_eff)
+ *   GR_PROBE_POYNTING         in (w.xyz, c.xyz, n_o, n_e)        out (S.xyz)
+ *   GR_PROBE_TIR_RETARDATION  in (n1, cos_i, sin_i)              out (cos delta, sin delta)
+ *   GR_PROBE_FRAME_ROTATION   in (prev.xyz, cur.xyz, axis.xyz)   out (cos 2psi, sin 2psi)
+ *   GR_PROBE_FIELD_QFORM      in (Q, U, S.xyz, P.xyz, ao)        out (e.A.e with ae = 1 - ao) */
+enum { GR_PROBE_NEFF = 0, GR_PROBE_POYNTING = 1, GR_PROBE_TIR_RETARDATION = 2, GR_PROBE_FRAME_ROTATION = 3,
+       GR_PROBE_FIELD_QFORM = 4, GR_PROBE_COUNT = 5 };
+gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out);
+
 /* Statistics of the last gr_accumulate / gr_trace_samples / gr_render call. */
 gr_status gr_last_stats(gr_ctx *ctx, gr_stats *out);
 

@@ -1250,6 +1250,29 @@ extern "C" int or_intersect(const float *planes, int n_planes, const float origi
 extern "C" float or_effective_extraordinary_index(float n_o, float n_e, float theta) {
     return effective_extraordinary_index(n_o, n_e, theta);
 }
+// The theta_c_for_bounce call site (refraction.rs:248-269): theta = acos(|cos|), then
+// effective_extraordinary_index.
+extern "C" float or_neff_from_cos(float n_o, float n_e, float c) {
+    return effective_extraordinary_index(n_o, n_e, tf_acos(std::fabs(clampf(c, -1.0f, 1.0f))));
+}
+// rotate_stokes_to_plane_of_incidence's matrix (raytracer/absorption.rs:71-89,
+// polarization.rs:95-101): frame_rotation(signed_frame_rotation_psi(prev, cur, axis)) as
+// (cos 2psi, sin 2psi), its Q<-Q and Q<-U entries.
+extern "C" float or_signed_frame_rotation_psi(const float prev[3], const float cur[3], const float axis[3]);
+extern "C" void or_frame_rotation_cs(const float prev[3], const float cur[3], const float axis[3], float out[2]) {
+    M4 m = frame_rotation(or_signed_frame_rotation_psi(prev, cur, axis));
+    out[0] = m.c[5];
+    out[1] = m.c[9];
+}
+// The polarization quadratic form on the reference's angle route (polarization.rs:58-60,
+// 172-192; birefringence.rs:252-270): psi = atan2(U, Q) / 2, e = cos psi s + sin psi p,
+// here with s and p given by their components in the tensor's principal frame, where
+// A = diag(ao, ao, ae).
+extern "C" float or_field_qform(float q, float u, const float s_l[3], const float p_l[3], float ao, float ae) {
+    float psi = 0.5f * tf_atan2(u, q);
+    V3 e = add(mul(load3(s_l), tf_cos(psi)), mul(load3(p_l), tf_sin(psi)));
+    return dot(v3(ao, ao, ae), mulv(e, e));
+}
 extern "C" float or_walk_off_angle(float n_o, float n_e, float theta) {
     return walk_off_angle(n_o, n_e, theta);
 }

@@ -104,6 +104,9 @@ int or_intersect(const float *planes, int n_planes, const float origin[3], const
                  float *t, float normal[3], int *facet);
 float or_effective_extraordinary_index(float n_o, float n_e, float theta);
 float or_walk_off_angle(float n_o, float n_e, float theta);
+float or_neff_from_cos(float n_o, float n_e, float c);
+void or_frame_rotation_cs(const float prev[3], const float cur[3], const float axis[3], float out[2]);
+float or_field_qform(float q, float u, const float s_l[3], const float p_l[3], float ao, float ae);
 void or_extraordinary_poynting_dir(const float w[3], const float c[3], float n_o, float n_e,
                                    float out[3]);
 float or_tir_phase_delta(float n1, float cos_i, float sin_i);

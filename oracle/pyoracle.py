@@ -130,6 +130,12 @@ def _declare(L):
     L.or_effective_extraordinary_index.restype = ctypes.c_float
     L.or_walk_off_angle.argtypes = [ctypes.c_float] * 3
     L.or_walk_off_angle.restype = ctypes.c_float
+    L.or_neff_from_cos.argtypes = [ctypes.c_float] * 3
+    L.or_neff_from_cos.restype = ctypes.c_float
+    L.or_frame_rotation_cs.argtypes = [fp, fp, fp, fp]
+    L.or_frame_rotation_cs.restype = None
+    L.or_field_qform.argtypes = [ctypes.c_float, ctypes.c_float, fp, fp, ctypes.c_float, ctypes.c_float]
+    L.or_field_qform.restype = ctypes.c_float
     L.or_extraordinary_poynting_dir.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float, fp]
     L.or_tir_phase_delta.argtypes = [ctypes.c_float] * 3
     L.or_tir_phase_delta.restype = ctypes.c_float
