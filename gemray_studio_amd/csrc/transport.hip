@@ -180,30 +180,44 @@ __device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_
 // the other planes give); here the flag is kept and the scan runs on, to the same outcome.
 // kCam: the origin is the camera's, whose per-plane sides are staged in cam_side.
 template <bool kCam>
+__device__ __forceinline__ void clip_plane(float4 p, float side_cam, int i, V3 ro, V3 rd, float &t_near, int &near_idx,
+                                           float &t_far, int &far_idx, bool &outside) {
+    const float denom = (p.x * rd.x + p.y * rd.y) + p.z * rd.z;
+    const float side = kCam ? side_cam : p.w + ((p.x * ro.x + p.y * ro.y) + p.z * ro.z);
+    const float a = (-side) * __builtin_amdgcn_rcpf(denom);
+    const float slack = fmaf(fabsf(a), 0x1p-19f, 0x1p-120f);
+    const bool steep = fabsf(denom) > 1e-7f;
+    outside = outside || (!steep && side > 0.0f);
+    const bool cand_near = steep && denom < 0.0f && a + slack > t_near;
+    const bool cand_far = steep && denom > 0.0f && a - slack < t_far;
+    if (cand_near || cand_far) {
+        const float t = (-side) / denom;
+        if (cand_near && t > t_near) {
+            t_near = t;
+            near_idx = i;
+        }
+        if (cand_far && t < t_far) {
+            t_far = t;
+            far_idx = i;
+        }
+    }
+}
+// Planes are read four at a time (the staged array is padded to a multiple of four with
+// planes 0.x + 0.y + 0.z - 1 <= 0, which are never steep, never outside and never win), so
+// one LDS wait covers four planes.
+template <bool kCam>
 __device__ __forceinline__ void clip_scan(const float4 *__restrict__ planes, const float *__restrict__ cam_side,
                                           uint32_t n, V3 ro, V3 rd, float &t_near, int &near_idx, float &t_far,
                                           int &far_idx, bool &outside) {
-    for (int i = 0; i < (int)n; ++i) {
-        const float4 p = planes[i];
-        const float denom = (p.x * rd.x + p.y * rd.y) + p.z * rd.z;
-        const float side = kCam ? cam_side[i] : p.w + ((p.x * ro.x + p.y * ro.y) + p.z * ro.z);
-        const float a = (-side) * __builtin_amdgcn_rcpf(denom);
-        const float slack = fmaf(fabsf(a), 0x1p-19f, 0x1p-120f);
-        const bool steep = fabsf(denom) > 1e-7f;
-        outside = outside || (!steep && side > 0.0f);
-        const bool cand_near = steep && denom < 0.0f && a + slack > t_near;
-        const bool cand_far = steep && denom > 0.0f && a - slack < t_far;
-        if (cand_near || cand_far) {
-            const float t = (-side) / denom;
-            if (cand_near && t > t_near) {
-                t_near = t;
-                near_idx = i;
-            }
-            if (cand_far && t < t_far) {
-                t_far = t;
-                far_idx = i;
-            }
-        }
+    const int n4 = (int)((n + 3u) & ~3u);
+    for (int i = 0; i < n4; i += 4) {
+        const float4 p0 = planes[i], p1 = planes[i + 1], p2 = planes[i + 2], p3 = planes[i + 3];
+        float4 cs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (kCam) cs = *reinterpret_cast<const float4 *>(cam_side + i);
+        clip_plane<kCam>(p0, cs.x, i, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane<kCam>(p1, cs.y, i + 1, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane<kCam>(p2, cs.z, i + 2, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane<kCam>(p3, cs.w, i + 3, ro, rd, t_near, near_idx, t_far, far_idx, outside);
     }
 }
 
@@ -693,12 +707,14 @@ __device__ __forceinline__ uint32_t path_segments(const PathState &ps, const Dev
 constexpr uint32_t kWaveTuples = 64u * GR_WAVE_ROUNDS;
 
 // Facet planes into LDS, with the camera origin's signed distance to each
-// (p.w + n . origin, the clip's operations).
+// (p.w + n . origin, the clip's operations), padded to a multiple of four with inert
+// planes (normal 0, d = -1) for the scan's four-plane reads.
 __device__ __forceinline__ void stage_planes(float4 *lds, float *cam_side, const float4 *__restrict__ g,
                                              const DevScene &sc) {
     const V3 o = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
-    for (uint32_t i = threadIdx.x; i < sc.n_planes; i += kBlock) {
-        const float4 p = g[i];
+    const uint32_t n4 = (sc.n_planes + 3u) & ~3u;
+    for (uint32_t i = threadIdx.x; i < n4; i += kBlock) {
+        const float4 p = i < sc.n_planes ? g[i] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
         lds[i] = p;
         cam_side[i] = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
     }
@@ -729,7 +745,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     uint32_t n_tuples, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
     unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ float cam_side[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes(planes, cam_side, planes_g, sc);
@@ -791,7 +807,7 @@ template <bool kAniso, bool kAbsorb, int kEnv>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     const BatchSlice *__restrict__ slices, uint32_t n_slices) {
     __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ float cam_side[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, false)];
@@ -847,7 +863,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, uint32_t rec_cap,
     float *__restrict__ rec, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ float cam_side[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes(planes, cam_side, planes_g, sc);
@@ -938,7 +954,7 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(const DevScene sc, uint3
 __global__ __launch_bounds__(kBlock) void cert_kernel(const DevScene sc, const float4 *__restrict__ planes_g,
                                                       uint32_t n_pixels, uint32_t *__restrict__ cert) {
     __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ float cam_side[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     stage_planes(planes, cam_side, planes_g, sc);
     __syncthreads();
     const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;

@@ -234,3 +234,17 @@ def test_camera_miss_certificates_only_prove_misses():
                     assert not oracle_intersect(planes, o[k], dj)[0], (name, k)
     assert checked > 1000
     assert proved > 0.5 * checked
+
+
+def test_padding_planes_are_inert():
+    # stage_planes pads the plane array to a multiple of four with (0, 0, 0, -1).
+    rng = np.random.default_rng(4)
+    for name, planes in STONES:
+        planes = np.ascontiguousarray(planes, F)
+        pad = (-len(planes)) % 4 or 4
+        padded = np.concatenate([planes, np.tile(np.array([[0, 0, 0, -1]], F), (pad, 1))]).astype(F)
+        o, d = random_rays(rng, 500)
+        a = model_scan(planes, o, d, np.random.default_rng(9))
+        b = model_scan(padded, o, d, np.random.default_rng(9))
+        for x, y in zip(a[:3], b[:3]):
+            assert np.array_equal(x, y), name
