@@ -110,7 +110,8 @@ EXPORTS = [
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
     "gr_serve_set_tick_tuples", "gr_probe",
 ]
-GR_PROBE_NEFF, GR_PROBE_POYNTING, GR_PROBE_TIR_RETARDATION, GR_PROBE_FRAME_ROTATION, GR_PROBE_FIELD_QFORM = range(5)
+(GR_PROBE_NEFF, GR_PROBE_POYNTING, GR_PROBE_TIR_RETARDATION, GR_PROBE_FRAME_ROTATION, GR_PROBE_FIELD_QFORM,
+ GR_PROBE_EXP, GR_PROBE_CMF) = range(7)
 
 _lib = None
 

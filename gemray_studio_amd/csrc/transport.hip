@@ -38,7 +38,7 @@ __device__ __forceinline__ float blackbody(float lambda_nm, const DevScene &sc) 
 __device__ __forceinline__ float asym_gauss(float x, float mu, float lo, float hi) {
     const bool below = x < mu;
     float t = div_by_const(x - mu, below ? lo : hi, below ? 1.0f / lo : 1.0f / hi);
-    return expf(-0.5f * t * t);
+    return exp_lobe(-0.5f * t * t);
 }
 __device__ __forceinline__ float rgb_to_spectral(V3 rgb, float l) {
     float r = fmaxf(rgb.x, 0.0f), g = fmaxf(rgb.y, 0.0f), b = fmaxf(rgb.z, 0.0f);
@@ -406,7 +406,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 const float unpol = fmaf(ao, u_xy, ae * u_z);
                 float p = clampf(degree_of_polarization(ps.st[k]), 0.0f, 1.0f);
                 float alpha = fmaf(p, qf_e - unpol, unpol);
-                ps.st[k] = scale(ps.st[k], exp_poly(-alpha * spl));
+                ps.st[k] = scale(ps.st[k], exp_lobe(-alpha * spl));
             }
         }
     }
@@ -1002,6 +1002,16 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__re
         case GR_PROBE_FRAME_ROTATION:
             frame_rotation_cs(v3(a[0], a[1], a[2]), v3(a[3], a[4], a[5]), v3(a[6], a[7], a[8]), &o[0], &o[1]);
             break;
+        case GR_PROBE_EXP:
+            o[0] = exp_lobe(a[0]);
+            break;
+        case GR_PROBE_CMF: {
+            const V3 c = cmf_x8_channel(a[0]);
+            o[0] = c.x;
+            o[1] = c.y;
+            o[2] = c.z;
+            break;
+        }
         case GR_PROBE_FIELD_QFORM: {
             // a = (q, u, S.x, S.y, S.z, P.x, P.y, P.z, ao); ae = 1 - ao.
             const QfInv qi = qf_invariants(v3(a[2], a[3], a[4]), v3(a[5], a[6], a[7]));

@@ -171,9 +171,11 @@ gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, g
  *   GR_PROBE_POYNTING         in (w.xyz, c.xyz, n_o, n_e)        out (S.xyz)
  *   GR_PROBE_TIR_RETARDATION  in (n1, cos_i, sin_i)              out (cos delta, sin delta)
  *   GR_PROBE_FRAME_ROTATION   in (prev.xyz, cur.xyz, axis.xyz)   out (cos 2psi, sin 2psi)
- *   GR_PROBE_FIELD_QFORM      in (Q, U, S.xyz, P.xyz, ao)        out (e.A.e with ae = 1 - ao) */
+ *   GR_PROBE_FIELD_QFORM      in (Q, U, S.xyz, P.xyz, ao)        out (e.A.e with ae = 1 - ao)
+ *   GR_PROBE_EXP              in (x)                             out (e^x as the lobes evaluate it)
+ *   GR_PROBE_CMF              in (lambda_nm)                     out (xbar, ybar, zbar) */
 enum { GR_PROBE_NEFF = 0, GR_PROBE_POYNTING = 1, GR_PROBE_TIR_RETARDATION = 2, GR_PROBE_FRAME_ROTATION = 3,
-       GR_PROBE_FIELD_QFORM = 4, GR_PROBE_COUNT = 5 };
+       GR_PROBE_FIELD_QFORM = 4, GR_PROBE_EXP = 5, GR_PROBE_CMF = 6, GR_PROBE_COUNT = 7 };
 gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out);
 
 /* Statistics of the last gr_accumulate / gr_trace_samples / gr_render call. */

@@ -11,6 +11,10 @@ physical inputs and compared with the reference's route through the transcendent
   TIR retardation        rational cos/sin delta from tan(delta/2)  8e-7
   frame rotation         double angle of (cos psi, sin psi)        8e-7
   field quadratic form   cos/sin 2psi from (Q, U) / |(Q, U)|       8e-7 x max(ao, ae)
+  lobe exponential       native exp (as the reference's shader)    2.4e-7 + |x| 1.2e-7 rel.
+  CIE matching functions lobes on the native exp                   2e-6 of the largest
+                                                                   component (the reference
+                                                                   CPU uses a Cephes exp)
 
 Budgets are a few ulp of the outputs' unit scale: the identities are exact in real
 arithmetic, and the reference's own route rounds at every transcendental.
@@ -116,3 +120,32 @@ def test_field_quadratic_form(gpu):
                                      float(ae[i])) for i in range(N)], F)
     scale = np.maximum(ao, ae)
     assert (np.abs(out - ref) / scale).max() <= 8e-7
+
+
+def test_lobe_exponential(gpu):
+    rng = np.random.default_rng(6)
+    x = np.concatenate([-(10.0 ** rng.uniform(-6, 1.7, N - 8)), np.zeros(8)]).astype(F)
+    out = gpu.probe(_lib.GR_PROBE_EXP, x[:, None])[:, 0].astype(np.float64)
+    exact = np.exp(x.astype(np.float64))
+    rel = np.abs(out - exact) / exact
+    assert np.all(rel <= 2.4e-7 + np.abs(x) * 1.2e-7)
+    # Against the reference CPU's Cephes polynomial (the oracle's exp_f32x8).
+    ref = np.zeros(len(x), F)
+    for i in range(0, len(x) - len(x) % 8, 8):
+        L.or_exp_f32x8(po.fptr(x[i:i + 8].copy()), po.fptr(ref[i:i + 8]))
+    m = (x > -5.0)[: len(x) - len(x) % 8]
+    assert ulp_dist(out[: len(m)][m].astype(F), ref[: len(m)][m]).max() <= 8
+
+
+def test_cie_matching_functions(gpu):
+    # Every comb wavelength the kernel can see lies in [380, 780).
+    rng = np.random.default_rng(7)
+    lam = rng.uniform(380.0, 780.0, N).astype(F)
+    out = gpu.probe(_lib.GR_PROBE_CMF, lam[:, None])
+    ref = np.zeros((N, 3), F)
+    for i in range(0, N, 8):
+        buf = np.zeros(24, F)
+        L.or_cie_1931_cmf_x8(po.fptr(lam[i:i + 8].copy()), po.fptr(buf))
+        ref[i:i + 8] = buf.reshape(8, 3)
+    err = np.abs(out.astype(np.float64) - ref) / np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1e-3)
+    assert err.max() <= 2e-6
