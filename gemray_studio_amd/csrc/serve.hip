@@ -70,6 +70,7 @@ struct Job {
     unsigned long long *d_stats = nullptr;
     // Progress (owned by the scheduler thread).
     uint64_t next_pixel = 0;
+    uint32_t inflight = 0;  // pieces in ticks that have not completed
     // Results.
     std::vector<float> film;
     unsigned long long events[GR_EV_COUNT] = {0};
@@ -81,6 +82,21 @@ struct Job {
 struct Piece {
     Job *job;
     uint64_t pixel_begin, n_pixels;
+};
+
+constexpr int kTickSlots = 2;
+
+struct TickSlot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float *staging = nullptr;
+    uint64_t staging_cap = 0;
+    BatchSlice *d_slices = nullptr;
+    uint32_t slices_cap = 0;
+    std::vector<BatchSlice> h_slices;  // kept until the tick completes
+    std::vector<Piece> pieces;
+    uint32_t launches = 0;
+    bool busy = false;
 };
 
 }  // namespace
@@ -97,13 +113,12 @@ struct ServeState {
     std::unordered_map<uint64_t, std::shared_ptr<Job>> jobs;
     uint64_t budget = 0, in_use = 0;
     uint64_t tick_tuples = kDefaultTickTuples;
-    // Device resources of the loop.
+    // Device resources of the loop: a control stream for uploads and read-backs, and two
+    // tick slots with their own streams and buffers so that one tick is queued while the
+    // other runs.
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float *staging = nullptr;
-    uint64_t staging_cap = 0;
-    BatchSlice *d_slices = nullptr;
-    uint32_t slices_cap = 0;
+    TickSlot slot[kTickSlots];
+    uint64_t ticks_launched = 0, ticks_done = 0;
     size_t rr = 0;  // round-robin start of the next tick
     // Metrics since the last reset.
     gr_serve_metrics m{};
@@ -184,21 +199,21 @@ gr_status finish_job(ServeState &S, Job &j) {
     return GR_OK;
 }
 
-gr_status ensure_staging(ServeState &S, uint64_t tuples, uint32_t slices) {
-    if (tuples > S.staging_cap) {
-        if (S.staging) SV_CHECK(hipFree(S.staging));
-        S.staging = nullptr;
-        S.staging_cap = 0;
-        SV_CHECK(hipMalloc(&S.staging, tuples * 3 * sizeof(float)));
-        S.staging_cap = tuples;
+gr_status ensure_staging(TickSlot &T, uint64_t tuples, uint32_t slices) {
+    if (tuples > T.staging_cap) {
+        if (T.staging) SV_CHECK(hipFree(T.staging));
+        T.staging = nullptr;
+        T.staging_cap = 0;
+        SV_CHECK(hipMalloc(&T.staging, tuples * 3 * sizeof(float)));
+        T.staging_cap = tuples;
     }
-    if (slices > S.slices_cap) {
-        if (S.d_slices) SV_CHECK(hipFree(S.d_slices));
-        S.d_slices = nullptr;
-        S.slices_cap = 0;
+    if (slices > T.slices_cap) {
+        if (T.d_slices) SV_CHECK(hipFree(T.d_slices));
+        T.d_slices = nullptr;
+        T.slices_cap = 0;
         uint32_t cap = std::max<uint32_t>(slices, 64);
-        SV_CHECK(hipMalloc(&S.d_slices, cap * sizeof(BatchSlice)));
-        S.slices_cap = cap;
+        SV_CHECK(hipMalloc(&T.d_slices, cap * sizeof(BatchSlice)));
+        T.slices_cap = cap;
     }
     return GR_OK;
 }
@@ -239,15 +254,17 @@ std::vector<Piece> plan_tick(ServeState &S, const std::vector<Job *> &jobs, uint
     return pieces;
 }
 
-// Traces one tick's pieces and adds them into their films; returns the tick's GPU time.
-gr_status run_tick(ServeState &S, std::vector<Piece> &pieces, float *gpu_ms, uint32_t *launches) {
+// Queues one tick's pieces on slot T: the batched trace launches (one per material class)
+// and the film reduce, bracketed by the slot's events.  Returns without waiting.
+gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces) {
     std::stable_sort(pieces.begin(), pieces.end(),
                      [](const Piece &a, const Piece &b) { return a.job->cls < b.job->cls; });
     uint64_t tuples = 0;
     for (const Piece &p : pieces) tuples += p.n_pixels * p.job->spp;
-    gr_status st = ensure_staging(S, std::max<uint64_t>(tuples, 1), (uint32_t)pieces.size());
+    gr_status st = ensure_staging(T, std::max<uint64_t>(tuples, 1), (uint32_t)pieces.size());
     if (st != GR_OK) return st;
-    std::vector<BatchSlice> sl(pieces.size());
+    std::vector<BatchSlice> &sl = T.h_slices;
+    sl.assign(pieces.size(), BatchSlice{});
     uint64_t off = 0;
     uint32_t reduce_blocks = 0;
     for (size_t i = 0; i < pieces.size(); ++i) {
@@ -258,7 +275,7 @@ gr_status run_tick(ServeState &S, std::vector<Piece> &pieces, float *gpu_ms, uin
         b.planes = j.d_planes;
         b.ri35 = j.d_ri35;
         b.cert = j.d_cert;
-        b.xyz = S.staging + 3 * off;
+        b.xyz = T.staging + 3 * off;
         b.film = j.d_film;
         b.stats = j.d_stats;
         b.pixel_begin = (uint32_t)p.pixel_begin;
@@ -283,16 +300,19 @@ gr_status run_tick(ServeState &S, std::vector<Piece> &pieces, float *gpu_ms, uin
         r.blocks += (uint32_t)(((uint64_t)sl[i].n_pixels * sl[i].spp + kBlockTuples - 1) / kBlockTuples);
         r.count++;
     }
-    SV_CHECK(hipMemcpyAsync(S.d_slices, sl.data(), sl.size() * sizeof(BatchSlice), hipMemcpyHostToDevice, S.stream));
-    SV_CHECK(hipEventRecord(S.ev0, S.stream));
+    T.launches = 0;
+    SV_CHECK(hipMemcpyAsync(T.d_slices, sl.data(), sl.size() * sizeof(BatchSlice), hipMemcpyHostToDevice, T.stream));
+    SV_CHECK(hipEventRecord(T.ev0, T.stream));
     for (const ClassRange &r : classes) {
-        SV_CHECK(gr_launch_trace_batch(S.d_slices + r.first, r.count, r.blocks, r.cls, S.stream));
-        ++*launches;
+        SV_CHECK(gr_launch_trace_batch(T.d_slices + r.first, r.count, r.blocks, r.cls, T.stream));
+        ++T.launches;
     }
-    SV_CHECK(gr_launch_film_reduce_batch(S.d_slices, (uint32_t)sl.size(), reduce_blocks, S.stream));
-    SV_CHECK(hipEventRecord(S.ev1, S.stream));
-    SV_CHECK(hipEventSynchronize(S.ev1));
-    SV_CHECK(hipEventElapsedTime(gpu_ms, S.ev0, S.ev1));
+    SV_CHECK(gr_launch_film_reduce_batch(T.d_slices, (uint32_t)sl.size(), reduce_blocks, T.stream));
+    ++T.launches;
+    SV_CHECK(hipEventRecord(T.ev1, T.stream));
+    for (const Piece &p : pieces) p.job->inflight++;
+    T.pieces = std::move(pieces);
+    T.busy = true;
     return GR_OK;
 }
 
@@ -333,16 +353,57 @@ void retire(ServeState &S, Job &j, int state, gr_status err) {
     }
 }
 
+// Waits for the tick on slot T, books its pieces, and reads back and retires the jobs it
+// completed (all pieces traced) or that were cancelled with nothing left in flight.
+gr_status complete_tick(ServeState &S, TickSlot &T) {
+    gr_status st = GR_OK;
+    float gpu_ms = 0.0f;
+    if (hipEventSynchronize(T.ev1) != hipSuccess || hipEventElapsedTime(&gpu_ms, T.ev0, T.ev1) != hipSuccess)
+        st = GR_ERR_DEVICE;
+    std::vector<Job *> touched;
+    for (const Piece &p : T.pieces) {
+        p.job->inflight--;
+        touched.push_back(p.job);
+    }
+    std::vector<Job *> finished;
+    for (Job *j : touched)
+        if (j->inflight == 0 && j->next_pixel == j->n_pixels && !j->cancel_req) finished.push_back(j);
+    // Films and counters of finished jobs are read back before the lock is taken; until
+    // retire() marks them done no API call reads either.
+    std::vector<gr_status> fin(finished.size(), st);
+    if (st == GR_OK)
+        for (size_t i = 0; i < finished.size(); ++i) fin[i] = finish_job(S, *finished[i]);
+    std::lock_guard<std::mutex> g(S.mu);
+    for (const Piece &p : T.pieces) p.job->paths_done += p.n_pixels * p.job->spp;
+    S.m.ticks++;
+    S.m.launches += T.launches;
+    S.m.busy_ms += gpu_ms;
+    S.ticks_done++;
+    T.busy = false;
+    T.pieces.clear();
+    if (st != GR_OK) return st;
+    for (size_t i = 0; i < finished.size(); ++i)
+        retire(S, *finished[i], fin[i] == GR_OK ? GR_JOB_DONE : GR_JOB_FAILED, fin[i]);
+    S.active.erase(std::remove_if(S.active.begin(), S.active.end(),
+                                  [](const std::shared_ptr<Job> &j) { return j->state != GR_JOB_RUNNING; }),
+                   S.active.end());
+    S.cv_done.notify_all();
+    return GR_OK;
+}
+
 void loop(ServeState *Sp) {
     ServeState &S = *Sp;
     (void)hipSetDevice(S.ctx->device);
     for (;;) {
+        const bool in_flight = S.slot[0].busy || S.slot[1].busy;
         std::vector<std::shared_ptr<Job>> admit;
         {
             std::unique_lock<std::mutex> lk(S.mu);
-            S.cv_work.wait(lk, [&] { return S.stop || !S.queue.empty() || !S.active.empty(); });
+            if (!in_flight)
+                S.cv_work.wait(lk, [&] { return S.stop || !S.queue.empty() || !S.active.empty(); });
             if (S.stop) return;
-            // Cancelled jobs leave first.
+            // Cancelled jobs leave: queued ones at once, running ones once none of their
+            // pieces is in flight.
             for (auto it = S.queue.begin(); it != S.queue.end();) {
                 if ((*it)->cancel_req) {
                     (*it)->state = GR_JOB_CANCELLED;
@@ -354,7 +415,7 @@ void loop(ServeState *Sp) {
                 }
             }
             for (auto it = S.active.begin(); it != S.active.end();) {
-                if ((*it)->cancel_req) {
+                if ((*it)->cancel_req && (*it)->inflight == 0) {
                     retire(S, **it, GR_JOB_CANCELLED, GR_OK);
                     it = S.active.erase(it);
                 } else {
@@ -386,46 +447,60 @@ void loop(ServeState *Sp) {
             S.m.admitted++;
             S.active.push_back(j);
         }
-        std::vector<Job *> running;
+        // Plan the next tick over the jobs that still have pixels to hand out.
+        std::vector<Job *> schedulable;
         uint64_t tick_tuples = 0;
         {
             std::lock_guard<std::mutex> g(S.mu);
-            for (auto &j : S.active) running.push_back(j.get());
+            for (auto &j : S.active)
+                if (!j->cancel_req && j->next_pixel < j->n_pixels) schedulable.push_back(j.get());
             tick_tuples = S.tick_tuples;
         }
-        if (running.empty()) continue;
-        std::vector<Piece> pieces = plan_tick(S, running, tick_tuples);
-        float gpu_ms = 0.0f;
-        uint32_t launches = 0;
-        gr_status st = run_tick(S, pieces, &gpu_ms, &launches);
-        // Films and counters of finished jobs are read back before the lock is taken; until
-        // retire() marks them done no API call reads either.
-        std::vector<Job *> finished;
-        for (Job *j : running)
-            if (j->next_pixel == j->n_pixels) finished.push_back(j);
-        std::vector<gr_status> fin(finished.size(), st);
-        if (st == GR_OK)
-            for (size_t i = 0; i < finished.size(); ++i) fin[i] = finish_job(S, *finished[i]);
-        {
-            std::lock_guard<std::mutex> g(S.mu);
-            for (const Piece &p : pieces) p.job->paths_done += p.n_pixels * p.job->spp;
-            S.m.ticks++;
-            S.m.launches += launches + 1;
-            S.m.busy_ms += gpu_ms;
-            if (st != GR_OK) {
-                // A device error fails every running job.
-                for (auto &j : S.active) retire(S, *j, GR_JOB_FAILED, st);
-                S.active.clear();
-            } else {
-                for (size_t i = 0; i < finished.size(); ++i)
-                    retire(S, *finished[i], fin[i] == GR_OK ? GR_JOB_DONE : GR_JOB_FAILED, fin[i]);
-                S.active.erase(std::remove_if(S.active.begin(), S.active.end(),
-                                              [](const std::shared_ptr<Job> &j) { return j->state != GR_JOB_RUNNING; }),
-                               S.active.end());
+        TickSlot &next = S.slot[S.ticks_launched % kTickSlots];
+        gr_status st = GR_OK;
+        if (!schedulable.empty()) {
+            // The slot's previous tick must be complete before its buffers are reused.
+            if (next.busy) st = complete_tick(S, next);
+            if (st == GR_OK) {
+                st = launch_tick(next, plan_tick(S, schedulable, tick_tuples));
+                if (st == GR_OK) S.ticks_launched++;
             }
+        } else {
+            // Nothing new to launch: complete the oldest tick in flight.
+            TickSlot &oldest = S.slot[S.ticks_done % kTickSlots];
+            if (oldest.busy) st = complete_tick(S, oldest);
+        }
+        if (st != GR_OK) {
+            // A device error fails every running job.
+            for (TickSlot &T : S.slot) {
+                if (T.busy) (void)hipStreamSynchronize(T.stream);
+                T.busy = false;
+                T.pieces.clear();
+            }
+            std::lock_guard<std::mutex> g(S.mu);
+            for (auto &j : S.active) {
+                j->inflight = 0;
+                retire(S, *j, GR_JOB_FAILED, st);
+            }
+            S.active.clear();
             S.cv_done.notify_all();
         }
     }
+}
+
+// Streams, events and buffers of the loop (all optional: partially created state too).
+void release_device(ServeState &S) {
+    for (TickSlot &T : S.slot) {
+        if (T.stream) (void)hipStreamSynchronize(T.stream);
+        if (T.staging) (void)hipFree(T.staging);
+        if (T.d_slices) (void)hipFree(T.d_slices);
+        if (T.ev0) (void)hipEventDestroy(T.ev0);
+        if (T.ev1) (void)hipEventDestroy(T.ev1);
+        if (T.stream) (void)hipStreamDestroy(T.stream);
+        T = TickSlot{};
+    }
+    if (S.stream) (void)hipStreamDestroy(S.stream);
+    S.stream = nullptr;
 }
 
 // Creates the loop on first use.
@@ -442,11 +517,12 @@ gr_status get_serve(gr_ctx *ctx, ServeState **out) {
             return GR_ERR_DEVICE;
         }
         S->budget = ctx->hbm_budget ? ctx->hbm_budget : free_b / 4 * 3;
-        if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess) {
-            if (S->ev0) (void)hipEventDestroy(S->ev0);
-            if (S->ev1) (void)hipEventDestroy(S->ev1);
-            if (S->stream) (void)hipStreamDestroy(S->stream);
+        bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess;
+        for (TickSlot &T : S->slot)
+            ok = ok && hipStreamCreateWithFlags(&T.stream, hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreate(&T.ev0) == hipSuccess && hipEventCreate(&T.ev1) == hipSuccess;
+        if (!ok) {
+            release_device(*S);
             delete S;
             return GR_ERR_DEVICE;
         }
@@ -489,12 +565,10 @@ void serve_shutdown(gr_ctx *ctx) {
     if (S->th.joinable()) S->th.join();
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(S->stream);
+    for (TickSlot &T : S->slot)
+        if (T.stream) (void)hipStreamSynchronize(T.stream);
     for (auto &j : S->active) free_job_device(*j);
-    if (S->staging) (void)hipFree(S->staging);
-    if (S->d_slices) (void)hipFree(S->d_slices);
-    (void)hipEventDestroy(S->ev0);
-    (void)hipEventDestroy(S->ev1);
-    (void)hipStreamDestroy(S->stream);
+    release_device(*S);
     delete S;
 }
 

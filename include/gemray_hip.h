@@ -237,7 +237,7 @@ typedef struct {
     uint64_t queued, running;      /* now                                                 */
     uint64_t paths, segments;      /* of completed jobs                                   */
     uint64_t ticks, launches;
-    double busy_ms;                /* GPU time of the ticks (HIP events)                  */
+    double busy_ms;                /* summed tick durations (HIP events; two overlap)     */
     double wall_ms;                /* first admission -> last completion                  */
     double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;
     uint64_t hbm_in_use, hbm_budget, tick_tuples;

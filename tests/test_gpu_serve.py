@@ -110,11 +110,12 @@ def test_job_counters_and_metrics(gpu):
 
 
 def test_late_small_job_overtakes_a_running_large_job(gpu):
-    big = B.SceneRef(B.camera_new(0.35, 0.28, 5.0, 18.0), 512, 512, scenes.oval_brilliant(),
+    # About a second of work for the large job; the small one arrives 10 ms in.
+    big = B.SceneRef(B.camera_new(0.35, 0.28, 5.0, 18.0), 1024, 1024, scenes.oval_brilliant(),
                      scenes.pleochroic_sapphire(3), 32, B.HdrMap(scenes.procedural_env(5, 128, 64)))
     small = _mix()[0]
-    a = gpu.submit(big, 0, 256)
-    time.sleep(0.02)
+    a = gpu.submit(big, 0, 1024)
+    time.sleep(0.01)
     b = gpu.submit(*small)
     assert gpu.wait(b, 60000) == _lib.GR_OK
     assert gpu.poll(a) == _lib.GR_PENDING, "the small job should join while the large one runs"
