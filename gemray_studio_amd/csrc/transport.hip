@@ -169,36 +169,40 @@ __device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_
 }
 
 // The convex facet-plane scan (simd/slab.rs:94-152, intersect.rs:93-132).  The reference
-// divides t = -side / denom for every plane and keeps the first strict maximum (entering)
-// and minimum (exiting).  Here a = -side * rcp(denom) (v_rcp_f32, 1 ulp) screens each
-// plane: |a - t| <= |a| * 2^-19, so a plane whose exact t could beat the running extreme
-// always passes the screen, and only those planes take the correctly rounded division and
-// the exact comparison (a 2^-120 absolute slack covers denormal products).  The selected
-// planes and t values are exactly the reference's.  The screen is branch-free; a lane
-// takes the one branch only when a plane may improve one of its extremes.  A parallel
-// plane with the origin outside makes the reference stop scanning (the ray misses whatever
-// the other planes give); here the flag is kept and the scan runs on, to the same outcome.
+// divides t = -side / denom for every plane and keeps the first strict maximum over
+// entering planes (denom < -1e-7) and the first strict minimum over exiting planes
+// (denom > 1e-7).  A plane can only change an extreme t_ext if its exact quotient is
+// beyond it: -side / denom > t_near with denom < 0, or < t_far with denom > 0, which are
+// both side + t_ext * denom > 0.  One fma evaluates that sum with a single rounding, which
+// keeps its sign (the operands' magnitudes keep the sum far from the denormal range), so
+// the screen is exact: a plane that fails it has a rounded quotient no better than t_ext,
+// and a plane that passes takes the correctly rounded division and the reference's strict
+// comparison.  A parallel plane (|denom| <= 1e-7) with the origin outside makes the
+// reference stop scanning (the ray misses whatever the other planes give); here the flag
+// is kept and the scan runs on, to the same outcome.
 // kCam: the origin is the camera's, whose per-plane sides are staged in cam_side.
 template <bool kCam>
 __device__ __forceinline__ void clip_plane(float4 p, float side_cam, int i, V3 ro, V3 rd, float &t_near, int &near_idx,
                                            float &t_far, int &far_idx, bool &outside) {
     const float denom = (p.x * rd.x + p.y * rd.y) + p.z * rd.z;
     const float side = kCam ? side_cam : p.w + ((p.x * ro.x + p.y * ro.y) + p.z * ro.z);
-    const float a = (-side) * __builtin_amdgcn_rcpf(denom);
-    const float slack = fmaf(fabsf(a), 0x1p-19f, 0x1p-120f);
     const bool steep = fabsf(denom) > 1e-7f;
-    outside = outside || (!steep && side > 0.0f);
-    const bool cand_near = steep && denom < 0.0f && a + slack > t_near;
-    const bool cand_far = steep && denom > 0.0f && a - slack < t_far;
-    if (cand_near || cand_far) {
-        const float t = (-side) / denom;
-        if (cand_near && t > t_near) {
-            t_near = t;
-            near_idx = i;
-        }
-        if (cand_far && t < t_far) {
-            t_far = t;
-            far_idx = i;
+    const bool entering = denom < 0.0f;
+    const float t_ext = entering ? t_near : t_far;
+    if (steep ? fmaf(t_ext, denom, side) > 0.0f : side > 0.0f) {
+        if (!steep) {
+            outside = true;
+        } else {
+            const float t = (-side) / denom;
+            if (entering) {
+                if (t > t_near) {
+                    t_near = t;
+                    near_idx = i;
+                }
+            } else if (t < t_far) {
+                t_far = t;
+                far_idx = i;
+            }
         }
     }
 }

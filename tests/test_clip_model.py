@@ -4,14 +4,14 @@ oracle's intersect_polyhedron restatement (CPU only).
 The kernel (gemray_studio_amd/csrc/transport.hip, clip_scan / exterior_miss) departs from
 the reference's scan in two ways that must not change any result:
 
-* the screen: a = -side * rcp(denom) with a 1-ulp reciprocal, and a plane takes the exact
-  division only if a +- (|a| 2^-19 + 2^-120) could beat the running extreme;
+* the screen: a plane takes the exact division only if side + t_ext * denom > 0 (one fma,
+  exact sign), i.e. its exact quotient is beyond the running extreme t_ext;
 * the exterior shortcut: a ray that starts outside the facet it just met and moves away
   from it is a miss without the scan.
 
-The model below reproduces the kernel's float32 operations (the same association order;
-the reciprocal is perturbed by -1, 0 or +1 ulp per plane, covering v_rcp_f32's error) and
-the tests require the oracle's exact hit, t bits and facet on randomized rays.
+The model below reproduces the kernel's float32 operations (the same association order)
+and the tests require the oracle's exact hit, t bits and facet on randomized rays,
+including planes duplicated with d moved by 0-2 ulp (ties and near-ties).
 """
 import ctypes
 
@@ -38,14 +38,10 @@ def dot_rows(p, v):
     return ((p[..., 0] * v[..., 0] + p[..., 1] * v[..., 1]) + p[..., 2] * v[..., 2]).astype(F)
 
 
-def fma32(x, y, z):
-    # x*y is exact in float64 for float32 operands; one rounding to float32 after the add
-    # (the double-rounding corner cases only move the slack by an ulp of itself).
-    return (x.astype(np.float64) * y.astype(np.float64) + np.float64(z)).astype(F)
-
-
-def model_scan(planes, ro, rd, rng):
-    """clip_scan<false> for a batch of rays (ro, rd: (R, 3) float32)."""
+def model_scan(planes, ro, rd, rng=None):
+    """clip_scan<false> for a batch of rays (ro, rd: (R, 3) float32).  The screen's fma
+    sign is evaluated in float64, where t_ext * denom is exact and adding side keeps the
+    exact sum's sign."""
     R = ro.shape[0]
     t_near = np.full(R, F(-1e30))
     t_far = np.full(R, F(1e30))
@@ -56,21 +52,18 @@ def model_scan(planes, ro, rd, rng):
     for i, p in enumerate(planes):
         denom = dot_rows(np.broadcast_to(p, (R, 4)), rd)
         side = (p[3] + dot_rows(np.broadcast_to(p, (R, 4)), ro)).astype(F)
+        steep = np.abs(denom) > F(1e-7)
+        entering = denom < 0
+        t_ext = np.where(entering, t_near, t_far)
+        g = t_ext.astype(np.float64) * denom.astype(np.float64) + side.astype(np.float64)
+        cand = np.where(steep, g > 0, side > 0)
+        outside |= cand & ~steep
         with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
-            rcp = (F(1.0) / denom).astype(F)
-            step = rng.integers(-1, 2, size=R)
-            rcp = np.where(step > 0, np.nextafter(rcp, F(np.inf)),
-                           np.where(step < 0, np.nextafter(rcp, F(-np.inf)), rcp)).astype(F)
-            a = ((-side) * rcp).astype(F)
-            slack = fma32(np.abs(a), F(2.0 ** -19), F(2.0 ** -120))
-            steep = np.abs(denom) > F(1e-7)
-            outside |= (~steep) & (side > 0)
-            cand_near = steep & (denom < 0) & ((a + slack).astype(F) > t_near)
-            cand_far = steep & (denom > 0) & ((a - slack).astype(F) < t_far)
             t = ((-side) / denom).astype(F)
-        divisions += int(np.count_nonzero(cand_near | cand_far))
-        upd_n = cand_near & (t > t_near)
-        upd_f = cand_far & (t < t_far)
+        work = cand & steep
+        divisions += int(np.count_nonzero(work))
+        upd_n = work & entering & (t > t_near)
+        upd_f = work & ~entering & (t < t_far)
         t_near = np.where(upd_n, t, t_near)
         near_idx = np.where(upd_n, i, near_idx)
         t_far = np.where(upd_f, t, t_far)
