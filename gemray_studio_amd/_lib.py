@@ -108,7 +108,7 @@ EXPORTS = [
     "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_camera_new",
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
-    "gr_serve_set_tick_tuples", "gr_probe",
+    "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce",
 ]
 (GR_PROBE_NEFF, GR_PROBE_POYNTING, GR_PROBE_TIR_RETARDATION, GR_PROBE_FRAME_ROTATION, GR_PROBE_FIELD_QFORM,
  GR_PROBE_EXP, GR_PROBE_CMF) = range(7)
@@ -173,6 +173,8 @@ def load():
     L.gr_metrics.argtypes = [vp, ctypes.POINTER(ServeMetrics)]
     L.gr_metrics_reset.argtypes = [vp]
     L.gr_serve_set_tick_tuples.argtypes = [vp, u64]
+    L.gr_set_split_bounce.argtypes = [vp, ctypes.c_uint32]
+    L.gr_set_split_bounce.restype = ctypes.c_int
     L.gr_probe.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t, fp]
     L.gr_probe.restype = ctypes.c_int
     for name in ("gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel", "gr_job_film",

@@ -346,6 +346,11 @@ class GpuBackend:
         code = {"plain": _lib.GR_TRACE_PLAIN, "wavefront": _lib.GR_TRACE_WAVEFRONT}[mode]
         check(self._L.gr_set_trace_mode(self._ctx, code), "gr_set_trace_mode")
 
+    def set_split_bounce(self, split_bounce: int) -> None:
+        """Plain mode: park paths running after this many segments for a second launch (0 = never)."""
+        self._require()
+        check(self._L.gr_set_split_bounce(self._ctx, split_bounce), "gr_set_split_bounce")
+
     def set_chunk_tuples(self, tuples: int) -> None:
         self._require()
         check(self._L.gr_set_chunk_tuples(self._ctx, tuples), "gr_set_chunk_tuples")

@@ -326,6 +326,20 @@ gr_status ensure_rec(gr_ctx *ctx, uint64_t tuples) {
     return GR_OK;
 }
 
+// Two halves of continuation records for a quarter of a launch's tuples each (paths
+// beyond that carry on in the launch that would have parked them).
+gr_status ensure_cont(gr_ctx *ctx, uint64_t tuples) {
+    const uint64_t cap = (tuples + 3) / 4;
+    if (cap <= ctx->cont_cap) return GR_OK;
+    if (ctx->cont) GR_CHECK(hipFree(ctx->cont));
+    ctx->cont = nullptr;
+    ctx->cont_cap = 0;
+    if (!ctx->cont_count) GR_CHECK(hipMalloc(&ctx->cont_count, (gr::kContLevels + 1) * sizeof(uint32_t)));
+    GR_CHECK(hipMalloc(&ctx->cont, 2 * cap * gr::kContFields * sizeof(float)));
+    ctx->cont_cap = (uint32_t)cap;
+    return GR_OK;
+}
+
 gr_status ensure_ri35(gr_ctx *ctx, uint32_t sample_offset, uint32_t spp) {
     if (ctx->ri35 && ctx->ri35_offset == sample_offset && ctx->ri35_n == spp) return GR_OK;
     if (spp > ctx->ri35_cap) {
@@ -396,6 +410,9 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     if (wavefront) {
         st = ensure_rec(ctx, cap);
         if (st != GR_OK) return st;
+    } else if (ctx->split_bounce) {
+        st = ensure_cont(ctx, cap);
+        if (st != GR_OK) return st;
     }
     st = ensure_ri35(ctx, sample_offset, spp);
     if (st != GR_OK) return st;
@@ -421,6 +438,10 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
         args.rec = ctx->rec;
         args.rec_cap = (uint32_t)ctx->rec_cap;
         args.trace_done = wavefront ? ctx->ev_k1 : nullptr;
+        args.split_bounce = wavefront ? 0u : ctx->split_bounce;
+        args.cont = ctx->cont;
+        args.cont_cap = ctx->cont_cap;
+        args.cont_count = ctx->cont_count;
         GR_CHECK(hipEventRecord(ctx->ev_k0, ctx->stream));
         GR_CHECK(gr_launch_trace(s->sc, args, ctx->stream));
         if (!wavefront) GR_CHECK(hipEventRecord(ctx->ev_k1, ctx->stream));
@@ -496,6 +517,8 @@ void gr_close(gr_ctx *ctx) {
     if (ctx->info) (void)hipFree(ctx->info);
     if (ctx->rec) (void)hipFree(ctx->rec);
     if (ctx->ri35) (void)hipFree(ctx->ri35);
+    if (ctx->cont) (void)hipFree(ctx->cont);
+    if (ctx->cont_count) (void)hipFree(ctx->cont_count);
     if (ctx->stats) (void)hipFree(ctx->stats);
     for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_k0, ctx->ev_k1, ctx->ev_k2})
         if (e) (void)hipEventDestroy(e);
@@ -514,6 +537,13 @@ gr_status gr_set_trace_mode(gr_ctx *ctx, int mode) {
     if (!ctx || (mode != GR_TRACE_PLAIN && mode != GR_TRACE_WAVEFRONT)) return GR_ERR_INVALID;
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->trace_mode = mode;
+    return GR_OK;
+}
+
+gr_status gr_set_split_bounce(gr_ctx *ctx, uint32_t split_bounce) {
+    if (!ctx || split_bounce > GR_MAX_BOUNCES) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->split_bounce = split_bounce;
     return GR_OK;
 }
 

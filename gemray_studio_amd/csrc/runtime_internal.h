@@ -44,6 +44,10 @@ struct gr_ctx {
     uint32_t ri35_cap = 0, ri35_offset = 0, ri35_n = 0;
     std::vector<float2> ri35_host;
     unsigned long long *stats = nullptr;
+    uint32_t split_bounce = 6;  // plain mode: park paths running after this many segments
+    float *cont = nullptr;      // continuation records, kContFields x cont_cap
+    uint32_t cont_cap = 0;
+    uint32_t *cont_count = nullptr;
     gr_stats last{};
     char label[256] = {0};
     uint64_t hbm_budget = 0;  // admission budget of the job loop; 0 = 3/4 of free HBM

@@ -41,6 +41,12 @@ struct DevScene {
     const float *map_rgb;     // device pointer
 };
 
+// Floats per continuation record (the parked state of one path), and the number of
+// continuation launches after the plain trace launch (paths are parked again at each
+// multiple of the split bounce until the last level).
+constexpr int kContFields = 86;
+constexpr int kContLevels = 3;
+
 // Arguments of one trace launch over tuples [0, n_tuples): tuple t is pixel
 // pixel_begin + t / spp, sample sample_offset + t % spp.
 struct TraceArgs {
@@ -55,6 +61,13 @@ struct TraceArgs {
     float *rec;                  // device escape records (wavefront mode), 20 x rec_cap
     uint32_t rec_cap;
     hipEvent_t trace_done;       // recorded between the trace and finish kernels, or null
+    // Plain mode: paths running after split_bounce segments are parked in cont (two halves
+    // of kContFields x cont_cap floats, kContLevels + 1 counters at cont_count) and
+    // continued by further launches; 0 = no split.
+    uint32_t split_bounce;
+    float *cont;
+    uint32_t cont_cap;
+    uint32_t *cont_count;
 };
 
 // One piece of one job in a batched launch: pixels [pixel_begin, pixel_begin + n_pixels)

@@ -688,6 +688,124 @@ __device__ __forceinline__ uint32_t path_segments(const PathState &ps, const Dev
     return ps.term_reason == 3u ? sc.max_bounces : ps.term_bounce + 1u;
 }
 
+// Continuation records (kContFields, transport.h): the complete state of a path still
+// running at the split bounce, field-major (field f of record r at cont[f * cap + r]).
+// Fields: 0 tuple index, 1-12 ray_o, ray_d, cur_k, prev_pn, 13-44 Stokes, 45-52 pdfs,
+// 53-76 the LDS caches (n_o, ao, ae), 77 hero offset, 78-84 seed, bounce, last facet,
+// termination bounce and reason and the two path hashes, 85 the mode flags.
+
+__device__ __forceinline__ void save_path(const PathState &ps, uint32_t t, const float *__restrict__ lcache,
+                                          float *__restrict__ cont, uint32_t cap, uint32_t r) {
+    float *c = cont + r;
+    auto put = [&](int f, float v) { c[(size_t)f * cap] = v; };
+    auto putu = [&](int f, uint32_t v) { c[(size_t)f * cap] = __uint_as_float(v); };
+    putu(0, t);
+    const V3 v[4] = {ps.ray_o, ps.ray_d, ps.cur_k, ps.prev_pn};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        put(1 + 3 * j, v[j].x);
+        put(2 + 3 * j, v[j].y);
+        put(3 + 3 * j, v[j].z);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        put(13 + 4 * k, ps.st[k].i);
+        put(14 + 4 * k, ps.st[k].q);
+        put(15 + 4 * k, ps.st[k].u);
+        put(16 + 4 * k, ps.st[k].v);
+        put(45 + k, ps.pdf[k]);
+        put(53 + k, LC_NO(k));
+        put(61 + k, LC_AO(k));
+        put(69 + k, LC_AE(k));
+    }
+    put(77, ps.hero_off);
+    putu(78, ps.seed);
+    putu(79, ps.bounce);
+    putu(80, ps.last_facet);
+    putu(81, ps.term_bounce);
+    putu(82, ps.term_reason);
+    putu(83, ps.fh);
+    putu(84, ps.fate_h);
+    putu(85, (ps.inside ? 1u : 0u) | (ps.is_extra ? 2u : 0u) | (ps.have_prev ? 4u : 0u) | (ps.have_n ? 8u : 0u) |
+                 (ps.have_abs ? 16u : 0u) | (ps.escaped ? 32u : 0u));
+}
+
+__device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__ lcache,
+                                              const float *__restrict__ cont, uint32_t cap, uint32_t r) {
+    const float *c = cont + r;
+    auto get = [&](int f) { return c[(size_t)f * cap]; };
+    auto getu = [&](int f) { return __float_as_uint(c[(size_t)f * cap]); };
+    ps.ray_o = v3(get(1), get(2), get(3));
+    ps.ray_d = v3(get(4), get(5), get(6));
+    ps.cur_k = v3(get(7), get(8), get(9));
+    ps.prev_pn = v3(get(10), get(11), get(12));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        ps.st[k] = Stokes{get(13 + 4 * k), get(14 + 4 * k), get(15 + 4 * k), get(16 + 4 * k)};
+        ps.pdf[k] = get(45 + k);
+        LC_NO(k) = get(53 + k);
+        LC_AO(k) = get(61 + k);
+        LC_AE(k) = get(69 + k);
+    }
+    ps.hero_off = get(77);
+    ps.seed = getu(78);
+    ps.bounce = getu(79);
+    ps.last_facet = getu(80);
+    ps.cert = 0xFFFFFFFFu;  // used at bounce 0 only
+    ps.term_bounce = getu(81);
+    ps.term_reason = getu(82);
+    ps.fh = getu(83);
+    ps.fate_h = getu(84);
+    const uint32_t fl = getu(85);
+    ps.inside = fl & 1u;
+    ps.is_extra = fl & 2u;
+    ps.have_prev = fl & 4u;
+    ps.have_n = fl & 8u;
+    ps.have_abs = fl & 16u;
+    ps.escaped = fl & 32u;
+    return getu(0);
+}
+
+// Reserves a continuation record for every calling lane (one atomic per wave) and saves
+// the path into it; false when the records have run out (the path then carries on).
+// Called by all live lanes of a wave together: in the plain and continuation kernels
+// every live lane reaches a given bounce count in the same loop iteration.
+__device__ __forceinline__ bool park(const PathState &ps, uint32_t t, const float *__restrict__ lcache,
+                                     float *__restrict__ cont, uint32_t cap, uint32_t *__restrict__ count) {
+    const uint64_t live = __ballot(1);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)live) - 1u;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(live));
+    base = __shfl(base, (int)leader, 64);
+    const uint32_t r = base + (uint32_t)__popcll(live & ((1ull << lane) - 1ull));
+    if (r >= cap) return false;
+    save_path(ps, t, lcache, cont, cap, r);
+    return true;
+}
+
+// XYZ (and, with kInfo, the per-path record) of a finished path into tuple t's slot.
+template <int kEnv, bool kInfo>
+__device__ __forceinline__ void write_result(const PathState &ps, const DevScene &sc, uint32_t t,
+                                             float *__restrict__ xyz_out, uint32_t *__restrict__ info_out) {
+    V3 xyz = v3(0.0f, 0.0f, 0.0f);
+    if (ps.escaped) {
+        float inten[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) inten[k] = intensity(ps.st[k]);
+        xyz = finish_escaped<kEnv>(ps.ray_d, inten, ps.pdf, ps.hero_off, sc);
+    }
+    xyz_out[3 * (size_t)t + 0] = xyz.x;
+    xyz_out[3 * (size_t)t + 1] = xyz.y;
+    xyz_out[3 * (size_t)t + 2] = xyz.z;
+    if (kInfo) {
+        info_out[4 * (size_t)t + 0] = path_segments(ps, sc);
+        info_out[4 * (size_t)t + 1] = ps.term_reason;
+        info_out[4 * (size_t)t + 2] = ps.fh;
+        info_out[4 * (size_t)t + 3] = ps.fate_h;
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Kernels.
 // ---------------------------------------------------------------------------------
@@ -742,12 +860,22 @@ __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block
 // The plain path: one lane per (pixel, sample) tuple t of the launch, pixel =
 // pixel_begin + t / spp, sample = sample_offset + t % spp.  Writes the tuple's XYZ (and,
 // with kInfo, segments, termination and the two path hashes).
+//
+// Path-length split: a wave runs until its longest path ends, and stone paths are short
+// with a long tail (on the oval-brilliant workload 69 % end after 3 segments, but most
+// 64-sample waves hold one path that runs much longer).  With split_bounce > 0, a path
+// still running after split_bounce segments is parked as a continuation record (one
+// atomic per wave reserves the records) and cont_kernel continues the parked paths in
+// fully populated waves, parking again at each further multiple of split_bounce for
+// kContLevels levels.  Parking moves state, not arithmetic: every path's result is
+// bit-identical to an unsplit run.  When the records run out, the lane carries on.
 template <bool kAniso, bool kAbsorb, int kEnv, bool kInfo>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
     const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp,
     uint32_t n_tuples, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, uint32_t split_bounce, float *__restrict__ cont, uint32_t cont_cap,
+    uint32_t *__restrict__ cont_count) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[kCacheFloats];
@@ -766,24 +894,54 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
         const uint32_t s = t - local_pixel * spp;
         PathState ps;
         init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
+        bool parked = false;
         while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+            if (ps.bounce == split_bounce && park(ps, t, lcache, cont, cont_cap, cont_count)) {
+                parked = true;
+                break;
+            }
         }
-        V3 xyz = v3(0.0f, 0.0f, 0.0f);
-        if (ps.escaped) {
-            float inten[8];
+        if (!parked) write_result<kEnv, kInfo>(ps, sc, t, xyz_out, info_out);
+    }
+    flush_events(ev, block_ev, stats);
+}
+
+// Continues the paths the previous launch parked: record r per lane, for r < *cont_count
+// (capped at cont_cap); workgroups past the count return at once.  With split_bounce > 0
+// the paths still running at that bounce are parked again, into next / next_count, for
+// the next level.
+template <bool kAniso, bool kAbsorb, int kEnv, bool kInfo>
+__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
+    const DevScene sc, const float4 *__restrict__ planes_g, const float *__restrict__ cont, uint32_t cont_cap,
+    const uint32_t *__restrict__ cont_count, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
+    unsigned long long *__restrict__ stats, uint32_t split_bounce, float *__restrict__ next,
+    uint32_t *__restrict__ next_count) {
+    const uint32_t count = min(*cont_count, cont_cap);
+    if (blockIdx.x * kBlock >= count) return;
+    __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
+    __shared__ float lcache[kCacheFloats];
+    __shared__ uint32_t block_ev[GR_EV_COUNT];
+    stage_planes(planes, cam_side, planes_g, sc);
+    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
+    __syncthreads();
+
+    uint32_t ev[kEvWords];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) inten[k] = intensity(ps.st[k]);
-            xyz = finish_escaped<kEnv>(ps.ray_d, inten, ps.pdf, ps.hero_off, sc);
+    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
+
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r < count) {
+        PathState ps;
+        const uint32_t t = load_path(ps, lcache, cont, cont_cap, r);
+        bool parked = false;
+        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+            if (ps.bounce == split_bounce && park(ps, t, lcache, next, cont_cap, next_count)) {
+                parked = true;
+                break;
+            }
         }
-        xyz_out[3 * (size_t)t + 0] = xyz.x;
-        xyz_out[3 * (size_t)t + 1] = xyz.y;
-        xyz_out[3 * (size_t)t + 2] = xyz.z;
-        if (kInfo) {
-            info_out[4 * (size_t)t + 0] = path_segments(ps, sc);
-            info_out[4 * (size_t)t + 1] = ps.term_reason;
-            info_out[4 * (size_t)t + 2] = ps.fh;
-            info_out[4 * (size_t)t + 3] = ps.fate_h;
-        }
+        if (!parked) write_result<kEnv, kInfo>(ps, sc, t, xyz_out, info_out);
     }
     flush_events(ev, block_ev, stats);
 }
@@ -1077,18 +1235,39 @@ __global__ __launch_bounds__(kBlock) void film_reduce_batch_kernel(const BatchSl
 
 namespace {
 
+template <bool A, bool B, int E, bool I>
+hipError_t launch_plain_i(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
+    dim3 grid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
+    const bool split = a.split_bounce != 0 && a.cont != nullptr && a.cont_cap != 0;
+    if (split) {
+        hipError_t e = hipMemsetAsync(a.cont_count, 0, (gr::kContLevels + 1) * sizeof(uint32_t), stream);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((gr::trace_kernel<A, B, E, I>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes, a.ri35,
+                       a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats,
+                       split ? a.split_bounce : 0u, a.cont, a.cont_cap, a.cont_count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !split) return e;
+    // Level k (k = 1..kContLevels) continues the records of level k-1 and, except the
+    // last, parks again at (k + 1) * split_bounce; records alternate between the two
+    // halves of cont.
+    dim3 cgrid((a.cont_cap + gr::kBlock - 1) / gr::kBlock);
+    for (int k = 1; k <= gr::kContLevels; ++k) {
+        float *in = a.cont + (size_t)((k - 1) & 1) * gr::kContFields * a.cont_cap;
+        float *out = a.cont + (size_t)(k & 1) * gr::kContFields * a.cont_cap;
+        const uint32_t next_split = k < gr::kContLevels ? (uint32_t)(k + 1) * a.split_bounce : 0u;
+        hipLaunchKernelGGL((gr::cont_kernel<A, B, E, I>), cgrid, dim3(gr::kBlock), 0, stream, sc, a.planes, in,
+                           a.cont_cap, a.cont_count + (k - 1), a.xyz, a.info, a.stats, next_split, out,
+                           a.cont_count + k);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 template <bool A, bool B, int E>
 hipError_t launch_plain(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
-    dim3 grid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
-    if (a.info)
-        hipLaunchKernelGGL((gr::trace_kernel<A, B, E, true>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
-                           a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info,
-                           a.stats);
-    else
-        hipLaunchKernelGGL((gr::trace_kernel<A, B, E, false>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
-                           a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info,
-                           a.stats);
-    return hipGetLastError();
+    return a.info ? launch_plain_i<A, B, E, true>(sc, a, stream) : launch_plain_i<A, B, E, false>(sc, a, stream);
 }
 
 template <bool A, bool B, int E>

@@ -141,6 +141,11 @@ gr_status gr_set_chunk_tuples(gr_ctx *ctx, uint64_t tuples);
  * counterpart: the reference has one dispatch shape (frame.rs:437-452). */
 gr_status gr_set_trace_mode(gr_ctx *ctx, int mode);
 
+/* Plain trace mode: paths still running after split_bounce segments are parked and
+ * continued by further, fully populated launches (default 6; 0 = never).  Results are
+ * bit-identical for every value.  No reference counterpart. */
+gr_status gr_set_split_bounce(gr_ctx *ctx, uint32_t split_bounce);
+
 /* GpuBackend::try_accumulate / GpuFrameRenderer::accumulate (gpu_backend.rs:160-193,
  * frame.rs:632-793): ADDS each pixel's sum over samples [sample_offset,
  * sample_offset + spp) into accum_xyz[3 * pixel + c].  accum_pixels must equal

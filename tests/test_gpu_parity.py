@@ -250,3 +250,29 @@ def test_plain_and_wavefront_kernels_agree_bit_for_bit(gpu, srb, idx):
     for a, b in zip(out["plain"][:3], out["wavefront"][:3]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert out["plain"][3] == out["wavefront"][3]
+
+
+@pytest.mark.parametrize("idx", [0, 2, 8, 10], ids=["diamond", "sapphire", "pleochroic-oval", "stress"])
+def test_path_length_split_is_bit_identical(gpu, srb, idx):
+    """Parking paths at the split bounce and finishing them in a second launch moves state
+    only: outputs, films and counters equal the unsplit run bit for bit."""
+    case = _cases(srb)[idx]
+    scene = case.product_scene()
+    n_pix = case.width * case.height
+    out = {}
+    try:
+        for split in (0, 1, 3, 5):
+            gpu.set_split_bounce(split)
+            xyz, info = gpu.trace_samples(scene, 0, n_pix, 2, 8)
+            film = np.zeros((n_pix, 3), np.float32)
+            assert gpu.try_accumulate(scene, 2, 8, film)
+            out[split] = (xyz, info, film, gpu.last_stats()["events"])
+    finally:
+        gpu.set_split_bounce(6)
+    ref = out[0]
+    for split in (1, 3, 5):
+        xyz, info, film, ev = out[split]
+        assert np.array_equal(xyz.view(np.uint32), ref[0].view(np.uint32)), split
+        assert np.array_equal(info, ref[1]), split
+        assert np.array_equal(film.view(np.uint32), ref[2].view(np.uint32)), split
+        assert ev == ref[3], split
