@@ -93,6 +93,12 @@ struct TickSlot {
     uint64_t staging_cap = 0;
     BatchSlice *d_slices = nullptr;
     uint32_t slices_cap = 0;
+    // Continuation records of the path-length split: per slice two halves of
+    // kContFields x ceil(tuples / 4) floats, and kContLevels + 1 counters.
+    float *cont = nullptr;
+    uint64_t cont_cap = 0;  // floats
+    uint32_t *cont_counts = nullptr;
+    uint32_t counts_cap = 0;  // slices
     std::vector<BatchSlice> h_slices;  // kept until the tick completes
     std::vector<Piece> pieces;
     uint32_t launches = 0;
@@ -218,6 +224,25 @@ gr_status ensure_staging(TickSlot &T, uint64_t tuples, uint32_t slices) {
     return GR_OK;
 }
 
+gr_status ensure_cont(TickSlot &T, uint64_t floats, uint32_t slices) {
+    if (floats > T.cont_cap) {
+        if (T.cont) SV_CHECK(hipFree(T.cont));
+        T.cont = nullptr;
+        T.cont_cap = 0;
+        SV_CHECK(hipMalloc(&T.cont, floats * sizeof(float)));
+        T.cont_cap = floats;
+    }
+    if (slices > T.counts_cap) {
+        if (T.cont_counts) SV_CHECK(hipFree(T.cont_counts));
+        T.cont_counts = nullptr;
+        T.counts_cap = 0;
+        uint32_t cap = std::max<uint32_t>(slices, 64);
+        SV_CHECK(hipMalloc(&T.cont_counts, (size_t)cap * (kContLevels + 1) * sizeof(uint32_t)));
+        T.counts_cap = cap;
+    }
+    return GR_OK;
+}
+
 // Pieces of one tick: every active job gets an equal share of the tick's tuples (at least
 // one pixel), then the unused budget extends the pieces in round-robin order.  A job has at
 // most one piece per tick, so no two reduce threads of a tick touch the same film pixel.
@@ -254,18 +279,27 @@ std::vector<Piece> plan_tick(ServeState &S, const std::vector<Job *> &jobs, uint
     return pieces;
 }
 
-// Queues one tick's pieces on slot T: the batched trace launches (one per material class)
-// and the film reduce, bracketed by the slot's events.  Returns without waiting.
-gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces) {
+// Queues one tick's pieces on slot T: the batched trace launches (one per material class,
+// each followed by its continuation launches when split_bounce > 0) and the film reduce,
+// bracketed by the slot's events.  Returns without waiting.
+gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_bounce) {
     std::stable_sort(pieces.begin(), pieces.end(),
                      [](const Piece &a, const Piece &b) { return a.job->cls < b.job->cls; });
-    uint64_t tuples = 0;
-    for (const Piece &p : pieces) tuples += p.n_pixels * p.job->spp;
+    uint64_t tuples = 0, cont_floats = 0;
+    for (const Piece &p : pieces) {
+        const uint64_t n = p.n_pixels * p.job->spp;
+        tuples += n;
+        cont_floats += 2 * ((n + 3) / 4) * kContFields;
+    }
     gr_status st = ensure_staging(T, std::max<uint64_t>(tuples, 1), (uint32_t)pieces.size());
     if (st != GR_OK) return st;
+    if (split_bounce) {
+        st = ensure_cont(T, cont_floats, (uint32_t)pieces.size());
+        if (st != GR_OK) return st;
+    }
     std::vector<BatchSlice> &sl = T.h_slices;
     sl.assign(pieces.size(), BatchSlice{});
-    uint64_t off = 0;
+    uint64_t off = 0, cont_off = 0;
     uint32_t reduce_blocks = 0;
     for (size_t i = 0; i < pieces.size(); ++i) {
         const Piece &p = pieces[i];
@@ -284,28 +318,42 @@ gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces) {
         b.spp = j.spp;
         b.reduce_block_begin = reduce_blocks;
         reduce_blocks += (uint32_t)((p.n_pixels + kBlockTuples - 1) / kBlockTuples);
+        if (split_bounce) {
+            const uint64_t cap = (p.n_pixels * j.spp + 3) / 4;
+            b.split_bounce = split_bounce;
+            b.cont_cap = (uint32_t)cap;
+            b.cont[0] = T.cont + cont_off;
+            b.cont[1] = T.cont + cont_off + cap * kContFields;
+            b.cont_count = T.cont_counts + i * (kContLevels + 1);
+            cont_off += 2 * cap * kContFields;
+        }
         off += p.n_pixels * j.spp;
     }
     // Class launches: slices of one class are contiguous; block numbering restarts per class.
     struct ClassRange {
         int cls;
-        uint32_t first, count, blocks;
+        uint32_t first, count, blocks, cont_blocks;
     };
     std::vector<ClassRange> classes;
     for (size_t i = 0; i < sl.size(); ++i) {
         const int c = pieces[i].job->cls;
-        if (classes.empty() || classes.back().cls != c) classes.push_back(ClassRange{c, (uint32_t)i, 0, 0});
+        if (classes.empty() || classes.back().cls != c) classes.push_back(ClassRange{c, (uint32_t)i, 0, 0, 0});
         ClassRange &r = classes.back();
         sl[i].trace_block_begin = r.blocks;
         r.blocks += (uint32_t)(((uint64_t)sl[i].n_pixels * sl[i].spp + kBlockTuples - 1) / kBlockTuples);
+        sl[i].cont_block_begin = r.cont_blocks;
+        r.cont_blocks += (sl[i].cont_cap + kBlockTuples - 1) / kBlockTuples;
         r.count++;
     }
     T.launches = 0;
     SV_CHECK(hipMemcpyAsync(T.d_slices, sl.data(), sl.size() * sizeof(BatchSlice), hipMemcpyHostToDevice, T.stream));
     SV_CHECK(hipEventRecord(T.ev0, T.stream));
+    if (split_bounce)
+        SV_CHECK(hipMemsetAsync(T.cont_counts, 0, sl.size() * (kContLevels + 1) * sizeof(uint32_t), T.stream));
     for (const ClassRange &r : classes) {
-        SV_CHECK(gr_launch_trace_batch(T.d_slices + r.first, r.count, r.blocks, r.cls, T.stream));
-        ++T.launches;
+        SV_CHECK(gr_launch_trace_batch(T.d_slices + r.first, r.count, r.blocks, split_bounce ? r.cont_blocks : 0u,
+                                       r.cls, T.stream));
+        T.launches += split_bounce ? 1 + kContLevels : 1;
     }
     SV_CHECK(gr_launch_film_reduce_batch(T.d_slices, (uint32_t)sl.size(), reduce_blocks, T.stream));
     ++T.launches;
@@ -450,11 +498,13 @@ void loop(ServeState *Sp) {
         // Plan the next tick over the jobs that still have pixels to hand out.
         std::vector<Job *> schedulable;
         uint64_t tick_tuples = 0;
+        uint32_t split_bounce = 0;
         {
             std::lock_guard<std::mutex> g(S.mu);
             for (auto &j : S.active)
                 if (!j->cancel_req && j->next_pixel < j->n_pixels) schedulable.push_back(j.get());
             tick_tuples = S.tick_tuples;
+            split_bounce = S.ctx->split_bounce;
         }
         TickSlot &next = S.slot[S.ticks_launched % kTickSlots];
         gr_status st = GR_OK;
@@ -462,7 +512,7 @@ void loop(ServeState *Sp) {
             // The slot's previous tick must be complete before its buffers are reused.
             if (next.busy) st = complete_tick(S, next);
             if (st == GR_OK) {
-                st = launch_tick(next, plan_tick(S, schedulable, tick_tuples));
+                st = launch_tick(next, plan_tick(S, schedulable, tick_tuples), split_bounce);
                 if (st == GR_OK) S.ticks_launched++;
             }
         } else {
@@ -494,6 +544,8 @@ void release_device(ServeState &S) {
         if (T.stream) (void)hipStreamSynchronize(T.stream);
         if (T.staging) (void)hipFree(T.staging);
         if (T.d_slices) (void)hipFree(T.d_slices);
+        if (T.cont) (void)hipFree(T.cont);
+        if (T.cont_counts) (void)hipFree(T.cont_counts);
         if (T.ev0) (void)hipEventDestroy(T.ev0);
         if (T.ev1) (void)hipEventDestroy(T.ev1);
         if (T.stream) (void)hipStreamDestroy(T.stream);

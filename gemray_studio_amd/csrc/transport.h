@@ -87,6 +87,13 @@ struct BatchSlice {
     uint32_t pixel_begin, n_pixels, sample_offset, spp;
     uint32_t trace_block_begin;  // first workgroup of this slice in its class launch
     uint32_t reduce_block_begin; // first workgroup of this slice in the film reduce
+    // Path-length split (see trace_kernel): this slice's continuation records, in two
+    // halves of kContFields x cont_cap floats, and its kContLevels + 1 counters.
+    uint32_t split_bounce;       // 0 = no split
+    uint32_t cont_cap;
+    uint32_t cont_block_begin;   // first workgroup of this slice in its class's continuation launches
+    float *cont[2];
+    uint32_t *cont_count;
 };
 
 // Material/environment class of a scene: bit 0 anisotropic, bit 1 absorbing, bit 2 HDR map.
@@ -102,9 +109,10 @@ hipError_t gr_launch_probe(int fn, const float *in, float *out, uint32_t n, hipS
 // Fills cert[0, width * height) with the frame's camera-ray miss certificates.
 hipError_t gr_launch_cert(const gr::DevScene &sc, const float4 *planes, uint32_t *cert, hipStream_t stream);
 // Traces slices [0, n_slices) of one class (all with scene_class == cls) in one launch of
-// n_blocks workgroups.
-hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, int cls,
-                                 hipStream_t stream);
+// n_blocks workgroups, then, when the slices split paths, kContLevels continuation
+// launches of cont_blocks workgroups.
+hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
+                                 uint32_t cont_blocks, int cls, hipStream_t stream);
 // Adds each slice's per-pixel ordered sample sums into its job's film.
 hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
                                        hipStream_t stream);

@@ -948,12 +948,15 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
 
 // Index of the slice that owns workgroup b: the last slice whose first block is <= b
 // (slices are in block order and the first starts at block 0).
+enum SliceBlocks { kTraceBlocks = 0, kReduceBlocks = 1, kContBlocks = 2 };
 __device__ __forceinline__ uint32_t find_slice(const BatchSlice *__restrict__ slices, uint32_t n, uint32_t b,
-                                               bool reduce) {
+                                               int which) {
     uint32_t lo = 0u, hi = n;
     while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t first = reduce ? slices[mid].reduce_block_begin : slices[mid].trace_block_begin;
+        const uint32_t first = which == kTraceBlocks    ? slices[mid].trace_block_begin
+                               : which == kReduceBlocks ? slices[mid].reduce_block_begin
+                                                        : slices[mid].cont_block_begin;
         if (first <= b) lo = mid;
         else hi = mid;
     }
@@ -972,7 +975,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, false)];
+    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, kTraceBlocks)];
     const DevScene &sc = *J.scene;
     stage_planes(planes, cam_side, J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
@@ -990,18 +993,54 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         PathState ps;
         init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, J.ri35[s],
                   J.cert[J.pixel_begin + local_pixel], sc);
+        bool parked = false;
         while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+            if (ps.bounce == J.split_bounce && park(ps, t, lcache, J.cont[0], J.cont_cap, J.cont_count)) {
+                parked = true;
+                break;
+            }
         }
-        V3 xyz = v3(0.0f, 0.0f, 0.0f);
-        if (ps.escaped) {
-            float inten[8];
+        if (!parked) write_result<kEnv, false>(ps, sc, t, J.xyz, nullptr);
+    }
+    flush_events(ev, block_ev, J.stats);
+}
+
+// Continuation level `level` (1..kContLevels) of a batched class launch: each workgroup
+// belongs to one slice (binary search of cont_block_begin) and continues that slice's
+// records of the previous level, parking again for the next level except at the last.
+template <bool kAniso, bool kAbsorb, int kEnv>
+__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
+    const BatchSlice *__restrict__ slices, uint32_t n_slices, int level) {
+    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, kContBlocks)];
+    const uint32_t count = min(J.cont_count[level - 1], J.cont_cap);
+    const uint32_t r = (blockIdx.x - J.cont_block_begin) * kBlock + threadIdx.x;
+    if ((blockIdx.x - J.cont_block_begin) * kBlock >= count) return;
+    __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
+    __shared__ float lcache[kCacheFloats];
+    __shared__ uint32_t block_ev[GR_EV_COUNT];
+    const DevScene &sc = *J.scene;
+    stage_planes(planes, cam_side, J.planes, sc);
+    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
+    __syncthreads();
+
+    uint32_t ev[kEvWords];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) inten[k] = intensity(ps.st[k]);
-            xyz = finish_escaped<kEnv>(ps.ray_d, inten, ps.pdf, ps.hero_off, sc);
+    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
+
+    if (r < count) {
+        PathState ps;
+        const uint32_t t = load_path(ps, lcache, J.cont[(level - 1) & 1], J.cont_cap, r);
+        const uint32_t next_split = level < kContLevels ? (uint32_t)(level + 1) * J.split_bounce : 0u;
+        bool parked = false;
+        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+            if (ps.bounce == next_split &&
+                park(ps, t, lcache, J.cont[level & 1], J.cont_cap, J.cont_count + level)) {
+                parked = true;
+                break;
+            }
         }
-        J.xyz[3 * (size_t)t + 0] = xyz.x;
-        J.xyz[3 * (size_t)t + 1] = xyz.y;
-        J.xyz[3 * (size_t)t + 2] = xyz.z;
+        if (!parked) write_result<kEnv, false>(ps, sc, t, J.xyz, nullptr);
     }
     flush_events(ev, block_ev, J.stats);
 }
@@ -1210,7 +1249,7 @@ __global__ __launch_bounds__(kBlock) void film_reduce_kernel(const float *__rest
 // spp staged tuples in sample order and add the sum into the slice's job film.
 __global__ __launch_bounds__(kBlock) void film_reduce_batch_kernel(const BatchSlice *__restrict__ slices,
                                                                    uint32_t n_slices) {
-    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, true)];
+    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, kReduceBlocks)];
     const uint32_t p = (blockIdx.x - J.reduce_block_begin) * kBlock + threadIdx.x;
     if (p >= J.n_pixels) return;
     const uint32_t spp = J.spp;
@@ -1317,25 +1356,36 @@ hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &a, hipSt
 
 namespace {
 template <int C>
-hipError_t launch_batch_class(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, hipStream_t stream) {
-    hipLaunchKernelGGL((gr::trace_batch_kernel<(C & 1) != 0, (C & 2) != 0, (C & 4) ? 1 : 0>), dim3(n_blocks),
-                       dim3(gr::kBlock), 0, stream, slices, n_slices);
-    return hipGetLastError();
+hipError_t launch_batch_class(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
+                              uint32_t cont_blocks, hipStream_t stream) {
+    constexpr bool kA = (C & 1) != 0, kB = (C & 2) != 0;
+    constexpr int kE = (C & 4) ? 1 : 0;
+    hipLaunchKernelGGL((gr::trace_batch_kernel<kA, kB, kE>), dim3(n_blocks), dim3(gr::kBlock), 0, stream, slices,
+                       n_slices);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || cont_blocks == 0) return e;
+    for (int level = 1; level <= gr::kContLevels; ++level) {
+        hipLaunchKernelGGL((gr::cont_batch_kernel<kA, kB, kE>), dim3(cont_blocks), dim3(gr::kBlock), 0, stream,
+                           slices, n_slices, level);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 }  // namespace
 
-hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, int cls,
-                                 hipStream_t stream) {
+hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
+                                 uint32_t cont_blocks, int cls, hipStream_t stream) {
     if (n_slices == 0 || n_blocks == 0) return hipSuccess;
     switch (cls) {
-        case 0: return launch_batch_class<0>(slices, n_slices, n_blocks, stream);
-        case 1: return launch_batch_class<1>(slices, n_slices, n_blocks, stream);
-        case 2: return launch_batch_class<2>(slices, n_slices, n_blocks, stream);
-        case 3: return launch_batch_class<3>(slices, n_slices, n_blocks, stream);
-        case 4: return launch_batch_class<4>(slices, n_slices, n_blocks, stream);
-        case 5: return launch_batch_class<5>(slices, n_slices, n_blocks, stream);
-        case 6: return launch_batch_class<6>(slices, n_slices, n_blocks, stream);
-        case 7: return launch_batch_class<7>(slices, n_slices, n_blocks, stream);
+        case 0: return launch_batch_class<0>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 1: return launch_batch_class<1>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 2: return launch_batch_class<2>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 3: return launch_batch_class<3>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 4: return launch_batch_class<4>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 5: return launch_batch_class<5>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 6: return launch_batch_class<6>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 7: return launch_batch_class<7>(slices, n_slices, n_blocks, cont_blocks, stream);
         default: return hipErrorInvalidValue;
     }
 }
