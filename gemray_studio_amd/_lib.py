@@ -98,7 +98,9 @@ class ServeMetrics(ctypes.Structure):
         (n, ctypes.c_double) for n in (
             "busy_ms", "wall_ms", "latency_p50_ms", "latency_p99_ms", "latency_mean_ms",
             "latency_max_ms")] + [
-        (n, ctypes.c_uint64) for n in ("hbm_in_use", "hbm_budget", "tick_tuples")]
+        (n, ctypes.c_uint64) for n in ("hbm_in_use", "hbm_budget", "tick_tuples")] + [
+        ("policy", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("slack", ctypes.c_double),
+        ("rate_tuples_per_ms", ctypes.c_double)]
 
 
 # Every symbol include/gemray_hip.h declares.
@@ -108,8 +110,9 @@ EXPORTS = [
     "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_camera_new",
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
-    "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce",
+    "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_serve_set_policy",
 ]
+GR_SERVE_EQUAL_SHARE, GR_SERVE_DEADLINE = 0, 1
 (GR_PROBE_NEFF, GR_PROBE_POYNTING, GR_PROBE_TIR_RETARDATION, GR_PROBE_FRAME_ROTATION, GR_PROBE_FIELD_QFORM,
  GR_PROBE_EXP, GR_PROBE_CMF) = range(7)
 
@@ -173,13 +176,14 @@ def load():
     L.gr_metrics.argtypes = [vp, ctypes.POINTER(ServeMetrics)]
     L.gr_metrics_reset.argtypes = [vp]
     L.gr_serve_set_tick_tuples.argtypes = [vp, u64]
+    L.gr_serve_set_policy.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     L.gr_set_split_bounce.argtypes = [vp, ctypes.c_uint32]
     L.gr_set_split_bounce.restype = ctypes.c_int
     L.gr_probe.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t, fp]
     L.gr_probe.restype = ctypes.c_int
     for name in ("gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel", "gr_job_film",
                  "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
-                 "gr_serve_set_tick_tuples"):
+                 "gr_serve_set_tick_tuples", "gr_serve_set_policy"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -329,6 +329,12 @@ class GpuBackend:
         self._require()
         check(self._L.gr_serve_set_tick_tuples(self._ctx, tuples), "gr_serve_set_tick_tuples")
 
+    def set_policy(self, policy: int, slack: float = 0.5) -> None:
+        """How ticks are shared among running jobs: _lib.GR_SERVE_DEADLINE (deadline =
+        submit + slack x estimated service) or _lib.GR_SERVE_EQUAL_SHARE."""
+        self._require()
+        check(self._L.gr_serve_set_policy(self._ctx, policy, slack), "gr_serve_set_policy")
+
     def probe(self, fn: int, inputs: np.ndarray) -> np.ndarray:
         """Evaluates device function fn (_lib.GR_PROBE_*) on rows of up to 9 float32 inputs;
         returns (n, 3) float32 outputs (a test probe of the kernel's own functions)."""

@@ -41,6 +41,8 @@ using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kDefaultTickTuples = 1ull << 24;
 constexpr uint64_t kMinTickTuples = GR_MAX_SAMPLES_PER_CALL;  // one pixel always fits
+constexpr double kDefaultSlack = 0.5;
+constexpr double kDefaultRate = 2.0e6;  // tuples per ms before any tick has completed
 constexpr uint64_t kMaxTickTuples = 1ull << 28;
 constexpr uint32_t kBlockTuples = 256;  // workgroup size of the batched kernels
 
@@ -76,6 +78,8 @@ struct Job {
     unsigned long long events[GR_EV_COUNT] = {0};
     uint64_t paths_done = 0;
     Clock::time_point t_submit, t_admit, t_done;
+    // Deadline policy: submit time (ms from the loop's start) + slack x estimated service.
+    double deadline_ms = 0.0;
     bool cancel_req = false;
 };
 
@@ -102,6 +106,8 @@ struct TickSlot {
     std::vector<BatchSlice> h_slices;  // kept until the tick completes
     std::vector<Piece> pieces;
     uint32_t launches = 0;
+    uint64_t tuples = 0;
+    Clock::time_point t_launch;
     bool busy = false;
 };
 
@@ -119,6 +125,13 @@ struct ServeState {
     std::unordered_map<uint64_t, std::shared_ptr<Job>> jobs;
     uint64_t budget = 0, in_use = 0;
     uint64_t tick_tuples = kDefaultTickTuples;
+    int policy = GR_SERVE_DEADLINE;
+    double slack = kDefaultSlack;
+    // Service-rate estimate for deadlines: tuples of completed ticks over the wall time the
+    // loop spent with ticks in flight (kDefaultRate until the first tick completes).
+    Clock::time_point t_start = Clock::now(), t_last_complete;
+    uint64_t rate_tuples = 0;
+    double rate_ms = 0.0;
     // Device resources of the loop: a control stream for uploads and read-backs, and two
     // tick slots with their own streams and buffers so that one tick is queued while the
     // other runs.
@@ -243,13 +256,33 @@ gr_status ensure_cont(TickSlot &T, uint64_t floats, uint32_t slices) {
     return GR_OK;
 }
 
-// Pieces of one tick: every active job gets an equal share of the tick's tuples (at least
-// one pixel), then the unused budget extends the pieces in round-robin order.  A job has at
-// most one piece per tick, so no two reduce threads of a tick touch the same film pixel.
-std::vector<Piece> plan_tick(ServeState &S, const std::vector<Job *> &jobs, uint64_t budget) {
+// Pieces of one tick.  A job has at most one piece per tick, so no two reduce threads of a
+// tick touch the same film pixel.
+//  * GR_SERVE_EQUAL_SHARE: every active job gets an equal share of the tick's tuples (at
+//    least one pixel), then the unused budget extends the pieces in round-robin order.
+//  * GR_SERVE_DEADLINE: the tick's tuples go to jobs in order of deadline (submit time +
+//    slack x estimated service time): slack 0 is first-come-first-served, which bounds the
+//    worst latency best; a larger slack lets short jobs overtake long ones.  The first job
+//    always gets at least one pixel.
+std::vector<Piece> plan_tick(ServeState &S, std::vector<Job *> jobs, uint64_t budget, int policy) {
     std::vector<Piece> pieces;
     const size_t n = jobs.size();
     if (n == 0) return pieces;
+    if (policy == GR_SERVE_DEADLINE) {
+        std::stable_sort(jobs.begin(), jobs.end(),
+                         [](const Job *a, const Job *b) { return a->deadline_ms < b->deadline_ms; });
+        uint64_t used = 0;
+        for (Job *j : jobs) {
+            uint64_t px = std::min<uint64_t>(j->n_pixels - j->next_pixel, (budget - used) / j->spp);
+            if (pieces.empty()) px = std::max<uint64_t>(px, 1);
+            if (px == 0) break;
+            pieces.push_back(Piece{j, j->next_pixel, px});
+            j->next_pixel += px;
+            used += px * j->spp;
+            if (used >= budget) break;
+        }
+        return pieces;
+    }
     const uint64_t share = std::max<uint64_t>(budget / n, 1);
     uint64_t used = 0;
     std::vector<uint64_t> take(n, 0);
@@ -360,6 +393,8 @@ gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_b
     SV_CHECK(hipEventRecord(T.ev1, T.stream));
     for (const Piece &p : pieces) p.job->inflight++;
     T.pieces = std::move(pieces);
+    T.tuples = tuples;
+    T.t_launch = Clock::now();
     T.busy = true;
     return GR_OK;
 }
@@ -401,42 +436,50 @@ void retire(ServeState &S, Job &j, int state, gr_status err) {
     }
 }
 
-// Waits for the tick on slot T, books its pieces, and reads back and retires the jobs it
-// completed (all pieces traced) or that were cancelled with nothing left in flight.
-gr_status complete_tick(ServeState &S, TickSlot &T) {
+// Waits for the tick on slot T and books its pieces; the jobs it completed (all pieces
+// traced, none cancelled) are appended to `finished` for finish_jobs.
+gr_status complete_tick(ServeState &S, TickSlot &T, std::vector<Job *> &finished) {
     gr_status st = GR_OK;
     float gpu_ms = 0.0f;
     if (hipEventSynchronize(T.ev1) != hipSuccess || hipEventElapsedTime(&gpu_ms, T.ev0, T.ev1) != hipSuccess)
         st = GR_ERR_DEVICE;
-    std::vector<Job *> touched;
     for (const Piece &p : T.pieces) {
         p.job->inflight--;
-        touched.push_back(p.job);
+        if (st == GR_OK && p.job->inflight == 0 && p.job->next_pixel == p.job->n_pixels && !p.job->cancel_req)
+            finished.push_back(p.job);
     }
-    std::vector<Job *> finished;
-    for (Job *j : touched)
-        if (j->inflight == 0 && j->next_pixel == j->n_pixels && !j->cancel_req) finished.push_back(j);
-    // Films and counters of finished jobs are read back before the lock is taken; until
-    // retire() marks them done no API call reads either.
-    std::vector<gr_status> fin(finished.size(), st);
-    if (st == GR_OK)
-        for (size_t i = 0; i < finished.size(); ++i) fin[i] = finish_job(S, *finished[i]);
+    const Clock::time_point now = Clock::now();
     std::lock_guard<std::mutex> g(S.mu);
     for (const Piece &p : T.pieces) p.job->paths_done += p.n_pixels * p.job->spp;
+    if (st == GR_OK) {
+        S.rate_tuples += T.tuples;
+        S.rate_ms += ms_between(std::max(T.t_launch, S.t_last_complete), now);
+    }
+    S.t_last_complete = now;
     S.m.ticks++;
     S.m.launches += T.launches;
     S.m.busy_ms += gpu_ms;
     S.ticks_done++;
     T.busy = false;
     T.pieces.clear();
-    if (st != GR_OK) return st;
+    return st;
+}
+
+// Reads back the films and counters of completed jobs and retires them.  The loop calls
+// this after queuing the next tick, so the read-backs overlap the GPU's work instead of
+// leaving it idle; until retire() marks a job done no API call reads its film.
+void finish_jobs(ServeState &S, std::vector<Job *> &finished) {
+    if (finished.empty()) return;
+    std::vector<gr_status> fin(finished.size());
+    for (size_t i = 0; i < finished.size(); ++i) fin[i] = finish_job(S, *finished[i]);
+    std::lock_guard<std::mutex> g(S.mu);
     for (size_t i = 0; i < finished.size(); ++i)
         retire(S, *finished[i], fin[i] == GR_OK ? GR_JOB_DONE : GR_JOB_FAILED, fin[i]);
     S.active.erase(std::remove_if(S.active.begin(), S.active.end(),
                                   [](const std::shared_ptr<Job> &j) { return j->state != GR_JOB_RUNNING; }),
                    S.active.end());
     S.cv_done.notify_all();
-    return GR_OK;
+    finished.clear();
 }
 
 void loop(ServeState *Sp) {
@@ -499,27 +542,32 @@ void loop(ServeState *Sp) {
         std::vector<Job *> schedulable;
         uint64_t tick_tuples = 0;
         uint32_t split_bounce = 0;
+        int policy = GR_SERVE_DEADLINE;
         {
             std::lock_guard<std::mutex> g(S.mu);
             for (auto &j : S.active)
                 if (!j->cancel_req && j->next_pixel < j->n_pixels) schedulable.push_back(j.get());
             tick_tuples = S.tick_tuples;
             split_bounce = S.ctx->split_bounce;
+            policy = S.policy;
         }
         TickSlot &next = S.slot[S.ticks_launched % kTickSlots];
         gr_status st = GR_OK;
+        std::vector<Job *> finished;
         if (!schedulable.empty()) {
-            // The slot's previous tick must be complete before its buffers are reused.
-            if (next.busy) st = complete_tick(S, next);
+            // The slot's previous tick must be complete before its buffers are reused.  Jobs
+            // that tick completed hold no pixels still to hand out, so the plan skips them.
+            if (next.busy) st = complete_tick(S, next, finished);
             if (st == GR_OK) {
-                st = launch_tick(next, plan_tick(S, schedulable, tick_tuples), split_bounce);
+                st = launch_tick(next, plan_tick(S, schedulable, tick_tuples, policy), split_bounce);
                 if (st == GR_OK) S.ticks_launched++;
             }
         } else {
             // Nothing new to launch: complete the oldest tick in flight.
             TickSlot &oldest = S.slot[S.ticks_done % kTickSlots];
-            if (oldest.busy) st = complete_tick(S, oldest);
+            if (oldest.busy) st = complete_tick(S, oldest, finished);
         }
+        if (st == GR_OK) finish_jobs(S, finished);
         if (st != GR_OK) {
             // A device error fails every running job.
             for (TickSlot &T : S.slot) {
@@ -659,6 +707,8 @@ gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) 
         S->m.declined++;
         return GR_DECLINE_BUDGET;
     }
+    const double rate = S->rate_ms > 0.0 ? (double)S->rate_tuples / S->rate_ms : gr::kDefaultRate;
+    j->deadline_ms = gr::ms_between(S->t_start, j->t_submit) + S->slack * (double)(j->n_pixels * j->spp) / rate;
     j->id = S->next_id++;
     S->jobs[j->id] = j;
     S->queue.push_back(j);
@@ -777,6 +827,9 @@ gr_status gr_metrics(gr_ctx *ctx, gr_serve_metrics *out) {
     out->queued = S->queue.size();
     out->running = S->active.size();
     out->tick_tuples = S->tick_tuples;
+    out->policy = (uint32_t)S->policy;
+    out->slack = S->slack;
+    out->rate_tuples_per_ms = S->rate_ms > 0.0 ? (double)S->rate_tuples / S->rate_ms : 0.0;
     return GR_OK;
 }
 
@@ -799,6 +852,19 @@ gr_status gr_serve_set_tick_tuples(gr_ctx *ctx, uint64_t tuples) {
     if (st != GR_OK) return st;
     std::lock_guard<std::mutex> g(S->mu);
     S->tick_tuples = tuples;
+    return GR_OK;
+}
+
+gr_status gr_serve_set_policy(gr_ctx *ctx, int policy, double slack) {
+    if (!ctx || (policy != GR_SERVE_EQUAL_SHARE && policy != GR_SERVE_DEADLINE) || !(slack >= 0.0) ||
+        slack > 1.0e6)
+        return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    S->policy = policy;
+    S->slack = slack;
     return GR_OK;
 }
 

@@ -17,6 +17,8 @@
 // No MFMA: there is no dense contraction.
 #include "transport.h"
 
+#include <cstddef>
+
 #include "device_physics.h"
 
 namespace gr {
@@ -946,10 +948,46 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
     flush_events(ev, block_ev, stats);
 }
 
+// The batched kernels' view of BatchSlice: the same layout with every pointer typed by the
+// memory it addresses.  The scene block is constant for the launch (address space 4), so its
+// uniform fields are read with scalar loads exactly as the plain kernel reads its by-value
+// scene from the kernel arguments; the buffers are global memory (address space 1), so the
+// kernels issue global rather than flat loads and stores.  Through generic pointers loaded
+// from the slice array the compiler could prove neither and fell back to per-lane flat
+// accesses.
+#define GR_AS_CONST __attribute__((address_space(4)))
+#define GR_AS_GLOBAL __attribute__((address_space(1)))
+struct BatchSliceView {
+    const GR_AS_CONST DevScene *scene;
+    const GR_AS_CONST float4 *planes;
+    const GR_AS_CONST float *ri35;  // float2 pairs
+    const GR_AS_CONST uint32_t *cert;
+    GR_AS_GLOBAL float *xyz;
+    GR_AS_GLOBAL float *film;
+    GR_AS_GLOBAL unsigned long long *stats;
+    uint32_t pixel_begin, n_pixels, sample_offset, spp;
+    uint32_t trace_block_begin;
+    uint32_t reduce_block_begin;
+    uint32_t split_bounce;
+    uint32_t cont_cap;
+    uint32_t cont_block_begin;
+    GR_AS_GLOBAL float *cont[2];
+    GR_AS_GLOBAL uint32_t *cont_count;
+};
+static_assert(sizeof(BatchSliceView) == sizeof(BatchSlice), "BatchSliceView mirrors BatchSlice");
+static_assert(offsetof(BatchSliceView, xyz) == offsetof(BatchSlice, xyz), "BatchSliceView mirrors BatchSlice");
+static_assert(offsetof(BatchSliceView, pixel_begin) == offsetof(BatchSlice, pixel_begin),
+              "BatchSliceView mirrors BatchSlice");
+static_assert(offsetof(BatchSliceView, split_bounce) == offsetof(BatchSlice, split_bounce),
+              "BatchSliceView mirrors BatchSlice");
+static_assert(offsetof(BatchSliceView, cont) == offsetof(BatchSlice, cont), "BatchSliceView mirrors BatchSlice");
+static_assert(offsetof(BatchSliceView, cont_count) == offsetof(BatchSlice, cont_count),
+              "BatchSliceView mirrors BatchSlice");
+
 // Index of the slice that owns workgroup b: the last slice whose first block is <= b
 // (slices are in block order and the first starts at block 0).
 enum SliceBlocks { kTraceBlocks = 0, kReduceBlocks = 1, kContBlocks = 2 };
-__device__ __forceinline__ uint32_t find_slice(const BatchSlice *__restrict__ slices, uint32_t n, uint32_t b,
+__device__ __forceinline__ uint32_t find_slice(const BatchSliceView *__restrict__ slices, uint32_t n, uint32_t b,
                                                int which) {
     uint32_t lo = 0u, hi = n;
     while (hi - lo > 1u) {
@@ -970,14 +1008,14 @@ __device__ __forceinline__ uint32_t find_slice(const BatchSlice *__restrict__ sl
 // launch of the same job.
 template <bool kAniso, bool kAbsorb, int kEnv>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
-    const BatchSlice *__restrict__ slices, uint32_t n_slices) {
+    const BatchSliceView *__restrict__ slices, uint32_t n_slices) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, kTraceBlocks)];
-    const DevScene &sc = *J.scene;
-    stage_planes(planes, cam_side, J.planes, sc);
+    const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kTraceBlocks)];
+    const DevScene &sc = *(const DevScene *)J.scene;
+    stage_planes(planes, cam_side, (const float4 *)J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -991,18 +1029,19 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         const uint32_t local_pixel = t / spp;
         const uint32_t s = t - local_pixel * spp;
         PathState ps;
-        init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, J.ri35[s],
+        init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, make_float2(J.ri35[2 * s], J.ri35[2 * s + 1]),
                   J.cert[J.pixel_begin + local_pixel], sc);
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == J.split_bounce && park(ps, t, lcache, J.cont[0], J.cont_cap, J.cont_count)) {
+            if (ps.bounce == J.split_bounce &&
+                park(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
                 parked = true;
                 break;
             }
         }
-        if (!parked) write_result<kEnv, false>(ps, sc, t, J.xyz, nullptr);
+        if (!parked) write_result<kEnv, false>(ps, sc, t, (float *)J.xyz, nullptr);
     }
-    flush_events(ev, block_ev, J.stats);
+    flush_events(ev, block_ev, (unsigned long long *)J.stats);
 }
 
 // Continuation level `level` (1..kContLevels) of a batched class launch: each workgroup
@@ -1010,8 +1049,8 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
 // records of the previous level, parking again for the next level except at the last.
 template <bool kAniso, bool kAbsorb, int kEnv>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
-    const BatchSlice *__restrict__ slices, uint32_t n_slices, int level) {
-    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, kContBlocks)];
+    const BatchSliceView *__restrict__ slices, uint32_t n_slices, int level) {
+    const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kContBlocks)];
     const uint32_t count = min(J.cont_count[level - 1], J.cont_cap);
     const uint32_t r = (blockIdx.x - J.cont_block_begin) * kBlock + threadIdx.x;
     if ((blockIdx.x - J.cont_block_begin) * kBlock >= count) return;
@@ -1019,8 +1058,8 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[kCacheFloats];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    const DevScene &sc = *J.scene;
-    stage_planes(planes, cam_side, J.planes, sc);
+    const DevScene &sc = *(const DevScene *)J.scene;
+    stage_planes(planes, cam_side, (const float4 *)J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1030,19 +1069,19 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
 
     if (r < count) {
         PathState ps;
-        const uint32_t t = load_path(ps, lcache, J.cont[(level - 1) & 1], J.cont_cap, r);
+        const uint32_t t = load_path(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
         const uint32_t next_split = level < kContLevels ? (uint32_t)(level + 1) * J.split_bounce : 0u;
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == next_split &&
-                park(ps, t, lcache, J.cont[level & 1], J.cont_cap, J.cont_count + level)) {
+                park(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
                 parked = true;
                 break;
             }
         }
-        if (!parked) write_result<kEnv, false>(ps, sc, t, J.xyz, nullptr);
+        if (!parked) write_result<kEnv, false>(ps, sc, t, (float *)J.xyz, nullptr);
     }
-    flush_events(ev, block_ev, J.stats);
+    flush_events(ev, block_ev, (unsigned long long *)J.stats);
 }
 
 // Escape records: field-major (record field f of tuple t at rec[f * cap + t]).
@@ -1247,20 +1286,20 @@ __global__ __launch_bounds__(kBlock) void film_reduce_kernel(const float *__rest
 
 // film_reduce_kernel over the slices of a batched tick: each slice's pixels sum their
 // spp staged tuples in sample order and add the sum into the slice's job film.
-__global__ __launch_bounds__(kBlock) void film_reduce_batch_kernel(const BatchSlice *__restrict__ slices,
+__global__ __launch_bounds__(kBlock) void film_reduce_batch_kernel(const BatchSliceView *__restrict__ slices,
                                                                    uint32_t n_slices) {
-    const BatchSlice &J = slices[find_slice(slices, n_slices, blockIdx.x, kReduceBlocks)];
+    const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kReduceBlocks)];
     const uint32_t p = (blockIdx.x - J.reduce_block_begin) * kBlock + threadIdx.x;
     if (p >= J.n_pixels) return;
     const uint32_t spp = J.spp;
-    const float *src = J.xyz + 3 * (size_t)p * spp;
+    const GR_AS_GLOBAL float *src = J.xyz + 3 * (size_t)p * spp;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for (uint32_t s = 0; s < spp; ++s) {
         sx = sx + src[3 * s + 0];
         sy = sy + src[3 * s + 1];
         sz = sz + src[3 * s + 2];
     }
-    float *dst = J.film + 3 * (size_t)(J.pixel_begin + p);
+    GR_AS_GLOBAL float *dst = J.film + 3 * (size_t)(J.pixel_begin + p);
     dst[0] = dst[0] + sx;
     dst[1] = dst[1] + sy;
     dst[2] = dst[2] + sz;
@@ -1360,13 +1399,14 @@ hipError_t launch_batch_class(const gr::BatchSlice *slices, uint32_t n_slices, u
                               uint32_t cont_blocks, hipStream_t stream) {
     constexpr bool kA = (C & 1) != 0, kB = (C & 2) != 0;
     constexpr int kE = (C & 4) ? 1 : 0;
-    hipLaunchKernelGGL((gr::trace_batch_kernel<kA, kB, kE>), dim3(n_blocks), dim3(gr::kBlock), 0, stream, slices,
+    const auto *view = reinterpret_cast<const gr::BatchSliceView *>(slices);
+    hipLaunchKernelGGL((gr::trace_batch_kernel<kA, kB, kE>), dim3(n_blocks), dim3(gr::kBlock), 0, stream, view,
                        n_slices);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || cont_blocks == 0) return e;
     for (int level = 1; level <= gr::kContLevels; ++level) {
         hipLaunchKernelGGL((gr::cont_batch_kernel<kA, kB, kE>), dim3(cont_blocks), dim3(gr::kBlock), 0, stream,
-                           slices, n_slices, level);
+                           view, n_slices, level);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -1393,7 +1433,8 @@ hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices
 hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
                                        hipStream_t stream) {
     if (n_slices == 0 || n_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(gr::film_reduce_batch_kernel, dim3(n_blocks), dim3(gr::kBlock), 0, stream, slices, n_slices);
+    hipLaunchKernelGGL(gr::film_reduce_batch_kernel, dim3(n_blocks), dim3(gr::kBlock), 0, stream,
+                       reinterpret_cast<const gr::BatchSliceView *>(slices), n_slices);
     return hipGetLastError();
 }
 

@@ -246,6 +246,9 @@ typedef struct {
     double wall_ms;                /* first admission -> last completion                  */
     double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;
     uint64_t hbm_in_use, hbm_budget, tick_tuples;
+    uint32_t policy, reserved;     /* GR_SERVE_*                                           */
+    double slack;
+    double rate_tuples_per_ms;     /* service-rate estimate behind the deadlines          */
 } gr_serve_metrics;
 
 /* Queues a job; *job_id identifies it until gr_job_release.  GR_DECLINE_* / GR_ERR_INVALID
@@ -270,6 +273,16 @@ gr_status gr_metrics(gr_ctx *ctx, gr_serve_metrics *out);
 gr_status gr_metrics_reset(gr_ctx *ctx);
 /* Tuples traced per tick (default 2^24; at least GR_MAX_SAMPLES_PER_CALL). */
 gr_status gr_serve_set_tick_tuples(gr_ctx *ctx, uint64_t tuples);
+/* How a tick's tuples are shared among running jobs (no counterpart in the reference, which
+ * serves one request at a time):
+ *   GR_SERVE_DEADLINE (default, slack 0.5): in order of deadline = submit time + slack x the
+ *     job's estimated service time (its tuples over the measured service rate).  Slack 0 is
+ *     first-come-first-served (the best worst-case latency); larger slacks let short jobs
+ *     overtake long ones (a lower median).  Deadlines are fixed at submission.
+ *   GR_SERVE_EQUAL_SHARE: every running job gets an equal share of every tick (slack unused).
+ * Films do not depend on the policy. */
+enum { GR_SERVE_EQUAL_SHARE = 0, GR_SERVE_DEADLINE = 1 };
+gr_status gr_serve_set_policy(gr_ctx *ctx, int policy, double slack);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,17 @@ def test_job_film_equals_accumulate_alone_and_in_the_mix(gpu):
     rev = _run_jobs(gpu, jobs[::-1])[::-1]
     for w, r in zip(want, rev):
         assert np.array_equal(w.view(np.uint32), r.view(np.uint32))
+    # The other scheduling policies cut the frames differently; the films stay the same.
+    gpu.set_tick_tuples(1 << 16)
+    try:
+        for policy, slack in ((_lib.GR_SERVE_EQUAL_SHARE, 0.0), (_lib.GR_SERVE_DEADLINE, 0.0),
+                              (_lib.GR_SERVE_DEADLINE, 50.0)):
+            gpu.set_policy(policy, slack)
+            for w, m in zip(want, _run_jobs(gpu, jobs)):
+                assert np.array_equal(w.view(np.uint32), m.view(np.uint32)), (policy, slack)
+    finally:
+        gpu.set_tick_tuples(1 << 24)
+        gpu.set_policy(_lib.GR_SERVE_DEADLINE)
 
 
 def test_job_counters_and_metrics(gpu):
@@ -123,6 +134,31 @@ def test_late_small_job_overtakes_a_running_large_job(gpu):
     assert gpu.job_info(b)["latency_ms"] < gpu.job_info(a)["latency_ms"]
     gpu.release(a)
     gpu.release(b)
+
+
+def test_deadline_slack_zero_is_first_come_first_served(gpu):
+    big = B.SceneRef(B.camera_new(0.35, 0.28, 5.0, 18.0), 512, 512, scenes.oval_brilliant(),
+                     scenes.pleochroic_sapphire(3), 32, B.HdrMap(scenes.procedural_env(5, 128, 64)))
+    small = _mix()[0]
+    gpu.set_policy(_lib.GR_SERVE_DEADLINE, 0.0)
+    try:
+        a = gpu.submit(big, 0, 1024)
+        time.sleep(0.01)
+        b = gpu.submit(*small)
+        assert gpu.wait(b, 60000) == _lib.GR_OK
+        # The earlier job has the earlier deadline: it was done before the small one.
+        assert gpu.poll(a) == _lib.GR_OK
+        m = gpu.metrics()
+        assert m["policy"] == _lib.GR_SERVE_DEADLINE and m["slack"] == 0.0
+        assert m["rate_tuples_per_ms"] > 0.0
+    finally:
+        gpu.set_policy(_lib.GR_SERVE_DEADLINE)
+    gpu.release(a)
+    gpu.release(b)
+    with pytest.raises(_lib.GrError):
+        gpu.set_policy(7, 1.0)
+    with pytest.raises(_lib.GrError):
+        gpu.set_policy(_lib.GR_SERVE_DEADLINE, -1.0)
 
 
 def test_cancel_frees_the_job(gpu):
