@@ -363,7 +363,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved_tf / PEAK_FP32_TFLOPS,
             "traffic": traffic,
-            "kernel": "trace_kernel" if args.trace_mode == "plain" else "trace_wave_kernel + finish_kernel",
+            "kernel": ("trace_kernel + its continuation launches (cont_kernel)" if args.trace_mode == "plain"
+                       else "trace_wave_kernel + finish_kernel"),
             "avg_launch_ms": avg_launch_s * 1e3,
             "avg_trace_kernel_ms": trace_ms / launches,
             "avg_finish_kernel_ms": finish_ms / launches,
