@@ -49,7 +49,7 @@ UNIAXIAL = ["Sapphire", "Ruby", "Emerald", "Zircon", "Quartz", "Tourmaline", "Sy
 PRESETS = ["Daylight", "Incandescent", "RingLights", "DarkSpotlight"]
 
 
-def batched_jobs(seed=3, n_jobs=64):
+def batched_jobs(seed=3, n_jobs=64, poses=None):
     """Batched workload: n_jobs seeded convex stones with 17-181 planes (cut angles jittered
     +-3 deg), an isotropic/uniaxial material mix (uniaxial ones on a seeded optic axis),
     square frames of log-uniform 128^2..1024^2 pixels, log-uniform 64..1024 spp, studio
@@ -72,11 +72,13 @@ def batched_jobs(seed=3, n_jobs=64):
             mat = replace(mat, c_axis=scenes.seeded_unit_vector(seed * 1000 + i))
         side = int(round(float(np.exp(rng.uniform(np.log(128), np.log(1024))))))
         spp = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1024))))))
-        cam = B.camera_new(float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.2, 0.6)), 5.0,
-                           float(rng.uniform(16.0, 28.0)))
+        cam_yaw, cam_pitch = float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.2, 0.6))
+        cam = B.camera_new(cam_yaw, cam_pitch, 5.0, float(rng.uniform(16.0, 28.0)))
         env = B.Studio(PRESETS[int(rng.integers(len(PRESETS)))], 1.0, float(rng.uniform(0, 2 * np.pi)),
                        float(rng.uniform(0.2, 1.0)))
         jobs.append((B.SceneRef(cam, side, side, planes, mat, 16, env), 0, spp))
+        if poses is not None:
+            poses.append((cam_yaw, cam_pitch, env.light_yaw, env.light_pitch))
     return jobs
 
 
@@ -206,6 +208,54 @@ def cpu_baseline(scene, cam_args, spp_total, budget_s):
                   f"{total_paths} paths, {total_segs} segments in {elapsed:.1f} s on {threads} threads",
         "paths_per_s": total_paths / elapsed,
     }
+
+
+def metrics_workload(gpu, args):
+    """Gemological metrics of the batched workload's 64 stones on the device: each stone at
+    its job's camera pose under its studio lights, plus its 19-pitch angular profile at that
+    azimuth (evaluate_gem_optical_metrics and evaluate_angular_profile_at_azimuth,
+    color/metrics.rs) -- 20 poses per stone, one gr_optical_metrics call per stone.  The
+    oracle (the C++ restatement, one thread) times a bounded sample of the same calls."""
+    from gemray_studio_amd import backend as B
+
+    poses = []
+    jobs = batched_jobs(n_jobs=args.batched_jobs, poses=poses)
+    calls = []
+    for (sc, _, _), (yaw, pitch, ly, lp) in zip(jobs, poses):
+        calls.append((sc.planes, sc.material, [(yaw, pitch, ly, lp)] +
+                      [(yaw, float(p), ly, lp) for p in B.PROFILE_PITCHES_RAD]))
+    gpu.optical_metrics(*calls[0])  # first-call setup outside the timed region
+    t0 = time.perf_counter()
+    results = [gpu.optical_metrics(*c) for c in calls]
+    gpu_s = time.perf_counter() - t0
+    n_eval = sum(len(c[2]) for c in calls)
+    out = {
+        "workload": f"{len(calls)} batched-workload stones (17-181 planes) x (job pose + 19-pitch angular "
+                    "profile) = 20 poses per stone, one device call per stone",
+        "evaluations": n_eval, "gpu_s": gpu_s, "evaluations_per_s": n_eval / gpu_s,
+        "fire_index_mean": float(np.mean([r["fire_index"] for rs in results for r in rs])),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        from gemray_studio_amd import materials as M
+
+        t0 = time.perf_counter()
+        n_cpu, identical = 0, True
+        for (planes, material, ps), rs in zip(calls, results):
+            m = M.to_struct(material)
+            for q, r in zip(ps, rs):
+                v = po.gem_optical_metrics(planes, m, *q)
+                identical &= all(np.float32(r[n]) == v[i] for i, n in enumerate(po.METRIC_NAMES))
+                n_cpu += 1
+            if time.perf_counter() - t0 > 3.0:
+                break
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"evaluations_per_s": n_cpu / cpu_s, "cores": 1, "kind": "port",
+                               "sample": f"oracle evaluate_gem_optical_metrics, first {n_cpu} of the same "
+                                         f"evaluations in {cpu_s:.1f} s on one thread"}
+        out["identical_to_oracle_on_sample"] = bool(identical)
+    return out
 
 
 def job_workloads(gpu, env_map, args):
@@ -374,6 +424,9 @@ def main():
     }
     if not args.no_jobs:
         out["batched"], out["served"] = job_workloads(gpu, scene.environment.rgb, args)
+        out["metrics"] = metrics_workload(gpu, args)
+        print(f"bench: metrics {out['metrics']['evaluations_per_s']:.0f} evaluations/s", file=sys.stderr,
+              flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, cam_args, args.spp, args.cpu_budget)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

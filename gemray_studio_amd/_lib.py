@@ -91,6 +91,15 @@ class JobStats(ctypes.Structure):
                 ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
 
 
+class MetricsPose(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("cam_yaw", "cam_pitch", "light_yaw", "light_pitch")]
+
+
+class GemMetrics(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in (
+        "brilliance_pct", "fire_index", "scintillation_pct", "windowing_pct", "extinction_pct")]
+
+
 class ServeMetrics(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "submitted", "admitted", "completed", "cancelled", "failed", "declined", "queued", "running",
@@ -111,10 +120,11 @@ EXPORTS = [
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
     "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_serve_set_policy",
+    "gr_optical_metrics",
 ]
 GR_SERVE_EQUAL_SHARE, GR_SERVE_DEADLINE = 0, 1
 (GR_PROBE_NEFF, GR_PROBE_POYNTING, GR_PROBE_TIR_RETARDATION, GR_PROBE_FRAME_ROTATION, GR_PROBE_FIELD_QFORM,
- GR_PROBE_EXP, GR_PROBE_CMF) = range(7)
+ GR_PROBE_EXP, GR_PROBE_CMF, GR_PROBE_ACOS) = range(8)
 
 _lib = None
 
@@ -179,6 +189,9 @@ def load():
     L.gr_serve_set_policy.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     L.gr_set_split_bounce.argtypes = [vp, ctypes.c_uint32]
     L.gr_set_split_bounce.restype = ctypes.c_int
+    L.gr_optical_metrics.argtypes = [vp, fp, ctypes.c_uint32, ctypes.POINTER(MaterialStruct),
+                                     ctypes.POINTER(MetricsPose), ctypes.c_uint32, ctypes.POINTER(GemMetrics)]
+    L.gr_optical_metrics.restype = ctypes.c_int
     L.gr_probe.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t, fp]
     L.gr_probe.restype = ctypes.c_int
     for name in ("gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel", "gr_job_film",

@@ -165,6 +165,12 @@ class Film:
             self.handle = None
 
 
+# PROFILE_ANGLES_DEG (color/metrics.rs:42-45) and f32::to_radians of each, in f32.
+PROFILE_ANGLES_DEG = np.arange(0.0, 91.0, 5.0).astype(np.float32)
+PROFILE_AZIMUTHS_DEG = np.array([0.0, 45.0, 90.0, 135.0], np.float32)
+PROFILE_PITCHES_RAD = PROFILE_ANGLES_DEG * (np.float32(np.pi) / np.float32(180.0))
+
+
 class GpuBackend:
     """``renderer::gpu_backend::GpuBackend`` over ``libgemray_hip.so``."""
 
@@ -345,6 +351,39 @@ class GpuBackend:
         check(self._L.gr_probe(self._ctx, fn, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(x),
                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "gr_probe")
         return out
+
+    def optical_metrics(self, planes: np.ndarray, material: GemMaterial, poses) -> list:
+        """evaluate_gem_optical_metrics (color/metrics.rs:1082-1248) for each pose
+        (cam_yaw, cam_pitch, light_yaw, light_pitch) of one stone, in one device call.
+        Returns one dict per pose with brilliance_pct, fire_index, scintillation_pct,
+        windowing_pct and extinction_pct."""
+        self._require()
+        p = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+        poses = list(poses)
+        arr = (_lib.MetricsPose * len(poses))()
+        for i, q in enumerate(poses):
+            arr[i].cam_yaw, arr[i].cam_pitch, arr[i].light_yaw, arr[i].light_pitch = (float(v) for v in q)
+        out = (_lib.GemMetrics * len(poses))()
+        m = to_struct(material)
+        check(self._L.gr_optical_metrics(self._ctx, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.shape[0],
+                                         ctypes.byref(m), arr, len(poses), out), "gr_optical_metrics")
+        return [{n: float(getattr(o, n)) for n, _ in _lib.GemMetrics._fields_} for o in out]
+
+    def gem_optical_metrics(self, planes, material, cam_yaw, cam_pitch, light_yaw, light_pitch) -> dict:
+        """One pose of ``optical_metrics`` (the reference's evaluate_gem_optical_metrics)."""
+        return self.optical_metrics(planes, material, [(cam_yaw, cam_pitch, light_yaw, light_pitch)])[0]
+
+    def angular_profile_at_azimuth(self, planes, material, cam_yaw, light_yaw, light_pitch):
+        """evaluate_angular_profile_at_azimuth (color/metrics.rs:1262-1290): brilliance,
+        extinction and windowing at camera pitches 0, 5, ..., 90 degrees (one device call)."""
+        ms = self.optical_metrics(planes, material,
+                                  [(cam_yaw, float(p), light_yaw, light_pitch) for p in PROFILE_PITCHES_RAD])
+        return ([m["brilliance_pct"] for m in ms], [m["extinction_pct"] for m in ms],
+                [m["windowing_pct"] for m in ms])
+
+    def angular_profile(self, planes, material, light_yaw, light_pitch):
+        """evaluate_angular_profile (color/metrics.rs:1298-1306): the canonical azimuth 0."""
+        return self.angular_profile_at_azimuth(planes, material, 0.0, light_yaw, light_pitch)
 
     def set_trace_mode(self, mode: str) -> None:
         """"plain" (one lane per tuple) or "wavefront" (persistent lanes + finish pass)."""

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgemray_hip.so")
-SOURCES = ["transport.hip", "runtime.hip", "serve.hip"]
+SOURCES = ["transport.hip", "runtime.hip", "serve.hip", "metrics.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

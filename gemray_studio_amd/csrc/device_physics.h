@@ -30,6 +30,8 @@ __device__ __forceinline__ V3 cross(V3 a, V3 b) {
 }
 __device__ __forceinline__ float len2(V3 a) { return dot(a, a); }
 __device__ __forceinline__ V3 normalize(V3 a) { return mul(a, 1.0f / sqrtf(dot(a, a))); }
+// f32::acos correctly rounded: the f64 arc cosine rounded once to f32.
+__device__ __forceinline__ float acos_cr(float x) { return (float)acos((double)x); }
 __device__ __forceinline__ V3 normalize_or_zero(V3 a) {
     float rcp = 1.0f / sqrtf(dot(a, a));
     if (isfinite(rcp) && rcp > 0.0f) return mul(a, rcp);

@@ -178,9 +178,10 @@ gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, g
  *   GR_PROBE_FRAME_ROTATION   in (prev.xyz, cur.xyz, axis.xyz)   out (cos 2psi, sin 2psi)
  *   GR_PROBE_FIELD_QFORM      in (Q, U, S.xyz, P.xyz, ao)        out (e.A.e with ae = 1 - ao)
  *   GR_PROBE_EXP              in (x)                             out (e^x as the lobes evaluate it)
- *   GR_PROBE_CMF              in (lambda_nm)                     out (xbar, ybar, zbar) */
+ *   GR_PROBE_CMF              in (lambda_nm)                     out (xbar, ybar, zbar)
+ *   GR_PROBE_ACOS             in (x)                             out (acos x as the metrics evaluate it) */
 enum { GR_PROBE_NEFF = 0, GR_PROBE_POYNTING = 1, GR_PROBE_TIR_RETARDATION = 2, GR_PROBE_FRAME_ROTATION = 3,
-       GR_PROBE_FIELD_QFORM = 4, GR_PROBE_EXP = 5, GR_PROBE_CMF = 6, GR_PROBE_COUNT = 7 };
+       GR_PROBE_FIELD_QFORM = 4, GR_PROBE_EXP = 5, GR_PROBE_CMF = 6, GR_PROBE_ACOS = 7, GR_PROBE_COUNT = 8 };
 gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out);
 
 /* Statistics of the last gr_accumulate / gr_trace_samples / gr_render call. */
@@ -204,6 +205,27 @@ gr_status gr_film_download(gr_ctx *ctx, gr_film *film, float *xyz_out, size_t n_
 /* Adds samples [sample_offset, sample_offset + spp) of every pixel into film. */
 gr_status gr_render(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset,
                     uint32_t spp, gr_film *film);
+
+/* ---- Gemological metrics ------------------------------------------------------------
+ * evaluate_gem_optical_metrics (crates/gemray/src/color/metrics.rs:1082-1248): brilliance,
+ * fire, scintillation, windowing and extinction of a cut stone seen from a camera pose
+ * under the studio rig's key/fill/ring lights.  Only the material's dispersion is read
+ * (the d, F and C line indices).  One call evaluates n_poses poses of one stone on the
+ * device; every result is bit-identical to the reference's for the same inputs.  The
+ * reference's angular profiles (evaluate_angular_profile_at_azimuth, metrics.rs:1262-1290:
+ * cam_pitch = 0, 5, ..., 90 degrees at one azimuth) are one call of 19 poses.
+ * n_planes = 0 gives the reference's placeholder values (85, 25, 75, 5, 5). */
+typedef struct {
+    float cam_yaw, cam_pitch;      /* radians                                             */
+    float light_yaw, light_pitch;  /* radians (StudioRig::new)                            */
+} gr_metrics_pose;
+
+typedef struct {
+    float brilliance_pct, fire_index, scintillation_pct, windowing_pct, extinction_pct;
+} gr_gem_metrics;
+
+gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes, const gr_material *material,
+                             const gr_metrics_pose *poses, uint32_t n_poses, gr_gem_metrics *out);
 
 /* ---- Batched and served rendering: the device-resident job loop ---------------------
  * The reference serves one RenderRequest{request_id, scene, first_sample, samples} at a

@@ -1157,6 +1157,57 @@ extern "C" void or_plane_new(const float n[3], float d, float out[4]) {
     out[2] = u.z;
     out[3] = d;
 }
+// geometry/cuts.rs:147-218 with its helpers :599-768 (34 planes).
+extern "C" int or_emerald_cut(float *p) {
+    int count = 0;
+    auto push = [&](V3 n, float d) {
+        float nn[3] = {n.x, n.y, n.z};
+        or_plane_new(nn, d, p + 4 * count);
+        ++count;
+    };
+    const float table_y = 0.30f, half_band = 0.03f, crown_crease_y = 0.15f, pav_crease = 0.35f;
+    const float r_z = 0.80f, r_x = 1.10f, r_diag = 1.00f, keel_y = -0.86f;
+    const float diag = 0.70710678118654752440f;
+    auto d_girdle = [](float a, float r, float hb) { return -fma_(tf_sin(a), r, tf_cos(a) * hb); };
+    auto d_crease = [](float ga, float ca, float r, float cm, float hb) {
+        float z_c = r - tf_cos(ga) * (cm - hb) / tf_sin(ga);
+        return -fma_(tf_sin(ca), z_c, tf_cos(ca) * cm);
+    };
+    auto corners = [&](float a, float ysign) {
+        float d = -fma_(tf_cos(a), half_band, tf_sin(a) * r_diag);
+        float y = ysign * tf_cos(a);
+        float sd = tf_sin(a) * diag;
+        push(v3(sd, y, sd), d);
+        push(v3(-sd, y, sd), d);
+        push(v3(sd, y, -sd), d);
+        push(v3(-sd, y, -sd), d);
+    };
+    auto tiers = [&](float a, float ysign, float dz, float dx) {
+        push(v3(0.0f, ysign * tf_cos(a), tf_sin(a)), dz);
+        push(v3(0.0f, ysign * tf_cos(a), -tf_sin(a)), dz);
+        push(v3(tf_sin(a), ysign * tf_cos(a), 0.0f), dx);
+        push(v3(-tf_sin(a), ysign * tf_cos(a), 0.0f), dx);
+    };
+    push(v3(0.0f, 1.0f, 0.0f), -table_y);
+    const float a1 = to_radians(35.0f), a2 = to_radians(45.0f);
+    tiers(a1, 1.0f, d_crease(a2, a1, r_z, crown_crease_y, half_band), d_crease(a2, a1, r_x, crown_crease_y, half_band));
+    tiers(a2, 1.0f, d_girdle(a2, r_z, half_band), d_girdle(a2, r_x, half_band));
+    corners(to_radians(40.0f), 1.0f);
+    push(v3(0.0f, 0.0f, 1.0f), -r_z);
+    push(v3(0.0f, 0.0f, -1.0f), -r_z);
+    push(v3(1.0f, 0.0f, 0.0f), -r_x);
+    push(v3(-1.0f, 0.0f, 0.0f), -r_x);
+    push(v3(diag, 0.0f, diag), -r_diag);
+    push(v3(-diag, 0.0f, diag), -r_diag);
+    push(v3(diag, 0.0f, -diag), -r_diag);
+    push(v3(-diag, 0.0f, -diag), -r_diag);
+    const float p1 = to_radians(53.0f), p2 = to_radians(43.0f);
+    tiers(p1, -1.0f, d_girdle(p1, r_z, half_band), d_girdle(p1, r_x, half_band));
+    tiers(p2, -1.0f, d_crease(p1, p2, r_z, pav_crease, half_band), d_crease(p1, p2, r_x, pav_crease, half_band));
+    corners(to_radians(48.0f), -1.0f);
+    push(v3(0.0f, -1.0f, 0.0f), keel_y);
+    return count;
+}
 // geometry/cuts.rs:45-129 (74 planes).
 extern "C" int or_standard_round_brilliant(float *p) {
     int count = 0;
@@ -1360,4 +1411,263 @@ extern "C" uint64_t or_render(const or_camera *cam, uint32_t width, uint32_t hei
     worker();
     for (auto &t : pool) t.join();
     return total_segments.load();
+}
+
+// ---------------------------------------------------------------------------------
+// Gemological metrics (crates/gemray/src/color/metrics.rs:10-1248).
+// ---------------------------------------------------------------------------------
+namespace {
+
+// camera_view_basis (metrics.rs:15-30).
+void view_basis(float cam_yaw, float cam_pitch, V3 *fwd, V3 *right, V3 *up) {
+    float cos_cp = tf_cos(cam_pitch), sin_cp = tf_sin(cam_pitch);
+    float cos_cy = tf_cos(cam_yaw), sin_cy = tf_sin(cam_yaw);
+    V3 f = normalize(v3(-cos_cp * sin_cy, -sin_cp, -cos_cp * cos_cy));
+    V3 world_up = std::fabs(cos_cp) < 1e-4f ? v3(0.0f, 0.0f, -1.0f) : v3(0.0f, 1.0f, 0.0f);
+    V3 r = normalize(cross(f, world_up));
+    *up = normalize(cross(r, f));
+    *fwd = f;
+    *right = r;
+}
+
+// fresnel_transmittance (metrics.rs:117-131).
+float fresnel_transmittance(float n1, float n2, float cos_i, float cos_t) {
+    float denom_s = fma_(n2, cos_t, n1 * cos_i);
+    float denom_p = fma_(n2, cos_i, n1 * cos_t);
+    if (std::fabs(denom_s) < 1e-6f || std::fabs(denom_p) < 1e-6f) return 0.0f;
+    float rs = fma_(n2, -cos_t, n1 * cos_i) / denom_s;
+    rs = rs * rs;
+    float rp = fma_(n2, -cos_i, n1 * cos_t) / denom_p;
+    rp = rp * rp;
+    float r = midpoint(rs, rp);
+    return clampf(1.0f - r, 0.0f, 1.0f);
+}
+
+enum { FATE_BLOCKED = 0, FATE_LEAKED = 1, FATE_UP = 2, FATE_ABSORBED = 3 };
+struct Fate {
+    int kind;
+    V3 dir;
+    float transmittance, exit_cos;
+    int facet;
+    uint32_t bounces;
+};
+
+// trace_wavelength (metrics.rs:138-222).
+Fate trace_wavelength(V3 entry_point, V3 incoming, V3 n_entry, float cos_i, const float *planes, int n,
+                      float index) {
+    Fate f{FATE_ABSORBED, v3(0, 0, 0), 0.0f, 0.0f, -1, 0};
+    float sin2_t = (1.0f / (index * index)) * fma_(cos_i, -cos_i, 1.0f);
+    if (sin2_t > 1.0f) {
+        f.kind = FATE_BLOCKED;
+        return f;
+    }
+    float cos_t = std::sqrt(1.0f - sin2_t);
+    float inv = 1.0f / index;
+    V3 dir = normalize(add(mul(incoming, inv), mul(n_entry, fma_(inv, cos_i, -cos_t))));
+    V3 hit_point = add(entry_point, mul(dir, 1e-4f));
+    float sin_crit = 1.0f / index;
+    float entry_t = fresnel_transmittance(1.0f, index, cos_i, cos_t);
+    for (uint32_t bounce = 0; bounce < 10; ++bounce) {
+        Hit h = intersect_planes(planes, n, hit_point, dir);
+        if (!h.hit) break;
+        V3 next_point = add(hit_point, mul(dir, h.t));
+        V3 n_out = h.normal;
+        float cos_theta = clampf(dot(dir, n_out), 0.0f, 1.0f);
+        float sin_theta = std::sqrt(maxf(fma_(cos_theta, -cos_theta, 1.0f), 0.0f));
+        if (sin_theta < sin_crit) {
+            float sin2_out = (index * index) * fma_(cos_theta, -cos_theta, 1.0f);
+            if (sin2_out <= 1.0f) {
+                float cos_out = std::sqrt(1.0f - sin2_out);
+                V3 out_dir = normalize(add(mul(dir, index), mul(n_out, fma_(index, -cos_theta, cos_out))));
+                if (n_out.y < -0.05f) {
+                    f.kind = FATE_LEAKED;
+                } else if (out_dir.y > 0.05f) {
+                    f.kind = FATE_UP;
+                    f.dir = out_dir;
+                    f.transmittance = entry_t * fresnel_transmittance(index, 1.0f, cos_theta, cos_out);
+                    f.exit_cos = cos_out;
+                    f.facet = h.facet;
+                    f.bounces = bounce;
+                }
+                return f;
+            }
+        }
+        dir = normalize(sub(dir, mul(n_out, 2.0f * cos_theta)));
+        hit_point = add(next_point, mul(dir, 1e-4f));
+    }
+    return f;
+}
+
+// ray_is_visibly_returned (metrics.rs:233-251).
+bool visibly_returned(V3 exit_dir, V3 fwd, V3 key, V3 fill, float sin_lp) {
+    bool head_shadow = dot(neg(exit_dir), fwd) > 0.96f;
+    float key_dot = maxf(dot(exit_dir, key), 0.0f);
+    float fill_dot = maxf(dot(exit_dir, fill), 0.0f);
+    bool ring = std::fabs(fma_(sin_lp, -0.8f, exit_dir.y)) < 0.35f;
+    bool lit = key_dot > 0.70f || fill_dot > 0.75f || (ring && exit_dir.y > 0.2f);
+    return !head_shadow && lit;
+}
+
+// f32::acos of the fire angle, correctly rounded (the f64 arc cosine rounded once).  The
+// reference calls the platform's acosf.  glibc 2.35's is not correctly rounded (it differs
+// by one ulp on 0.25 % of [-1, 1], none of them in [0.9999, 1], where nearly all fire angles
+// lie); the reference's golden metrics and every value its tests report hold with either,
+// and the correctly rounded value is the platform-independent one (newer glibc releases
+// round acosf correctly too).
+inline float acos_cr(float x) { return (float)std::acos((double)x); }
+
+constexpr int kGrid = 18;
+constexpr float kAperture[5][2] = {{0.0f, 0.0f}, {0.08f, 0.0f}, {-0.08f, 0.0f}, {0.0f, 0.08f}, {0.0f, -0.08f}};
+constexpr float kYawOffsetsDeg[5] = {-3.0f, -1.5f, 0.0f, 1.5f, 3.0f};
+
+inline float grid_coord(int i) { return fma_(((float)i + 0.5f) / (float)kGrid, 2.0f, -1.0f); }
+
+}  // namespace
+
+// Per-ray records of evaluate_gem_optical_metrics: for cell c = ix * 18 + iz, aperture
+// sample k: cls[5c + k] (OR_MX_*), and for a returned ray with a Fire sample its
+// (angle_deg, weight); temporal[5c + k] = 1 when the cell's ray at yaw offset k is visibly
+// returned (classify_aperture_sample metrics.rs:519-771, cell_returned_at_yaw_offset
+// :309-334).  Cells outside the perimeter get OR_MX_NONE and temporal 0.
+extern "C" void or_metrics_items(const float *planes, int n_planes, const or_material *m, float cam_yaw,
+                                 float cam_pitch, float light_yaw, float light_pitch, int32_t *cls,
+                                 float *angle, float *weight, uint8_t *temporal) {
+    const float nd = maxf(dispersion_evaluate(m, 589.3f), 1.1f);
+    const float n_f = maxf(dispersion_evaluate(m, 486.1f), 1.001f);
+    const float n_c = maxf(dispersion_evaluate(m, 656.3f), 1.001f);
+    V3 fwd, right, up;
+    view_basis(cam_yaw, cam_pitch, &fwd, &right, &up);
+    Rig rig = studio_rig(light_yaw, light_pitch);
+    const float sin_lp = tf_sin(light_pitch);
+    V3 tf[5], tr[5], tu[5];
+    for (int k = 0; k < 5; ++k) view_basis(cam_yaw + to_radians(kYawOffsetsDeg[k]), cam_pitch, &tf[k], &tr[k], &tu[k]);
+    for (int ix = 0; ix < kGrid; ++ix) {
+        for (int iz = 0; iz < kGrid; ++iz) {
+            const int c = ix * kGrid + iz;
+            const float u = grid_coord(ix), v = grid_coord(iz);
+            for (int k = 0; k < 5; ++k) {
+                cls[5 * c + k] = OR_MX_NONE;
+                angle[5 * c + k] = 0.0f;
+                weight[5 * c + k] = 0.0f;
+                temporal[5 * c + k] = 0;
+            }
+            if (fma_(v, v, u * u) > 0.70f) continue;
+            for (int k = 0; k < 5; ++k) {
+                V3 dir = normalize(add(add(fwd, mul(right, kAperture[k][0])), mul(up, kAperture[k][1])));
+                V3 org = add(add(mul(neg(fwd), 2.5f), mul(right, u * 0.95f)), mul(up, v * 0.95f));
+                Hit h = intersect_planes(planes, n_planes, org, dir);
+                if (!h.hit) continue;
+                V3 entry = add(org, mul(dir, h.t));
+                float cos_i = clampf(dot(neg(dir), h.normal), 0.0f, 1.0f);
+                Fate fd = trace_wavelength(entry, dir, h.normal, cos_i, planes, n_planes, nd);
+                int out = OR_MX_EXTINCT;
+                if (fd.kind == FATE_BLOCKED) {
+                    out = OR_MX_BLOCKED;
+                } else if (fd.kind == FATE_LEAKED) {
+                    out = OR_MX_WINDOWED;
+                } else if (fd.kind == FATE_UP && visibly_returned(fd.dir, fwd, rig.key, rig.fill, sin_lp)) {
+                    out = OR_MX_RETURNED;
+                    Fate ff = trace_wavelength(entry, dir, h.normal, cos_i, planes, n_planes, n_f);
+                    Fate fc = trace_wavelength(entry, dir, h.normal, cos_i, planes, n_planes, n_c);
+                    if (ff.kind == FATE_UP && fc.kind == FATE_UP && ff.facet == fc.facet &&
+                        ff.bounces == fc.bounces) {
+                        float cos_sep = clampf(dot(ff.dir, fc.dir), -1.0f, 1.0f);
+                        angle[5 * c + k] = acos_cr(cos_sep) * 57.2957795130823208767981548141051703f;
+                        float energy = fd.transmittance * ff.transmittance * fc.transmittance;
+                        float radiance = maxf(ff.exit_cos, 0.0f) * maxf(fc.exit_cos, 0.0f);
+                        weight[5 * c + k] = energy * radiance;
+                        out = OR_MX_RETURNED_FIRE;
+                    }
+                }
+                cls[5 * c + k] = out;
+            }
+            for (int k = 0; k < 5; ++k) {
+                V3 org = add(add(mul(neg(tf[k]), 2.5f), mul(tr[k], u * 0.95f)), mul(tu[k], v * 0.95f));
+                Hit h = intersect_planes(planes, n_planes, org, tf[k]);
+                if (!h.hit) continue;
+                V3 entry = add(org, mul(tf[k], h.t));
+                float cos_i = clampf(dot(neg(tf[k]), h.normal), 0.0f, 1.0f);
+                Fate fd = trace_wavelength(entry, tf[k], h.normal, cos_i, planes, n_planes, nd);
+                temporal[5 * c + k] = fd.kind == FATE_UP && visibly_returned(fd.dir, tf[k], rig.key, rig.fill, sin_lp);
+            }
+        }
+    }
+}
+
+// The accumulation and final metrics of evaluate_gem_optical_metrics (metrics.rs:1082-1248,
+// spatial/temporal/combined scintillation :366-450) over records in or_metrics_items'
+// layout, in the reference's (ix, iz, sample) order.
+extern "C" void or_metrics_reduce(const int32_t *cls, const float *angle, const float *weight,
+                                  const uint8_t *temporal, float out[5]) {
+    uint32_t total = 0, windowed = 0, extinct = 0, returned = 0, cell_count = 0;
+    float fire_sum = 0.0f, cfs = 0.0f, cfs2 = 0.0f, tvs = 0.0f;
+    for (int c = 0; c < kGrid * kGrid; ++c) {
+        uint32_t cell_total = 0, cell_returned = 0;
+        for (int k = 0; k < 5; ++k) {
+            const int r = cls[5 * c + k];
+            if (r == OR_MX_NONE) continue;
+            ++total;
+            ++cell_total;
+            if (r == OR_MX_WINDOWED) ++windowed;
+            else if (r == OR_MX_EXTINCT) ++extinct;
+            else if (r == OR_MX_RETURNED || r == OR_MX_RETURNED_FIRE) {
+                ++returned;
+                ++cell_returned;
+                if (r == OR_MX_RETURNED_FIRE) fire_sum = fma_(angle[5 * c + k], weight[5 * c + k], fire_sum);
+            }
+        }
+        if (cell_total > 0) {
+            float frac = (float)cell_returned / (float)cell_total;
+            cfs += frac;
+            cfs2 = fma_(frac, frac, cfs2);
+            ++cell_count;
+            uint32_t tr = 0;
+            for (int k = 0; k < 5; ++k) tr += temporal[5 * c + k] ? 1u : 0u;
+            float p = (float)tr / 5.0f;
+            tvs += fma_(p, -p, p);
+        }
+    }
+    const float n_total = (float)(total > 1u ? total : 1u);
+    const float windowing = clampf((float)windowed / n_total * 100.0f, 0.0f, 100.0f);
+    const float extinction = clampf((float)extinct / n_total * 100.0f, 0.0f, 100.0f);
+    const float brilliance = clampf((float)returned / n_total * 100.0f, 0.0f, 100.0f);
+    const float fire = maxf(fire_sum / n_total * 275.0f, 0.1f);
+    float spatial = 0.0f;
+    if (cell_count > 0) {
+        float mean = cfs / (float)cell_count;
+        if (mean > 1e-4f) {
+            float var = maxf(fma_(mean, -mean, cfs2 / (float)cell_count), 0.0f);
+            float cv = std::sqrt(var) / mean;
+            spatial = clampf(cv / (1.0f + cv) * 100.0f, 0.0f, 100.0f);
+        }
+    }
+    float temporal_pct = 0.0f;
+    if (cell_count > 0) temporal_pct = clampf((tvs / (float)cell_count) / 0.24f * 100.0f, 0.0f, 100.0f);
+    const float scint = clampf(fma_(0.4f, temporal_pct, 0.6f * spatial), 0.0f, 100.0f);
+    out[0] = brilliance;
+    out[1] = fire;
+    out[2] = scint;
+    out[3] = windowing;
+    out[4] = extinction;
+}
+
+// evaluate_gem_optical_metrics (metrics.rs:1082-1248): out = brilliance_pct, fire_index,
+// scintillation_pct, windowing_pct, extinction_pct.
+extern "C" void or_gem_optical_metrics(const float *planes, int n_planes, const or_material *m, float cam_yaw,
+                                       float cam_pitch, float light_yaw, float light_pitch, float out[5]) {
+    if (n_planes == 0) {
+        out[0] = 85.0f;
+        out[1] = 25.0f;
+        out[2] = 75.0f;
+        out[3] = 5.0f;
+        out[4] = 5.0f;
+        return;
+    }
+    const int n = 5 * kGrid * kGrid;
+    std::vector<int32_t> cls(n);
+    std::vector<float> angle(n), weight(n);
+    std::vector<uint8_t> temporal(n);
+    or_metrics_items(planes, n_planes, m, cam_yaw, cam_pitch, light_yaw, light_pitch, cls.data(), angle.data(),
+                     weight.data(), temporal.data());
+    or_metrics_reduce(cls.data(), angle.data(), weight.data(), temporal.data(), out);
 }

@@ -131,6 +131,22 @@ uint64_t or_render(const or_camera *cam, uint32_t width, uint32_t height, const 
                    uint32_t pixel_end, int threads, float *accum, float *per_sample_xyz,
                    or_path_info *per_sample_info, uint64_t *events);
 
+/* StandardGemCuts::emerald_cut (geometry/cuts.rs:147-218): writes 34 planes, returns 34. */
+int or_emerald_cut(float *p);
+/* evaluate_gem_optical_metrics (crates/gemray/src/color/metrics.rs:1082-1248): out =
+ * {brilliance_pct, fire_index, scintillation_pct, windowing_pct, extinction_pct}. */
+enum { OR_MX_NONE = 0, OR_MX_BLOCKED = 1, OR_MX_WINDOWED = 2, OR_MX_EXTINCT = 3, OR_MX_RETURNED = 4,
+       OR_MX_RETURNED_FIRE = 5 };
+#define OR_MX_ITEMS (5 * 18 * 18)
+void or_gem_optical_metrics(const float *planes, int n_planes, const or_material *m, float cam_yaw,
+                            float cam_pitch, float light_yaw, float light_pitch, float out[5]);
+/* Its per-ray records (OR_MX_ITEMS each; see the definition) and the ordered reduction. */
+void or_metrics_items(const float *planes, int n_planes, const or_material *m, float cam_yaw, float cam_pitch,
+                      float light_yaw, float light_pitch, int32_t *cls, float *angle, float *weight,
+                      uint8_t *temporal);
+void or_metrics_reduce(const int32_t *cls, const float *angle, const float *weight, const uint8_t *temporal,
+                       float out[5]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -151,6 +151,12 @@ def _declare(L):
         ctypes.POINTER(ctypes.c_uint64),
     ]
     L.or_render.restype = ctypes.c_uint64
+    L.or_emerald_cut.argtypes = [fp]
+    L.or_emerald_cut.restype = ctypes.c_int
+    L.or_gem_optical_metrics.argtypes = [fp, ctypes.c_int, ctypes.c_void_p] + [ctypes.c_float] * 4 + [fp]
+    L.or_metrics_items.argtypes = [fp, ctypes.c_int, ctypes.c_void_p] + [ctypes.c_float] * 4 + [
+        ctypes.c_void_p, fp, fp, ctypes.c_void_p]
+    L.or_metrics_reduce.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_void_p, fp]
 
 
 def fptr(a: np.ndarray):
@@ -209,6 +215,47 @@ def standard_round_brilliant() -> np.ndarray:
     out = np.zeros((74, 4), dtype=np.float32)
     n = lib().or_standard_round_brilliant(fptr(out))
     assert n == 74
+    return out
+
+
+def emerald_cut() -> np.ndarray:
+    out = np.zeros((34, 4), dtype=np.float32)
+    n = lib().or_emerald_cut(fptr(out))
+    assert n == 34
+    return out
+
+
+METRIC_NAMES = ("brilliance_pct", "fire_index", "scintillation_pct", "windowing_pct", "extinction_pct")
+MX_ITEMS = 5 * 18 * 18
+
+
+def gem_optical_metrics(planes, material_struct, cam_yaw, cam_pitch, light_yaw, light_pitch) -> np.ndarray:
+    """evaluate_gem_optical_metrics restated: float32[5] in METRIC_NAMES order."""
+    p = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+    out = np.zeros(5, dtype=np.float32)
+    lib().or_gem_optical_metrics(fptr(p), p.shape[0], ctypes.byref(material_struct), cam_yaw, cam_pitch,
+                                 light_yaw, light_pitch, fptr(out))
+    return out
+
+
+def metrics_items(planes, material_struct, cam_yaw, cam_pitch, light_yaw, light_pitch):
+    """Per-ray records (cls int32, angle, weight, temporal uint8), MX_ITEMS each."""
+    p = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+    cls = np.zeros(MX_ITEMS, np.int32)
+    angle = np.zeros(MX_ITEMS, np.float32)
+    weight = np.zeros(MX_ITEMS, np.float32)
+    temporal = np.zeros(MX_ITEMS, np.uint8)
+    lib().or_metrics_items(fptr(p), p.shape[0], ctypes.byref(material_struct), cam_yaw, cam_pitch, light_yaw,
+                           light_pitch, cls.ctypes.data, fptr(angle), fptr(weight), temporal.ctypes.data)
+    return cls, angle, weight, temporal
+
+
+def metrics_reduce(cls, angle, weight, temporal) -> np.ndarray:
+    out = np.zeros(5, dtype=np.float32)
+    cls = np.ascontiguousarray(cls, np.int32)
+    temporal = np.ascontiguousarray(temporal, np.uint8)
+    lib().or_metrics_reduce(cls.ctypes.data, fptr(np.ascontiguousarray(angle, np.float32)),
+                            fptr(np.ascontiguousarray(weight, np.float32)), temporal.ctypes.data, fptr(out))
     return out
 
 
