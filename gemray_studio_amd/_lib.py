@@ -80,12 +80,14 @@ class Stats(ctypes.Structure):
 
 class JobDesc(ctypes.Structure):
     _fields_ = [("scene", ctypes.POINTER(Scene)), ("sample_offset", ctypes.c_uint32),
-                ("spp", ctypes.c_uint32)]
+                ("spp", ctypes.c_uint32), ("subbatch_ms", ctypes.c_uint32), ("subbatch_samples", ctypes.c_uint32),
+                ("subbatch_fixed", ctypes.c_uint32)]
 
 
 class JobStats(ctypes.Structure):
     _fields_ = [("state", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("sample_offset", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("samples_done", ctypes.c_uint32), ("subbatches", ctypes.c_uint32),
                 ("footprint_bytes", ctypes.c_uint64), ("paths", ctypes.c_uint64),
                 ("events", ctypes.c_uint64 * GR_EV_COUNT), ("queue_ms", ctypes.c_double),
                 ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
@@ -120,7 +122,7 @@ EXPORTS = [
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
     "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_serve_set_policy",
-    "gr_optical_metrics",
+    "gr_optical_metrics", "gr_job_take_delta",
 ]
 GR_SERVE_EQUAL_SHARE, GR_SERVE_DEADLINE = 0, 1
 (GR_PROBE_NEFF, GR_PROBE_POYNTING, GR_PROBE_TIR_RETARDATION, GR_PROBE_FRAME_ROTATION, GR_PROBE_FIELD_QFORM,
@@ -183,6 +185,9 @@ def load():
     L.gr_job_film.argtypes = [vp, u64, fp, ctypes.c_size_t]
     L.gr_job_info.argtypes = [vp, u64, ctypes.POINTER(JobStats)]
     L.gr_job_release.argtypes = [vp, u64]
+    L.gr_job_take_delta.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), fp,
+                                    ctypes.c_size_t]
+    L.gr_job_take_delta.restype = ctypes.c_int
     L.gr_metrics.argtypes = [vp, ctypes.POINTER(ServeMetrics)]
     L.gr_metrics_reset.argtypes = [vp]
     L.gr_serve_set_tick_tuples.argtypes = [vp, u64]

@@ -274,12 +274,17 @@ class GpuBackend:
 
     # ---- batched / served jobs (gr_job_*) -------------------------------------------
 
-    def submit(self, scene: SceneRef, sample_offset: int, spp: int) -> int:
+    def submit(self, scene: SceneRef, sample_offset: int, spp: int, subbatch_ms: int = 0,
+               subbatch_samples: int = 0, subbatch_fixed: bool = False) -> int:
         """Queues a render job (a RenderRequest: scene, first_sample, samples); returns its id.
+        With subbatch_ms > 0 the job is traced in contiguous sample sub-batches sized toward
+        that many milliseconds each (from subbatch_samples, or always subbatch_samples when
+        subbatch_fixed), whose sums ``take_delta`` drains as the worker's tracer does.
         Raises GrError on a decline (GR_DECLINE_*) or invalid input."""
         self._require()
         packed = _Packed(scene)
-        desc = JobDesc(ctypes.pointer(packed.scene), sample_offset, spp)
+        desc = JobDesc(ctypes.pointer(packed.scene), sample_offset, spp, subbatch_ms, subbatch_samples,
+                       1 if subbatch_fixed else 0)
         jid = ctypes.c_uint64()
         check(self._L.gr_job_submit(self._ctx, ctypes.byref(desc), ctypes.byref(jid)), "gr_job_submit")
         return int(jid.value)
@@ -311,10 +316,22 @@ class GpuBackend:
         check(self._L.gr_job_info(self._ctx, job, ctypes.byref(s)), "gr_job_info")
         return {
             "state": _lib.JOB_STATES[s.state], "width": s.width, "height": s.height,
-            "sample_offset": s.sample_offset, "spp": s.spp, "footprint_bytes": int(s.footprint_bytes),
+            "sample_offset": s.sample_offset, "spp": s.spp, "samples_done": s.samples_done,
+            "subbatches": s.subbatches, "footprint_bytes": int(s.footprint_bytes),
             "paths": int(s.paths), "events": {n: int(s.events[i]) for i, n in enumerate(_lib.EV_NAMES)},
             "queue_ms": s.queue_ms, "run_ms": s.run_ms, "latency_ms": s.latency_ms,
         }
+
+    def take_delta(self, job: int, n_pixels: int):
+        """PendingDelta::take: (first_sample, samples, xyz[n_pixels, 3]) summed over the
+        sub-batches finished since the last take, or None when nothing is pending."""
+        self._require()
+        out = np.zeros((n_pixels, 3), np.float32)
+        first, n = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self._L.gr_job_take_delta(self._ctx, job, ctypes.byref(first), ctypes.byref(n),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n_pixels),
+              "gr_job_take_delta")
+        return None if n.value == 0 else (int(first.value), int(n.value), out)
 
     def release(self, job: int) -> None:
         self._require()

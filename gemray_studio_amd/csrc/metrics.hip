@@ -434,7 +434,7 @@ gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes
 
     std::lock_guard<std::mutex> g(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return GR_ERR_DEVICE;
-    // One allocation for the call: planes, poses, per-ray records and results.
+    // One buffer, kept by the context between calls: planes, poses, per-ray records, results.
     const size_t items = (size_t)n_poses * gr::kMxItems;
     const size_t off_poses = ((size_t)n_planes * sizeof(float4) + 255) & ~(size_t)255;
     const size_t off_fire = (off_poses + n_poses * sizeof(gr::MetricsPose) + 255) & ~(size_t)255;
@@ -442,8 +442,14 @@ gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes
     const size_t off_temporal = off_cls + items;
     const size_t off_out = (off_temporal + items + 255) & ~(size_t)255;
     const size_t bytes = off_out + (size_t)n_poses * 5 * sizeof(float);
-    char *buf = nullptr;
-    if (hipMalloc(&buf, bytes) != hipSuccess) return GR_ERR_DEVICE;
+    if (bytes > ctx->mx_cap) {
+        if (ctx->mx_buf) (void)hipFree(ctx->mx_buf);
+        ctx->mx_buf = nullptr;
+        ctx->mx_cap = 0;
+        if (hipMalloc(&ctx->mx_buf, bytes) != hipSuccess) return GR_ERR_DEVICE;
+        ctx->mx_cap = bytes;
+    }
+    char *buf = ctx->mx_buf;
     std::vector<float> res((size_t)n_poses * 5);
     gr_status st = GR_OK;
     if (hipMemcpyAsync(buf, planes, (size_t)n_planes * sizeof(float4), hipMemcpyHostToDevice, ctx->stream) !=
@@ -459,7 +465,6 @@ gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes
             hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = GR_ERR_DEVICE;
-    (void)hipFree(buf);
     if (st != GR_OK) return st;
     for (uint32_t i = 0; i < n_poses; ++i)
         out[i] = gr_gem_metrics{res[5 * i + 0], res[5 * i + 1], res[5 * i + 2], res[5 * i + 3], res[5 * i + 4]};

@@ -519,6 +519,7 @@ void gr_close(gr_ctx *ctx) {
     if (ctx->ri35) (void)hipFree(ctx->ri35);
     if (ctx->cont) (void)hipFree(ctx->cont);
     if (ctx->cont_count) (void)hipFree(ctx->cont_count);
+    if (ctx->mx_buf) (void)hipFree(ctx->mx_buf);
     if (ctx->stats) (void)hipFree(ctx->stats);
     for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_k0, ctx->ev_k1, ctx->ev_k2})
         if (e) (void)hipEventDestroy(e);

@@ -48,6 +48,8 @@ struct gr_ctx {
     float *cont = nullptr;      // continuation records, kContFields x cont_cap
     uint32_t cont_cap = 0;
     uint32_t *cont_count = nullptr;
+    char *mx_buf = nullptr;     // gr_optical_metrics: planes, poses, per-ray records, results
+    size_t mx_cap = 0;          // bytes
     gr_stats last{};
     char label[256] = {0};
     uint64_t hbm_budget = 0;  // admission budget of the job loop; 0 = 3/4 of free HBM

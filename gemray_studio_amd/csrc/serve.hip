@@ -59,7 +59,19 @@ struct Job {
     std::vector<float> map;
     DevScene host_sc{};
     int cls = 0;
-    uint32_t width = 0, height = 0, sample_offset = 0, spp = 0;
+    uint32_t width = 0, height = 0;
+    // The requested range, and the sub-batch being traced: pieces cover samples
+    // [sample_offset, sample_offset + spp) of their pixels.  A one-batch job has
+    // sample_offset = first_sample and spp = total_spp.
+    uint32_t first_sample = 0, total_spp = 0, sample_offset = 0, spp = 0;
+    // Progressive delivery: sub-batch sizing, samples folded so far, and the host-side
+    // pending delta and running total (the worker's PendingDelta and running_total).
+    bool progressive = false, fixed_batches = false;
+    uint32_t subbatch_ms = 0, produced = 0, subbatches = 0;
+    Clock::time_point t_batch;
+    std::vector<float> pending;
+    uint32_t pending_first = 0, pending_samples = 0;
+    bool delta_taken = false;
     uint64_t n_pixels = 0;
     uint64_t footprint = 0;
     // Device copies (from admission to completion).
@@ -86,6 +98,7 @@ struct Job {
 struct Piece {
     Job *job;
     uint64_t pixel_begin, n_pixels;
+    uint32_t sample_offset, spp;  // the job's sub-batch when the piece was planned
 };
 
 constexpr int kTickSlots = 2;
@@ -161,7 +174,7 @@ uint64_t job_footprint(const Job &j) {
     uint64_t b = j.n_pixels * 3 * sizeof(float);             // film
     b += j.planes.size() * sizeof(float);                    // planes
     b += j.map.size() * sizeof(float);                       // environment map
-    b += (uint64_t)j.spp * sizeof(float2);                   // sample table
+    b += (uint64_t)j.total_spp * sizeof(float2);             // sample table
     b += j.n_pixels * sizeof(uint32_t);                      // miss certificates
     b += sizeof(DevScene) + GR_EV_COUNT * sizeof(unsigned long long);
     return b;
@@ -184,15 +197,15 @@ gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMalloc(&j.d_scene, sizeof(DevScene)));
     SV_CHECK(hipMalloc(&j.d_planes, j.planes.size() * sizeof(float)));
     if (!j.map.empty()) SV_CHECK(hipMalloc(&j.d_map, j.map.size() * sizeof(float)));
-    SV_CHECK(hipMalloc(&j.d_ri35, (size_t)j.spp * sizeof(float2)));
+    SV_CHECK(hipMalloc(&j.d_ri35, (size_t)j.total_spp * sizeof(float2)));
     SV_CHECK(hipMalloc(&j.d_cert, j.n_pixels * sizeof(uint32_t)));
     SV_CHECK(hipMalloc(&j.d_film, j.n_pixels * 3 * sizeof(float)));
     SV_CHECK(hipMalloc(&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long)));
     DevScene sc = j.host_sc;
     sc.map_rgb = j.d_map;
-    std::vector<float2> ri(j.spp);
-    for (uint32_t i = 0; i < j.spp; ++i)
-        ri[i] = make_float2(h_radical_inverse(j.sample_offset + i, 3), h_radical_inverse(j.sample_offset + i, 5));
+    std::vector<float2> ri(j.total_spp);
+    for (uint32_t i = 0; i < j.total_spp; ++i)
+        ri[i] = make_float2(h_radical_inverse(j.first_sample + i, 3), h_radical_inverse(j.first_sample + i, 5));
     SV_CHECK(hipMemcpyAsync(j.d_scene, &sc, sizeof(DevScene), hipMemcpyHostToDevice, S.stream));
     SV_CHECK(hipMemcpyAsync(j.d_planes, j.planes.data(), j.planes.size() * sizeof(float), hipMemcpyHostToDevice,
                             S.stream));
@@ -209,13 +222,25 @@ gr_status upload_job(ServeState &S, Job &j) {
     return GR_OK;
 }
 
-gr_status finish_job(ServeState &S, Job &j) {
-    j.film.resize(j.n_pixels * 3);
-    SV_CHECK(hipMemcpyAsync(j.film.data(), j.d_film, j.n_pixels * 3 * sizeof(float), hipMemcpyDeviceToHost,
-                            S.stream));
-    SV_CHECK(hipMemcpyAsync(j.events, j.d_stats, sizeof(j.events), hipMemcpyDeviceToHost, S.stream));
+// Reads back the film of a job's finished (sub-)batch into `out` and, after its last
+// batch, the job's counters; before a further sub-batch the device film is cleared.
+gr_status read_batch(ServeState &S, Job &j, std::vector<float> &out, bool last) {
+    out.resize(j.n_pixels * 3);
+    SV_CHECK(hipMemcpyAsync(out.data(), j.d_film, j.n_pixels * 3 * sizeof(float), hipMemcpyDeviceToHost, S.stream));
+    if (last) SV_CHECK(hipMemcpyAsync(j.events, j.d_stats, sizeof(j.events), hipMemcpyDeviceToHost, S.stream));
+    if (!last) SV_CHECK(hipMemsetAsync(j.d_film, 0, j.n_pixels * 3 * sizeof(float), S.stream));
     SV_CHECK(hipStreamSynchronize(S.stream));
     return GR_OK;
+}
+
+// next_batch_size (apps/gemray-worker/src/stream_emit/sizing.rs:21-30): scales the previous
+// sub-batch toward the target duration, growing at most x4 and never returning 0.
+uint32_t next_batch_size(uint32_t prev, double elapsed_ms, uint32_t target_ms) {
+    const uint64_t grow64 = std::max<uint64_t>(std::min<uint64_t>((uint64_t)prev * 4u, 0xFFFFFFFFull), 1u);
+    if (!(elapsed_ms > 0.0)) return (uint32_t)grow64;
+    const double scaled = (double)prev * ((double)target_ms / elapsed_ms);
+    const double c = std::min(std::max(scaled, 1.0), (double)grow64);
+    return (uint32_t)std::round(c);
 }
 
 gr_status ensure_staging(TickSlot &T, uint64_t tuples, uint32_t slices) {
@@ -276,7 +301,7 @@ std::vector<Piece> plan_tick(ServeState &S, std::vector<Job *> jobs, uint64_t bu
             uint64_t px = std::min<uint64_t>(j->n_pixels - j->next_pixel, (budget - used) / j->spp);
             if (pieces.empty()) px = std::max<uint64_t>(px, 1);
             if (px == 0) break;
-            pieces.push_back(Piece{j, j->next_pixel, px});
+            pieces.push_back(Piece{j, j->next_pixel, px, j->sample_offset, j->spp});
             j->next_pixel += px;
             used += px * j->spp;
             if (used >= budget) break;
@@ -306,7 +331,7 @@ std::vector<Piece> plan_tick(ServeState &S, std::vector<Job *> jobs, uint64_t bu
     S.rr = (S.rr + 1) % n;
     for (size_t i = 0; i < n; ++i) {
         if (take[i] == 0) continue;
-        pieces.push_back(Piece{jobs[i], jobs[i]->next_pixel, take[i]});
+        pieces.push_back(Piece{jobs[i], jobs[i]->next_pixel, take[i], jobs[i]->sample_offset, jobs[i]->spp});
         jobs[i]->next_pixel += take[i];
     }
     return pieces;
@@ -320,7 +345,7 @@ gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_b
                      [](const Piece &a, const Piece &b) { return a.job->cls < b.job->cls; });
     uint64_t tuples = 0, cont_floats = 0;
     for (const Piece &p : pieces) {
-        const uint64_t n = p.n_pixels * p.job->spp;
+        const uint64_t n = p.n_pixels * p.spp;
         tuples += n;
         cont_floats += 2 * ((n + 3) / 4) * kContFields;
     }
@@ -340,19 +365,19 @@ gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_b
         BatchSlice &b = sl[i];
         b.scene = j.d_scene;
         b.planes = j.d_planes;
-        b.ri35 = j.d_ri35;
+        b.ri35 = j.d_ri35 + (p.sample_offset - j.first_sample);
         b.cert = j.d_cert;
         b.xyz = T.staging + 3 * off;
         b.film = j.d_film;
         b.stats = j.d_stats;
         b.pixel_begin = (uint32_t)p.pixel_begin;
         b.n_pixels = (uint32_t)p.n_pixels;
-        b.sample_offset = j.sample_offset;
-        b.spp = j.spp;
+        b.sample_offset = p.sample_offset;
+        b.spp = p.spp;
         b.reduce_block_begin = reduce_blocks;
         reduce_blocks += (uint32_t)((p.n_pixels + kBlockTuples - 1) / kBlockTuples);
         if (split_bounce) {
-            const uint64_t cap = (p.n_pixels * j.spp + 3) / 4;
+            const uint64_t cap = (p.n_pixels * p.spp + 3) / 4;
             b.split_bounce = split_bounce;
             b.cont_cap = (uint32_t)cap;
             b.cont[0] = T.cont + cont_off;
@@ -360,7 +385,7 @@ gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_b
             b.cont_count = T.cont_counts + i * (kContLevels + 1);
             cont_off += 2 * cap * kContFields;
         }
-        off += p.n_pixels * j.spp;
+        off += p.n_pixels * p.spp;
     }
     // Class launches: slices of one class are contiguous; block numbering restarts per class.
     struct ClassRange {
@@ -450,7 +475,7 @@ gr_status complete_tick(ServeState &S, TickSlot &T, std::vector<Job *> &finished
     }
     const Clock::time_point now = Clock::now();
     std::lock_guard<std::mutex> g(S.mu);
-    for (const Piece &p : T.pieces) p.job->paths_done += p.n_pixels * p.job->spp;
+    for (const Piece &p : T.pieces) p.job->paths_done += p.n_pixels * p.spp;
     if (st == GR_OK) {
         S.rate_tuples += T.tuples;
         S.rate_ms += ms_between(std::max(T.t_launch, S.t_last_complete), now);
@@ -471,10 +496,46 @@ gr_status complete_tick(ServeState &S, TickSlot &T, std::vector<Job *> &finished
 void finish_jobs(ServeState &S, std::vector<Job *> &finished) {
     if (finished.empty()) return;
     std::vector<gr_status> fin(finished.size());
-    for (size_t i = 0; i < finished.size(); ++i) fin[i] = finish_job(S, *finished[i]);
+    std::vector<std::vector<float>> batch(finished.size());
+    for (size_t i = 0; i < finished.size(); ++i) {
+        Job &j = *finished[i];
+        const bool last = j.produced + j.spp >= j.total_spp;
+        fin[i] = read_batch(S, j, j.progressive ? batch[i] : j.film, last);
+    }
+    const Clock::time_point now = Clock::now();
     std::lock_guard<std::mutex> g(S.mu);
-    for (size_t i = 0; i < finished.size(); ++i)
-        retire(S, *finished[i], fin[i] == GR_OK ? GR_JOB_DONE : GR_JOB_FAILED, fin[i]);
+    for (size_t i = 0; i < finished.size(); ++i) {
+        Job &j = *finished[i];
+        if (fin[i] != GR_OK) {
+            retire(S, j, GR_JOB_FAILED, fin[i]);
+            continue;
+        }
+        j.subbatches++;
+        if (j.progressive) {
+            // PendingDelta::add and the running total (tracer.rs:93-103): the first
+            // sub-batch since a take is copied, later ones are added in sample order.
+            const std::vector<float> &b = batch[i];
+            if (j.pending_samples == 0) {
+                j.pending = b;
+                j.pending_first = j.sample_offset;
+            } else {
+                for (size_t k = 0; k < b.size(); ++k) j.pending[k] += b[k];
+            }
+            j.pending_samples += j.spp;
+            for (size_t k = 0; k < b.size(); ++k) j.film[k] += b[k];
+        }
+        j.produced += j.spp;
+        if (j.produced >= j.total_spp) {
+            retire(S, j, GR_JOB_DONE, GR_OK);
+            continue;
+        }
+        // The next sub-batch: sized from how long this one took, over the whole frame.
+        const uint32_t want = j.fixed_batches ? j.spp : next_batch_size(j.spp, ms_between(j.t_batch, now), j.subbatch_ms);
+        j.sample_offset += j.spp;
+        j.spp = std::min(want, j.total_spp - j.produced);
+        j.next_pixel = 0;
+        j.t_batch = now;
+    }
     S.active.erase(std::remove_if(S.active.begin(), S.active.end(),
                                   [](const std::shared_ptr<Job> &j) { return j->state != GR_JOB_RUNNING; }),
                    S.active.end());
@@ -531,6 +592,7 @@ void loop(ServeState *Sp) {
             }
             j->state = GR_JOB_RUNNING;
             j->t_admit = Clock::now();
+            j->t_batch = j->t_admit;
             if (!S.have_first) {
                 S.have_first = true;
                 S.t_first = j->t_admit;
@@ -698,8 +760,15 @@ gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) 
     j->width = s->width;
     j->height = s->height;
     j->n_pixels = (uint64_t)s->width * s->height;
-    j->sample_offset = desc->sample_offset;
-    j->spp = desc->spp;
+    j->first_sample = j->sample_offset = desc->sample_offset;
+    j->total_spp = j->spp = desc->spp;
+    if (desc->subbatch_ms > 0) {
+        j->progressive = true;
+        j->subbatch_ms = desc->subbatch_ms;
+        j->fixed_batches = desc->subbatch_fixed != 0;
+        j->spp = std::min(std::max(desc->subbatch_samples, 1u), j->total_spp);
+        j->film.assign(j->n_pixels * 3, 0.0f);
+    }
     j->footprint = gr::job_footprint(*j);
     j->t_submit = gr::Clock::now();
     std::lock_guard<std::mutex> g(S->mu);
@@ -708,7 +777,8 @@ gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) 
         return GR_DECLINE_BUDGET;
     }
     const double rate = S->rate_ms > 0.0 ? (double)S->rate_tuples / S->rate_ms : gr::kDefaultRate;
-    j->deadline_ms = gr::ms_between(S->t_start, j->t_submit) + S->slack * (double)(j->n_pixels * j->spp) / rate;
+    j->deadline_ms =
+        gr::ms_between(S->t_start, j->t_submit) + S->slack * (double)(j->n_pixels * j->total_spp) / rate;
     j->id = S->next_id++;
     S->jobs[j->id] = j;
     S->queue.push_back(j);
@@ -784,8 +854,10 @@ gr_status gr_job_info(gr_ctx *ctx, uint64_t job_id, gr_job_stats *out) {
     out->state = (uint32_t)j->state;
     out->width = j->width;
     out->height = j->height;
-    out->sample_offset = j->sample_offset;
-    out->spp = j->spp;
+    out->sample_offset = j->first_sample;
+    out->spp = j->total_spp;
+    out->samples_done = j->produced;
+    out->subbatches = j->subbatches;
     out->footprint_bytes = j->footprint;
     out->paths = j->paths_done;
     if (j->state == GR_JOB_DONE)
@@ -797,6 +869,34 @@ gr_status gr_job_info(gr_ctx *ctx, uint64_t job_id, gr_job_stats *out) {
             out->run_ms = gr::ms_between(j->t_admit, j->t_done);
         }
     }
+    return GR_OK;
+}
+
+gr_status gr_job_take_delta(gr_ctx *ctx, uint64_t job_id, uint32_t *first_sample, uint32_t *samples,
+                            float *delta_xyz, size_t n_pixels) {
+    if (!ctx || !first_sample || !samples || !delta_xyz) return GR_ERR_INVALID;
+    ServeState *S = nullptr;
+    gr_status st = gr::get_serve(ctx, &S);
+    if (st != GR_OK) return st;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto j = gr::find_job(*S, job_id);
+    if (!j) return GR_ERR_NOT_FOUND;
+    if (n_pixels != j->n_pixels) return GR_ERR_INVALID;
+    *first_sample = j->first_sample;
+    *samples = 0;
+    if (j->progressive) {
+        if (j->pending_samples == 0) return GR_OK;
+        std::memcpy(delta_xyz, j->pending.data(), j->pending.size() * sizeof(float));
+        *first_sample = j->pending_first;
+        *samples = j->pending_samples;
+        j->pending_samples = 0;
+        return GR_OK;
+    }
+    // One batch: the whole film once the job is done.
+    if (j->state != GR_JOB_DONE || j->delta_taken) return GR_OK;
+    std::memcpy(delta_xyz, j->film.data(), j->film.size() * sizeof(float));
+    *samples = j->total_spp;
+    j->delta_taken = true;
     return GR_OK;
 }
 

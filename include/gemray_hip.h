@@ -246,11 +246,23 @@ typedef struct {
     const gr_scene *scene;  /* copied during gr_job_submit                               */
     uint32_t sample_offset; /* first absolute sample (RenderRequest.first_sample)        */
     uint32_t spp;           /* samples per pixel (RenderRequest.samples)                 */
+    /* Progressive delivery (the worker's tracer, stream_emit/tracer.rs:64-117): with
+     * subbatch_ms > 0 the job traces its range in contiguous sample sub-batches, each
+     * over the whole frame, sized toward subbatch_ms of tracing each (next_batch_size,
+     * stream_emit/sizing.rs:12-30: at most x4 growth per step, never 0), starting from
+     * subbatch_samples (0 = 1) -- or always subbatch_samples when subbatch_fixed is set.
+     * Every finished sub-batch is folded into a pending delta (gr_job_take_delta) and
+     * into the running total that gr_job_film returns.  subbatch_ms = 0: one batch.    */
+    uint32_t subbatch_ms;
+    uint32_t subbatch_samples;
+    uint32_t subbatch_fixed;
 } gr_job_desc;
 
 typedef struct {
     uint32_t state;                /* GR_JOB_*                                            */
     uint32_t width, height, sample_offset, spp;
+    uint32_t samples_done;         /* samples of every pixel folded into the film so far  */
+    uint32_t subbatches;           /* sub-batches finished (1 for a one-batch job)       */
     uint64_t footprint_bytes;      /* HBM held from admission to completion               */
     uint64_t paths;                /* paths traced so far                                 */
     uint64_t events[GR_EV_COUNT];  /* per-class counters (when done)                      */
@@ -288,6 +300,13 @@ gr_status gr_job_cancel(gr_ctx *ctx, uint64_t job_id);
  * gr_accumulate.  GR_PENDING if the job has not finished. */
 gr_status gr_job_film(gr_ctx *ctx, uint64_t job_id, float *accum_xyz, size_t accum_pixels);
 gr_status gr_job_info(gr_ctx *ctx, uint64_t job_id, gr_job_stats *out);
+/* PendingDelta::take (stream_emit/emitter.rs:46-95): the sum, per pixel, of the samples
+ * of every sub-batch finished since the last take, as one contiguous range
+ * [*first_sample, *first_sample + *samples); written (not added) into delta_xyz
+ * (n_pixels = width * height).  *samples = 0 and delta_xyz untouched when nothing is
+ * pending.  Works in any state, so sub-batches finished before a cancel can be drained. */
+gr_status gr_job_take_delta(gr_ctx *ctx, uint64_t job_id, uint32_t *first_sample, uint32_t *samples,
+                            float *delta_xyz, size_t n_pixels);
 /* Forgets a settled job (GR_PENDING if it has not settled). */
 gr_status gr_job_release(gr_ctx *ctx, uint64_t job_id);
 /* Loop metrics since the last reset: throughput, p50/p99 submit-to-completion latency. */

@@ -1,7 +1,7 @@
 """The trace kernel's clip shortcuts, modelled in float32 numpy and checked against the
 oracle's intersect_polyhedron restatement (CPU only).
 
-The kernel (gemray_studio_amd/csrc/transport.hip, clip_scan / exterior_miss) departs from
+The kernels (gemray_studio_amd/csrc/clip.h, clip_scan / exterior_miss) depart from
 the reference's scan in two ways that must not change any result:
 
 * the screen: a plane takes the exact division only if side + t_ext * denom > 0 (one fma,

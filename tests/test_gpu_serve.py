@@ -205,3 +205,151 @@ def test_admission_follows_the_hbm_budget():
             assert np.array_equal(_accumulate(b, sc, first, spp).view(np.uint32), f.view(np.uint32))
     finally:
         b.close()
+
+
+def _drain(gpu, jid, n_pixels, timeout_s=60.0):
+    """take_delta until the job settles (the worker's emitter loop), then once more."""
+    deltas = []
+    t0 = time.time()
+    while gpu.poll(jid) == _lib.GR_PENDING:
+        d = gpu.take_delta(jid, n_pixels)
+        if d is not None:
+            deltas.append(d)
+        time.sleep(0.002)
+        assert time.time() - t0 < timeout_s
+    d = gpu.take_delta(jid, n_pixels)
+    if d is not None:
+        deltas.append(d)
+    return deltas
+
+
+def _batch_films(gpu, scene, first, spp, k):
+    """gr_accumulate of each fixed sub-batch [first + i k, ...) of the range."""
+    out = []
+    s = first
+    while s < first + spp:
+        n = min(k, first + spp - s)
+        out.append((s, n, _accumulate(gpu, scene, s, n)))
+        s += n
+    return out
+
+
+def _check_fixed_deltas(deltas, batches, first, spp):
+    # The taken deltas tile the range in order, and each is its sub-batches' films summed
+    # in sample order (PendingDelta::add), bit for bit.
+    assert deltas[0][0] == first and sum(d[1] for d in deltas) == sum(b[1] for b in batches)
+    bi = 0
+    for d_first, d_n, d in deltas:
+        assert d_first == batches[bi][0]
+        acc, covered = None, 0
+        while covered < d_n:
+            _, n, f = batches[bi]
+            acc = f.copy() if acc is None else (acc + f).astype(np.float32)
+            covered += n
+            bi += 1
+        assert covered == d_n
+        assert np.array_equal(acc.view(np.uint32), d.view(np.uint32))
+    return bi
+
+
+def test_progressive_fixed_subbatches_are_bit_exact(gpu):
+    sc, first, spp = _mix()[2][0], 5, 10
+    n = sc.width * sc.height
+    batches = _batch_films(gpu, sc, first, spp, 3)
+    jid = gpu.submit(sc, first, spp, subbatch_ms=100, subbatch_samples=3, subbatch_fixed=True)
+    deltas = _drain(gpu, jid, n)
+    assert gpu.wait(jid, 60000) == _lib.GR_OK
+    assert _check_fixed_deltas(deltas, batches, first, spp) == len(batches)
+    # The running total: the sub-batch films added in order (tracer.rs:98-100).
+    total = np.zeros((n, 3), np.float32)
+    for _, _, f in batches:
+        total = (total + f).astype(np.float32)
+    film = np.zeros((n, 3), np.float32)
+    gpu.job_film(jid, film)
+    assert np.array_equal(film.view(np.uint32), total.view(np.uint32))
+    info = gpu.job_info(jid)
+    assert info["samples_done"] == spp and info["subbatches"] == len(batches) == 4
+    assert gpu.take_delta(jid, n) is None
+    gpu.release(jid)
+
+
+def test_progressive_subbatches_in_a_mix_equal_alone(gpu):
+    jobs = _mix()
+    sc, first, spp = jobs[4][0], 2, 9
+    n = sc.width * sc.height
+    batches = _batch_films(gpu, sc, first, spp, 2)
+    gpu.set_tick_tuples(1 << 16)
+    try:
+        others = [gpu.submit(*j) for j in jobs[:4]]
+        jid = gpu.submit(sc, first, spp, subbatch_ms=100, subbatch_samples=2, subbatch_fixed=True)
+        deltas = _drain(gpu, jid, n)
+        for o in others:
+            assert gpu.wait(o, 60000) == _lib.GR_OK
+            gpu.release(o)
+    finally:
+        gpu.set_tick_tuples(1 << 24)
+    assert _check_fixed_deltas(deltas, batches, first, spp) == len(batches)
+    gpu.release(jid)
+
+
+def test_progressive_adaptive_subbatches_cover_the_range(gpu):
+    sc = B.SceneRef(B.camera_new(0.35, 0.28, 5.0, 18.0), 192, 128, scenes.oval_brilliant(),
+                    scenes.pleochroic_sapphire(3), 32, B.HdrMap(scenes.procedural_env(5, 128, 64)))
+    first, spp = 11, 96
+    n = sc.width * sc.height
+    jid = gpu.submit(sc, first, spp, subbatch_ms=4)
+    deltas = _drain(gpu, jid, n)
+    assert gpu.wait(jid, 60000) == _lib.GR_OK
+    # Contiguous ranges from the first sample to the end.
+    at = first
+    for d_first, d_n, _ in deltas:
+        assert d_first == at and d_n > 0
+        at += d_n
+    assert at == first + spp
+    info = gpu.job_info(jid)
+    assert info["samples_done"] == spp and info["subbatches"] >= 3
+    # Sub-batch sums add up to the whole range (coalesced_deltas_sum_identically_to_un_coalesced_ones,
+    # stream_emit/tests.rs:34-55: within 1e-3 relative).
+    direct = _accumulate(gpu, sc, first, spp)
+    total = np.zeros((n, 3), np.float32)
+    for _, _, d in deltas:
+        total = (total + d).astype(np.float32)
+    scale = np.maximum(np.maximum(np.abs(total), np.abs(direct)), 1e-6)
+    assert np.max(np.abs(total - direct) / scale) < 1e-3
+    film = np.zeros((n, 3), np.float32)
+    gpu.job_film(jid, film)
+    assert np.max(np.abs(film - direct) / scale) < 1e-3
+    gpu.release(jid)
+
+
+def test_progressive_cancel_keeps_finished_subbatches(gpu):
+    sc = B.SceneRef(B.camera_new(0.35, 0.28, 5.0, 18.0), 512, 512, scenes.oval_brilliant(),
+                    scenes.pleochroic_sapphire(3), 32, B.HdrMap(scenes.procedural_env(5, 128, 64)))
+    n = sc.width * sc.height
+    jid = gpu.submit(sc, 0, 1024, subbatch_ms=100, subbatch_samples=4, subbatch_fixed=True)
+    got = []
+    t0 = time.time()
+    while not got:
+        d = gpu.take_delta(jid, n)
+        if d is not None:
+            got.append(d)
+        time.sleep(0.002)
+        assert time.time() - t0 < 60
+    gpu.cancel(jid)
+    assert gpu.wait(jid, 60000) == _lib.GR_CANCELLED
+    d = gpu.take_delta(jid, n)
+    if d is not None:
+        got.append(d)
+    info = gpu.job_info(jid)
+    assert 4 <= info["samples_done"] < 1024 and info["samples_done"] % 4 == 0
+    assert sum(x[1] for x in got) == info["samples_done"]
+    # What was delivered is exact: the first delivered range equals its sub-batches' films.
+    f0, n0, d0 = got[0]
+    assert f0 == 0
+    acc = None
+    for s in range(0, n0, 4):
+        f = _accumulate(gpu, sc, s, 4)
+        acc = f if acc is None else (acc + f).astype(np.float32)
+    assert np.array_equal(acc.view(np.uint32), d0.view(np.uint32))
+    gpu.release(jid)
+    assert gpu.metrics()["hbm_in_use"] == 0
