@@ -75,4 +75,29 @@ __device__ __forceinline__ bool exterior_miss(float4 p, V3 ro, V3 rd) {
     return fabsf(denom) > 1e-7f && denom > 0.0f && side > 0.0f;
 }
 
+// shading_normal_near_edge (optics/raytracer/intersect.rs:196-229): within
+// rounding_radius of the edge shared with the nearest other facet, the flat normal is
+// blended (smoothstep) toward the bisector of the two facet normals.  The caller skips it
+// for radius <= 0, where the reference returns the flat normal unchanged.
+__device__ __forceinline__ V3 shading_normal_near_edge(const float4 *__restrict__ planes, uint32_t n, V3 hit_point,
+                                                    int hit_facet, V3 hit_normal, float rounding_radius) {
+    float nearest_dist = __builtin_huge_valf();
+    V3 nearest_normal = hit_normal;
+    for (uint32_t i = 0; i < n; ++i) {
+        if ((int)i == hit_facet) continue;
+        const float4 p = planes[i];
+        const float dist = -(p.w + ((p.x * hit_point.x + p.y * hit_point.y) + p.z * hit_point.z));
+        if (dist < nearest_dist) {
+            nearest_dist = dist;
+            nearest_normal = v3(p.x, p.y, p.z);
+        }
+    }
+    if (nearest_dist >= rounding_radius) return hit_normal;
+    float t = 1.0f - nearest_dist / rounding_radius;
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    const float smooth_t = t * t * fmaf(2.0f, -t, 3.0f);
+    const V3 bisector = normalize_or_zero(add(hit_normal, nearest_normal));
+    return normalize_or_zero(add(mul(hit_normal, 1.0f - smooth_t), mul(bisector, smooth_t)));
+}
+
 }  // namespace gr

@@ -175,7 +175,7 @@ gr_status validate_scene(const gr_scene *s) {
     // Branches not built here: decline so the caller falls back to its CPU tracer.
     if (!std::isnan(m->biaxial_delta_beta_alpha)) return GR_DECLINE_MATERIAL;
     if (m->scattering_sigma_s > 0.0f) return GR_DECLINE_MATERIAL;
-    if (m->edge_rounding_radius > 0.0f) return GR_DECLINE_MATERIAL;
+    if (!std::isfinite(m->edge_rounding_radius)) return GR_ERR_INVALID;
     if (s->finishes)
         for (uint32_t i = 0; i < s->n_planes; ++i)
             if (s->finishes[i] != 0) return GR_DECLINE_MATERIAL;
@@ -219,6 +219,7 @@ void fill_dev_scene(const gr_scene *s, DevScene *d) {
     std::memcpy(d->o_bands, m->o_bands, sizeof(d->o_bands));
     std::memcpy(d->e_bands, m->e_bands, sizeof(d->e_bands));
     d->absorption_path_scale = m->absorption_path_scale;
+    d->edge_rounding_radius = m->edge_rounding_radius;
     const gr_env *e = s->env;
     d->env_kind = e->kind;
     if (e->kind == GR_ENV_STUDIO) {

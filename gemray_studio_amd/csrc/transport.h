@@ -32,6 +32,7 @@ struct DevScene {
     float o_bands[GR_MAX_BANDS][3];
     float e_bands[GR_MAX_BANDS][3];
     float absorption_path_scale;
+    float edge_rounding_radius;   // > 0: edge-rounded shading normals
     // environment
     int32_t env_kind;         // GR_ENV_STUDIO / GR_ENV_MAP
     float temp_k_clamped, spot_mult, exposure, denom_560;

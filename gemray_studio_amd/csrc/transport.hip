@@ -281,6 +281,9 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         facet = 0;
     }
     V3 hit_point = add(ro, mul(rd, hit_t));
+    // Facet edge rounding (transport.rs:850-857); off (the flat normal) for radius <= 0.
+    if (sc.edge_rounding_radius > 0.0f)
+        normal = shading_normal_near_edge(planes, sc.n_planes, hit_point, facet, normal, sc.edge_rounding_radius);
     V3 cur_k = ps.cur_k;
     const bool inside = ps.inside;
 

@@ -79,7 +79,8 @@ typedef struct {
     /* Branches this build declines with GR_DECLINE_MATERIAL when enabled: */
     float biaxial_delta_beta_alpha; /* NaN = None (uniaxial or isotropic)             */
     float scattering_sigma_s;       /* > 0: Henyey-Greenstein inclusions              */
-    float edge_rounding_radius;     /* > 0: edge-rounded shading normals              */
+    /* Traced: edge-rounded shading normals (intersect.rs:196-229) when > 0.          */
+    float edge_rounding_radius;
 } gr_material;
 
 /* EnvironmentSource (optics/raytracer/environment.rs:188-196). */

@@ -621,6 +621,30 @@ struct Hit {
     V3 normal;
     int facet;
 };
+
+// shading_normal_near_edge (optics/raytracer/intersect.rs:196-229): blends the flat normal
+// toward the bisector with the nearest other facet's normal within rounding_radius of
+// their shared edge (smoothstep ease); radius <= 0 returns the flat normal unchanged.
+V3 shading_normal_near_edge(const float *planes, int n, V3 hit_point, int hit_facet, V3 hit_normal,
+                            float rounding_radius) {
+    if (rounding_radius <= 0.0f) return hit_normal;
+    float nearest_dist = INFINITY;
+    V3 nearest_normal = hit_normal;
+    for (int i = 0; i < n; ++i) {
+        if (i == hit_facet) continue;
+        V3 nrm = load3(planes + 4 * i);
+        float dist = -(planes[4 * i + 3] + dot(nrm, hit_point));
+        if (dist < nearest_dist) {
+            nearest_dist = dist;
+            nearest_normal = nrm;
+        }
+    }
+    if (nearest_dist >= rounding_radius) return hit_normal;
+    float t = clampf(1.0f - nearest_dist / rounding_radius, 0.0f, 1.0f);
+    float smooth_t = t * t * fma_(2.0f, -t, 3.0f);
+    V3 bisector = normalize_or_zero(add(hit_normal, nearest_normal));
+    return normalize_or_zero(add(mul(hit_normal, 1.0f - smooth_t), mul(bisector, smooth_t)));
+}
 Hit intersect_planes(const float *planes, int n, V3 o, V3 d) {
     float t_near = -1e30f, t_far = 1e30f;
     int near_idx = -1, far_idx = -1;
@@ -944,7 +968,9 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
         }
         if (ev) ev[OR_EV_HIT]++;
         V3 hit_point = add(ray_o, mul(ray_d, hit.t));
-        V3 normal = hit.normal;  // edge rounding radius <= 0: flat normal
+        // Facet edge rounding (transport.rs:850-857).
+        V3 normal = shading_normal_near_edge(planes, n_planes, hit_point, hit.facet, hit.normal,
+                                             m->edge_rounding_radius);
 
         // rotate_stokes_to_plane_of_incidence (raytracer/absorption.rs:71-89).
         V3 cpn = normalize_or_zero(cross(current_k, normal));
@@ -1287,6 +1313,15 @@ extern "C" void or_generate_ray(const or_camera *cam, float sx, float sy, float 
     dir[0] = d.x;
     dir[1] = d.y;
     dir[2] = d.z;
+}
+extern "C" void or_shading_normal_near_edge(const float *planes, int n_planes, const float hit_point[3],
+                                            int hit_facet, const float hit_normal[3], float rounding_radius,
+                                            float out[3]) {
+    V3 r = shading_normal_near_edge(planes, n_planes, load3(hit_point), hit_facet, load3(hit_normal),
+                                    rounding_radius);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
 }
 extern "C" int or_intersect(const float *planes, int n_planes, const float origin[3],
                             const float dir[3], float *t, float normal[3], int *facet) {

@@ -25,7 +25,7 @@ extern "C" {
 enum { OR_DISP_SELLMEIER1 = 0, OR_DISP_SELLMEIER3 = 1, OR_DISP_CAUCHY = 2 };
 
 /* The subset of GemMaterial (optics/materials.rs:32-120) the polished, non-scattering,
- * non-biaxial estimator reads. */
+ * non-biaxial estimator reads (laid out as gr_material of include/gemray_hip.h). */
 typedef struct {
     int32_t disp_kind;
     float disp[6];          /* S1: b1,c1 | S3: b0,b1,b2,c0,c1,c2 | Cauchy: a,b,c */
@@ -36,6 +36,9 @@ typedef struct {
     float o_bands[OR_MAX_BANDS][3]; /* center_nm, width_nm, peak */
     float e_bands[OR_MAX_BANDS][3];
     float absorption_path_scale;
+    float biaxial_delta_beta_alpha;  /* unused (biaxial stones are not restated)         */
+    float scattering_sigma_s;        /* unused (no volume scattering)                    */
+    float edge_rounding_radius;      /* > 0: edge-rounded shading normals                */
 } or_material;
 
 /* EnvironmentSource (optics/raytracer/environment.rs:188-196).  kind 0 = Studio,
@@ -100,6 +103,8 @@ void or_camera_new(float yaw, float pitch, float distance, float fov_deg, or_cam
 void or_generate_ray(const or_camera *cam, float sx, float sy, float w, float h, float jx,
                      float jy, float origin[3], float dir[3]);
 /* Unit-level probes used by the tests. */
+void or_shading_normal_near_edge(const float *planes, int n_planes, const float hit_point[3], int hit_facet,
+                                 const float hit_normal[3], float rounding_radius, float out[3]);
 int or_intersect(const float *planes, int n_planes, const float origin[3], const float dir[3],
                  float *t, float normal[3], int *facet);
 float or_effective_extraordinary_index(float n_o, float n_e, float theta);

@@ -151,6 +151,7 @@ def _declare(L):
         ctypes.POINTER(ctypes.c_uint64),
     ]
     L.or_render.restype = ctypes.c_uint64
+    L.or_shading_normal_near_edge.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, ctypes.c_float, fp]
     L.or_emerald_cut.argtypes = [fp]
     L.or_emerald_cut.restype = ctypes.c_int
     L.or_gem_optical_metrics.argtypes = [fp, ctypes.c_int, ctypes.c_void_p] + [ctypes.c_float] * 4 + [fp]

@@ -14,6 +14,8 @@ Tiers (after the reference's harness, crates/gemray/docs/gpu.md:68-95):
   T3  statistical image comparison against the oracle on disjoint sample sets
       (estimator_check/image_comparison.rs:52-58, 221-238).
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -61,12 +63,19 @@ def _cases(srb):
         Case("pleochroic-oval-map", pleo, scenes.oval_brilliant(), B.HdrMap(env_small), 24, 24, 32),
         Case("furnace-uniform", M.furnace_material(), srb, B.HdrMap.uniform(2.5), 24, 24, 12),
         Case("stress-stepcut", scenes.stress_material(), scenes.step_cut(), B.HdrMap(env_small), 16, 16, 64),
+        # Edge-rounded shading normals (intersect.rs:196-229), on an isotropic and a
+        # uniaxial stone: hits within the radius of an edge blend toward the bisector.
+        Case("diamond-rounded-edges", dataclasses.replace(M.by_name("Diamond"), edge_rounding_radius=0.03), srb,
+             B.Studio("RingLights", 1.0, 0.85, 0.95), 24, 24, 12),
+        Case("sapphire-rounded-edges-map", dataclasses.replace(M.by_name("Sapphire"), edge_rounding_radius=0.05),
+             scenes.oval_brilliant(), B.HdrMap(env_small), 24, 24, 16),
     ]
 
 
 CASE_IDS = ["diamond-ringlights", "cz-daylight", "sapphire-ringlights", "ruby-incandescent",
             "moissanite-dark", "zircon-daylight", "tourmaline-map", "spinel-map",
-            "pleochroic-oval-map", "furnace-uniform", "stress-stepcut"]
+            "pleochroic-oval-map", "furnace-uniform", "stress-stepcut", "diamond-rounded-edges",
+            "sapphire-rounded-edges-map"]
 
 
 @pytest.mark.parametrize("idx", range(len(CASE_IDS)), ids=CASE_IDS)

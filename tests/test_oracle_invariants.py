@@ -4,6 +4,7 @@ streams, the exp polynomial and CMF batching, material data, and the furnace anc
 Each check is independent of the oracle's own formulas (closed forms, textbook values,
 float64 references or the reference's recorded expectations)."""
 import ctypes
+import dataclasses
 import math
 
 import numpy as np
@@ -279,3 +280,62 @@ def test_channel_fates_depend_on_libm_rounding():
     with po.use_library("liboracle_cr.so"):
         _, _, idd, _ = case_d.oracle_render(0, 4, per_sample=True)
     assert np.array_equal(ic["fate_hash"], idd["fate_hash"])
+
+
+# ---- edge-rounded shading normals (optics/raytracer/intersect.rs:309-475) ----
+
+CORNER = np.array([[0.0, 1.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0]], np.float32)
+
+
+def shading_normal(planes, hit_point, facet, normal, radius):
+    out = np.zeros(3, np.float32)
+    po.lib().or_shading_normal_near_edge(po.fptr(np.ascontiguousarray(planes, np.float32)), len(planes),
+                                         po.fptr(po.f32arr(hit_point)), facet, po.fptr(po.f32arr(normal)),
+                                         radius, po.fptr(out))
+    return out
+
+
+def test_edge_rounding_off_returns_the_flat_normal():
+    for r in (0.0, -1.0, -0.001):
+        assert shading_normal(CORNER, (0, 0, 5), 0, (0, 1, 0), r).tolist() == [0.0, 1.0, 0.0]
+
+
+def test_edge_rounding_is_continuous_across_a_shared_edge():
+    a = shading_normal(CORNER, (0, 0, 5), 0, (0, 1, 0), 1.0)
+    b = shading_normal(CORNER, (0, 0, 5), 1, (1, 0, 0), 1.0)
+    bis = np.array([1, 1, 0], np.float64) / np.sqrt(2.0)
+    assert np.linalg.norm(a - b) < 1e-5 and np.linalg.norm(a - bis) < 1e-5
+
+
+def test_edge_rounding_leaves_the_facet_interior_flat():
+    assert shading_normal(CORNER, (-10, 0, 5), 0, (0, 1, 0), 1.0).tolist() == [0.0, 1.0, 0.0]
+
+
+def test_edge_rounding_partial_blend_is_the_smoothstep_toward_the_bisector():
+    # 0.5 from the other facet with radius 1: t = 0.5, smoothstep 0.5 -> the normal halfway
+    # (in the blend weights) between the flat normal and the bisector, renormalized.
+    out = shading_normal(CORNER, (-0.5, 0, 5), 0, (0, 1, 0), 1.0).astype(np.float64)
+    t = 0.5
+    s = t * t * (3 - 2 * t)
+    bis = np.array([1, 1, 0]) / np.sqrt(2.0)
+    want = np.array([0, 1, 0]) * (1 - s) + bis * s
+    want /= np.linalg.norm(want)
+    assert np.linalg.norm(out - want) < 1e-6
+    # The blend angle grows monotonically toward the edge.
+    angles = [np.arccos(shading_normal(CORNER, (-d, 0, 5), 0, (0, 1, 0), 1.0)[1]) for d in (0.9, 0.6, 0.3, 0.0)]
+    assert all(a < b for a, b in zip(angles, angles[1:]))
+    assert abs(angles[-1] - np.pi / 4) < 1e-6
+
+
+def test_edge_rounding_off_is_bit_identical_in_the_tracer():
+    srb = po.standard_round_brilliant()
+    cam = po.camera_new(0.35, 0.28, 5.0, 18.0)
+    env = po.studio_env("Daylight", 1.0, 0.4, 0.35)
+    outs = []
+    for r in (0.0, -1.0, 0.02):
+        m = M.to_struct(dataclasses.replace(M.by_name("Diamond"), edge_rounding_radius=r))
+        acc, _, _, _ = po.render(cam, 6, 6, srb, m, 10, env, 0, 4, threads=1)
+        outs.append(acc)
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    # A positive radius changes the hits near edges (and only those).
+    assert not np.array_equal(outs[0], outs[2])
