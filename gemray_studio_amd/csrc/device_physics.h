@@ -38,6 +38,21 @@ __device__ __forceinline__ V3 normalize_or_zero(V3 a) {
     return v3(0.0f, 0.0f, 0.0f);
 }
 
+// cosine_weighted_hemisphere with frosted_orthonormal_basis
+// (optics/raytracer/scattering.rs:339-355): a cosine-distributed direction about n.  Out
+// of line: only frosted facets call it, and its sincos would otherwise add to the
+// register pressure of every bounce.
+__device__ __noinline__ V3 cosine_weighted_hemisphere(float u1, float u2, V3 n) {
+    const float r = sqrtf(u1);
+    const float theta = (2.0f * 3.14159265358979323846f) * u2;
+    float sin_t, cos_t;
+    sincosf(theta, &sin_t, &cos_t);
+    const V3 a = fabsf(n.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t = normalize_or_zero(sub(a, mul(n, dot(n, a))));
+    const V3 b = cross(n, t);
+    return normalize_or_zero(add(add(mul(t, r * cos_t), mul(b, r * sin_t)), mul(n, sqrtf(fmaxf(1.0f - u1, 0.0f)))));
+}
+
 // f32::clamp keeps NaN; fmaxf/fminf are IEEE maxNum/minNum like f32::max/min.
 __device__ __forceinline__ float clampf(float x, float lo, float hi) {
     if (x < lo) x = lo;
@@ -149,6 +164,8 @@ __device__ __forceinline__ V3 cmf_x8_channel(float l) {
 // ---- Sampling streams (optics/raytracer/sampling.rs) -----------------------------------
 
 constexpr uint32_t FRESNEL_BRANCH_STREAM = 0x9E3779B1u;
+constexpr uint32_t FROSTED_DIR_U_STREAM = 0x27D4EB2Fu;  // sampling.rs:28-29
+constexpr uint32_t FROSTED_DIR_V_STREAM = 0x165667B1u;
 constexpr uint32_t RUSSIAN_ROULETTE_STREAM = 0x517CC1B7u;
 constexpr uint32_t BIREFRINGENT_SPLIT_STREAM = 0x2545F491u;
 constexpr uint32_t MODE_COUPLING_STREAM = 0xCC9E2D51u;

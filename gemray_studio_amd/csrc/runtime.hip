@@ -178,7 +178,7 @@ gr_status validate_scene(const gr_scene *s) {
     if (!std::isfinite(m->edge_rounding_radius)) return GR_ERR_INVALID;
     if (s->finishes)
         for (uint32_t i = 0; i < s->n_planes; ++i)
-            if (s->finishes[i] != 0) return GR_DECLINE_MATERIAL;
+            if (s->finishes[i] > GR_FINISH_FROSTED) return GR_ERR_INVALID;
     return GR_OK;
 }
 
@@ -220,6 +220,10 @@ void fill_dev_scene(const gr_scene *s, DevScene *d) {
     std::memcpy(d->e_bands, m->e_bands, sizeof(d->e_bands));
     d->absorption_path_scale = m->absorption_path_scale;
     d->edge_rounding_radius = m->edge_rounding_radius;
+    d->n_frosted = 0;
+    if (s->finishes)
+        for (uint32_t i = 0; i < s->n_planes; ++i) d->n_frosted += s->finishes[i] == GR_FINISH_FROSTED ? 1u : 0u;
+    d->finishes = nullptr;  // device copy set by the uploader
     const gr_env *e = s->env;
     d->env_kind = e->kind;
     if (e->kind == GR_ENV_STUDIO) {
@@ -278,6 +282,7 @@ using gr::validate_scene;
 struct gr_scene_dev {
     gr::DevScene sc;
     float4 *planes = nullptr;
+    uint32_t *finishes = nullptr;  // only when some facet is frosted
     float *map = nullptr;
     uint32_t *cert = nullptr;  // per-pixel camera-ray miss certificates
 };
@@ -366,6 +371,12 @@ gr_status upload_scene(gr_ctx *ctx, const gr_scene *s, gr_scene_dev *out) {
     GR_CHECK(hipMalloc(&out->planes, s->n_planes * sizeof(float4)));
     GR_CHECK(hipMemcpyAsync(out->planes, s->planes, s->n_planes * sizeof(float4), hipMemcpyHostToDevice,
                             ctx->stream));
+    if (out->sc.n_frosted) {
+        GR_CHECK(hipMalloc(&out->finishes, s->n_planes * sizeof(uint32_t)));
+        GR_CHECK(hipMemcpyAsync(out->finishes, s->finishes, s->n_planes * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                ctx->stream));
+        out->sc.finishes = out->finishes;
+    }
     if (s->env->kind == GR_ENV_MAP) {
         size_t bytes = (size_t)s->env->map_w * s->env->map_h * 3 * sizeof(float);
         GR_CHECK(hipMalloc(&out->map, bytes));
@@ -381,9 +392,11 @@ gr_status upload_scene(gr_ctx *ctx, const gr_scene *s, gr_scene_dev *out) {
 void free_scene(gr_scene_dev *s) {
     if (!s) return;
     if (s->planes) (void)hipFree(s->planes);
+    if (s->finishes) (void)hipFree(s->finishes);
     if (s->map) (void)hipFree(s->map);
     if (s->cert) (void)hipFree(s->cert);
     s->planes = nullptr;
+    s->finishes = nullptr;
     s->map = nullptr;
     s->cert = nullptr;
 }

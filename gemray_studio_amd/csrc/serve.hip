@@ -56,6 +56,7 @@ struct Job {
     gr_status error = GR_OK;
     // Host copies of the request (inputs are borrowed only during gr_job_submit).
     std::vector<float> planes;
+    std::vector<uint32_t> finishes;  // only when some facet is frosted
     std::vector<float> map;
     DevScene host_sc{};
     int cls = 0;
@@ -77,6 +78,7 @@ struct Job {
     // Device copies (from admission to completion).
     DevScene *d_scene = nullptr;
     float4 *d_planes = nullptr;
+    uint32_t *d_finishes = nullptr;
     float *d_map = nullptr;
     float2 *d_ri35 = nullptr;
     uint32_t *d_cert = nullptr;
@@ -173,6 +175,7 @@ namespace {
 uint64_t job_footprint(const Job &j) {
     uint64_t b = j.n_pixels * 3 * sizeof(float);             // film
     b += j.planes.size() * sizeof(float);                    // planes
+    b += j.finishes.size() * sizeof(uint32_t);               // facet finishes
     b += j.map.size() * sizeof(float);                       // environment map
     b += (uint64_t)j.total_spp * sizeof(float2);             // sample table
     b += j.n_pixels * sizeof(uint32_t);                      // miss certificates
@@ -181,11 +184,12 @@ uint64_t job_footprint(const Job &j) {
 }
 
 void free_job_device(Job &j) {
-    for (void *p : {(void *)j.d_scene, (void *)j.d_planes, (void *)j.d_map, (void *)j.d_ri35, (void *)j.d_cert,
-                    (void *)j.d_film, (void *)j.d_stats})
+    for (void *p : {(void *)j.d_scene, (void *)j.d_planes, (void *)j.d_finishes, (void *)j.d_map, (void *)j.d_ri35,
+                    (void *)j.d_cert, (void *)j.d_film, (void *)j.d_stats})
         if (p) (void)hipFree(p);
     j.d_scene = nullptr;
     j.d_planes = nullptr;
+    j.d_finishes = nullptr;
     j.d_map = nullptr;
     j.d_ri35 = nullptr;
     j.d_cert = nullptr;
@@ -196,6 +200,7 @@ void free_job_device(Job &j) {
 gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMalloc(&j.d_scene, sizeof(DevScene)));
     SV_CHECK(hipMalloc(&j.d_planes, j.planes.size() * sizeof(float)));
+    if (!j.finishes.empty()) SV_CHECK(hipMalloc(&j.d_finishes, j.finishes.size() * sizeof(uint32_t)));
     if (!j.map.empty()) SV_CHECK(hipMalloc(&j.d_map, j.map.size() * sizeof(float)));
     SV_CHECK(hipMalloc(&j.d_ri35, (size_t)j.total_spp * sizeof(float2)));
     SV_CHECK(hipMalloc(&j.d_cert, j.n_pixels * sizeof(uint32_t)));
@@ -203,12 +208,16 @@ gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMalloc(&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long)));
     DevScene sc = j.host_sc;
     sc.map_rgb = j.d_map;
+    sc.finishes = j.d_finishes;
     std::vector<float2> ri(j.total_spp);
     for (uint32_t i = 0; i < j.total_spp; ++i)
         ri[i] = make_float2(h_radical_inverse(j.first_sample + i, 3), h_radical_inverse(j.first_sample + i, 5));
     SV_CHECK(hipMemcpyAsync(j.d_scene, &sc, sizeof(DevScene), hipMemcpyHostToDevice, S.stream));
     SV_CHECK(hipMemcpyAsync(j.d_planes, j.planes.data(), j.planes.size() * sizeof(float), hipMemcpyHostToDevice,
                             S.stream));
+    if (!j.finishes.empty())
+        SV_CHECK(hipMemcpyAsync(j.d_finishes, j.finishes.data(), j.finishes.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, S.stream));
     if (!j.map.empty())
         SV_CHECK(hipMemcpyAsync(j.d_map, j.map.data(), j.map.size() * sizeof(float), hipMemcpyHostToDevice,
                                 S.stream));
@@ -755,6 +764,7 @@ gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) 
     gr::fill_dev_scene(s, &j->host_sc);
     j->cls = gr::scene_class(j->host_sc);
     j->planes.assign(s->planes, s->planes + 4 * (size_t)s->n_planes);
+    if (j->host_sc.n_frosted) j->finishes.assign(s->finishes, s->finishes + s->n_planes);
     if (s->env->kind == GR_ENV_MAP)
         j->map.assign(s->env->map_rgb, s->env->map_rgb + (size_t)s->env->map_w * s->env->map_h * 3);
     j->width = s->width;

@@ -33,6 +33,8 @@ struct DevScene {
     float e_bands[GR_MAX_BANDS][3];
     float absorption_path_scale;
     float edge_rounding_radius;   // > 0: edge-rounded shading normals
+    uint32_t n_frosted;           // facets with FacetFinish::Frosted (0: finishes unused)
+    const uint32_t *finishes;     // device, n_planes FacetFinish values, or null
     // environment
     int32_t env_kind;         // GR_ENV_STUDIO / GR_ENV_MAP
     float temp_k_clamped, spot_mult, exposure, denom_560;
@@ -97,9 +99,11 @@ struct BatchSlice {
     uint32_t *cont_count;
 };
 
-// Material/environment class of a scene: bit 0 anisotropic, bit 1 absorbing, bit 2 HDR map.
+// Material/environment class of a scene: bit 0 anisotropic, bit 1 absorbing, bit 2 HDR map,
+// bit 3 some facet frosted.
 inline int scene_class(const DevScene &sc) {
-    return (sc.anisotropic ? 1 : 0) | (sc.has_absorption ? 2 : 0) | (sc.env_kind == GR_ENV_MAP ? 4 : 0);
+    return (sc.anisotropic ? 1 : 0) | (sc.has_absorption ? 2 : 0) | (sc.env_kind == GR_ENV_MAP ? 4 : 0) |
+           (sc.n_frosted ? 8 : 0);
 }
 
 }  // namespace gr

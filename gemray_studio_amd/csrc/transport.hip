@@ -192,7 +192,7 @@ __device__ __forceinline__ bool camera_miss_certified(const float4 *__restrict__
 
 // One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
 // has ended (escape, Russian roulette, or the bounce cap).
-template <bool kAniso, bool kAbsorb>
+template <bool kAniso, bool kAbsorb, bool kFrost>
 __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, const float4 *__restrict__ planes,
                                             const float *__restrict__ cam_side, float *__restrict__ lcache,
                                             uint32_t *ev) {
@@ -427,7 +427,50 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         r_unpol = clampf(0.5f * fmaf(r_p, r_p, r_s * r_s), 1e-4f, 1.0f - 1e-4f);
         reflect = stream_draw(ps.seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol;
     }
-    if (reflect) {
+    if (kFrost && sc.n_frosted != 0u && sc.finishes[facet] == GR_FINISH_FROSTED) {
+        // apply_frosted_bounce (scattering.rs:522-590): the hero's TIR or Fresnel draw
+        // (above) picks a diffuse reflection or transmission, cosine-distributed about the
+        // facet normal; every channel is depolarized, its intensity divided by the hero's
+        // branch probability, and k' = S'.
+        const float u1 = stream_draw(ps.seed, bounce, FROSTED_DIR_U_STREAM);
+        const float u2 = stream_draw(ps.seed, bounce, FROSTED_DIR_V_STREAM);
+        if (reflect) {
+            if (tir) {
+                ev_inc(ev, GR_EV_TIR);
+                event = 4u;
+            } else {
+                ev_inc(ev, GR_EV_REFLECT);
+                event = 5u;
+            }
+            new_k = cosine_weighted_hemisphere(u1, u2, normal);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float i = intensity(ps.st[k]);
+                ps.st[k] = Stokes{tir ? i : i / r_unpol, 0.0f, 0.0f, 0.0f};
+                if (!tir) ps.pdf[k] *= r_unpol;
+            }
+        } else {
+            ev_inc(ev, GR_EV_REFRACT);
+            event = 6u;
+            new_k = cosine_weighted_hemisphere(u1, u2, neg(normal));
+            const bool entering = kAniso && !inside;
+            bool use_extra = ps.is_extra;
+            if (entering) {
+                ev_inc(ev, GR_EV_ENTRY_ANISO);
+                use_extra = stream_draw(ps.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
+            }
+            const float t_unpol = 1.0f - r_unpol;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                ps.st[k] = Stokes{intensity(ps.st[k]) / t_unpol, 0.0f, 0.0f, 0.0f};
+                ps.pdf[k] *= t_unpol;
+            }
+            new_inside = !inside;
+            has_update = entering;
+            update = use_extra;
+        }
+        new_s = new_k;
+    } else if (reflect) {
         // apply_tir_bounce (refraction.rs:599-637) and apply_partial_reflect_bounce
         // (:650-690) share their per-channel work: retardation for channels past their
         // own critical angle, Fresnel reflection (and its pdf factor) otherwise.  The
@@ -810,7 +853,7 @@ __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block
 // fully populated waves, parking again at each further multiple of split_bounce for
 // kContLevels levels.  Parking moves state, not arithmetic: every path's result is
 // bit-identical to an unsplit run.  When the records run out, the lane carries on.
-template <bool kAniso, bool kAbsorb, int kEnv, bool kInfo>
+template <bool kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
     const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp,
@@ -836,7 +879,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
         PathState ps;
         init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
         bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == split_bounce && park(ps, t, lcache, cont, cont_cap, cont_count)) {
                 parked = true;
                 break;
@@ -851,7 +894,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
 // (capped at cont_cap); workgroups past the count return at once.  With split_bounce > 0
 // the paths still running at that bounce are parked again, into next / next_count, for
 // the next level.
-template <bool kAniso, bool kAbsorb, int kEnv, bool kInfo>
+template <bool kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float *__restrict__ cont, uint32_t cont_cap,
     const uint32_t *__restrict__ cont_count, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
@@ -876,7 +919,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
         PathState ps;
         const uint32_t t = load_path(ps, lcache, cont, cont_cap, r);
         bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == split_bounce && park(ps, t, lcache, next, cont_cap, next_count)) {
                 parked = true;
                 break;
@@ -945,7 +988,7 @@ __device__ __forceinline__ uint32_t find_slice(const BatchSliceView *__restrict_
 // belongs to one slice, stages that job's planes and reads that job's scene block; the
 // per-lane work is the plain kernel's, so each tuple's result is bit-identical to a plain
 // launch of the same job.
-template <bool kAniso, bool kAbsorb, int kEnv>
+template <bool kAniso, bool kAbsorb, int kEnv, bool kFrost>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     const BatchSliceView *__restrict__ slices, uint32_t n_slices) {
     __shared__ float4 planes[GR_MAX_PLANES];
@@ -971,7 +1014,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, make_float2(J.ri35[2 * s], J.ri35[2 * s + 1]),
                   J.cert[J.pixel_begin + local_pixel], sc);
         bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == J.split_bounce &&
                 park(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
                 parked = true;
@@ -986,7 +1029,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
 // Continuation level `level` (1..kContLevels) of a batched class launch: each workgroup
 // belongs to one slice (binary search of cont_block_begin) and continues that slice's
 // records of the previous level, parking again for the next level except at the last.
-template <bool kAniso, bool kAbsorb, int kEnv>
+template <bool kAniso, bool kAbsorb, int kEnv, bool kFrost>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
     const BatchSliceView *__restrict__ slices, uint32_t n_slices, int level) {
     const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kContBlocks)];
@@ -1011,7 +1054,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
         const uint32_t t = load_path(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
         const uint32_t next_split = level < kContLevels ? (uint32_t)(level + 1) * J.split_bounce : 0u;
         bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == next_split &&
                 park(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
                 parked = true;
@@ -1068,7 +1111,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
         init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], cert[pixel_begin + lp], sc);
     }
     while (__ballot(busy) != 0ull) {
-        if (busy && bounce_step<kAniso, kAbsorb>(ps, sc, planes, cam_side, lcache, ev)) {
+        if (busy && bounce_step<kAniso, kAbsorb, true>(ps, sc, planes, cam_side, lcache, ev)) {
             // Path ended: escape record (and per-tuple info).
             float *r = rec + t;
             r[0 * (size_t)rec_cap] = ps.ray_d.x;
@@ -1255,7 +1298,7 @@ __global__ __launch_bounds__(kBlock) void film_reduce_batch_kernel(const BatchSl
 
 namespace {
 
-template <bool A, bool B, int E, bool I>
+template <bool A, bool B, int E, bool F, bool I>
 hipError_t launch_plain_i(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
     dim3 grid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
     const bool split = a.split_bounce != 0 && a.cont != nullptr && a.cont_cap != 0;
@@ -1263,7 +1306,7 @@ hipError_t launch_plain_i(const gr::DevScene &sc, const gr::TraceArgs &a, hipStr
         hipError_t e = hipMemsetAsync(a.cont_count, 0, (gr::kContLevels + 1) * sizeof(uint32_t), stream);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((gr::trace_kernel<A, B, E, I>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes, a.ri35,
+    hipLaunchKernelGGL((gr::trace_kernel<A, B, E, F, I>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes, a.ri35,
                        a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats,
                        split ? a.split_bounce : 0u, a.cont, a.cont_cap, a.cont_count);
     hipError_t e = hipGetLastError();
@@ -1276,7 +1319,7 @@ hipError_t launch_plain_i(const gr::DevScene &sc, const gr::TraceArgs &a, hipStr
         float *in = a.cont + (size_t)((k - 1) & 1) * gr::kContFields * a.cont_cap;
         float *out = a.cont + (size_t)(k & 1) * gr::kContFields * a.cont_cap;
         const uint32_t next_split = k < gr::kContLevels ? (uint32_t)(k + 1) * a.split_bounce : 0u;
-        hipLaunchKernelGGL((gr::cont_kernel<A, B, E, I>), cgrid, dim3(gr::kBlock), 0, stream, sc, a.planes, in,
+        hipLaunchKernelGGL((gr::cont_kernel<A, B, E, F, I>), cgrid, dim3(gr::kBlock), 0, stream, sc, a.planes, in,
                            a.cont_cap, a.cont_count + (k - 1), a.xyz, a.info, a.stats, next_split, out,
                            a.cont_count + k);
         e = hipGetLastError();
@@ -1287,7 +1330,13 @@ hipError_t launch_plain_i(const gr::DevScene &sc, const gr::TraceArgs &a, hipStr
 
 template <bool A, bool B, int E>
 hipError_t launch_plain(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
-    return a.info ? launch_plain_i<A, B, E, true>(sc, a, stream) : launch_plain_i<A, B, E, false>(sc, a, stream);
+    // Scenes with frosted facets take the kernels built with the frosted bounce; the others
+    // keep kernels without it (its code raises the register pressure of every bounce).
+    if (sc.n_frosted)
+        return a.info ? launch_plain_i<A, B, E, true, true>(sc, a, stream)
+                      : launch_plain_i<A, B, E, true, false>(sc, a, stream);
+    return a.info ? launch_plain_i<A, B, E, false, true>(sc, a, stream)
+                  : launch_plain_i<A, B, E, false, false>(sc, a, stream);
 }
 
 template <bool A, bool B, int E>
@@ -1339,15 +1388,15 @@ namespace {
 template <int C>
 hipError_t launch_batch_class(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
                               uint32_t cont_blocks, hipStream_t stream) {
-    constexpr bool kA = (C & 1) != 0, kB = (C & 2) != 0;
+    constexpr bool kA = (C & 1) != 0, kB = (C & 2) != 0, kF = (C & 8) != 0;
     constexpr int kE = (C & 4) ? 1 : 0;
     const auto *view = reinterpret_cast<const gr::BatchSliceView *>(slices);
-    hipLaunchKernelGGL((gr::trace_batch_kernel<kA, kB, kE>), dim3(n_blocks), dim3(gr::kBlock), 0, stream, view,
+    hipLaunchKernelGGL((gr::trace_batch_kernel<kA, kB, kE, kF>), dim3(n_blocks), dim3(gr::kBlock), 0, stream, view,
                        n_slices);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || cont_blocks == 0) return e;
     for (int level = 1; level <= gr::kContLevels; ++level) {
-        hipLaunchKernelGGL((gr::cont_batch_kernel<kA, kB, kE>), dim3(cont_blocks), dim3(gr::kBlock), 0, stream,
+        hipLaunchKernelGGL((gr::cont_batch_kernel<kA, kB, kE, kF>), dim3(cont_blocks), dim3(gr::kBlock), 0, stream,
                            view, n_slices, level);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -1368,6 +1417,14 @@ hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices
         case 5: return launch_batch_class<5>(slices, n_slices, n_blocks, cont_blocks, stream);
         case 6: return launch_batch_class<6>(slices, n_slices, n_blocks, cont_blocks, stream);
         case 7: return launch_batch_class<7>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 8: return launch_batch_class<8>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 9: return launch_batch_class<9>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 10: return launch_batch_class<10>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 11: return launch_batch_class<11>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 12: return launch_batch_class<12>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 13: return launch_batch_class<13>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 14: return launch_batch_class<14>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 15: return launch_batch_class<15>(slices, n_slices, n_blocks, cont_blocks, stream);
         default: return hipErrorInvalidValue;
     }
 }

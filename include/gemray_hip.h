@@ -83,6 +83,10 @@ typedef struct {
     float edge_rounding_radius;
 } gr_material;
 
+/* FacetFinish (optics/raytracer/camera.rs:43-51): a frosted facet scatters diffusely
+ * (scattering.rs:522-590) instead of reflecting and refracting specularly. */
+enum { GR_FINISH_POLISHED = 0, GR_FINISH_FROSTED = 1 };
+
 /* EnvironmentSource (optics/raytracer/environment.rs:188-196). */
 enum { GR_ENV_STUDIO = 0, GR_ENV_MAP = 1 };
 typedef struct {
@@ -101,7 +105,7 @@ typedef struct {
     uint32_t width, height;
     const float *planes;       /* n_planes x {normal.x, normal.y, normal.z, d}           */
     uint32_t n_planes;
-    const uint32_t *finishes;  /* NULL = all polished; else n_planes, 0 = polished       */
+    const uint32_t *finishes;  /* NULL = all polished; else n_planes GR_FINISH_* values  */
     const gr_material *material;
     uint32_t max_bounces;
     const gr_env *env;

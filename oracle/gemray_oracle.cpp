@@ -271,6 +271,8 @@ void cmf_x8(const float l[8], float out[8][3]) {
 // ---------------------------------------------------------------------------------
 
 constexpr uint32_t FRESNEL_BRANCH_STREAM = 0x9E3779B1u;
+constexpr uint32_t FROSTED_DIR_U_STREAM = 0x27D4EB2Fu;  // sampling.rs:28-29
+constexpr uint32_t FROSTED_DIR_V_STREAM = 0x165667B1u;
 constexpr uint32_t RUSSIAN_ROULETTE_STREAM = 0x517CC1B7u;
 constexpr uint32_t BIREFRINGENT_SPLIT_STREAM = 0x2545F491u;
 constexpr uint32_t MODE_COUPLING_STREAM = 0xCC9E2D51u;
@@ -792,6 +794,18 @@ inline void fresnel_rs_rp(float n1k, float n2k, float cos_i, float cos_t, float 
     *r_p = fma_(n1k, -cos_t, n2k * cos_i) / fma_(n1k, cos_t, n2k * cos_i);
 }
 
+// frosted_orthonormal_basis and cosine_weighted_hemisphere
+// (optics/raytracer/scattering.rs:339-355).
+V3 cosine_weighted_hemisphere(float u1, float u2, V3 n) {
+    float r = std::sqrt(u1);
+    float theta = 2.0f * PI_F * u2;
+    float sin_t = tf_sin(theta), cos_t = tf_cos(theta);
+    V3 a = std::fabs(n.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    V3 t = normalize_or_zero(sub(a, mul(n, dot(n, a))));
+    V3 b = cross(n, t);
+    return normalize_or_zero(add(add(mul(t, r * cos_t), mul(b, r * sin_t)), mul(n, std::sqrt(maxf(1.0f - u1, 0.0f)))));
+}
+
 // refraction.rs:599-637.
 V3 apply_tir_bounce(const Geo &g, V3 k_hat, V3 normal, Stokes *st, float *pdf) {
     for (int k = 0; k < 8; ++k) {
@@ -914,10 +928,10 @@ inline void fnv(uint32_t &h, uint32_t v) {
 // ---------------------------------------------------------------------------------
 
 extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
-                                      const float *planes, int n_planes, const or_material *m,
-                                      uint32_t max_bounces, const or_env *env, uint32_t rng_seed,
-                                      float hero_rand, float out_xyz[3], or_path_info *info,
-                                      uint64_t *ev) {
+                                      const float *planes, int n_planes, const uint32_t *finishes,
+                                      const or_material *m, uint32_t max_bounces, const or_env *env,
+                                      uint32_t rng_seed, float hero_rand, float out_xyz[3],
+                                      or_path_info *info, uint64_t *ev) {
     float lambdas[8];
     wrapped_hero_wavelengths(hero_rand, lambdas);
     Stokes stokes[8];
@@ -1007,12 +1021,57 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
 
         Geo g = compute_geo(ctx, normal, current_k, inside, is_extra);
 
-        // dispatch_bounce (transport.rs:128-193), polished finish.
+        // dispatch_bounce (transport.rs:128-193): FacetFinish::Frosted (1) takes
+        // apply_frosted_bounce (scattering.rs:522-590), Polished (0 / no finishes) the
+        // specular TIR / partial Fresnel dispatch.
         V3 new_k, new_s;
         bool new_inside;
         bool has_update = false, update = false;
         uint32_t event;
-        if (g.sin2_t > 1.0f) {
+        if (finishes && finishes[hit.facet] == 1u) {
+            float u1 = stream_draw(rng_seed, bounce, FROSTED_DIR_U_STREAM);
+            float u2 = stream_draw(rng_seed, bounce, FROSTED_DIR_V_STREAM);
+            new_inside = inside;
+            if (g.sin2_t > 1.0f) {
+                if (ev) ev[OR_EV_TIR]++;
+                event = 4;
+                new_k = cosine_weighted_hemisphere(u1, u2, normal);
+                for (int k = 0; k < 8; ++k) stokes[k] = Stokes{intensity(stokes[k]), 0.0f, 0.0f, 0.0f};
+            } else {
+                float cos_t = std::sqrt(maxf(1.0f - g.sin2_t, 0.0f));
+                float r_s, r_p;
+                fresnel_rs_rp(g.n1, g.n2, g.cos_i, cos_t, &r_s, &r_p);
+                float r_unpol = clampf(0.5f * fma_(r_p, r_p, r_s * r_s), 1e-4f, 1.0f - 1e-4f);
+                if (stream_draw(rng_seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol) {
+                    if (ev) ev[OR_EV_REFLECT]++;
+                    event = 5;
+                    new_k = cosine_weighted_hemisphere(u1, u2, normal);
+                    for (int k = 0; k < 8; ++k) {
+                        stokes[k] = Stokes{intensity(stokes[k]) / r_unpol, 0.0f, 0.0f, 0.0f};
+                        path_pdf[k] *= r_unpol;
+                    }
+                } else {
+                    if (ev) ev[OR_EV_REFRACT]++;
+                    event = 6;
+                    new_k = cosine_weighted_hemisphere(u1, u2, neg(normal));
+                    bool entering = !inside && ctx.is_anisotropic;
+                    bool use_extra = is_extra;
+                    if (entering) {
+                        if (ev) ev[OR_EV_ENTRY_ANISO]++;
+                        use_extra = stream_draw(rng_seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
+                    }
+                    float t_unpol = 1.0f - r_unpol;
+                    for (int k = 0; k < 8; ++k) {
+                        stokes[k] = Stokes{intensity(stokes[k]) / t_unpol, 0.0f, 0.0f, 0.0f};
+                        path_pdf[k] *= t_unpol;
+                    }
+                    new_inside = !inside;
+                    has_update = entering;
+                    update = use_extra;
+                }
+            }
+            new_s = new_k;
+        } else if (g.sin2_t > 1.0f) {
             if (ev) ev[OR_EV_TIR]++;
             event = 1;
             new_k = apply_tir_bounce(g, current_k, normal, stokes, path_pdf);
@@ -1384,7 +1443,8 @@ extern "C" float or_signed_frame_rotation_psi(const float prev[3], const float c
 // render_core::trace_into: rows claimed from an atomic counter, every pixel sums its own
 // samples in order and adds the sum into accum.
 extern "C" uint64_t or_render(const or_camera *cam, uint32_t width, uint32_t height,
-                              const float *planes, int n_planes, const or_material *m,
+                              const float *planes, int n_planes, const uint32_t *finishes,
+                              const or_material *m,
                               uint32_t max_bounces, const or_env *env, uint32_t first_sample,
                               uint32_t spp, uint32_t pixel_begin, uint32_t pixel_end,
                               int threads, float *accum, float *per_sample_xyz,
@@ -1416,7 +1476,7 @@ extern "C" uint64_t or_render(const or_camera *cam, uint32_t width, uint32_t hei
                     or_generate_ray(cam, (float)x, (float)y, (float)width, (float)height, d.jx,
                                     d.jy, o, dir);
                     or_path_info info;
-                    or_trace_spectral_ray(o, dir, planes, n_planes, m, max_bounces, env, d.seed,
+                    or_trace_spectral_ray(o, dir, planes, n_planes, finishes, m, max_bounces, env, d.seed,
                                           d.hero, xyz, &info, events ? local_ev : nullptr);
                     segs += info.segments;
                     sum = add(sum, load3(xyz));

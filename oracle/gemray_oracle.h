@@ -118,10 +118,11 @@ float or_tir_phase_delta(float n1, float cos_i, float sin_i);
 float or_signed_frame_rotation_psi(const float prev[3], const float cur[3], const float axis[3]);
 
 /* trace_spectral_ray_with_finish_instrumented (transport.rs:476-537) with
- * enable_internal_mode_coupling = true and all-polished finishes.  events may be NULL;
- * when given it receives OR_EV_COUNT counters (added into). */
+ * enable_internal_mode_coupling = true.  finishes: per-plane FacetFinish (0 polished,
+ * 1 frosted), or NULL for all polished.  events may be NULL; when given it receives
+ * OR_EV_COUNT counters (added into). */
 void or_trace_spectral_ray(const float origin[3], const float dir[3], const float *planes,
-                           int n_planes, const or_material *m, uint32_t max_bounces,
+                           int n_planes, const uint32_t *finishes, const or_material *m, uint32_t max_bounces,
                            const or_env *env, uint32_t rng_seed, float hero_rand,
                            float out_xyz[3], or_path_info *info, uint64_t *events);
 
@@ -131,7 +132,8 @@ void or_trace_spectral_ray(const float origin[3], const float dir[3], const floa
  * bounded CPU-baseline sample).  per_sample_xyz/per_sample_info, when non-NULL, receive
  * one entry per (pixel - pixel_begin) * spp + s.  Returns total segments traced. */
 uint64_t or_render(const or_camera *cam, uint32_t width, uint32_t height, const float *planes,
-                   int n_planes, const or_material *m, uint32_t max_bounces, const or_env *env,
+                   int n_planes, const uint32_t *finishes, const or_material *m, uint32_t max_bounces,
+                   const or_env *env,
                    uint32_t first_sample, uint32_t spp, uint32_t pixel_begin,
                    uint32_t pixel_end, int threads, float *accum, float *per_sample_xyz,
                    or_path_info *per_sample_info, uint64_t *events);

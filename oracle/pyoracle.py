@@ -142,11 +142,11 @@ def _declare(L):
     L.or_signed_frame_rotation_psi.argtypes = [fp, fp, fp]
     L.or_signed_frame_rotation_psi.restype = ctypes.c_float
     L.or_trace_spectral_ray.argtypes = [
-        fp, fp, fp, ctypes.c_int, ctypes.c_void_p, u32, ctypes.POINTER(OrEnv), u32,
+        fp, fp, fp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.POINTER(OrEnv), u32,
         ctypes.c_float, fp, ctypes.POINTER(OrPathInfo), ctypes.POINTER(ctypes.c_uint64),
     ]
     L.or_render.argtypes = [
-        ctypes.POINTER(OrCamera), u32, u32, fp, ctypes.c_int, ctypes.c_void_p, u32,
+        ctypes.POINTER(OrCamera), u32, u32, fp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, u32,
         ctypes.POINTER(OrEnv), u32, u32, u32, u32, ctypes.c_int, fp, fp, ctypes.c_void_p,
         ctypes.POINTER(ctypes.c_uint64),
     ]
@@ -266,8 +266,16 @@ def camera_new(yaw, pitch, distance, fov_deg) -> OrCamera:
     return c
 
 
+def _finishes(finishes, n):
+    if finishes is None:
+        return None
+    f = np.ascontiguousarray(finishes, dtype=np.uint32)
+    assert f.shape == (n,)
+    return f
+
+
 def trace(origin, direction, planes, material_struct, max_bounces, env: Env, seed, hero_rand,
-          events: Optional[np.ndarray] = None):
+          events: Optional[np.ndarray] = None, finishes=None):
     o = f32arr(origin)
     d = f32arr(direction)
     p = np.ascontiguousarray(planes, dtype=np.float32)
@@ -276,8 +284,10 @@ def trace(origin, direction, planes, material_struct, max_bounces, env: Env, see
     ev = None
     if events is not None:
         ev = events.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    fin = _finishes(finishes, p.shape[0])
     lib().or_trace_spectral_ray(
-        fptr(o), fptr(d), fptr(p), p.shape[0], ctypes.byref(material_struct), max_bounces,
+        fptr(o), fptr(d), fptr(p), p.shape[0], fin.ctypes.data if fin is not None else None,
+        ctypes.byref(material_struct), max_bounces,
         ctypes.byref(env.struct), seed, hero_rand, fptr(out), ctypes.byref(info), ev,
     )
     return out, info
@@ -285,7 +295,8 @@ def trace(origin, direction, planes, material_struct, max_bounces, env: Env, see
 
 def render(camera: OrCamera, width, height, planes, material_struct, max_bounces, env: Env,
            first_sample, spp, accum: Optional[np.ndarray] = None, pixel_begin=0,
-           pixel_end=None, threads=None, per_sample=False, events: Optional[np.ndarray] = None):
+           pixel_end=None, threads=None, per_sample=False, events: Optional[np.ndarray] = None,
+           finishes=None):
     """render_core::trace_into restated; returns (accum, per_sample_xyz, per_sample_info,
     total_segments)."""
     if pixel_end is None:
@@ -301,8 +312,10 @@ def render(camera: OrCamera, width, height, planes, material_struct, max_bounces
     ev = None
     if events is not None:
         ev = events.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    fin = _finishes(finishes, p.shape[0])
     segs = lib().or_render(
-        ctypes.byref(camera), width, height, fptr(p), p.shape[0], ctypes.byref(material_struct),
+        ctypes.byref(camera), width, height, fptr(p), p.shape[0], fin.ctypes.data if fin is not None else None,
+        ctypes.byref(material_struct),
         max_bounces, ctypes.byref(env.struct), first_sample, spp, pixel_begin, pixel_end, threads,
         fptr(accum), fptr(xyz) if xyz is not None else None,
         info.ctypes.data if info is not None else None, ev,

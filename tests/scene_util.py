@@ -21,12 +21,13 @@ class Case:
     height: int
     max_bounces: int
     camera: tuple = (0.35, 0.28, 5.0, 18.0)  # test_camera() (estimator_check/mod.rs:132-138)
+    finishes: object = None  # per-plane FacetFinish (0 polished, 1 frosted) or None
 
     def product_scene(self) -> B.SceneRef:
         cam = B.camera_new(*self.camera)
         return B.SceneRef(camera=cam, width=self.width, height=self.height, planes=self.planes,
                           material=self.material, max_bounces=self.max_bounces,
-                          environment=self.env)
+                          environment=self.env, facet_finishes=self.finishes)
 
     def oracle_env(self):
         if isinstance(self.env, B.Studio):
@@ -40,7 +41,15 @@ class Case:
         return po.render(cam, self.width, self.height, self.planes, M.to_struct(self.material),
                          self.max_bounces, self.oracle_env(), first_sample, spp,
                          per_sample=per_sample, events=events, threads=threads,
-                         pixel_begin=pixel_begin, pixel_end=pixel_end)
+                         pixel_begin=pixel_begin, pixel_end=pixel_end, finishes=self.finishes)
+
+
+def frosted_girdle(n_planes: int, girdle=range(33, 49)):
+    """Finishes with the standard round brilliant's girdle band frosted
+    (STANDARD_ROUND_BRILLIANT_GIRDLE_FACETS, geometry/cuts.rs:37), the rest polished."""
+    f = np.zeros(n_planes, np.uint32)
+    f[list(girdle)] = 1
+    return f
 
 
 def camera_bits_equal(a, b) -> bool:

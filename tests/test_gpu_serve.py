@@ -14,6 +14,7 @@ from gemray_studio_amd import _lib
 from gemray_studio_amd import backend as B
 from gemray_studio_amd import materials as M
 from gemray_studio_amd import scenes
+from scene_util import frosted_girdle
 
 pytestmark = pytest.mark.gpu
 
@@ -43,6 +44,8 @@ def _mix():
         (B.SceneRef(cam, 20, 20, srb, M.furnace_material(), 12, B.HdrMap.uniform(2.5)), 0, 16),
         (B.SceneRef(cam2, 30, 18, scenes.random_convex_stone(9, 120), M.by_name("Tourmaline"), 16,
                     B.Studio("Incandescent", 1.3, 0.0, 1.2)), 7, 9),
+        (B.SceneRef(cam, 22, 26, srb, M.by_name("Zircon"), 16, B.HdrMap(env), facet_finishes=frosted_girdle(74)),
+         4, 7),
     ]
 
 

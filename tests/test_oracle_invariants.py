@@ -339,3 +339,57 @@ def test_edge_rounding_off_is_bit_identical_in_the_tracer():
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
     # A positive radius changes the hits near edges (and only those).
     assert not np.array_equal(outs[0], outs[2])
+
+
+# ---- frosted facets (FacetFinish::Frosted; raytracer_tests.rs:2246-2760) ----
+
+def test_all_polished_finishes_are_bit_identical_to_none():
+    from scene_util import Case
+    srb = po.standard_round_brilliant()
+    for name in ("Diamond", "Sapphire"):
+        base = Case("p", M.by_name(name), srb, None, 8, 8, 12)
+        cam = po.camera_new(0.35, 0.28, 5.0, 18.0)
+        env = po.studio_env("RingLights", 1.0, 0.85, 0.95)
+        a, _, _, _ = po.render(cam, 8, 8, srb, M.to_struct(base.material), 12, env, 0, 4, threads=1)
+        b, _, _, _ = po.render(cam, 8, 8, srb, M.to_struct(base.material), 12, env, 0, 4, threads=1,
+                               finishes=np.zeros(74, np.uint32))
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _furnace_target(l0):
+    t = np.zeros(3, np.float32)
+    po.lib().or_analytic_furnace_target(l0, po.fptr(t))
+    return t.astype(np.float64)
+
+
+@pytest.mark.parametrize("birefringence", [0.0, 0.05])
+def test_frosted_girdle_white_furnace(birefringence):
+    # A colourless n = 1.5 stone (isotropic, or uniaxial with delta n = 0.05) with a frosted
+    # girdle in a uniform environment stays near the environment's radiance
+    # (raytracer_tests.rs:2322-2409, 2710-2760).  The reference holds this scene to 0.06,
+    # not the polished furnace's 0.02: its frosted bounce divides each channel by the
+    # branch probability without a Fresnel factor, and its own measurements put the frosted
+    # furnace 3.6-4.9 % off the analytic target (estimator_check/furnace.rs:44-85); the
+    # restatement lands there too.
+    from scene_util import frosted_girdle
+    srb = po.standard_round_brilliant()
+    m = M.new_custom("furnace", 1.5, 0.0, birefringence, (0.0, 0.0, 0.0))
+    cam = po.camera_new(0.35, 0.28, 5.0, 18.0)
+    acc, _, _, _ = po.render(cam, 24, 24, srb, M.to_struct(m), 12, po.uniform_env(2.5), 0, 256,
+                             finishes=frosted_girdle(74))
+    mean = acc.astype(np.float64).sum(0) / (24 * 24 * 256)
+    assert np.all(np.abs(mean - _furnace_target(2.5)) / _furnace_target(2.5) < 0.06), mean
+
+
+def test_frosted_girdle_changes_the_face_up_appearance():
+    from scene_util import frosted_girdle
+    srb = po.standard_round_brilliant()
+    cam = po.camera_new(0.35, 0.28, 5.0, 18.0)
+    env = po.studio_env("RingLights", 1.0, 0.85, 0.95)
+    m = M.to_struct(M.by_name("Diamond"))
+    a, _, ia, _ = po.render(cam, 16, 16, srb, m, 12, env, 0, 32, per_sample=True)
+    b, _, ib, _ = po.render(cam, 16, 16, srb, m, 12, env, 0, 32, per_sample=True, finishes=frosted_girdle(74))
+    assert not np.array_equal(a, b)
+    # Only paths that met a girdle facet changed: the others are bit-identical.
+    same = ia["facet_hash"] == ib["facet_hash"]
+    assert 0.0 < same.mean() < 1.0
