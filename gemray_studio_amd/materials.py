@@ -218,6 +218,9 @@ class MaterialStruct(ctypes.Structure):
         ("biaxial_delta_beta_alpha", ctypes.c_float),
         ("scattering_sigma_s", ctypes.c_float),
         ("edge_rounding_radius", ctypes.c_float),
+        ("has_beta_bands", ctypes.c_int32),
+        ("n_beta_bands", ctypes.c_int32),
+        ("beta_bands", (ctypes.c_float * 3) * MAX_BANDS),
     ]
 
 
@@ -245,4 +248,12 @@ def to_struct(m: GemMaterial) -> MaterialStruct:
         float("nan") if m.biaxial_delta_beta_alpha is None else m.biaxial_delta_beta_alpha)
     s.scattering_sigma_s = m.scattering_sigma_s
     s.edge_rounding_radius = m.edge_rounding_radius
+    if m.beta_bands is not None:
+        if len(m.beta_bands) > MAX_BANDS:
+            raise ValueError(f"{m.name}: more than {MAX_BANDS} beta absorption bands")
+        s.has_beta_bands = 1
+        s.n_beta_bands = len(m.beta_bands)
+        for i, b in enumerate(m.beta_bands):
+            for j in range(3):
+                s.beta_bands[i][j] = b[j]
     return s

@@ -76,11 +76,19 @@ typedef struct {
     float o_bands[GR_MAX_BANDS][3];
     float e_bands[GR_MAX_BANDS][3];
     float absorption_path_scale;
-    /* Branches this build declines with GR_DECLINE_MATERIAL when enabled: */
-    float biaxial_delta_beta_alpha; /* NaN = None (uniaxial or isotropic)             */
-    float scattering_sigma_s;       /* > 0: Henyey-Greenstein inclusions              */
+    /* Biaxial indicatrix (materials.rs:1508-1519): n_beta = n(lambda), n_alpha = n_beta -
+     * biaxial_delta_beta_alpha, n_gamma = n_alpha + birefringence_delta, gamma axis =
+     * c_axis.  NaN = None (uniaxial or isotropic).  Traced when the stone is anisotropic;
+     * a biaxial value on an isotropic stone is declined (GR_DECLINE_MATERIAL).        */
+    float biaxial_delta_beta_alpha;
+    /* Declined (GR_DECLINE_MATERIAL) when > 0: Henyey-Greenstein inclusions.          */
+    float scattering_sigma_s;
     /* Traced: edge-rounded shading normals (intersect.rs:196-229) when > 0.          */
     float edge_rounding_radius;
+    /* AbsorptionTensor::beta_ray (optics/absorption.rs): the third band set of a
+     * trichroic biaxial stone, read only when biaxial; has_beta_bands = 0 is None.    */
+    int32_t has_beta_bands, n_beta_bands;
+    float beta_bands[GR_MAX_BANDS][3];
 } gr_material;
 
 /* FacetFinish (optics/raytracer/camera.rs:43-51): a frosted facet scatters diffusely

@@ -18,10 +18,8 @@
  *   - every other transcendental -> the platform libm (sinf, cosf, tanf, acosf, atanf,
  *     atan2f, expf, floorf, fmodf).
  *
- * Scope: polished facets, no volume scattering, no edge rounding, isotropic or uniaxial
- * materials (GemMaterial::biaxial_delta_beta_alpha == None).  Those are the branches the
- * reference takes for every golden vector it records and for every material the
- * benchmark workloads use.
+ * Scope: polished and frosted facets, edge-rounded shading normals, isotropic, uniaxial
+ * and biaxial materials; no volume scattering (GemMaterial::scattering == None).
  */
 #include "gemray_oracle.h"
 
@@ -60,6 +58,7 @@ inline float tf_exp(float x) { return std::exp(x); }
 
 constexpr int NUM_CHANNELS = 8;
 constexpr float PI_F = 3.14159265358979323846f;
+constexpr float FLT_EPSILON_ = 1.1920928955078125e-7f;  // f32::EPSILON
 
 // ---------------------------------------------------------------------------------
 // Scalar helpers with Rust f32 semantics.
@@ -428,7 +427,7 @@ V3 extraordinary_eigen_polarization(V3 w, V3 c) {
     return normalize_or_zero(cross(w, o));
 }
 
-// AbsorptionTensor3 (birefringence.rs:185-255), uniaxial constructor only.
+// AbsorptionTensor3 (birefringence.rs:185-255).
 struct Tensor3 {
     V3 alpha;
     V3 a1, a2, a3;  // columns of `axes`
@@ -439,6 +438,16 @@ inline Tensor3 tensor_uniaxial(float alpha_o, float alpha_e, V3 c_axis) {
     stable_orthonormal_basis(c, &t.a1, &t.a2);
     t.a3 = c;
     t.alpha = v3(alpha_o, alpha_o, alpha_e);
+    return t;
+}
+// AbsorptionTensor3::biaxial (birefringence.rs:238-245): (alpha, beta, gamma) on the
+// same frame as the uniaxial tensor and the biaxial indicatrix.
+inline Tensor3 tensor_biaxial(float alpha, float beta, float gamma, V3 gamma_axis) {
+    V3 g = normalize_or_zero(gamma_axis);
+    Tensor3 t;
+    stable_orthonormal_basis(g, &t.a1, &t.a2);
+    t.a3 = g;
+    t.alpha = v3(alpha, beta, gamma);
     return t;
 }
 inline float quadratic_form(const Tensor3 &t, V3 e) {
@@ -453,6 +462,195 @@ inline float effective_pleochroic_alpha(const Tensor3 &t, V3 e_hat, V3 ea, V3 eb
     float alpha_pol = quadratic_form(t, e_hat);
     float alpha_unpol = midpoint(quadratic_form(t, ea), quadratic_form(t, eb));
     return fma_(p, alpha_pol - alpha_unpol, alpha_unpol);
+}
+
+// ---------------------------------------------------------------------------------
+// Biaxial indicatrix (optics/birefringence.rs:139-170, 349-870).
+// ---------------------------------------------------------------------------------
+
+// birefringence.rs:139-150.
+inline V3 canonicalize_eigenvector_sign(V3 v) {
+    float ax = std::fabs(v.x), ay = std::fabs(v.y), az = std::fabs(v.z);
+    float largest = (ax >= ay && ax >= az) ? v.x : (ay >= az ? v.y : v.z);
+    return largest < 0.0f ? neg(v) : v;
+}
+// birefringence.rs:157-170.
+inline V3 cross_fma(V3 a, V3 b) {
+    return v3(fma_(a.y, b.z, -(a.z * b.y)), fma_(a.z, b.x, -(a.x * b.z)), fma_(a.x, b.y, -(a.y * b.x)));
+}
+inline float dot_fma(V3 a, V3 b) { return fma_(a.x, b.x, fma_(a.y, b.y, a.z * b.z)); }
+
+struct Biaxial {
+    float n_alpha, n_beta, n_gamma;
+    V3 ax, ay, az;  // columns of `axes`: the alpha, beta and gamma principal directions
+};
+// axes.transpose() * v (glam Mat3 * Vec3: lane j is dot(column j, v)).
+inline V3 to_local(const Biaxial &b, V3 v) { return v3(dot(b.ax, v), dot(b.ay, v), dot(b.az, v)); }
+// axes * v: ((ax * v.x) + (ay * v.y)) + (az * v.z) per lane.
+inline V3 to_world(const Biaxial &b, V3 v) { return add(add(mul(b.ax, v.x), mul(b.ay, v.y)), mul(b.az, v.z)); }
+
+// BiaxialIndicatrix::from_gamma_axis (birefringence.rs:386-391).
+Biaxial biaxial_from_gamma_axis(float n_alpha, float n_beta, float n_gamma, V3 gamma_axis) {
+    Biaxial b;
+    b.n_alpha = n_alpha;
+    b.n_beta = n_beta;
+    b.n_gamma = n_gamma;
+    b.az = normalize_or_zero(gamma_axis);
+    stable_orthonormal_basis(b.az, &b.ax, &b.ay);
+    return b;
+}
+// GemMaterial::biaxial_indicatrix (materials.rs:1508-1519); false for None.
+bool material_biaxial_indicatrix(const or_material *m, float lambda_nm, Biaxial *out) {
+    if (std::isnan(m->biaxial_delta_beta_alpha)) return false;
+    float n_beta = dispersion_evaluate(m, lambda_nm);
+    float n_alpha = n_beta - m->biaxial_delta_beta_alpha;
+    float n_gamma = n_alpha + m->birefringence_delta;
+    *out = biaxial_from_gamma_axis(n_alpha, n_beta, n_gamma, load3(m->c_axis));
+    return true;
+}
+// birefringence.rs:394-400.
+inline void b_coeffs(const Biaxial &b, float out[3]) {
+    out[0] = 1.0f / (b.n_alpha * b.n_alpha);
+    out[1] = 1.0f / (b.n_beta * b.n_beta);
+    out[2] = 1.0f / (b.n_gamma * b.n_gamma);
+}
+// birefringence.rs:519-522.
+inline bool indices_are_degenerate(float a, float b) {
+    float scale = maxf(maxf(std::fabs(a), std::fabs(b)), 1.0f);
+    return std::fabs(a - b) <= std::sqrt(FLT_EPSILON_) * scale;
+}
+// birefringence.rs:495-499.
+inline void uniaxial_wave_indices(V3 k_hat, V3 c_axis, float n_o, float n_e, float *n_slow, float *n_fast) {
+    float theta = tf_acos(std::fabs(clampf(dot(k_hat, c_axis), -1.0f, 1.0f)));
+    float n_eff = effective_extraordinary_index(n_o, n_e, theta);
+    *n_slow = maxf(n_o, n_eff);
+    *n_fast = minf(n_o, n_eff);
+}
+// BiaxialIndicatrix::wave_indices (birefringence.rs:435-487): (n_slow, n_fast).
+void wave_indices(const Biaxial &b, V3 wave_normal, float *n_slow, float *n_fast) {
+    V3 k = normalize_or_zero(wave_normal);
+    bool ab = indices_are_degenerate(b.n_alpha, b.n_beta);
+    bool bg = indices_are_degenerate(b.n_beta, b.n_gamma);
+    if (ab && bg) {
+        float n = (b.n_alpha + b.n_beta + b.n_gamma) / 3.0f;
+        *n_slow = n;
+        *n_fast = n;
+        return;
+    }
+    if (ab) {
+        uniaxial_wave_indices(k, b.az, midpoint(b.n_alpha, b.n_beta), b.n_gamma, n_slow, n_fast);
+        return;
+    }
+    if (bg) {
+        uniaxial_wave_indices(k, b.ax, midpoint(b.n_beta, b.n_gamma), b.n_alpha, n_slow, n_fast);
+        return;
+    }
+    V3 l = to_local(b, k);
+    float a2 = l.x * l.x, b2 = l.y * l.y, g2 = l.z * l.z;
+    float bc[3];
+    b_coeffs(b, bc);
+    float big_b = fma_(a2, bc[1] + bc[2], fma_(b2, bc[0] + bc[2], g2 * (bc[0] + bc[1])));
+    float big_c = fma_(a2, bc[1] * bc[2], fma_(b2, bc[0] * bc[2], g2 * bc[0] * bc[1]));
+    float disc = std::sqrt(maxf(fma_(big_b, big_b, -4.0f * big_c), 0.0f));
+    float x_lo = 0.5f * (big_b - disc);
+    float x_hi = midpoint(big_b, disc);
+    *n_slow = 1.0f / std::sqrt(maxf(x_lo, 1e-12f));
+    *n_fast = 1.0f / std::sqrt(maxf(x_hi, 1e-12f));
+}
+// birefringence.rs:728-756.
+float precise_root_near(V3 l, const float bc[3], float x) {
+    float a = l.x * l.x, bb = l.y * l.y, cc = l.z * l.z;
+    float big_b = fma_(a, bc[1] + bc[2], fma_(bb, bc[0] + bc[2], cc * (bc[0] + bc[1])));
+    float xdiff = bc[0] - bc[1];
+    float ydiff = bc[2] - bc[0];
+    float a_plus_c = a + cc;
+    float a_plus_bb = a + bb;
+    float two_a_minus_bc = 2.0f * fma_(bb, -cc, a);
+    float disc_sq = fma_(a_plus_c * a_plus_c, xdiff * xdiff,
+                         fma_(a_plus_bb * a_plus_bb, ydiff * ydiff, two_a_minus_bc * xdiff * ydiff));
+    float disc = std::sqrt(maxf(disc_sq, 0.0f));
+    float x_lo = 0.5f * (big_b - disc);
+    float x_hi = midpoint(big_b, disc);
+    return std::fabs(x - x_lo) <= std::fabs(x - x_hi) ? x_lo : x_hi;
+}
+// birefringence.rs:659-700.
+V3 eigenvector_world(const Biaxial &b, V3 l, const float bc[3], float x_in, V3 k_hat) {
+    float x = precise_root_near(l, bc, x_in);
+    float s = fma_(bc[0], l.x * l.x, fma_(bc[1], l.y * l.y, bc[2] * l.z * l.z));
+    float m00 = fma_(l.x * l.x, fma_(2.0f, -bc[0], s), bc[0] - x);
+    float m11 = fma_(l.y * l.y, fma_(2.0f, -bc[1], s), bc[1] - x);
+    float m22 = fma_(l.z * l.z, fma_(2.0f, -bc[2], s), bc[2] - x);
+    float m01 = l.x * l.y * (s - bc[0] - bc[1]);
+    float m02 = l.x * l.z * (s - bc[0] - bc[2]);
+    float m12 = l.y * l.z * (s - bc[1] - bc[2]);
+    V3 row0 = v3(m00, m01, m02), row1 = v3(m01, m11, m12), row2 = v3(m02, m12, m22);
+    V3 c01 = cross_fma(row0, row1);
+    V3 c02 = cross_fma(row0, row2);
+    V3 c12 = cross_fma(row1, row2);
+    float sign02 = dot_fma(c01, c02) < 0.0f ? -1.0f : 1.0f;
+    float sign12 = dot_fma(c01, c12) < 0.0f ? -1.0f : 1.0f;
+    V3 v_local = add(add(c01, mul(c02, sign02)), mul(c12, sign12));
+    V3 v_world = to_world(b, v_local);
+    if (length_squared(v_world) > 1e-12f) return normalize(canonicalize_eigenvector_sign(v_world));
+    V3 t, bb;
+    stable_orthonormal_basis(k_hat, &t, &bb);
+    return t;
+}
+// BiaxialIndicatrix::eigen_polarizations (birefringence.rs:542-553): (d_slow, d_fast).
+void eigen_polarizations(const Biaxial &b, V3 wave_normal, V3 *d_slow, V3 *d_fast) {
+    V3 k = normalize_or_zero(wave_normal);
+    V3 l = to_local(b, k);
+    float bc[3];
+    b_coeffs(b, bc);
+    float n_slow, n_fast;
+    wave_indices(b, wave_normal, &n_slow, &n_fast);
+    float x_slow = 1.0f / (n_slow * n_slow);
+    float x_fast = 1.0f / (n_fast * n_fast);
+    *d_slow = eigenvector_world(b, l, bc, x_slow, k);
+    *d_fast = eigenvector_world(b, l, bc, x_fast, k);
+}
+// birefringence.rs:763-771.
+V3 d_to_e_direction(const Biaxial &b, V3 d_hat) {
+    V3 d = to_local(b, d_hat);
+    V3 e = v3(d.x / (b.n_alpha * b.n_alpha), d.y / (b.n_beta * b.n_beta), d.z / (b.n_gamma * b.n_gamma));
+    return normalize_or_zero(to_world(b, e));
+}
+// birefringence.rs:855-865.
+V3 poynting_direction(V3 wave_normal, V3 e_field_hat) {
+    V3 k_hat = normalize_or_zero(wave_normal);
+    V3 e_hat = normalize_or_zero(e_field_hat);
+    V3 perp = sub(k_hat, mul(e_hat, dot(e_hat, k_hat)));
+    float len2 = length_squared(perp);
+    if (len2 > 1e-10f) {
+        float len = std::sqrt(len2);
+        return v3(perp.x / len, perp.y / len, perp.z / len);
+    }
+    return k_hat;
+}
+// birefringence.rs:831-836.
+V3 mode_poynting_dir(const Biaxial &b, V3 wave_normal, bool want_slow) {
+    V3 d_slow, d_fast;
+    eigen_polarizations(b, wave_normal, &d_slow, &d_fast);
+    V3 e_hat = d_to_e_direction(b, want_slow ? d_slow : d_fast);
+    return poynting_direction(wave_normal, e_hat);
+}
+// birefringence.rs:795-818: (n, wave_normal) of one mode at an air->crystal entry.
+float resolve_entry_mode(const Biaxial &b, V3 incident, V3 normal, float cos_i, float n_seed, bool want_slow,
+                         V3 *wave_out) {
+    float n_guess = n_seed;
+    V3 wave_dir = incident;
+    for (int it = 0; it < 2; ++it) {
+        float eta = 1.0f / n_guess;
+        float sin2 = eta * eta * fma_(cos_i, -cos_i, 1.0f);
+        if (sin2 > 1.0f) break;
+        float cos_t = std::sqrt(maxf(1.0f - sin2, 0.0f));
+        wave_dir = normalize(add(mul(incident, eta), mul(normal, fma_(eta, cos_i, -cos_t))));
+        float ns, nf;
+        wave_indices(b, wave_dir, &ns, &nf);
+        n_guess = want_slow ? ns : nf;
+    }
+    *wave_out = wave_dir;
+    return n_guess;
 }
 
 // ---------------------------------------------------------------------------------
@@ -688,7 +886,9 @@ struct RayCtx {
     float lambdas[8];
     V3 c_axis;
     bool is_anisotropic;
+    bool is_biaxial;        // hero_indicatrix.is_some()
     float n_o_ch[8];
+    Biaxial biaxial_ch[8];  // material.biaxial_indicatrix(lambdas[k]) when is_biaxial
     Tensor3 tensor_ch[8];
 };
 
@@ -701,10 +901,22 @@ void build_ray_ctx(const or_material *m, const float lambdas[8], RayCtx *ctx) {
         // per_channel_uniaxial_indices(ctx, 0.0): the n_eff half is discarded.
         ctx->n_o_ch[k] = dispersion_evaluate(m, lambdas[k]);
     }
+    // build_ray_wavelength_cache (refraction.rs:128-177): per-channel indicatrices, and a
+    // trichroic tensor when the stone is biaxial and carries a beta band set.
+    ctx->is_biaxial = false;
+    for (int k = 0; k < 8; ++k) {
+        bool bx = material_biaxial_indicatrix(m, lambdas[k], &ctx->biaxial_ch[k]);
+        if (k == 0) ctx->is_biaxial = bx;
+    }
     for (int k = 0; k < 8; ++k) {
         float ao = spectral_absorption(m->o_bands, m->n_o_bands, lambdas[k]);
         float ae = spectral_absorption(m->e_bands, m->n_e_bands, lambdas[k]);
-        ctx->tensor_ch[k] = tensor_uniaxial(ao, ae, ctx->c_axis);
+        if (ctx->is_biaxial && m->has_beta_bands) {
+            float ab = spectral_absorption(m->beta_bands, m->n_beta_bands, lambdas[k]);
+            ctx->tensor_ch[k] = tensor_biaxial(ao, ab, ae, ctx->c_axis);
+        } else {
+            ctx->tensor_ch[k] = tensor_uniaxial(ao, ae, ctx->c_axis);
+        }
     }
 }
 
@@ -712,6 +924,8 @@ struct Geo {
     float cos_i, sin_i;
     float n_o_hero, n_e_hero;
     float n_o_ch[8];
+    float n_biax_a_ch[8], n_biax_b_ch[8];  // biaxial fast (A) / slow (B) mode indices
+    float n_biax_a_hero, n_biax_b_hero;
     float n1, n2, sin2_t;
     float n1_ch[8], n2_ch[8], sin2_t_ch[8];
 };
@@ -719,7 +933,7 @@ struct Geo {
 // refraction.rs:235-273.
 float theta_c_for_bounce(const RayCtx &ctx, V3 normal, V3 k_hat, float cos_i, bool inside,
                          float n_o_hero_seed) {
-    if (!inside && ctx.is_anisotropic) {
+    if (!inside && ctx.is_anisotropic && !ctx.is_biaxial) {
         float n_e_hero_seed = n_o_hero_seed + ctx.m->birefringence_delta;
         float n_guess = n_o_hero_seed;
         float theta = 0.0f;
@@ -754,7 +968,29 @@ Geo compute_geo(const RayCtx &ctx, V3 normal, V3 k_hat, bool inside, bool is_ext
     }
     g.n_o_hero = ctx.n_o_ch[0];
     g.n_e_hero = g.n_o_hero + ctx.m->birefringence_delta;
-    const float *n_medium = (ctx.is_anisotropic && inside && !is_extra) ? ctx.n_o_ch : n_eff_ch;
+    // hero_biaxial_wave_dirs + per_channel_biaxial_indices (refraction.rs:345-400): the
+    // hero's indicatrix resolves one wave normal per mode (inside the stone, the tracked
+    // wave normal), and every channel's own indicatrix gives its mode index there.
+    for (int k = 0; k < 8; ++k) g.n_biax_a_ch[k] = g.n_biax_b_ch[k] = 0.0f;
+    if (ctx.is_biaxial) {
+        V3 dir_a = k_hat, dir_b = k_hat;
+        if (!inside) {
+            resolve_entry_mode(ctx.biaxial_ch[0], k_hat, normal, g.cos_i, ctx.n_o_ch[0], false, &dir_a);
+            resolve_entry_mode(ctx.biaxial_ch[0], k_hat, normal, g.cos_i, ctx.n_o_ch[0], true, &dir_b);
+        }
+        for (int k = 0; k < 8; ++k) {
+            float ns, nf;
+            wave_indices(ctx.biaxial_ch[k], dir_a, &ns, &nf);
+            g.n_biax_a_ch[k] = nf;
+            wave_indices(ctx.biaxial_ch[k], dir_b, &ns, &nf);
+            g.n_biax_b_ch[k] = ns;
+        }
+    }
+    g.n_biax_a_hero = g.n_biax_a_ch[0];
+    g.n_biax_b_hero = g.n_biax_b_ch[0];
+    const float *n_mode_a = ctx.is_biaxial ? g.n_biax_a_ch : ctx.n_o_ch;
+    const float *n_mode_b = ctx.is_biaxial ? g.n_biax_b_ch : n_eff_ch;
+    const float *n_medium = (ctx.is_anisotropic && inside && !is_extra) ? n_mode_a : n_mode_b;
     float n_medium_hero = n_medium[0];
     g.n1 = inside ? n_medium_hero : 1.0f;
     g.n2 = inside ? 1.0f : n_medium_hero;
@@ -782,6 +1018,11 @@ inline V3 reflect(V3 k, V3 n) { return sub(k, mul(n, 2.0f * dot(k, n))); }
 inline V3 poynting_dir_for_mode(const RayCtx &ctx, const Geo &g, V3 k, bool inside, bool is_extra,
                                 uint64_t *ev) {
     if (!inside || !ctx.is_anisotropic) return k;
+    if (ctx.is_biaxial) {
+        // Both biaxial modes walk off (refraction.rs:574-576).
+        if (ev) ev[OR_EV_WALKOFF]++;
+        return mode_poynting_dir(ctx.biaxial_ch[0], k, is_extra);
+    }
     if (is_extra) {
         if (ev) ev[OR_EV_WALKOFF]++;
         return extraordinary_poynting_dir(k, ctx.c_axis, g.n_o_hero, g.n_e_hero);
@@ -851,7 +1092,7 @@ struct RefractOut {
     bool update;
 };
 
-// refraction.rs:748-1011 (non-biaxial arms).
+// refraction.rs:748-1011.
 RefractOut apply_refract_bounce(const RayCtx &ctx, const Geo &g, float r_unpol, V3 k_hat,
                                 V3 normal, bool inside, bool is_extra, uint32_t seed,
                                 uint32_t bounce, Stokes *st, float *pdf, uint64_t *ev) {
@@ -861,14 +1102,23 @@ RefractOut apply_refract_bounce(const RayCtx &ctx, const Geo &g, float r_unpol, 
     bool use_extra = entering_aniso ? (stream_draw(seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f)
                                     : is_extra;
     if (ev && entering_aniso) ev[OR_EV_ENTRY_ANISO]++;
-    float n2_hero_dir = (entering_aniso && !use_extra) ? g.n_o_hero : g.n2;
+    const bool entering_biaxial = entering_aniso && ctx.is_biaxial;
+    float n2_hero_dir;
+    if (entering_biaxial)
+        n2_hero_dir = use_extra ? g.n_biax_b_hero : g.n_biax_a_hero;
+    else
+        n2_hero_dir = (entering_aniso && !use_extra) ? g.n_o_hero : g.n2;
     float eta_dir = g.n1 / n2_hero_dir;
     float sin2_t_dir = minf(eta_dir * eta_dir * fma_(g.cos_i, -g.cos_i, 1.0f), 1.0f);
     float cos_t_dir = std::sqrt(maxf(1.0f - sin2_t_dir, 0.0f));
     V3 refr_wave_dir =
         normalize(add(mul(k_hat, eta_dir), mul(normal, fma_(eta_dir, g.cos_i, -cos_t_dir))));
     V3 s = refr_wave_dir;
-    if (entering_aniso && use_extra) {
+    if (entering_biaxial) {
+        // Both biaxial modes walk off at entry (refraction.rs:803-818).
+        if (ev) ev[OR_EV_WALKOFF]++;
+        s = mode_poynting_dir(ctx.biaxial_ch[0], refr_wave_dir, use_extra);
+    } else if (entering_aniso && use_extra) {
         if (ev) ev[OR_EV_WALKOFF]++;
         s = extraordinary_poynting_dir(refr_wave_dir, c_axis, g.n_o_hero, g.n_e_hero);
     }
@@ -876,7 +1126,8 @@ RefractOut apply_refract_bounce(const RayCtx &ctx, const Geo &g, float r_unpol, 
 
     for (int k = 0; k < 8; ++k) {
         float n1k = g.n1_ch[k];
-        float n2k = (entering_aniso && !use_extra) ? g.n_o_ch[k] : g.n2_ch[k];
+        float n2k = (entering_aniso && !use_extra) ? (ctx.is_biaxial ? g.n_biax_a_ch[k] : g.n_o_ch[k])
+                                                   : g.n2_ch[k];
         float q = n1k / n2k;
         float sin2_t_k = powi(q, 2) * fma_(g.cos_i, -g.cos_i, 1.0f);
         if (sin2_t_k > 1.0f) {
@@ -889,7 +1140,9 @@ RefractOut apply_refract_bounce(const RayCtx &ctx, const Geo &g, float r_unpol, 
         V3 refr_k = normalize(
             add(mul(k_hat, eta_dir_k), mul(normal, fma_(eta_dir_k, g.cos_i, -cos_t_k))));
         V3 final_k = refr_k;
-        if (entering_aniso && use_extra) {
+        if (entering_biaxial) {
+            final_k = mode_poynting_dir(ctx.biaxial_ch[k], refr_k, use_extra);
+        } else if (entering_aniso && use_extra) {
             float n_e_k = g.n_o_ch[k] + ctx.m->birefringence_delta;
             final_k = extraordinary_poynting_dir(refr_k, c_axis, g.n_o_ch[k], n_e_k);
         }
@@ -1004,8 +1257,15 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
         if (inside) {
             normal = neg(normal);
             if (ev) ev[OR_EV_ABSORB]++;
-            V3 ea = ordinary_eigen_polarization(current_k, ctx.c_axis);
-            V3 eb = extraordinary_eigen_polarization(current_k, ctx.c_axis);
+            // channel_absorption_alphas (raytracer/absorption.rs:174-214): the biaxial
+            // eigenmodes of the hero's indicatrix, else the uniaxial o/e pair.
+            V3 ea, eb;
+            if (ctx.is_biaxial) {
+                eigen_polarizations(ctx.biaxial_ch[0], current_k, &ea, &eb);
+            } else {
+                ea = ordinary_eigen_polarization(current_k, ctx.c_axis);
+                eb = extraordinary_eigen_polarization(current_k, ctx.c_axis);
+            }
             float alphas[8];
             for (int k = 0; k < 8; ++k) {
                 V3 e_hat = electric_field_direction(stokes[k], cpn, current_k);
@@ -1430,6 +1690,83 @@ extern "C" void or_extraordinary_poynting_dir(const float w[3], const float c[3]
 }
 extern "C" float or_tir_phase_delta(float n1, float cos_i, float sin_i) {
     return tir_phase_delta(n1, cos_i, sin_i);
+}
+// Biaxial indicatrix probes: an indicatrix travels as 12 floats (n_alpha, n_beta,
+// n_gamma, then the alpha, beta and gamma principal axes).
+namespace {
+Biaxial load_biaxial(const float p[12]) {
+    Biaxial b;
+    b.n_alpha = p[0];
+    b.n_beta = p[1];
+    b.n_gamma = p[2];
+    b.ax = load3(p + 3);
+    b.ay = load3(p + 6);
+    b.az = load3(p + 9);
+    return b;
+}
+void store_biaxial(const Biaxial &b, float p[12]) {
+    p[0] = b.n_alpha;
+    p[1] = b.n_beta;
+    p[2] = b.n_gamma;
+    const V3 v[3] = {b.ax, b.ay, b.az};
+    for (int j = 0; j < 3; ++j) {
+        p[3 + 3 * j] = v[j].x;
+        p[4 + 3 * j] = v[j].y;
+        p[5 + 3 * j] = v[j].z;
+    }
+}
+void store3(V3 v, float *p) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+}  // namespace
+extern "C" int or_material_biaxial_indicatrix(const or_material *m, float lambda_nm, float out[12]) {
+    Biaxial b;
+    if (!material_biaxial_indicatrix(m, lambda_nm, &b)) return 0;
+    store_biaxial(b, out);
+    return 1;
+}
+extern "C" void or_biaxial_from_gamma_axis(float n_alpha, float n_beta, float n_gamma, const float g[3],
+                                           float out[12]) {
+    store_biaxial(biaxial_from_gamma_axis(n_alpha, n_beta, n_gamma, load3(g)), out);
+}
+extern "C" void or_biaxial_wave_indices(const float ind[12], const float w[3], float out[2]) {
+    wave_indices(load_biaxial(ind), load3(w), &out[0], &out[1]);
+}
+extern "C" void or_biaxial_eigen_polarizations(const float ind[12], const float w[3], float out[6]) {
+    V3 a, b;
+    eigen_polarizations(load_biaxial(ind), load3(w), &a, &b);
+    store3(a, out);
+    store3(b, out + 3);
+}
+extern "C" void or_biaxial_d_to_e(const float ind[12], const float d[3], float out[3]) {
+    store3(d_to_e_direction(load_biaxial(ind), load3(d)), out);
+}
+extern "C" void or_biaxial_mode_poynting_dir(const float ind[12], const float w[3], int want_slow, float out[3]) {
+    store3(mode_poynting_dir(load_biaxial(ind), load3(w), want_slow != 0), out);
+}
+extern "C" float or_biaxial_resolve_entry_mode(const float ind[12], const float incident[3], const float normal[3],
+                                               float cos_i, float n_seed, int want_slow, float wave[3]) {
+    V3 w;
+    float n = resolve_entry_mode(load_biaxial(ind), load3(incident), load3(normal), cos_i, n_seed, want_slow != 0, &w);
+    store3(w, wave);
+    return n;
+}
+extern "C" void or_poynting_direction(const float k[3], const float e[3], float out[3]) {
+    store3(poynting_direction(load3(k), load3(e)), out);
+}
+extern "C" void or_uniaxial_eigen_polarizations(const float w[3], const float c[3], float out[6]) {
+    store3(ordinary_eigen_polarization(load3(w), load3(c)), out);
+    store3(extraordinary_eigen_polarization(load3(w), load3(c)), out + 3);
+}
+// effective_pleochroic_alpha (birefringence.rs:272-286) on AbsorptionTensor3::uniaxial
+// (alpha[0], alpha[2]) when trichroic == 0, else ::biaxial (alpha[0], alpha[1], alpha[2]).
+extern "C" float or_effective_pleochroic_alpha(const float alpha[3], int trichroic, const float axis[3],
+                                               const float e[3], const float ea[3], const float eb[3], float dop) {
+    Tensor3 t = trichroic ? tensor_biaxial(alpha[0], alpha[1], alpha[2], load3(axis))
+                          : tensor_uniaxial(alpha[0], alpha[2], load3(axis));
+    return effective_pleochroic_alpha(t, load3(e), load3(ea), load3(eb), dop);
 }
 // raytracer/absorption.rs:44-54.
 extern "C" float or_signed_frame_rotation_psi(const float prev[3], const float cur[3],

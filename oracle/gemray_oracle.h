@@ -24,8 +24,8 @@ extern "C" {
 /* DispersionModel (crates/gemray/src/optics/dispersion.rs:3-7). */
 enum { OR_DISP_SELLMEIER1 = 0, OR_DISP_SELLMEIER3 = 1, OR_DISP_CAUCHY = 2 };
 
-/* The subset of GemMaterial (optics/materials.rs:32-120) the polished, non-scattering,
- * non-biaxial estimator reads (laid out as gr_material of include/gemray_hip.h). */
+/* The subset of GemMaterial (optics/materials.rs:32-120) the non-scattering estimator
+ * reads (laid out as gr_material of include/gemray_hip.h). */
 typedef struct {
     int32_t disp_kind;
     float disp[6];          /* S1: b1,c1 | S3: b0,b1,b2,c0,c1,c2 | Cauchy: a,b,c */
@@ -36,9 +36,12 @@ typedef struct {
     float o_bands[OR_MAX_BANDS][3]; /* center_nm, width_nm, peak */
     float e_bands[OR_MAX_BANDS][3];
     float absorption_path_scale;
-    float biaxial_delta_beta_alpha;  /* unused (biaxial stones are not restated)         */
+    float biaxial_delta_beta_alpha;  /* NaN = None; else a biaxial indicatrix            */
     float scattering_sigma_s;        /* unused (no volume scattering)                    */
     float edge_rounding_radius;      /* > 0: edge-rounded shading normals                */
+    int32_t has_beta_bands;          /* AbsorptionTensor::beta_ray is Some               */
+    int32_t n_beta_bands;
+    float beta_bands[OR_MAX_BANDS][3];
 } or_material;
 
 /* EnvironmentSource (optics/raytracer/environment.rs:188-196).  kind 0 = Studio,
@@ -116,6 +119,20 @@ void or_extraordinary_poynting_dir(const float w[3], const float c[3], float n_o
                                    float out[3]);
 float or_tir_phase_delta(float n1, float cos_i, float sin_i);
 float or_signed_frame_rotation_psi(const float prev[3], const float cur[3], const float axis[3]);
+/* BiaxialIndicatrix probes (optics/birefringence.rs:349-870).  An indicatrix is 12 floats:
+ * n_alpha, n_beta, n_gamma, then the alpha, beta and gamma principal axes. */
+int or_material_biaxial_indicatrix(const or_material *m, float lambda_nm, float out[12]);
+void or_biaxial_from_gamma_axis(float n_alpha, float n_beta, float n_gamma, const float g[3], float out[12]);
+void or_biaxial_wave_indices(const float ind[12], const float w[3], float out[2] /* slow, fast */);
+void or_biaxial_eigen_polarizations(const float ind[12], const float w[3], float out[6] /* d_slow, d_fast */);
+void or_biaxial_d_to_e(const float ind[12], const float d[3], float out[3]);
+void or_biaxial_mode_poynting_dir(const float ind[12], const float w[3], int want_slow, float out[3]);
+float or_biaxial_resolve_entry_mode(const float ind[12], const float incident[3], const float normal[3],
+                                    float cos_i, float n_seed, int want_slow, float wave[3]);
+void or_poynting_direction(const float k[3], const float e[3], float out[3]);
+void or_uniaxial_eigen_polarizations(const float w[3], const float c[3], float out[6] /* o, e */);
+float or_effective_pleochroic_alpha(const float alpha[3], int trichroic, const float axis[3], const float e[3],
+                                    const float ea[3], const float eb[3], float dop);
 
 /* trace_spectral_ray_with_finish_instrumented (transport.rs:476-537) with
  * enable_internal_mode_coupling = true.  finishes: per-plane FacetFinish (0 polished,

@@ -141,6 +141,19 @@ def _declare(L):
     L.or_tir_phase_delta.restype = ctypes.c_float
     L.or_signed_frame_rotation_psi.argtypes = [fp, fp, fp]
     L.or_signed_frame_rotation_psi.restype = ctypes.c_float
+    L.or_material_biaxial_indicatrix.argtypes = [ctypes.c_void_p, ctypes.c_float, fp]
+    L.or_material_biaxial_indicatrix.restype = ctypes.c_int
+    L.or_biaxial_from_gamma_axis.argtypes = [ctypes.c_float] * 3 + [fp, fp]
+    L.or_biaxial_wave_indices.argtypes = [fp, fp, fp]
+    L.or_biaxial_eigen_polarizations.argtypes = [fp, fp, fp]
+    L.or_biaxial_d_to_e.argtypes = [fp, fp, fp]
+    L.or_biaxial_mode_poynting_dir.argtypes = [fp, fp, ctypes.c_int, fp]
+    L.or_biaxial_resolve_entry_mode.argtypes = [fp, fp, fp, ctypes.c_float, ctypes.c_float, ctypes.c_int, fp]
+    L.or_biaxial_resolve_entry_mode.restype = ctypes.c_float
+    L.or_poynting_direction.argtypes = [fp, fp, fp]
+    L.or_uniaxial_eigen_polarizations.argtypes = [fp, fp, fp]
+    L.or_effective_pleochroic_alpha.argtypes = [fp, ctypes.c_int, fp, fp, fp, fp, ctypes.c_float]
+    L.or_effective_pleochroic_alpha.restype = ctypes.c_float
     L.or_trace_spectral_ray.argtypes = [
         fp, fp, fp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.POINTER(OrEnv), u32,
         ctypes.c_float, fp, ctypes.POINTER(OrPathInfo), ctypes.POINTER(ctypes.c_uint64),
@@ -258,6 +271,80 @@ def metrics_reduce(cls, angle, weight, temporal) -> np.ndarray:
     lib().or_metrics_reduce(cls.ctypes.data, fptr(np.ascontiguousarray(angle, np.float32)),
                             fptr(np.ascontiguousarray(weight, np.float32)), temporal.ctypes.data, fptr(out))
     return out
+
+
+class Indicatrix:
+    """A BiaxialIndicatrix (optics/birefringence.rs:349-870) as the oracle's 12 floats."""
+
+    def __init__(self, p):
+        self.p = f32arr(p)
+        assert self.p.shape == (12,)
+
+    @property
+    def n(self):
+        return self.p[:3]
+
+    @property
+    def axes(self):
+        """3 x 3, columns = the alpha, beta, gamma principal axes."""
+        return self.p[3:].reshape(3, 3).T.copy()
+
+    @classmethod
+    def from_gamma_axis(cls, n_alpha, n_beta, n_gamma, gamma_axis):
+        out = np.zeros(12, np.float32)
+        lib().or_biaxial_from_gamma_axis(n_alpha, n_beta, n_gamma, fptr(f32arr(gamma_axis)), fptr(out))
+        return cls(out)
+
+    @classmethod
+    def of_material(cls, material_struct, lambda_nm):
+        out = np.zeros(12, np.float32)
+        if not lib().or_material_biaxial_indicatrix(ctypes.byref(material_struct), lambda_nm, fptr(out)):
+            return None
+        return cls(out)
+
+    def wave_indices(self, w):
+        out = np.zeros(2, np.float32)
+        lib().or_biaxial_wave_indices(fptr(self.p), fptr(f32arr(w)), fptr(out))
+        return out
+
+    def eigen_polarizations(self, w):
+        out = np.zeros(6, np.float32)
+        lib().or_biaxial_eigen_polarizations(fptr(self.p), fptr(f32arr(w)), fptr(out))
+        return out[:3], out[3:]
+
+    def d_to_e(self, d):
+        out = np.zeros(3, np.float32)
+        lib().or_biaxial_d_to_e(fptr(self.p), fptr(f32arr(d)), fptr(out))
+        return out
+
+    def mode_poynting_dir(self, w, want_slow):
+        out = np.zeros(3, np.float32)
+        lib().or_biaxial_mode_poynting_dir(fptr(self.p), fptr(f32arr(w)), int(want_slow), fptr(out))
+        return out
+
+    def resolve_entry_mode(self, incident, normal, cos_i, n_seed, want_slow):
+        wave = np.zeros(3, np.float32)
+        n = lib().or_biaxial_resolve_entry_mode(fptr(self.p), fptr(f32arr(incident)), fptr(f32arr(normal)),
+                                                cos_i, n_seed, int(want_slow), fptr(wave))
+        return np.float32(n), wave
+
+
+def poynting_direction(k, e) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().or_poynting_direction(fptr(f32arr(k)), fptr(f32arr(e)), fptr(out))
+    return out
+
+
+def uniaxial_eigen_polarizations(w, c):
+    out = np.zeros(6, np.float32)
+    lib().or_uniaxial_eigen_polarizations(fptr(f32arr(w)), fptr(f32arr(c)), fptr(out))
+    return out[:3], out[3:]
+
+
+def effective_pleochroic_alpha(alpha, trichroic, axis, e, ea, eb, dop) -> np.float32:
+    return np.float32(lib().or_effective_pleochroic_alpha(
+        fptr(f32arr(alpha)), int(trichroic), fptr(f32arr(axis)), fptr(f32arr(e)), fptr(f32arr(ea)),
+        fptr(f32arr(eb)), dop))
 
 
 def camera_new(yaw, pitch, distance, fov_deg) -> OrCamera:

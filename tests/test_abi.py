@@ -33,8 +33,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes_match_header_layout():
-    # gr_material: 1 + 6 + 1 + 1 + 3 + 2 + 24 + 24 + 1 + 3 four-byte fields.
-    assert ctypes.sizeof(_lib.MaterialStruct) == 4 * 66
+    # gr_material: 1 + 6 + 1 + 1 + 3 + 2 + 24 + 24 + 1 + 3 + 2 + 24 four-byte fields.
+    assert ctypes.sizeof(_lib.MaterialStruct) == 4 * 92
     assert ctypes.sizeof(_lib.Camera) == 4 * 13
     assert ctypes.sizeof(_lib.Env) == 4 * 8 + 8
     assert ctypes.sizeof(_lib.Stats) == 8 * 11 + 8 + 8 + 8 + 8 + 8 + 8
