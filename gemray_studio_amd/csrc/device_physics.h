@@ -282,6 +282,35 @@ __device__ __forceinline__ float qf_channel(const QfInv &v, float q, float u, fl
     const float x = fmaf(ao, v.x_xy, ae * v.x_z);
     return fmaf(c2, d, fmaf(s2, x, h));
 }
+// The same form for a trichroic tensor, w = (ao, ab, ae): the a1 and a2 sums kept apart.
+struct QfInv3 {
+    float h_x, h_y, h_z, d_x, d_y, d_z, x_x, x_y, x_z;
+};
+__device__ __forceinline__ QfInv3 qf_invariants3(V3 s_l, V3 p_l) {
+    QfInv3 v;
+    const float sx = s_l.x * s_l.x, sy = s_l.y * s_l.y, sz = s_l.z * s_l.z;
+    const float px = p_l.x * p_l.x, py = p_l.y * p_l.y, pz = p_l.z * p_l.z;
+    v.h_x = (sx + px) * 0.5f;
+    v.h_y = (sy + py) * 0.5f;
+    v.h_z = (sz + pz) * 0.5f;
+    v.d_x = (sx - px) * 0.5f;
+    v.d_y = (sy - py) * 0.5f;
+    v.d_z = (sz - pz) * 0.5f;
+    v.x_x = s_l.x * p_l.x;
+    v.x_y = s_l.y * p_l.y;
+    v.x_z = s_l.z * p_l.z;
+    return v;
+}
+__device__ __forceinline__ float qf_channel3(const QfInv3 &v, float q, float u, float ao, float ab, float ae) {
+    const float h = fmaf(ao, v.h_x, fmaf(ab, v.h_y, ae * v.h_z));
+    const float d = fmaf(ao, v.d_x, fmaf(ab, v.d_y, ae * v.d_z));
+    const float r2 = fmaf(q, q, u * u);
+    if (!(r2 > 0.0f)) return h + d;
+    const float inv_r = 1.0f / sqrtf(r2);
+    const float c2 = q * inv_r, s2 = u * inv_r;
+    const float x = fmaf(ao, v.x_x, fmaf(ab, v.x_y, ae * v.x_z));
+    return fmaf(c2, d, fmaf(s2, x, h));
+}
 
 // extraordinary_poynting_dir (birefringence.rs:74-100) without the angle round trips.  The
 // reference takes theta = acos(|cos theta|), the walk-off delta = atan(tan rho) from sin and

@@ -172,8 +172,21 @@ gr_status validate_scene(const gr_scene *s) {
     } else {
         return GR_DECLINE_ENV;
     }
-    // Branches not built here: decline so the caller falls back to its CPU tracer.
-    if (!std::isnan(m->biaxial_delta_beta_alpha)) return GR_DECLINE_MATERIAL;
+    if (!std::isnan(m->biaxial_delta_beta_alpha)) {
+        // A biaxial indicatrix: finite, with physical principal indices over the visible.
+        if (!std::isfinite(m->biaxial_delta_beta_alpha)) return GR_ERR_INVALID;
+        for (double l : {380.0, 589.3, 780.0}) {
+            double nb = dispersion_f64(m, l), na = nb - m->biaxial_delta_beta_alpha,
+                   ng = na + m->birefringence_delta;
+            if (!(na >= 1.0 && na <= 6.0 && ng >= 1.0 && ng <= 6.0)) return GR_ERR_INVALID;
+        }
+        if (m->has_beta_bands != 0 && m->has_beta_bands != 1) return GR_ERR_INVALID;
+        if (m->has_beta_bands && (m->n_beta_bands < 0 || m->n_beta_bands > GR_MAX_BANDS)) return GR_ERR_INVALID;
+        // Branch not built here: the biaxial arms of an isotropic stone (a cubic crystal,
+        // or |delta n| <= 1e-4).  Decline so the caller falls back to its CPU tracer.
+        if (m->crystal_cubic || !(std::fabs(m->birefringence_delta) > 1e-4f)) return GR_DECLINE_MATERIAL;
+    }
+    // Branch not built here: decline so the caller falls back to its CPU tracer.
     if (m->scattering_sigma_s > 0.0f) return GR_DECLINE_MATERIAL;
     if (!std::isfinite(m->edge_rounding_radius)) return GR_ERR_INVALID;
     if (s->finishes)
@@ -219,6 +232,14 @@ void fill_dev_scene(const gr_scene *s, DevScene *d) {
     std::memcpy(d->o_bands, m->o_bands, sizeof(d->o_bands));
     std::memcpy(d->e_bands, m->e_bands, sizeof(d->e_bands));
     d->absorption_path_scale = m->absorption_path_scale;
+    d->biaxial = (d->anisotropic && !std::isnan(m->biaxial_delta_beta_alpha)) ? 1 : 0;
+    if (d->biaxial) {
+        d->delta_beta_alpha = m->biaxial_delta_beta_alpha;
+        d->has_beta_bands = m->has_beta_bands ? 1 : 0;
+        d->n_beta_bands = m->has_beta_bands ? m->n_beta_bands : 0;
+        std::memcpy(d->beta_bands, m->beta_bands, sizeof(d->beta_bands));
+        if (d->n_beta_bands > 0) d->has_absorption = 1;
+    }
     d->edge_rounding_radius = m->edge_rounding_radius;
     d->n_frosted = 0;
     if (s->finishes)

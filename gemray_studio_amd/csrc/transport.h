@@ -32,6 +32,12 @@ struct DevScene {
     float o_bands[GR_MAX_BANDS][3];
     float e_bands[GR_MAX_BANDS][3];
     float absorption_path_scale;
+    // Biaxial indicatrix (materials.rs:1508-1519): its principal frame is abs_frame.
+    int32_t biaxial;              // anisotropic with biaxial_delta_beta_alpha set
+    float delta_beta_alpha;       // n_beta - n_alpha
+    int32_t has_beta_bands;       // a trichroic tensor (biaxial with beta_ray Some)
+    int32_t n_beta_bands;
+    float beta_bands[GR_MAX_BANDS][3];
     float edge_rounding_radius;   // > 0: edge-rounded shading normals
     uint32_t n_frosted;           // facets with FacetFinish::Frosted (0: finishes unused)
     const uint32_t *finishes;     // device, n_planes FacetFinish values, or null
@@ -47,7 +53,7 @@ struct DevScene {
 // Floats per continuation record (the parked state of one path), and the number of
 // continuation launches after the plain trace launch (paths are parked again at each
 // multiple of the split bounce until the last level).
-constexpr int kContFields = 86;
+constexpr int kContFields = 94;
 constexpr int kContLevels = 3;
 
 // Arguments of one trace launch over tuples [0, n_tuples): tuple t is pixel
@@ -100,10 +106,10 @@ struct BatchSlice {
 };
 
 // Material/environment class of a scene: bit 0 anisotropic, bit 1 absorbing, bit 2 HDR map,
-// bit 3 some facet frosted.
+// bit 3 some facet frosted, bit 4 biaxial (set only with bit 0).
 inline int scene_class(const DevScene &sc) {
     return (sc.anisotropic ? 1 : 0) | (sc.has_absorption ? 2 : 0) | (sc.env_kind == GR_ENV_MAP ? 4 : 0) |
-           (sc.n_frosted ? 8 : 0);
+           (sc.n_frosted ? 8 : 0) | (sc.biaxial ? 16 : 0);
 }
 
 }  // namespace gr

@@ -20,6 +20,7 @@
 #include <cstddef>
 
 #include "device_physics.h"
+#include "biaxial.h"
 #include "clip.h"
 
 namespace gr {
@@ -108,14 +109,19 @@ constexpr int kEvWords = (GR_EV_COUNT + 1) / 2;
 __device__ __forceinline__ void ev_inc(uint32_t *ev, int i) { ev[i >> 1] += 1u << (16 * (i & 1)); }
 
 // The per-path dispersion and absorption caches (refraction.rs:127-174): 24 floats per
-// lane in LDS, channel-major so that a wave's accesses hit distinct banks.  They are
-// filled lazily and read a few times per bounce; keeping them out of VGPRs leaves
+// lane in LDS (32 in the kernels built for biaxial stones, whose trichroic tensor adds the
+// beta-axis absorption), channel-major so that a wave's accesses hit distinct banks.  They
+// are filled lazily and read a few times per bounce; keeping them out of VGPRs leaves
 // registers to the Stokes state.
 constexpr int kBlock = 256;
 #define LC_NO(k) lcache[(k) * kBlock + threadIdx.x]
 #define LC_AO(k) lcache[(8 + (k)) * kBlock + threadIdx.x]
 #define LC_AE(k) lcache[(16 + (k)) * kBlock + threadIdx.x]
-constexpr int kCacheFloats = 24 * kBlock;
+#define LC_AB(k) lcache[(24 + (k)) * kBlock + threadIdx.x]
+template <int kAniso>
+struct CacheFloats {
+    static constexpr int value = (kAniso == 2 ? 32 : 24) * kBlock;
+};
 
 // ---------------------------------------------------------------------------------
 // Per-path state.
@@ -191,11 +197,13 @@ __device__ __forceinline__ bool camera_miss_certified(const float4 *__restrict__
 }
 
 // One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
-// has ended (escape, Russian roulette, or the bounce cap).
-template <bool kAniso, bool kAbsorb, bool kFrost>
+// has ended (escape, Russian roulette, or the bounce cap).  kAniso: 0 isotropic, 1
+// uniaxial, 2 biaxial (an anisotropic stone with a biaxial indicatrix).
+template <int kAniso, bool kAbsorb, bool kFrost>
 __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, const float4 *__restrict__ planes,
                                             const float *__restrict__ cam_side, float *__restrict__ lcache,
                                             uint32_t *ev) {
+    constexpr bool kBiax = kAniso == 2;
     const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
     const float dn = sc.birefringence_delta;
     const uint32_t bounce = ps.bounce;
@@ -323,11 +331,32 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                     }
                     LC_AO(k) = so;
                     LC_AE(k) = se;
+                    if (kBiax) {
+                        // The beta band set of a trichroic stone (refraction.rs:151-166);
+                        // otherwise the tensor is uniaxial, (ao, ao, ae).
+                        float sb = so;
+                        if (sc.has_beta_bands) {
+                            sb = -0.0f;
+                            for (int b = 0; b < sc.n_beta_bands; ++b) {
+                                float t = (lk - sc.beta_bands[b][0]) / sc.beta_bands[b][1];
+                                sb = sb + sc.beta_bands[b][2] * expf(-0.5f * t * t);
+                            }
+                        }
+                        LC_AB(k) = sb;
+                    }
                 }
                 ps.have_abs = true;
             }
-            V3 ea = ordinary_eigen_polarization(cur_k, c_axis);
-            V3 eb = extraordinary_eigen_polarization(cur_k, c_axis);
+            // The eigenmodes: the hero's biaxial indicatrix (raytracer/absorption.rs:174-196),
+            // else the uniaxial ordinary / extraordinary pair.
+            V3 ea, eb;
+            if (kBiax) {
+                const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
+                bx_eigen_polarizations(bx_of(LC_NO(0), sc.delta_beta_alpha, dn), fr, cur_k, &ea, &eb);
+            } else {
+                ea = ordinary_eigen_polarization(cur_k, c_axis);
+                eb = extraordinary_eigen_polarization(cur_k, c_axis);
+            }
             // Channel-invariant parts of electric_field_direction and of the quadratic
             // forms: the s/p axes and the projections on the absorption frame (a1, a2, c).
             V3 kk = normalize_or_zero(cur_k);
@@ -338,20 +367,39 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             V3 eb_l = v3(dot(afr(sc, 0), eb), dot(afr(sc, 1), eb), dot(afr(sc, 2), eb));
             V3 s_l = v3(dot(afr(sc, 0), s_hat), dot(afr(sc, 1), s_hat), dot(afr(sc, 2), s_hat));
             V3 p_l = v3(dot(afr(sc, 0), p_hat), dot(afr(sc, 1), p_hat), dot(afr(sc, 2), p_hat));
-            const QfInv qi = qf_invariants(s_l, p_l);
-            // The unpolarized term midpoint(ea.A.ea, eb.A.eb) with its channel-invariant
-            // parts gathered: ao (|ea_xy|^2 + |eb_xy|^2) / 2 + ae (ea_z^2 + eb_z^2) / 2.
-            const float u_xy = ((ea_l.x * ea_l.x + ea_l.y * ea_l.y) + (eb_l.x * eb_l.x + eb_l.y * eb_l.y)) * 0.5f;
-            const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
             float spl = hit_t * sc.absorption_path_scale;
+            if (kBiax) {
+                // The trichroic tensor diag(ao, ab, ae) (birefringence.rs:238-286): the same
+                // sums as below with the a1 and a2 axes weighted apart.
+                const QfInv3 qi = qf_invariants3(s_l, p_l);
+                const float u_x = (ea_l.x * ea_l.x + eb_l.x * eb_l.x) * 0.5f;
+                const float u_y = (ea_l.y * ea_l.y + eb_l.y * eb_l.y) * 0.5f;
+                const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float ao = LC_AO(k), ae = LC_AE(k);
-                const float qf_e = qf_channel(qi, ps.st[k].q, ps.st[k].u, ao, ae);
-                const float unpol = fmaf(ao, u_xy, ae * u_z);
-                float p = clampf(degree_of_polarization(ps.st[k]), 0.0f, 1.0f);
-                float alpha = fmaf(p, qf_e - unpol, unpol);
-                ps.st[k] = scale(ps.st[k], exp_lobe(-alpha * spl));
+                for (int k = 0; k < 8; ++k) {
+                    const float ao = LC_AO(k), ab = LC_AB(k), ae = LC_AE(k);
+                    const float qf_e = qf_channel3(qi, ps.st[k].q, ps.st[k].u, ao, ab, ae);
+                    const float unpol = fmaf(ao, u_x, fmaf(ab, u_y, ae * u_z));
+                    float p = clampf(degree_of_polarization(ps.st[k]), 0.0f, 1.0f);
+                    float alpha = fmaf(p, qf_e - unpol, unpol);
+                    ps.st[k] = scale(ps.st[k], exp_lobe(-alpha * spl));
+                }
+            } else {
+                const QfInv qi = qf_invariants(s_l, p_l);
+                // The unpolarized term midpoint(ea.A.ea, eb.A.eb) with its channel-invariant
+                // parts gathered: ao (|ea_xy|^2 + |eb_xy|^2) / 2 + ae (ea_z^2 + eb_z^2) / 2.
+                const float u_xy =
+                    ((ea_l.x * ea_l.x + ea_l.y * ea_l.y) + (eb_l.x * eb_l.x + eb_l.y * eb_l.y)) * 0.5f;
+                const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const float ao = LC_AO(k), ae = LC_AE(k);
+                    const float qf_e = qf_channel(qi, ps.st[k].q, ps.st[k].u, ao, ae);
+                    const float unpol = fmaf(ao, u_xy, ae * u_z);
+                    float p = clampf(degree_of_polarization(ps.st[k]), 0.0f, 1.0f);
+                    float alpha = fmaf(p, qf_e - unpol, unpol);
+                    ps.st[k] = scale(ps.st[k], exp_lobe(-alpha * spl));
+                }
             }
         }
     }
@@ -361,7 +409,42 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     float sin_i = sqrtf(fmaxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
     float one_m_c2 = fmaf(cos_i, -cos_i, 1.0f);
     float n_med[8];
-    if (kAniso) {
+    float n_biax_a[8];  // biaxial entry: each channel's mode-A (fast) index
+    if (kBiax) {
+        // hero_biaxial_wave_dirs + per_channel_biaxial_indices (refraction.rs:345-400): one
+        // wave normal per mode from the hero's indicatrix -- at an entry the two-step
+        // fixed point of resolve_entry_mode seeded with n_beta, inside the stone the
+        // tracked wave normal -- and each channel's own mode indices there.  Inside, the
+        // medium is mode A (fast) unless the path took mode B; at an entry, mode B.
+        const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
+        const float dba = sc.delta_beta_alpha;
+        if (inside) {
+            const bool use_a = !ps.is_extra;
+            const V3 l = bx_local(fr, normalize_or_zero(cur_k));
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                float ns, nf;
+                bx_wave_indices(bx_of(LC_NO(k), dba, dn), l, &ns, &nf);
+                n_med[k] = use_a ? nf : ns;
+                n_biax_a[k] = nf;
+            }
+        } else {
+            const Bx hero = bx_of(LC_NO(0), dba, dn);
+            const V3 dir_a = bx_entry_wave_dir(hero, fr, cur_k, normal, cos_i, one_m_c2, LC_NO(0), false);
+            const V3 dir_b = bx_entry_wave_dir(hero, fr, cur_k, normal, cos_i, one_m_c2, LC_NO(0), true);
+            const V3 la = bx_local(fr, normalize_or_zero(dir_a));
+            const V3 lb = bx_local(fr, normalize_or_zero(dir_b));
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const Bx bk = bx_of(LC_NO(k), dba, dn);
+                float ns, nf;
+                bx_wave_indices(bk, la, &ns, &nf);
+                n_biax_a[k] = nf;
+                bx_wave_indices(bk, lb, &ns, &nf);
+                n_med[k] = ns;
+            }
+        }
+    } else if (kAniso) {
         const bool use_o = inside && !ps.is_extra;
         if (use_o) {
 #pragma unroll
@@ -502,7 +585,12 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         }
         new_k = sub(cur_k, mul(normal, 2.0f * dot(cur_k, normal)));
         new_s = new_k;
-        if (kAniso && inside && ps.is_extra) {
+        if (kBiax && inside) {
+            // Both biaxial modes walk off (refraction.rs:574-576).
+            ev_inc(ev, GR_EV_WALKOFF);
+            const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
+            new_s = bx_mode_poynting_dir(bx_of(LC_NO(0), sc.delta_beta_alpha, dn), fr, new_k, ps.is_extra);
+        } else if (kAniso && inside && ps.is_extra) {
             ev_inc(ev, GR_EV_WALKOFF);
             new_s = extraordinary_poynting_dir(new_k, c_axis, n_o_hero, n_e_hero);
         }
@@ -517,13 +605,24 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             ev_inc(ev, GR_EV_ENTRY_ANISO);
             use_extra = stream_draw(ps.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
         }
-        float n2_dir = (entering && !use_extra) ? n_o_hero : n2;
+        // At a biaxial entry the hero refracts with its own mode index (mode B: n_med, mode
+        // A: n_biax_a) and both modes walk off (refraction.rs:803-818).
+        const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
+        const float dba = sc.delta_beta_alpha;
+        float n2_dir;
+        if (kBiax && entering)
+            n2_dir = use_extra ? n_med[0] : n_biax_a[0];
+        else
+            n2_dir = (entering && !use_extra) ? n_o_hero : n2;
         float eta_dir = n1 / n2_dir;
         float sin2_dir = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
         float cos_t_dir = sqrtf(fmaxf(1.0f - sin2_dir, 0.0f));
         V3 refr = normalize(add(mul(cur_k, eta_dir), mul(normal, fmaf(eta_dir, cos_i, -cos_t_dir))));
         V3 final_dir = refr;
-        if (entering && use_extra) {
+        if (kBiax && entering) {
+            ev_inc(ev, GR_EV_WALKOFF);
+            final_dir = bx_mode_poynting_dir(bx_of(LC_NO(0), dba, dn), fr, refr, use_extra);
+        } else if (entering && use_extra) {
             ev_inc(ev, GR_EV_WALKOFF);
             final_dir = extraordinary_poynting_dir(refr, c_axis, n_o_hero, n_e_hero);
         }
@@ -531,14 +630,16 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             float n1k = inside ? n_med[k] : 1.0f;
-            float n2k = (entering && !use_extra) ? LC_NO(k) : (inside ? 1.0f : n_med[k]);
+            float n2k = (entering && !use_extra) ? (kBiax ? n_biax_a[k] : LC_NO(k)) : (inside ? 1.0f : n_med[k]);
             float q = n1k / n2k;
             float s2k = q * q * one_m_c2;
             bool alive = !(s2k > 1.0f);
             float cos_t_k = sqrtf(fmaxf(1.0f - s2k, 0.0f));
             if (alive) {
                 V3 dk = normalize(add(mul(cur_k, q), mul(normal, fmaf(q, cos_i, -cos_t_k))));
-                if (entering && use_extra)
+                if (kBiax && entering)
+                    dk = bx_mode_poynting_dir(bx_of(LC_NO(k), dba, dn), fr, dk, use_extra);
+                else if (entering && use_extra)
                     dk = extraordinary_poynting_dir(dk, c_axis, LC_NO(k), LC_NO(k) + dn);
                 alive = dot(dk, final_dir) >= DIRECTION_MATCH_COS_TOL;
             }
@@ -676,8 +777,10 @@ __device__ __forceinline__ uint32_t path_segments(const PathState &ps, const Dev
 // running at the split bounce, field-major (field f of record r at cont[f * cap + r]).
 // Fields: 0 tuple index, 1-12 ray_o, ray_d, cur_k, prev_pn, 13-44 Stokes, 45-52 pdfs,
 // 53-76 the LDS caches (n_o, ao, ae), 77 hero offset, 78-84 seed, bounce, last facet,
-// termination bounce and reason and the two path hashes, 85 the mode flags.
+// termination bounce and reason and the two path hashes, 85 the mode flags, and in the
+// kernels built for biaxial stones (kBx) 86-93 the beta-axis absorption cache.
 
+template <bool kBx>
 __device__ __forceinline__ void save_path(const PathState &ps, uint32_t t, const float *__restrict__ lcache,
                                           float *__restrict__ cont, uint32_t cap, uint32_t r) {
     float *c = cont + r;
@@ -712,8 +815,13 @@ __device__ __forceinline__ void save_path(const PathState &ps, uint32_t t, const
     putu(84, ps.fate_h);
     putu(85, (ps.inside ? 1u : 0u) | (ps.is_extra ? 2u : 0u) | (ps.have_prev ? 4u : 0u) | (ps.have_n ? 8u : 0u) |
                  (ps.have_abs ? 16u : 0u) | (ps.escaped ? 32u : 0u));
+    if (kBx) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) put(86 + k, LC_AB(k));
+    }
 }
 
+template <bool kBx>
 __device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__ lcache,
                                               const float *__restrict__ cont, uint32_t cap, uint32_t r) {
     const float *c = cont + r;
@@ -747,6 +855,10 @@ __device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__
     ps.have_n = fl & 8u;
     ps.have_abs = fl & 16u;
     ps.escaped = fl & 32u;
+    if (kBx) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) LC_AB(k) = get(86 + k);
+    }
     return getu(0);
 }
 
@@ -754,6 +866,7 @@ __device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__
 // the path into it; false when the records have run out (the path then carries on).
 // Called by all live lanes of a wave together: in the plain and continuation kernels
 // every live lane reaches a given bounce count in the same loop iteration.
+template <bool kBx>
 __device__ __forceinline__ bool park(const PathState &ps, uint32_t t, const float *__restrict__ lcache,
                                      float *__restrict__ cont, uint32_t cap, uint32_t *__restrict__ count) {
     const uint64_t live = __ballot(1);
@@ -764,7 +877,7 @@ __device__ __forceinline__ bool park(const PathState &ps, uint32_t t, const floa
     base = __shfl(base, (int)leader, 64);
     const uint32_t r = base + (uint32_t)__popcll(live & ((1ull << lane) - 1ull));
     if (r >= cap) return false;
-    save_path(ps, t, lcache, cont, cap, r);
+    save_path<kBx>(ps, t, lcache, cont, cap, r);
     return true;
 }
 
@@ -853,7 +966,7 @@ __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block
 // fully populated waves, parking again at each further multiple of split_bounce for
 // kContLevels levels.  Parking moves state, not arithmetic: every path's result is
 // bit-identical to an unsplit run.  When the records run out, the lane carries on.
-template <bool kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
+template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
     const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp,
@@ -862,7 +975,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
     uint32_t *__restrict__ cont_count) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[kCacheFloats];
+    __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
@@ -880,7 +993,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
         init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == split_bounce && park(ps, t, lcache, cont, cont_cap, cont_count)) {
+            if (ps.bounce == split_bounce && park<kAniso == 2>(ps, t, lcache, cont, cont_cap, cont_count)) {
                 parked = true;
                 break;
             }
@@ -894,7 +1007,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
 // (capped at cont_cap); workgroups past the count return at once.  With split_bounce > 0
 // the paths still running at that bounce are parked again, into next / next_count, for
 // the next level.
-template <bool kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
+template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float *__restrict__ cont, uint32_t cont_cap,
     const uint32_t *__restrict__ cont_count, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
@@ -904,7 +1017,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
     if (blockIdx.x * kBlock >= count) return;
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[kCacheFloats];
+    __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
@@ -917,10 +1030,10 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r < count) {
         PathState ps;
-        const uint32_t t = load_path(ps, lcache, cont, cont_cap, r);
+        const uint32_t t = load_path<kAniso == 2>(ps, lcache, cont, cont_cap, r);
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == split_bounce && park(ps, t, lcache, next, cont_cap, next_count)) {
+            if (ps.bounce == split_bounce && park<kAniso == 2>(ps, t, lcache, next, cont_cap, next_count)) {
                 parked = true;
                 break;
             }
@@ -988,12 +1101,12 @@ __device__ __forceinline__ uint32_t find_slice(const BatchSliceView *__restrict_
 // belongs to one slice, stages that job's planes and reads that job's scene block; the
 // per-lane work is the plain kernel's, so each tuple's result is bit-identical to a plain
 // launch of the same job.
-template <bool kAniso, bool kAbsorb, int kEnv, bool kFrost>
+template <int kAniso, bool kAbsorb, int kEnv, bool kFrost>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     const BatchSliceView *__restrict__ slices, uint32_t n_slices) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[kCacheFloats];
+    __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kTraceBlocks)];
     const DevScene &sc = *(const DevScene *)J.scene;
@@ -1016,7 +1129,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == J.split_bounce &&
-                park(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
+                park<kAniso == 2>(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
                 parked = true;
                 break;
             }
@@ -1029,7 +1142,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
 // Continuation level `level` (1..kContLevels) of a batched class launch: each workgroup
 // belongs to one slice (binary search of cont_block_begin) and continues that slice's
 // records of the previous level, parking again for the next level except at the last.
-template <bool kAniso, bool kAbsorb, int kEnv, bool kFrost>
+template <int kAniso, bool kAbsorb, int kEnv, bool kFrost>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
     const BatchSliceView *__restrict__ slices, uint32_t n_slices, int level) {
     const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kContBlocks)];
@@ -1038,7 +1151,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
     if ((blockIdx.x - J.cont_block_begin) * kBlock >= count) return;
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[kCacheFloats];
+    __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const DevScene &sc = *(const DevScene *)J.scene;
     stage_planes(planes, cam_side, (const float4 *)J.planes, sc);
@@ -1051,12 +1164,12 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
 
     if (r < count) {
         PathState ps;
-        const uint32_t t = load_path(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
+        const uint32_t t = load_path<kAniso == 2>(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
         const uint32_t next_split = level < kContLevels ? (uint32_t)(level + 1) * J.split_bounce : 0u;
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == next_split &&
-                park(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
+                park<kAniso == 2>(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
                 parked = true;
                 break;
             }
@@ -1079,14 +1192,14 @@ constexpr int kRecFields = 20;
 // iteration advances every busy lane by one bounce, and every path ends within
 // max_bounces iterations, so the loop ends after at most kWaveTuples * max_bounces
 // iterations.
-template <bool kAniso, bool kAbsorb, bool kInfo>
+template <int kAniso, bool kAbsorb, bool kInfo>
 __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
     const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, uint32_t rec_cap,
     float *__restrict__ rec, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[kCacheFloats];
+    __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
@@ -1243,6 +1356,33 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__re
             o[0] = qf_channel(qi, a[0], a[1], a[8], 1.0f - a[8]);
             break;
         }
+        case GR_PROBE_BX_WAVE_INDICES:
+        case GR_PROBE_BX_EIGEN_SLOW:
+        case GR_PROBE_BX_EIGEN_FAST:
+        case GR_PROBE_BX_POYNTING_SLOW:
+        case GR_PROBE_BX_POYNTING_FAST: {
+            // a = (n_alpha, n_beta, n_gamma, w.xyz, gamma axis.xyz); the frame is the
+            // gamma axis normalized and its stable orthonormal completion.
+            BxFrame fr;
+            fr.az = normalize_or_zero(v3(a[6], a[7], a[8]));
+            stable_orthonormal_basis(fr.az, &fr.ax, &fr.ay);
+            const Bx b{a[0], a[1], a[2]};
+            const V3 w = v3(a[3], a[4], a[5]);
+            V3 r = v3(0.0f, 0.0f, 0.0f);
+            if (fn == GR_PROBE_BX_WAVE_INDICES) {
+                bx_wave_indices(b, bx_local(fr, normalize_or_zero(w)), &r.x, &r.y);
+            } else if (fn == GR_PROBE_BX_EIGEN_SLOW || fn == GR_PROBE_BX_EIGEN_FAST) {
+                V3 ds, df;
+                bx_eigen_polarizations(b, fr, w, &ds, &df);
+                r = fn == GR_PROBE_BX_EIGEN_SLOW ? ds : df;
+            } else {
+                r = bx_mode_poynting_dir(b, fr, w, fn == GR_PROBE_BX_POYNTING_SLOW);
+            }
+            o[0] = r.x;
+            o[1] = r.y;
+            o[2] = r.z;
+            break;
+        }
         default:
             break;
     }
@@ -1298,7 +1438,7 @@ __global__ __launch_bounds__(kBlock) void film_reduce_batch_kernel(const BatchSl
 
 namespace {
 
-template <bool A, bool B, int E, bool F, bool I>
+template <int A, bool B, int E, bool F, bool I>
 hipError_t launch_plain_i(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
     dim3 grid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
     const bool split = a.split_bounce != 0 && a.cont != nullptr && a.cont_cap != 0;
@@ -1328,7 +1468,7 @@ hipError_t launch_plain_i(const gr::DevScene &sc, const gr::TraceArgs &a, hipStr
     return hipSuccess;
 }
 
-template <bool A, bool B, int E>
+template <int A, bool B, int E>
 hipError_t launch_plain(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
     // Scenes with frosted facets take the kernels built with the frosted bounce; the others
     // keep kernels without it (its code raises the register pressure of every bounce).
@@ -1339,7 +1479,7 @@ hipError_t launch_plain(const gr::DevScene &sc, const gr::TraceArgs &a, hipStrea
                   : launch_plain_i<A, B, E, false, false>(sc, a, stream);
 }
 
-template <bool A, bool B, int E>
+template <int A, bool B, int E>
 hipError_t launch_wave(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
     const uint32_t waves = (a.n_tuples + gr::kWaveTuples - 1) / gr::kWaveTuples;
     dim3 grid((waves + gr::kBlock / 64 - 1) / (gr::kBlock / 64));
@@ -1368,7 +1508,7 @@ hipError_t launch_wave(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream
     return hipGetLastError();
 }
 
-template <bool A, bool B, int E>
+template <int A, bool B, int E>
 hipError_t launch_one(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
     return a.wavefront ? launch_wave<A, B, E>(sc, a, stream) : launch_plain<A, B, E>(sc, a, stream);
 }
@@ -1378,6 +1518,10 @@ hipError_t launch_one(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_
 hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
     const bool an = sc.anisotropic != 0, ab = sc.has_absorption != 0;
     const int e = sc.env_kind;
+    if (an && sc.biaxial) {
+        if (ab) return e == 0 ? launch_one<2, true, 0>(sc, a, stream) : launch_one<2, true, 1>(sc, a, stream);
+        return e == 0 ? launch_one<2, false, 0>(sc, a, stream) : launch_one<2, false, 1>(sc, a, stream);
+    }
     if (an && ab) return e == 0 ? launch_one<true, true, 0>(sc, a, stream) : launch_one<true, true, 1>(sc, a, stream);
     if (an && !ab) return e == 0 ? launch_one<true, false, 0>(sc, a, stream) : launch_one<true, false, 1>(sc, a, stream);
     if (!an && ab) return e == 0 ? launch_one<false, true, 0>(sc, a, stream) : launch_one<false, true, 1>(sc, a, stream);
@@ -1388,7 +1532,8 @@ namespace {
 template <int C>
 hipError_t launch_batch_class(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
                               uint32_t cont_blocks, hipStream_t stream) {
-    constexpr bool kA = (C & 1) != 0, kB = (C & 2) != 0, kF = (C & 8) != 0;
+    constexpr int kA = (C & 16) ? 2 : (C & 1);
+    constexpr bool kB = (C & 2) != 0, kF = (C & 8) != 0;
     constexpr int kE = (C & 4) ? 1 : 0;
     const auto *view = reinterpret_cast<const gr::BatchSliceView *>(slices);
     hipLaunchKernelGGL((gr::trace_batch_kernel<kA, kB, kE, kF>), dim3(n_blocks), dim3(gr::kBlock), 0, stream, view,
@@ -1425,6 +1570,15 @@ hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices
         case 13: return launch_batch_class<13>(slices, n_slices, n_blocks, cont_blocks, stream);
         case 14: return launch_batch_class<14>(slices, n_slices, n_blocks, cont_blocks, stream);
         case 15: return launch_batch_class<15>(slices, n_slices, n_blocks, cont_blocks, stream);
+        // Biaxial classes (bit 4, always with bit 0).
+        case 17: return launch_batch_class<17>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 19: return launch_batch_class<19>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 21: return launch_batch_class<21>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 23: return launch_batch_class<23>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 25: return launch_batch_class<25>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 27: return launch_batch_class<27>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 29: return launch_batch_class<29>(slices, n_slices, n_blocks, cont_blocks, stream);
+        case 31: return launch_batch_class<31>(slices, n_slices, n_blocks, cont_blocks, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -192,9 +192,16 @@ gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, g
  *   GR_PROBE_FIELD_QFORM      in (Q, U, S.xyz, P.xyz, ao)        out (e.A.e with ae = 1 - ao)
  *   GR_PROBE_EXP              in (x)                             out (e^x as the lobes evaluate it)
  *   GR_PROBE_CMF              in (lambda_nm)                     out (xbar, ybar, zbar)
- *   GR_PROBE_ACOS             in (x)                             out (acos x as the metrics evaluate it) */
+ *   GR_PROBE_ACOS             in (x)                             out (acos x as the metrics evaluate it)
+ * The biaxial indicatrix (optics/birefringence.rs:435-836), in (n_alpha, n_beta, n_gamma,
+ * w.xyz, gamma_axis.xyz) -- the principal frame is BiaxialIndicatrix::from_gamma_axis's:
+ *   GR_PROBE_BX_WAVE_INDICES  out (n_slow, n_fast)             wave_indices(w)
+ *   GR_PROBE_BX_EIGEN_SLOW/_FAST      out (D.xyz)              eigen_polarizations(w).0 / .1
+ *   GR_PROBE_BX_POYNTING_SLOW/_FAST   out (S.xyz)              mode_poynting_dir(w, slow / fast) */
 enum { GR_PROBE_NEFF = 0, GR_PROBE_POYNTING = 1, GR_PROBE_TIR_RETARDATION = 2, GR_PROBE_FRAME_ROTATION = 3,
-       GR_PROBE_FIELD_QFORM = 4, GR_PROBE_EXP = 5, GR_PROBE_CMF = 6, GR_PROBE_ACOS = 7, GR_PROBE_COUNT = 8 };
+       GR_PROBE_FIELD_QFORM = 4, GR_PROBE_EXP = 5, GR_PROBE_CMF = 6, GR_PROBE_ACOS = 7,
+       GR_PROBE_BX_WAVE_INDICES = 8, GR_PROBE_BX_EIGEN_SLOW = 9, GR_PROBE_BX_EIGEN_FAST = 10,
+       GR_PROBE_BX_POYNTING_SLOW = 11, GR_PROBE_BX_POYNTING_FAST = 12, GR_PROBE_COUNT = 13 };
 gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out);
 
 /* Statistics of the last gr_accumulate / gr_trace_samples / gr_render call. */

@@ -208,6 +208,40 @@ def test_discriminant_reformulation_is_the_naive_discriminant():
             assert abs(x0 - x1) / max(abs(x0), abs(x1), 1e-12) < 1e-9
 
 
+def test_optic_axis_amplifies_last_ulp_differences():
+    # Next to an optic axis of a biaxial stone, one ulp of the wave normal moves the mode
+    # indices by orders of magnitude more than an ulp (the discriminant's square root has
+    # infinite slope at zero) and can collapse the eigenmode pair to one mode; a uniaxial
+    # stone's indices stay within a few ulp.  This is the reference's own arithmetic, and
+    # why a device path whose wave normal differs from the oracle's in the last ulp near an
+    # optic axis can differ by ~1e-3-1e-2 at the end.
+    ind = po.Indicatrix.of_material(M.to_struct(M.by_name("Alexandrite")), 589.3)
+    b = 1.0 / ind.n.astype(np.float64) ** 2
+    vz = math.atan(math.sqrt((b[0] - b[1]) / (b[1] - b[2])))
+    ax = ind.axes.astype(np.float64)
+    axis = unit(math.cos(vz) * ax[:, 2] + math.sin(vz) * ax[:, 0])
+    worst_n, collapsed = 0.0, set()
+    for j in range(3):
+        for step in range(1, 40):
+            w = axis.copy()
+            for _ in range(step):
+                w[j] = np.nextafter(w[j], np.float32(2))
+            n0, n1 = ind.wave_indices(axis), ind.wave_indices(w)
+            worst_n = max(worst_n, float(np.abs(n1 - n0).max()))
+    assert worst_n > 1e-5  # > 100 ulp of n ~ 1.75 from a few ulp of direction
+    # Walking off the axis, the eigenmode pair is collapsed, then distinct.
+    for t in (0.0, 1e-4, 1e-3, 1e-2):
+        w = unit(math.cos(vz + t) * ax[:, 2] + math.sin(vz + t) * ax[:, 0])
+        d_slow, d_fast = ind.eigen_polarizations(w)
+        collapsed.add(bool(np.array_equal(d_slow, d_fast)))
+    assert collapsed == {True, False}
+    uni = po.Indicatrix.from_gamma_axis(1.65, 1.65, 1.72, [0.0, 1.0, 0.0])
+    w0 = unit([0.3, 0.8, 0.52])
+    w1 = w0.copy()
+    w1[0] = np.nextafter(w1[0], np.float32(2))
+    assert np.abs(uni.wave_indices(w1) - uni.wave_indices(w0)).max() <= 4 * np.spacing(np.float32(1.72))
+
+
 def test_d_to_e_and_poynting_direction():
     for label, ind in indicatrices():
         for k in sphere(60, 5):

@@ -74,13 +74,24 @@ def _cases(srb):
              24, 24, 12, finishes=frosted_girdle(74)),
         Case("zircon-frosted-girdle-map", M.by_name("Zircon"), srb, B.HdrMap(env_small), 24, 24, 16,
              finishes=frosted_girdle(74)),
+        # Biaxial stones (BiaxialIndicatrix, birefringence.rs:349-870): trichroic
+        # Alexandrite and Tanzanite (a beta band set), Topaz (uniaxial tensor on biaxial
+        # eigenmodes), and a frosted girdle on a biaxial stone.
+        Case("alexandrite-ringlights", M.by_name("Alexandrite"), srb, B.Studio("RingLights", 1.0, 0.85, 0.95),
+             24, 24, 12),
+        Case("topaz-map", M.by_name("Topaz"), srb, B.HdrMap(env_small), 24, 24, 16),
+        Case("tanzanite-oval-map", dataclasses.replace(M.by_name("Tanzanite"), absorption_path_scale=0.6),
+             scenes.oval_brilliant(), B.HdrMap(env_small), 24, 24, 32),
+        Case("alexandrite-frosted-girdle-map", M.by_name("Alexandrite"), srb, B.HdrMap(env_small), 24, 24, 16,
+             finishes=frosted_girdle(74)),
     ]
 
 
 CASE_IDS = ["diamond-ringlights", "cz-daylight", "sapphire-ringlights", "ruby-incandescent",
             "moissanite-dark", "zircon-daylight", "tourmaline-map", "spinel-map",
             "pleochroic-oval-map", "furnace-uniform", "stress-stepcut", "diamond-rounded-edges",
-            "sapphire-rounded-edges-map", "diamond-frosted-girdle", "zircon-frosted-girdle-map"]
+            "sapphire-rounded-edges-map", "diamond-frosted-girdle", "zircon-frosted-girdle-map",
+            "alexandrite-ringlights", "topaz-map", "tanzanite-oval-map", "alexandrite-frosted-girdle-map"]
 
 
 @pytest.mark.parametrize("idx", range(len(CASE_IDS)), ids=CASE_IDS)
@@ -122,7 +133,21 @@ def test_per_path_parity_with_oracle(gpu, srb, idx):
     print(f"[parity] {case.name}: seq={frac:.5f} fates={fate_frac:.5f} agree={agree:.5f} max_rel={err.max():.3e} "
           f"median_rel={np.median(err):.3e} segments gpu={stats['segments']} oracle={segs_o}", flush=True)
     assert np.all(np.isfinite(xyz_g))
-    assert err.max() <= XYZ_RTOL, f"{case.name}: worst matched-path rel err {err.max():.3e}"
+    if case.material.biaxial_delta_beta_alpha is None:
+        assert err.max() <= XYZ_RTOL, f"{case.name}: worst matched-path rel err {err.max():.3e}"
+    else:
+        # A biaxial stone's mode indices come from sqrt(B^2 - 4C), whose slope is infinite
+        # at an optic axis, and its eigenmode pair collapses to one mode where that
+        # discriminant rounds to zero (birefringence.rs:435-487, 542-553): a last-ulp change
+        # of a wave normal near an optic axis moves a path's absorption and Fresnel factors
+        # by up to ~1e-2 with every decision unchanged.  Nudging the frosted direction's
+        # cosine by one ulp in the oracle itself moves 4 of ~9200 matched Alexandrite paths
+        # by more than 1e-3 (uniaxial stones: none above 2e-5), and
+        # tests/test_biaxial_oracle.py::test_optic_axis_amplifies_last_ulp_differences pins
+        # the amplification.  Held like the full-size frames: 99.9 % of matched paths within
+        # XYZ_RTOL, every one within 1e-2.
+        assert np.quantile(err, 0.999) <= XYZ_RTOL, f"{case.name}: p99.9 rel err {np.quantile(err, 0.999):.3e}"
+        assert err.max() <= 1e-2, f"{case.name}: worst matched-path rel err {err.max():.3e}"
     assert np.median(err) <= XYZ_RTOL_MEDIAN
     # Segment accounting: device counter == sum of per-path segments; oracle agrees on
     # matched paths.
@@ -231,6 +256,26 @@ def test_frosted_girdle_furnace(gpu, srb):
     assert np.all(np.abs((mg - mo) / se) <= 4.0), (mg, mo, se)
 
 
+def test_biaxial_furnace(gpu, srb):
+    """A colourless biaxial stone (Topaz's indicatrix without absorption bands) in a uniform
+    environment, 32 bounces, 32x32 x 400 spp: both engines within 0.02 of the analytic
+    target and |z| <= 4 between them (the birefringent furnace, raytracer_tests.rs:2539)."""
+    m = dataclasses.replace(M.by_name("Topaz"), o_bands=(), e_bands=(), beta_bands=None)
+    case = Case("furnace-biaxial", m, srb, B.HdrMap.uniform(2.5), 32, 32, 32)
+    spp = 400
+    n_pix = case.width * case.height
+    xyz_g, _ = gpu.trace_samples(case.product_scene(), 0, n_pix, 0, spp, with_info=False)
+    _, xyz_o, _, _ = case.oracle_render(0, spp, per_sample=True)
+    target = np.zeros(3, np.float32)
+    po.lib().or_analytic_furnace_target(2.5, po.fptr(target))
+    for name, x in (("gpu", xyz_g), ("oracle", xyz_o)):
+        rel = np.abs(x.astype(np.float64).mean(axis=0) - target) / np.abs(target)
+        assert rel.max() <= 0.02, (name, rel)
+    mg, mo = xyz_g.astype(np.float64).mean(0), xyz_o.astype(np.float64).mean(0)
+    se = np.sqrt(xyz_g.astype(np.float64).var(0) / len(xyz_g) + xyz_o.astype(np.float64).var(0) / len(xyz_o))
+    assert np.all(np.abs((mg - mo) / se) <= 4.0), (mg, mo, se)
+
+
 def test_statistical_image_comparison(gpu, srb):
     """T3 on disjoint samples: 32x32 px, 256 spp per engine; per-pixel Welch z; pass if
     |z| > 3 pixels <= max(5 x binomial expectation, 10) (image_comparison.rs:52-58)."""
@@ -255,7 +300,9 @@ def test_declines_leave_film_untouched(gpu, srb):
                 max_bounces=8, environment=B.Studio())
     scattering = dataclasses.replace(M.by_name("Diamond"), scattering_sigma_s=0.5)
     assert not gpu.try_accumulate(B.SceneRef(material=scattering, **base), 0, 2, accum)
-    assert not gpu.try_accumulate(B.SceneRef(material=M.by_name("Alexandrite"), **base), 0, 2, accum)
+    # A biaxial indicatrix on an isotropic stone (a cubic crystal).
+    biaxial_cubic = dataclasses.replace(M.by_name("Diamond"), biaxial_delta_beta_alpha=0.002)
+    assert not gpu.try_accumulate(B.SceneRef(material=biaxial_cubic, **base), 0, 2, accum)
     assert np.all(accum == 3.0)
     assert gpu.try_accumulate(B.SceneRef(material=M.by_name("Diamond"), **base), 0, 2, accum)
     assert not np.all(accum == 3.0)
@@ -263,7 +310,8 @@ def test_declines_leave_film_untouched(gpu, srb):
         gpu.try_accumulate(B.SceneRef(material=M.by_name("Diamond"), **base), 0, 2, np.zeros((5, 3), np.float32))
 
 
-@pytest.mark.parametrize("idx", [0, 2, 8, 10], ids=["diamond", "sapphire", "pleochroic-oval", "stress"])
+@pytest.mark.parametrize("idx", [0, 2, 8, 10, 17], ids=["diamond", "sapphire", "pleochroic-oval", "stress",
+                                                        "tanzanite-oval"])
 def test_plain_and_wavefront_kernels_agree_bit_for_bit(gpu, srb, idx):
     """The persistent-wavefront kernel only reschedules tuples: every tuple's XYZ, segment
     count, termination and path hashes equal the plain kernel's, and so does the film."""
@@ -285,7 +333,8 @@ def test_plain_and_wavefront_kernels_agree_bit_for_bit(gpu, srb, idx):
     assert out["plain"][3] == out["wavefront"][3]
 
 
-@pytest.mark.parametrize("idx", [0, 2, 8, 10], ids=["diamond", "sapphire", "pleochroic-oval", "stress"])
+@pytest.mark.parametrize("idx", [0, 2, 8, 10, 17], ids=["diamond", "sapphire", "pleochroic-oval", "stress",
+                                                        "tanzanite-oval"])
 def test_path_length_split_is_bit_identical(gpu, srb, idx):
     """Parking paths at the split bounce and finishing them in a second launch moves state
     only: outputs, films and counters equal the unsplit run bit for bit."""

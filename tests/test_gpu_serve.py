@@ -46,6 +46,8 @@ def _mix():
                     B.Studio("Incandescent", 1.3, 0.0, 1.2)), 7, 9),
         (B.SceneRef(cam, 22, 26, srb, M.by_name("Zircon"), 16, B.HdrMap(env), facet_finishes=frosted_girdle(74)),
          4, 7),
+        (B.SceneRef(cam2, 26, 22, scenes.oval_brilliant(), M.by_name("Tanzanite"), 24, B.HdrMap(env)), 3, 9),
+        (B.SceneRef(cam, 18, 24, srb, M.by_name("Topaz"), 12, B.Studio("Daylight", 1.0, 0.4, 0.35)), 6, 5),
     ]
 
 
