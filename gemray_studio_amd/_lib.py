@@ -35,7 +35,7 @@ GR_ENV_MAP = 1
 GR_TRACE_PLAIN = 0
 GR_TRACE_WAVEFRONT = 1
 EV_NAMES = ["segment", "miss", "hit", "rotate", "absorb", "tir", "reflect", "refract",
-            "entry_aniso", "walkoff", "rr"]
+            "entry_aniso", "walkoff", "rr", "scatter"]
 GR_EV_COUNT = len(EV_NAMES)
 
 

@@ -221,6 +221,7 @@ class MaterialStruct(ctypes.Structure):
         ("has_beta_bands", ctypes.c_int32),
         ("n_beta_bands", ctypes.c_int32),
         ("beta_bands", (ctypes.c_float * 3) * MAX_BANDS),
+        ("scattering_g", ctypes.c_float),
     ]
 
 
@@ -247,6 +248,7 @@ def to_struct(m: GemMaterial) -> MaterialStruct:
     s.biaxial_delta_beta_alpha = (
         float("nan") if m.biaxial_delta_beta_alpha is None else m.biaxial_delta_beta_alpha)
     s.scattering_sigma_s = m.scattering_sigma_s
+    s.scattering_g = m.scattering_g
     s.edge_rounding_radius = m.edge_rounding_radius
     if m.beta_bands is not None:
         if len(m.beta_bands) > MAX_BANDS:

@@ -81,7 +81,8 @@ typedef struct {
      * c_axis.  NaN = None (uniaxial or isotropic).  Traced when the stone is anisotropic;
      * a biaxial value on an isotropic stone is declined (GR_DECLINE_MATERIAL).        */
     float biaxial_delta_beta_alpha;
-    /* Declined (GR_DECLINE_MATERIAL) when > 0: Henyey-Greenstein inclusions.          */
+    /* Henyey-Greenstein volume scattering (scattering.rs:132-330) when > 0: extinction
+     * coefficient sigma_s, in the units of the absorption bands.                      */
     float scattering_sigma_s;
     /* Traced: edge-rounded shading normals (intersect.rs:196-229) when > 0.          */
     float edge_rounding_radius;
@@ -89,6 +90,8 @@ typedef struct {
      * trichroic biaxial stone, read only when biaxial; has_beta_bands = 0 is None.    */
     int32_t has_beta_bands, n_beta_bands;
     float beta_bands[GR_MAX_BANDS][3];
+    /* Henyey-Greenstein asymmetry g of the scattering phase function, |g| < 1.        */
+    float scattering_g;
 } gr_material;
 
 /* FacetFinish (optics/raytracer/camera.rs:43-51): a frosted facet scatters diffusely
@@ -122,7 +125,7 @@ typedef struct {
 /* Per-class event counters of the last render (device-side, exact). */
 enum {
     GR_EV_SEGMENT = 0, GR_EV_MISS, GR_EV_HIT, GR_EV_ROTATE, GR_EV_ABSORB, GR_EV_TIR,
-    GR_EV_REFLECT, GR_EV_REFRACT, GR_EV_ENTRY_ANISO, GR_EV_WALKOFF, GR_EV_RR, GR_EV_COUNT
+    GR_EV_REFLECT, GR_EV_REFRACT, GR_EV_ENTRY_ANISO, GR_EV_WALKOFF, GR_EV_RR, GR_EV_SCATTER, GR_EV_COUNT
 };
 typedef struct {
     uint64_t events[GR_EV_COUNT]; /* events[GR_EV_SEGMENT] = 8-lambda path segments     */

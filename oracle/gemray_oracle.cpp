@@ -19,7 +19,7 @@
  *     atan2f, expf, floorf, fmodf).
  *
  * Scope: polished and frosted facets, edge-rounded shading normals, isotropic, uniaxial
- * and biaxial materials; no volume scattering (GemMaterial::scattering == None).
+ * and biaxial materials, Henyey-Greenstein volume scattering.
  */
 #include "gemray_oracle.h"
 
@@ -46,6 +46,7 @@ inline float tf_acos(float x) { return (float)std::acos((double)x); }
 inline float tf_atan(float x) { return (float)std::atan((double)x); }
 inline float tf_atan2(float y, float x) { return (float)std::atan2((double)y, (double)x); }
 inline float tf_exp(float x) { return (float)std::exp((double)x); }
+inline float tf_log(float x) { return (float)std::log((double)x); }
 #else
 inline float tf_sin(float x) { return std::sin(x); }
 inline float tf_cos(float x) { return std::cos(x); }
@@ -54,6 +55,7 @@ inline float tf_acos(float x) { return std::acos(x); }
 inline float tf_atan(float x) { return std::atan(x); }
 inline float tf_atan2(float y, float x) { return std::atan2(y, x); }
 inline float tf_exp(float x) { return std::exp(x); }
+inline float tf_log(float x) { return std::log(x); }
 #endif
 
 constexpr int NUM_CHANNELS = 8;
@@ -272,6 +274,9 @@ void cmf_x8(const float l[8], float out[8][3]) {
 constexpr uint32_t FRESNEL_BRANCH_STREAM = 0x9E3779B1u;
 constexpr uint32_t FROSTED_DIR_U_STREAM = 0x27D4EB2Fu;  // sampling.rs:28-29
 constexpr uint32_t FROSTED_DIR_V_STREAM = 0x165667B1u;
+constexpr uint32_t DISTANCE_SAMPLE_STREAM = 0xA24BAED4u;  // sampling.rs:30-41
+constexpr uint32_t PHASE_DIR_U_STREAM = 0x9FB21C65u;
+constexpr uint32_t PHASE_DIR_V_STREAM = 0x1CE4E5B9u;
 constexpr uint32_t RUSSIAN_ROULETTE_STREAM = 0x517CC1B7u;
 constexpr uint32_t BIREFRINGENT_SPLIT_STREAM = 0x2545F491u;
 constexpr uint32_t MODE_COUPLING_STREAM = 0xCC9E2D51u;
@@ -1047,6 +1052,29 @@ V3 cosine_weighted_hemisphere(float u1, float u2, V3 n) {
     return normalize_or_zero(add(add(mul(t, r * cos_t), mul(b, r * sin_t)), mul(n, std::sqrt(maxf(1.0f - u1, 0.0f)))));
 }
 
+// sample_henyey_greenstein_direction (scattering.rs:428-447): HG-distributed cosine about
+// `forward` (isotropic for |g| < 1e-3), azimuth 2 pi u2, on the frosted basis of `forward`.
+V3 sample_henyey_greenstein_direction(float u1, float u2, float g, V3 forward) {
+    float cos_theta;
+    if (std::fabs(g) < 1e-3f) {
+        cos_theta = fma_(2.0f, -u1, 1.0f);
+    } else {
+        float one_minus_g2 = fma_(g, -g, 1.0f);
+        float denom = fma_(2.0f * g, u1, 1.0f - g);
+        float sq = one_minus_g2 / denom;
+        cos_theta = fma_(sq, -sq, fma_(g, g, 1.0f)) / (2.0f * g);
+    }
+    cos_theta = clampf(cos_theta, -1.0f, 1.0f);
+    float sin_theta = std::sqrt(maxf(fma_(cos_theta, -cos_theta, 1.0f), 0.0f));
+    float phi = 2.0f * PI_F * u2;
+    float sin_p = tf_sin(phi), cos_p = tf_cos(phi);
+    V3 a = std::fabs(forward.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    V3 t = normalize_or_zero(sub(a, mul(forward, dot(forward, a))));
+    V3 b = cross(forward, t);
+    return normalize_or_zero(
+        add(add(mul(t, sin_theta * cos_p), mul(b, sin_theta * sin_p)), mul(forward, cos_theta)));
+}
+
 // refraction.rs:599-637.
 V3 apply_tir_bounce(const Geo &g, V3 k_hat, V3 normal, Stokes *st, float *pdf) {
     for (int k = 0; k < 8; ++k) {
@@ -1169,6 +1197,61 @@ RefractOut apply_refract_bounce(const RayCtx &ctx, const Geo &g, float r_unpol, 
     return out;
 }
 
+// channel_absorption_alphas (raytracer/absorption.rs:174-214): each channel's pleochroic
+// coefficient for its own Stokes vector, s axis s_axis and wave normal k_hat.
+void channel_absorption_alphas(const RayCtx &ctx, V3 s_axis, V3 k_hat, const Stokes *st, float alphas[8]) {
+    V3 ea, eb;
+    if (ctx.is_biaxial) {
+        eigen_polarizations(ctx.biaxial_ch[0], k_hat, &ea, &eb);
+    } else {
+        ea = ordinary_eigen_polarization(k_hat, ctx.c_axis);
+        eb = extraordinary_eigen_polarization(k_hat, ctx.c_axis);
+    }
+    for (int k = 0; k < 8; ++k) {
+        V3 e_hat = electric_field_direction(st[k], s_axis, k_hat);
+        alphas[k] = effective_pleochroic_alpha(ctx.tensor_ch[k], e_hat, ea, eb, degree_of_polarization(st[k]));
+    }
+}
+
+// maybe_scatter_or_extinguish (scattering.rs:132-232): an exponential free path on the
+// hero's extinction alphas[0] + sigma_s (absorption-length units; hit_t is in model units,
+// scaled by path_scale).  Before the boundary: every channel is depolarized and weighted
+// tr_k sigma_s / (sigma_t,hero (1 - u)), its pdf multiplied by sigma_t,k tr_k, and the
+// new direction is HG-sampled; returns true with the free path in model units.  Else every
+// channel takes survive_k / survive_hero and pdf *= survive_k; returns false.
+bool maybe_scatter_or_extinguish(const float alphas[8], float sigma_s, float g, V3 ray_dir, float hit_t,
+                                 float path_scale, uint32_t seed, uint32_t bounce, Stokes *st, float *pdf,
+                                 float *t_model, V3 *new_dir) {
+    float sigma_t_hero = alphas[0] + sigma_s;
+    float dist_rand = stream_draw(seed, bounce, DISTANCE_SAMPLE_STREAM);
+    float one_minus_u = maxf(1.0f - dist_rand, 1e-7f);
+    float t_free = -(tf_log(one_minus_u)) / sigma_t_hero;
+    float hit_t_scaled = hit_t * path_scale;
+    if (t_free < hit_t_scaled) {
+        float pdf_hero = sigma_t_hero * one_minus_u;
+        for (int k = 0; k < 8; ++k) {
+            float sigma_t_k = alphas[k] + sigma_s;
+            float tr_k = exp_lane(-(alphas[k] + sigma_s) * t_free);
+            float weight = tr_k * sigma_s / pdf_hero;
+            st[k] = Stokes{intensity(st[k]) * weight, 0.0f, 0.0f, 0.0f};
+            pdf[k] *= sigma_t_k * tr_k;
+        }
+        float u1 = stream_draw(seed, bounce, PHASE_DIR_U_STREAM);
+        float u2 = stream_draw(seed, bounce, PHASE_DIR_V_STREAM);
+        *new_dir = sample_henyey_greenstein_direction(u1, u2, g, ray_dir);
+        *t_model = t_free / path_scale;
+        return true;
+    }
+    float survive[8];
+    for (int k = 0; k < 8; ++k) survive[k] = exp_lane(-(alphas[k] + sigma_s) * hit_t_scaled);
+    float survive_hero = maxf(survive[0], 1e-30f);
+    for (int k = 0; k < 8; ++k) {
+        st[k] = scale(st[k], survive[k] / survive_hero);
+        pdf[k] *= survive[k];
+    }
+    return false;
+}
+
 inline void fnv(uint32_t &h, uint32_t v) {
     h ^= v;
     h *= 16777619u;
@@ -1234,6 +1317,42 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
             break;
         }
         if (ev) ev[OR_EV_HIT]++;
+        // try_scatter_step (scattering.rs:291-336) -> maybe_scatter_or_extinguish
+        // (:132-232), inside a scattering stone before its facet hit is processed: an
+        // exponential free path on the hero's extinction sigma_a + sigma_s; a scatter before
+        // the facet depolarizes, reweights every channel, turns the ray by the HG phase
+        // function and skips the facet (transport.rs:805-848); otherwise every channel
+        // takes its extinction over the segment relative to the hero's.
+        if (inside && !(m->scattering_sigma_s <= 0.0f)) {
+            if (ev) ev[OR_EV_SCATTER]++;
+            float alphas[8];
+            channel_absorption_alphas(ctx, have_prev ? prev_plane_normal : v3(0, 0, 0), current_k, stokes, alphas);
+            float t_free_model;
+            V3 new_dir;
+            if (maybe_scatter_or_extinguish(alphas, m->scattering_sigma_s, m->scattering_g, ray_d, hit.t,
+                                            m->absorption_path_scale, rng_seed, bounce, stokes, path_pdf,
+                                            &t_free_model, &new_dir)) {
+                ray_o = add(ray_o, mul(ray_d, t_free_model));
+                ray_d = new_dir;
+                current_k = new_dir;
+                fnv(fh, 0xFFFEu);
+                have_prev = false;
+                prev_plane_normal = v3(0, 0, 0);
+                if (bounce > 4) {
+                    if (ev) ev[OR_EV_RR]++;
+                    float max_i = 0.0f;
+                    for (int k = 0; k < 8; ++k) max_i = maxf(max_i, intensity(stokes[k]));
+                    float q = clampf(max_i, 0.05f, 1.0f);
+                    if (stream_draw(rng_seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) {
+                        term_bounce = bounce;
+                        term_reason = OR_TERM_SCATTER;
+                        break;
+                    }
+                    for (int k = 0; k < 8; ++k) stokes[k] = scale(stokes[k], 1.0f / q);
+                }
+                continue;
+            }
+        }
         V3 hit_point = add(ray_o, mul(ray_d, hit.t));
         // Facet edge rounding (transport.rs:850-857).
         V3 normal = shading_normal_near_edge(planes, n_planes, hit_point, hit.facet, hit.normal,
@@ -1254,24 +1373,14 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
 
         // apply_interior_segment (transport.rs:538-557) -> apply_absorption
         // (raytracer/absorption.rs:115-141).
+        // (A scattering stone's extinction was taken by the scatter step above.)
         if (inside) {
             normal = neg(normal);
+        }
+        if (inside && m->scattering_sigma_s <= 0.0f) {
             if (ev) ev[OR_EV_ABSORB]++;
-            // channel_absorption_alphas (raytracer/absorption.rs:174-214): the biaxial
-            // eigenmodes of the hero's indicatrix, else the uniaxial o/e pair.
-            V3 ea, eb;
-            if (ctx.is_biaxial) {
-                eigen_polarizations(ctx.biaxial_ch[0], current_k, &ea, &eb);
-            } else {
-                ea = ordinary_eigen_polarization(current_k, ctx.c_axis);
-                eb = extraordinary_eigen_polarization(current_k, ctx.c_axis);
-            }
             float alphas[8];
-            for (int k = 0; k < 8; ++k) {
-                V3 e_hat = electric_field_direction(stokes[k], cpn, current_k);
-                alphas[k] = effective_pleochroic_alpha(ctx.tensor_ch[k], e_hat, ea, eb,
-                                                       degree_of_polarization(stokes[k]));
-            }
+            channel_absorption_alphas(ctx, cpn, current_k, stokes, alphas);
             float spl = hit.t * m->absorption_path_scale;
             for (int k = 0; k < 8; ++k) {
                 float trans = exp_lane(-alphas[k] * spl);
@@ -1759,6 +1868,44 @@ extern "C" void or_poynting_direction(const float k[3], const float e[3], float 
 extern "C" void or_uniaxial_eigen_polarizations(const float w[3], const float c[3], float out[6]) {
     store3(ordinary_eigen_polarization(load3(w), load3(c)), out);
     store3(extraordinary_eigen_polarization(load3(w), load3(c)), out + 3);
+}
+// Scattering probes (scattering.rs:132-232, 428-447; raytracer/absorption.rs:174-214).
+extern "C" void or_sample_hg_batch(int n, const float *u1, const float *u2, float g, const float forward[3],
+                                   float *out) {
+    for (int i = 0; i < n; ++i) {
+        V3 d = sample_henyey_greenstein_direction(u1[i], u2[i], g, load3(forward));
+        out[3 * i] = d.x;
+        out[3 * i + 1] = d.y;
+        out[3 * i + 2] = d.z;
+    }
+}
+extern "C" void or_channel_absorption_alphas(const or_material *m, const float lambdas[8], const float s_axis[3],
+                                             const float k_hat[3], const float stokes[32], float out[8]) {
+    RayCtx ctx;
+    build_ray_ctx(m, lambdas, &ctx);
+    Stokes st[8];
+    for (int k = 0; k < 8; ++k) st[k] = Stokes{stokes[4 * k], stokes[4 * k + 1], stokes[4 * k + 2], stokes[4 * k + 3]};
+    channel_absorption_alphas(ctx, load3(s_axis), load3(k_hat), st, out);
+}
+// stokes (32) and pdf (8) are updated in place; returns 1 on a scatter (t_model and dir set).
+extern "C" int or_maybe_scatter_or_extinguish(const float alphas[8], float sigma_s, float g, const float dir[3],
+                                              float hit_t, float path_scale, uint32_t seed, uint32_t bounce,
+                                              float stokes[32], float pdf[8], float *t_model, float new_dir[3]) {
+    Stokes st[8];
+    for (int k = 0; k < 8; ++k) st[k] = Stokes{stokes[4 * k], stokes[4 * k + 1], stokes[4 * k + 2], stokes[4 * k + 3]};
+    V3 nd = v3(0, 0, 0);
+    float t = 0.0f;
+    bool sc = maybe_scatter_or_extinguish(alphas, sigma_s, g, load3(dir), hit_t, path_scale, seed, bounce, st, pdf,
+                                          &t, &nd);
+    for (int k = 0; k < 8; ++k) {
+        stokes[4 * k] = st[k].i;
+        stokes[4 * k + 1] = st[k].q;
+        stokes[4 * k + 2] = st[k].u;
+        stokes[4 * k + 3] = st[k].v;
+    }
+    *t_model = t;
+    store3(nd, new_dir);
+    return sc ? 1 : 0;
 }
 // effective_pleochroic_alpha (birefringence.rs:272-286) on AbsorptionTensor3::uniaxial
 // (alpha[0], alpha[2]) when trichroic == 0, else ::biaxial (alpha[0], alpha[1], alpha[2]).

@@ -37,11 +37,12 @@ typedef struct {
     float e_bands[OR_MAX_BANDS][3];
     float absorption_path_scale;
     float biaxial_delta_beta_alpha;  /* NaN = None; else a biaxial indicatrix            */
-    float scattering_sigma_s;        /* unused (no volume scattering)                    */
+    float scattering_sigma_s;        /* > 0: Henyey-Greenstein volume scattering         */
     float edge_rounding_radius;      /* > 0: edge-rounded shading normals                */
     int32_t has_beta_bands;          /* AbsorptionTensor::beta_ray is Some               */
     int32_t n_beta_bands;
     float beta_bands[OR_MAX_BANDS][3];
+    float scattering_g;              /* Henyey-Greenstein asymmetry                     */
 } or_material;
 
 /* EnvironmentSource (optics/raytracer/environment.rs:188-196).  kind 0 = Studio,
@@ -74,6 +75,7 @@ enum {
     OR_EV_ENTRY_ANISO,   /* air->crystal entries of an anisotropic stone    */
     OR_EV_WALKOFF,       /* extraordinary Poynting evaluations (hero)       */
     OR_EV_RR,            /* Russian-roulette evaluations                    */
+    OR_EV_SCATTER,       /* free-path draws of a scattering stone           */
     OR_EV_COUNT
 };
 
@@ -133,6 +135,14 @@ void or_poynting_direction(const float k[3], const float e[3], float out[3]);
 void or_uniaxial_eigen_polarizations(const float w[3], const float c[3], float out[6] /* o, e */);
 float or_effective_pleochroic_alpha(const float alpha[3], int trichroic, const float axis[3], const float e[3],
                                     const float ea[3], const float eb[3], float dop);
+/* Henyey-Greenstein scattering probes (optics/raytracer/scattering.rs:132-232, 428-447;
+ * raytracer/absorption.rs:174-214).  Stokes vectors travel as 8 x (I, Q, U, V). */
+void or_sample_hg_batch(int n, const float *u1, const float *u2, float g, const float forward[3], float *out);
+void or_channel_absorption_alphas(const or_material *m, const float lambdas[8], const float s_axis[3],
+                                  const float k_hat[3], const float stokes[32], float out[8]);
+int or_maybe_scatter_or_extinguish(const float alphas[8], float sigma_s, float g, const float dir[3], float hit_t,
+                                   float path_scale, uint32_t seed, uint32_t bounce, float stokes[32], float pdf[8],
+                                   float *t_model, float new_dir[3]);
 
 /* trace_spectral_ray_with_finish_instrumented (transport.rs:476-537) with
  * enable_internal_mode_coupling = true.  finishes: per-plane FacetFinish (0 polished,

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 OR_EV_NAMES = [
     "segment", "miss", "hit", "rotate", "absorb", "tir", "reflect", "refract",
-    "entry_aniso", "walkoff", "rr",
+    "entry_aniso", "walkoff", "rr", "scatter",
 ]
 OR_EV_COUNT = len(OR_EV_NAMES)
 
@@ -154,6 +154,11 @@ def _declare(L):
     L.or_uniaxial_eigen_polarizations.argtypes = [fp, fp, fp]
     L.or_effective_pleochroic_alpha.argtypes = [fp, ctypes.c_int, fp, fp, fp, fp, ctypes.c_float]
     L.or_effective_pleochroic_alpha.restype = ctypes.c_float
+    L.or_sample_hg_batch.argtypes = [ctypes.c_int, fp, fp, ctypes.c_float, fp, fp]
+    L.or_channel_absorption_alphas.argtypes = [ctypes.c_void_p, fp, fp, fp, fp, fp]
+    L.or_maybe_scatter_or_extinguish.argtypes = [fp, ctypes.c_float, ctypes.c_float, fp, ctypes.c_float,
+                                                 ctypes.c_float, u32, u32, fp, fp, fp, fp]
+    L.or_maybe_scatter_or_extinguish.restype = ctypes.c_int
     L.or_trace_spectral_ray.argtypes = [
         fp, fp, fp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, u32, ctypes.POINTER(OrEnv), u32,
         ctypes.c_float, fp, ctypes.POINTER(OrPathInfo), ctypes.POINTER(ctypes.c_uint64),

@@ -33,11 +33,11 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes_match_header_layout():
-    # gr_material: 1 + 6 + 1 + 1 + 3 + 2 + 24 + 24 + 1 + 3 + 2 + 24 four-byte fields.
-    assert ctypes.sizeof(_lib.MaterialStruct) == 4 * 92
+    # gr_material: 1 + 6 + 1 + 1 + 3 + 2 + 24 + 24 + 1 + 3 + 2 + 24 + 1 four-byte fields.
+    assert ctypes.sizeof(_lib.MaterialStruct) == 4 * 93
     assert ctypes.sizeof(_lib.Camera) == 4 * 13
     assert ctypes.sizeof(_lib.Env) == 4 * 8 + 8
-    assert ctypes.sizeof(_lib.Stats) == 8 * 11 + 8 + 8 + 8 + 8 + 8 + 8
+    assert ctypes.sizeof(_lib.Stats) == 8 * 12 + 8 + 8 + 8 + 8 + 8 + 8
 
 
 # ctypes mirror of each C struct of the header, checked field by field against the C
