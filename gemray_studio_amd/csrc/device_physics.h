@@ -172,6 +172,35 @@ constexpr uint32_t MODE_COUPLING_STREAM = 0xCC9E2D51u;
 constexpr uint32_t PIXEL_JITTER_X_ROTATION_STREAM = 0xA511E9B3u;
 constexpr uint32_t PIXEL_JITTER_Y_ROTATION_STREAM = 0x63D81B23u;
 constexpr uint32_t HERO_WAVELENGTH_ROTATION_STREAM = 0x1B873593u;
+constexpr uint32_t DISTANCE_SAMPLE_STREAM = 0xA24BAED4u;  // sampling.rs:30-41
+constexpr uint32_t PHASE_DIR_U_STREAM = 0x9FB21C65u;
+constexpr uint32_t PHASE_DIR_V_STREAM = 0x1CE4E5B9u;
+
+// sample_henyey_greenstein_direction (optics/raytracer/scattering.rs:428-447): the
+// Henyey-Greenstein cosine about `forward` by inversion of its CDF (isotropic for
+// |g| < 1e-3), azimuth 2 pi u2, on frosted_orthonormal_basis(forward).  Out of line, like
+// cosine_weighted_hemisphere: only scattering stones call it.
+__device__ __noinline__ V3 sample_henyey_greenstein_direction(float u1, float u2, float g, V3 forward) {
+    float cos_theta;
+    if (fabsf(g) < 1e-3f) {
+        cos_theta = fmaf(2.0f, -u1, 1.0f);
+    } else {
+        const float one_minus_g2 = fmaf(g, -g, 1.0f);
+        const float denom = fmaf(2.0f * g, u1, 1.0f - g);
+        const float sq = one_minus_g2 / denom;
+        cos_theta = fmaf(sq, -sq, fmaf(g, g, 1.0f)) / (2.0f * g);
+    }
+    cos_theta = clampf(cos_theta, -1.0f, 1.0f);
+    const float sin_theta = sqrtf(fmaxf(fmaf(cos_theta, -cos_theta, 1.0f), 0.0f));
+    const float phi = (2.0f * 3.14159265358979323846f) * u2;
+    float sin_p, cos_p;
+    sincosf(phi, &sin_p, &cos_p);
+    const V3 a = fabsf(forward.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t = normalize_or_zero(sub(a, mul(forward, dot(forward, a))));
+    const V3 b = cross(forward, t);
+    return normalize_or_zero(
+        add(add(mul(t, sin_theta * cos_p), mul(b, sin_theta * sin_p)), mul(forward, cos_theta)));
+}
 
 __host__ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
     x *= 0x85ebca6bu;

@@ -186,8 +186,9 @@ gr_status validate_scene(const gr_scene *s) {
         // or |delta n| <= 1e-4).  Decline so the caller falls back to its CPU tracer.
         if (m->crystal_cubic || !(std::fabs(m->birefringence_delta) > 1e-4f)) return GR_DECLINE_MATERIAL;
     }
-    // Branch not built here: decline so the caller falls back to its CPU tracer.
-    if (m->scattering_sigma_s > 0.0f) return GR_DECLINE_MATERIAL;
+    // Henyey-Greenstein scattering: a finite extinction (<= 0 is off) and |g| <= 1.
+    if (!std::isfinite(m->scattering_sigma_s)) return GR_ERR_INVALID;
+    if (m->scattering_sigma_s > 0.0f && !(std::fabs(m->scattering_g) <= 1.0f)) return GR_ERR_INVALID;
     if (!std::isfinite(m->edge_rounding_radius)) return GR_ERR_INVALID;
     if (s->finishes)
         for (uint32_t i = 0; i < s->n_planes; ++i)
@@ -241,6 +242,8 @@ void fill_dev_scene(const gr_scene *s, DevScene *d) {
         if (d->n_beta_bands > 0) d->has_absorption = 1;
     }
     d->edge_rounding_radius = m->edge_rounding_radius;
+    d->sigma_s = m->scattering_sigma_s > 0.0f ? m->scattering_sigma_s : 0.0f;
+    d->scattering_g = m->scattering_g;
     d->n_frosted = 0;
     if (s->finishes)
         for (uint32_t i = 0; i < s->n_planes; ++i) d->n_frosted += s->finishes[i] == GR_FINISH_FROSTED ? 1u : 0u;

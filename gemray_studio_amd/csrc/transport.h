@@ -39,6 +39,7 @@ struct DevScene {
     int32_t n_beta_bands;
     float beta_bands[GR_MAX_BANDS][3];
     float edge_rounding_radius;   // > 0: edge-rounded shading normals
+    float sigma_s, scattering_g;  // Henyey-Greenstein volume scattering when sigma_s > 0
     uint32_t n_frosted;           // facets with FacetFinish::Frosted (0: finishes unused)
     const uint32_t *finishes;     // device, n_planes FacetFinish values, or null
     // environment
@@ -105,11 +106,15 @@ struct BatchSlice {
     uint32_t *cont_count;
 };
 
+// Whether a scene needs the kernels built for the rare features (kFrost): some facet
+// frosted, or a scattering material.
+inline bool rare_features(const DevScene &sc) { return sc.n_frosted != 0 || sc.sigma_s > 0.0f; }
+
 // Material/environment class of a scene: bit 0 anisotropic, bit 1 absorbing, bit 2 HDR map,
-// bit 3 some facet frosted, bit 4 biaxial (set only with bit 0).
+// bit 3 rare features (frosted facets, volume scattering), bit 4 biaxial (only with bit 0).
 inline int scene_class(const DevScene &sc) {
     return (sc.anisotropic ? 1 : 0) | (sc.has_absorption ? 2 : 0) | (sc.env_kind == GR_ENV_MAP ? 4 : 0) |
-           (sc.n_frosted ? 8 : 0) | (sc.biaxial ? 16 : 0);
+           (rare_features(sc) ? 8 : 0) | (sc.biaxial ? 16 : 0);
 }
 
 }  // namespace gr

@@ -196,6 +196,101 @@ __device__ __forceinline__ bool camera_miss_certified(const float4 *__restrict__
     return ti > tj;
 }
 
+// spectral_absorption per eigen-axis (raytracer/absorption.rs:28-30, optics/absorption.rs:68-71)
+// into the lane's LDS caches, summed from -0.0 like Iterator::sum; the kernels built for
+// biaxial stones add the beta band set of a trichroic stone (refraction.rs:151-166; otherwise
+// their tensor is uniaxial, (ao, ao, ae)).  Called once per path (PathState::have_abs) by
+// the scatter step; the interior Beer-Lambert block spells the same loop out.
+template <bool kBiax>
+__device__ __forceinline__ void fill_absorption_cache(float hero_off, const DevScene &sc, float *__restrict__ lcache) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float so = -0.0f, se = -0.0f;
+        const float lk = comb_lambda(hero_off, k);
+        for (int b = 0; b < sc.n_o_bands; ++b) {
+            float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
+            so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
+        }
+        for (int b = 0; b < sc.n_e_bands; ++b) {
+            float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
+            se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
+        }
+        LC_AO(k) = so;
+        LC_AE(k) = se;
+        if (kBiax) {
+            float sb = so;
+            if (sc.has_beta_bands) {
+                sb = -0.0f;
+                for (int b = 0; b < sc.n_beta_bands; ++b) {
+                    float t = (lk - sc.beta_bands[b][0]) / sc.beta_bands[b][1];
+                    sb = sb + sc.beta_bands[b][2] * expf(-0.5f * t * t);
+                }
+            }
+            LC_AB(k) = sb;
+        }
+    }
+}
+
+// channel_absorption_alphas (raytracer/absorption.rs:174-214): each channel's pleochroic
+// coefficient p alpha_pol + (1 - p) alpha_unpol for wave normal k and plane-of-incidence
+// normal s_axis, the eigenmodes being the hero's biaxial indicatrix's or the uniaxial
+// ordinary / extraordinary pair.  The channel-invariant parts of electric_field_direction
+// and of the quadratic forms (the s/p axes and their projections on the absorption frame
+// (a1, a2, c)) are formed once; per channel the form follows from (Q, U) (qf_channel).
+// Each coefficient is handed to use(c, alpha) as soon as it is formed, so that Beer-Lambert
+// consumes it at once instead of holding all eight (register pressure of the hot kernels).
+template <bool kBiax, class Use>
+__device__ __forceinline__ void channel_alphas(const PathState &ps, const DevScene &sc,
+                                               const float *__restrict__ lcache, V3 k, V3 s_axis, Use &&use) {
+    V3 ea, eb;
+    if (kBiax) {
+        const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
+        bx_eigen_polarizations(bx_of(LC_NO(0), sc.delta_beta_alpha, sc.birefringence_delta), fr, k, &ea, &eb);
+    } else {
+        const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
+        ea = ordinary_eigen_polarization(k, c_axis);
+        eb = extraordinary_eigen_polarization(k, c_axis);
+    }
+    V3 kk = normalize_or_zero(k);
+    V3 s_raw = sub(s_axis, mul(kk, dot(kk, s_axis)));
+    V3 s_hat = len2(s_raw) > 1e-8f ? normalize(s_raw) : arbitrary_perpendicular(kk);
+    V3 p_hat = cross(kk, s_hat);
+    V3 ea_l = v3(dot(afr(sc, 0), ea), dot(afr(sc, 1), ea), dot(afr(sc, 2), ea));
+    V3 eb_l = v3(dot(afr(sc, 0), eb), dot(afr(sc, 1), eb), dot(afr(sc, 2), eb));
+    V3 s_l = v3(dot(afr(sc, 0), s_hat), dot(afr(sc, 1), s_hat), dot(afr(sc, 2), s_hat));
+    V3 p_l = v3(dot(afr(sc, 0), p_hat), dot(afr(sc, 1), p_hat), dot(afr(sc, 2), p_hat));
+    if (kBiax) {
+        // The trichroic tensor diag(ao, ab, ae) (birefringence.rs:238-286): the same sums as
+        // below with the a1 and a2 axes weighted apart.
+        const QfInv3 qi = qf_invariants3(s_l, p_l);
+        const float u_x = (ea_l.x * ea_l.x + eb_l.x * eb_l.x) * 0.5f;
+        const float u_y = (ea_l.y * ea_l.y + eb_l.y * eb_l.y) * 0.5f;
+        const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float ao = LC_AO(c), ab = LC_AB(c), ae = LC_AE(c);
+            const float qf_e = qf_channel3(qi, ps.st[c].q, ps.st[c].u, ao, ab, ae);
+            const float unpol = fmaf(ao, u_x, fmaf(ab, u_y, ae * u_z));
+            const float p = clampf(degree_of_polarization(ps.st[c]), 0.0f, 1.0f);
+            use(c, fmaf(p, qf_e - unpol, unpol));
+        }
+    } else {
+        const QfInv qi = qf_invariants(s_l, p_l);
+        // The unpolarized term midpoint(ea.A.ea, eb.A.eb) with its channel-invariant parts
+        // gathered: ao (|ea_xy|^2 + |eb_xy|^2) / 2 + ae (ea_z^2 + eb_z^2) / 2.
+        const float u_xy = ((ea_l.x * ea_l.x + ea_l.y * ea_l.y) + (eb_l.x * eb_l.x + eb_l.y * eb_l.y)) * 0.5f;
+        const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float ao = LC_AO(c), ae = LC_AE(c);
+            const float qf_e = qf_channel(qi, ps.st[c].q, ps.st[c].u, ao, ae);
+            const float unpol = fmaf(ao, u_xy, ae * u_z);
+            const float p = clampf(degree_of_polarization(ps.st[c]), 0.0f, 1.0f);
+            use(c, fmaf(p, qf_e - unpol, unpol));
+        }
+    }
+}
+
 // One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
 // has ended (escape, Russian roulette, or the bounce cap).  kAniso: 0 isotropic, 1
 // uniaxial, 2 biaxial (an anisotropic stone with a biaxial indicatrix).
@@ -279,6 +374,83 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         }
         ps.have_n = true;
     }
+    // ---- volume scattering (try_scatter_step + maybe_scatter_or_extinguish,
+    //      scattering.rs:132-336; call site transport.rs:805-848) ----
+    // Inside a scattering stone, before the facet hit is processed: an exponential free path
+    // on the hero's extinction alpha_hero + sigma_s, the alphas taken in the previous plane
+    // of incidence (none after a scatter).  A scatter before the facet depolarizes and
+    // reweights every channel, turns the ray by the Henyey-Greenstein phase function, rolls
+    // the roulette past bounce 4 and ends the iteration; otherwise every channel takes its
+    // extinction over the segment relative to the hero's, in place of Beer-Lambert.  Traced
+    // by the kernels built for the rare features (kFrost).
+    const bool scattering = kFrost && !(sc.sigma_s <= 0.0f);
+    if (scattering && ps.inside) {
+        ev_inc(ev, GR_EV_SCATTER);
+        const float sigma_s = sc.sigma_s;
+        float alphas[8];
+        if (kAbsorb) {
+            if (!ps.have_abs) {
+                fill_absorption_cache<kBiax>(ps.hero_off, sc, lcache);
+                ps.have_abs = true;
+            }
+            channel_alphas<kBiax>(ps, sc, lcache, ps.cur_k, ps.have_prev ? ps.prev_pn : v3(0.0f, 0.0f, 0.0f),
+                                  [&](int c, float a) { alphas[c] = a; });
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) alphas[k] = 0.0f;  // no band: every coefficient is zero
+        }
+        const float sigma_t_hero = alphas[0] + sigma_s;
+        const float one_minus_u = fmaxf(1.0f - stream_draw(ps.seed, bounce, DISTANCE_SAMPLE_STREAM), 1e-7f);
+        const float t_free = -(logf(one_minus_u)) / sigma_t_hero;
+        const float hit_t_scaled = hit_t * sc.absorption_path_scale;
+        if (t_free < hit_t_scaled) {
+            const float pdf_hero = sigma_t_hero * one_minus_u;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float sigma_t_k = alphas[k] + sigma_s;
+                const float tr_k = exp_lobe(-(alphas[k] + sigma_s) * t_free);
+                const float weight = tr_k * sigma_s / pdf_hero;
+                ps.st[k] = Stokes{intensity(ps.st[k]) * weight, 0.0f, 0.0f, 0.0f};
+                ps.pdf[k] *= sigma_t_k * tr_k;
+            }
+            const float u1 = stream_draw(ps.seed, bounce, PHASE_DIR_U_STREAM);
+            const float u2 = stream_draw(ps.seed, bounce, PHASE_DIR_V_STREAM);
+            const V3 new_dir = sample_henyey_greenstein_direction(u1, u2, sc.scattering_g, rd);
+            ps.ray_o = add(ro, mul(rd, t_free / sc.absorption_path_scale));
+            ps.ray_d = new_dir;
+            ps.cur_k = new_dir;
+            ps.have_prev = false;
+            fnv(ps.fh, 0xFFFEu);
+            if (bounce > 4) {
+                // apply_russian_roulette (transport.rs:204-220); an ended path is
+                // PathTermination::ScatterAbsorbed.
+                ev_inc(ev, GR_EV_RR);
+                float max_i = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) max_i = fmaxf(max_i, intensity(ps.st[k]));
+                const float q = clampf(max_i, 0.05f, 1.0f);
+                if (stream_draw(ps.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) {
+                    ps.term_bounce = bounce;
+                    ps.term_reason = 1u;
+                    return true;
+                }
+                const float inv_q = 1.0f / q;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) ps.st[k] = scale(ps.st[k], inv_q);
+            }
+            ps.bounce = bounce + 1u;
+            return ps.bounce >= sc.max_bounces;
+        }
+        float survive[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) survive[k] = exp_lobe(-(alphas[k] + sigma_s) * hit_t_scaled);
+        const float survive_hero = fmaxf(survive[0], 1e-30f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ps.st[k] = scale(ps.st[k], survive[k] / survive_hero);
+            ps.pdf[k] *= survive[k];
+        }
+    }
     // An unbounded side (no plane on it) reports facet 0 with a zero normal, as the
     // reference's HitRecord does for a negative index (intersect.rs:105-126).
     V3 normal = v3(0.0f, 0.0f, 0.0f);
@@ -309,97 +481,48 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     ps.have_prev = true;
 
     // ---- interior segment: pleochroic Beer-Lambert (raytracer/absorption.rs:115-216,
-    //      birefringence.rs:252-286, polarization.rs:172-192) ----
+    //      birefringence.rs:252-286, polarization.rs:172-192); a scattering stone's
+    //      extinction was taken by the scatter step above (transport.rs:586-606) ----
     if (inside) {
         normal = neg(normal);
-        ev_inc(ev, GR_EV_ABSORB);
-        if (kAbsorb) {
-            if (!ps.have_abs) {
-                // spectral_absorption per eigen-axis (raytracer/absorption.rs:28-30,
-                // optics/absorption.rs:68-71), summed from -0.0 like Iterator::sum.
+        if (!scattering) {
+            ev_inc(ev, GR_EV_ABSORB);
+            if (kAbsorb) {
+                // The cache fill is written out here rather than calling
+                // fill_absorption_cache: with the call, the compiler's allocation of this
+                // (hot) block spilled 178 VGPRs instead of 30 (same arithmetic).
+                if (!ps.have_abs) {
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    float so = -0.0f, se = -0.0f;
-                    const float lk = comb_lambda(ps.hero_off, k);
-                    for (int b = 0; b < sc.n_o_bands; ++b) {
-                        float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
-                        so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
-                    }
-                    for (int b = 0; b < sc.n_e_bands; ++b) {
-                        float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
-                        se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
-                    }
-                    LC_AO(k) = so;
-                    LC_AE(k) = se;
-                    if (kBiax) {
-                        // The beta band set of a trichroic stone (refraction.rs:151-166);
-                        // otherwise the tensor is uniaxial, (ao, ao, ae).
-                        float sb = so;
-                        if (sc.has_beta_bands) {
-                            sb = -0.0f;
-                            for (int b = 0; b < sc.n_beta_bands; ++b) {
-                                float t = (lk - sc.beta_bands[b][0]) / sc.beta_bands[b][1];
-                                sb = sb + sc.beta_bands[b][2] * expf(-0.5f * t * t);
-                            }
+                    for (int k = 0; k < 8; ++k) {
+                        float so = -0.0f, se = -0.0f;
+                        const float lk = comb_lambda(ps.hero_off, k);
+                        for (int b = 0; b < sc.n_o_bands; ++b) {
+                            float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
+                            so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
                         }
-                        LC_AB(k) = sb;
+                        for (int b = 0; b < sc.n_e_bands; ++b) {
+                            float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
+                            se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
+                        }
+                        LC_AO(k) = so;
+                        LC_AE(k) = se;
+                        if (kBiax) {
+                            float sb = so;
+                            if (sc.has_beta_bands) {
+                                sb = -0.0f;
+                                for (int b = 0; b < sc.n_beta_bands; ++b) {
+                                    float t = (lk - sc.beta_bands[b][0]) / sc.beta_bands[b][1];
+                                    sb = sb + sc.beta_bands[b][2] * expf(-0.5f * t * t);
+                                }
+                            }
+                            LC_AB(k) = sb;
+                        }
                     }
+                    ps.have_abs = true;
                 }
-                ps.have_abs = true;
-            }
-            // The eigenmodes: the hero's biaxial indicatrix (raytracer/absorption.rs:174-196),
-            // else the uniaxial ordinary / extraordinary pair.
-            V3 ea, eb;
-            if (kBiax) {
-                const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
-                bx_eigen_polarizations(bx_of(LC_NO(0), sc.delta_beta_alpha, dn), fr, cur_k, &ea, &eb);
-            } else {
-                ea = ordinary_eigen_polarization(cur_k, c_axis);
-                eb = extraordinary_eigen_polarization(cur_k, c_axis);
-            }
-            // Channel-invariant parts of electric_field_direction and of the quadratic
-            // forms: the s/p axes and the projections on the absorption frame (a1, a2, c).
-            V3 kk = normalize_or_zero(cur_k);
-            V3 s_raw = sub(cpn, mul(kk, dot(kk, cpn)));
-            V3 s_hat = len2(s_raw) > 1e-8f ? normalize(s_raw) : arbitrary_perpendicular(kk);
-            V3 p_hat = cross(kk, s_hat);
-            V3 ea_l = v3(dot(afr(sc, 0), ea), dot(afr(sc, 1), ea), dot(afr(sc, 2), ea));
-            V3 eb_l = v3(dot(afr(sc, 0), eb), dot(afr(sc, 1), eb), dot(afr(sc, 2), eb));
-            V3 s_l = v3(dot(afr(sc, 0), s_hat), dot(afr(sc, 1), s_hat), dot(afr(sc, 2), s_hat));
-            V3 p_l = v3(dot(afr(sc, 0), p_hat), dot(afr(sc, 1), p_hat), dot(afr(sc, 2), p_hat));
-            float spl = hit_t * sc.absorption_path_scale;
-            if (kBiax) {
-                // The trichroic tensor diag(ao, ab, ae) (birefringence.rs:238-286): the same
-                // sums as below with the a1 and a2 axes weighted apart.
-                const QfInv3 qi = qf_invariants3(s_l, p_l);
-                const float u_x = (ea_l.x * ea_l.x + eb_l.x * eb_l.x) * 0.5f;
-                const float u_y = (ea_l.y * ea_l.y + eb_l.y * eb_l.y) * 0.5f;
-                const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const float ao = LC_AO(k), ab = LC_AB(k), ae = LC_AE(k);
-                    const float qf_e = qf_channel3(qi, ps.st[k].q, ps.st[k].u, ao, ab, ae);
-                    const float unpol = fmaf(ao, u_x, fmaf(ab, u_y, ae * u_z));
-                    float p = clampf(degree_of_polarization(ps.st[k]), 0.0f, 1.0f);
-                    float alpha = fmaf(p, qf_e - unpol, unpol);
-                    ps.st[k] = scale(ps.st[k], exp_lobe(-alpha * spl));
-                }
-            } else {
-                const QfInv qi = qf_invariants(s_l, p_l);
-                // The unpolarized term midpoint(ea.A.ea, eb.A.eb) with its channel-invariant
-                // parts gathered: ao (|ea_xy|^2 + |eb_xy|^2) / 2 + ae (ea_z^2 + eb_z^2) / 2.
-                const float u_xy =
-                    ((ea_l.x * ea_l.x + ea_l.y * ea_l.y) + (eb_l.x * eb_l.x + eb_l.y * eb_l.y)) * 0.5f;
-                const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const float ao = LC_AO(k), ae = LC_AE(k);
-                    const float qf_e = qf_channel(qi, ps.st[k].q, ps.st[k].u, ao, ae);
-                    const float unpol = fmaf(ao, u_xy, ae * u_z);
-                    float p = clampf(degree_of_polarization(ps.st[k]), 0.0f, 1.0f);
-                    float alpha = fmaf(p, qf_e - unpol, unpol);
-                    ps.st[k] = scale(ps.st[k], exp_lobe(-alpha * spl));
-                }
+                const float spl = hit_t * sc.absorption_path_scale;
+                channel_alphas<kBiax>(ps, sc, lcache, cur_k, cpn,
+                                      [&](int c, float a) { ps.st[c] = scale(ps.st[c], exp_lobe(-a * spl)); });
             }
         }
     }
@@ -1472,7 +1595,7 @@ template <int A, bool B, int E>
 hipError_t launch_plain(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
     // Scenes with frosted facets take the kernels built with the frosted bounce; the others
     // keep kernels without it (its code raises the register pressure of every bounce).
-    if (sc.n_frosted)
+    if (gr::rare_features(sc))
         return a.info ? launch_plain_i<A, B, E, true, true>(sc, a, stream)
                       : launch_plain_i<A, B, E, true, false>(sc, a, stream);
     return a.info ? launch_plain_i<A, B, E, false, true>(sc, a, stream)

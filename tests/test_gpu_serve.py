@@ -48,6 +48,8 @@ def _mix():
          4, 7),
         (B.SceneRef(cam2, 26, 22, scenes.oval_brilliant(), M.by_name("Tanzanite"), 24, B.HdrMap(env)), 3, 9),
         (B.SceneRef(cam, 18, 24, srb, M.by_name("Topaz"), 12, B.Studio("Daylight", 1.0, 0.4, 0.35)), 6, 5),
+        (B.SceneRef(cam2, 20, 16, srb, dataclasses.replace(M.by_name("Ruby"), scattering_sigma_s=0.8,
+                                                           scattering_g=-0.3), 16, B.HdrMap(env)), 2, 7),
     ]
 
 
