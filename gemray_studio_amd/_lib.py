@@ -122,7 +122,7 @@ EXPORTS = [
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
     "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_serve_set_policy",
-    "gr_optical_metrics", "gr_job_take_delta",
+    "gr_optical_metrics", "gr_job_take_delta", "gr_guide_buffers",
 ]
 GR_SERVE_EQUAL_SHARE, GR_SERVE_DEADLINE = 0, 1
 (GR_PROBE_NEFF, GR_PROBE_POYNTING, GR_PROBE_TIR_RETARDATION, GR_PROBE_FRAME_ROTATION, GR_PROBE_FIELD_QFORM,
@@ -200,6 +200,9 @@ def load():
     L.gr_optical_metrics.restype = ctypes.c_int
     L.gr_probe.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t, fp]
     L.gr_probe.restype = ctypes.c_int
+    L.gr_guide_buffers.argtypes = [vp, ctypes.POINTER(Camera), ctypes.c_uint32, ctypes.c_uint32, fp, ctypes.c_uint32,
+                                   fp, fp, ctypes.POINTER(ctypes.c_int32)]
+    L.gr_guide_buffers.restype = ctypes.c_int
     for name in ("gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel", "gr_job_film",
                  "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
                  "gr_serve_set_tick_tuples", "gr_serve_set_policy"):

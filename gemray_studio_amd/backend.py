@@ -369,6 +369,24 @@ class GpuBackend:
                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "gr_probe")
         return out
 
+    def guide_buffers(self, camera: Camera, width: int, height: int, planes: np.ndarray):
+        """The denoiser's primary-hit guide buffers (generate_guide_buffers,
+        apps/diagram-gui/src/bridge/guide_pass.rs:37-110): per pixel the hit distance of the
+        pixel-corner camera ray, the hit facet's normal and its index (1e6, zeros and -1 on a
+        miss).  Returns (depth (h*w,), normal (h*w, 3), facet (h*w,) int32)."""
+        self._require()
+        p = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+        n = width * height
+        depth = np.zeros(n, np.float32)
+        normal = np.zeros((n, 3), np.float32)
+        facet = np.zeros(n, np.int32)
+        check(self._L.gr_guide_buffers(self._ctx, ctypes.byref(camera), width, height,
+                                       p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.shape[0],
+                                       depth.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       normal.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       facet.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "gr_guide_buffers")
+        return depth, normal, facet
+
     def optical_metrics(self, planes: np.ndarray, material: GemMaterial, poses) -> list:
         """evaluate_gem_optical_metrics (color/metrics.rs:1082-1248) for each pose
         (cam_yaw, cam_pitch, light_yaw, light_pitch) of one stone, in one device call.

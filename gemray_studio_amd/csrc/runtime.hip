@@ -634,6 +634,54 @@ gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out) {
     return st;
 }
 
+gr_status gr_guide_buffers(gr_ctx *ctx, const gr_camera *camera, uint32_t width, uint32_t height, const float *planes,
+                           uint32_t n_planes, float *depth, float *normal, int32_t *facet) {
+    if (!ctx || !camera) return GR_ERR_INVALID;
+    if (width == 0 || height == 0) return GR_OK;
+    if (!depth || !normal || !facet || (n_planes > 0 && !planes)) return GR_ERR_INVALID;
+    if ((uint64_t)width * height > GR_MAX_PIXELS || n_planes > GR_MAX_PLANES) return GR_ERR_INVALID;
+    if (!finite3(camera->origin) || !finite3(camera->forward) || !finite3(camera->right) || !finite3(camera->up) ||
+        !std::isfinite(camera->fov_tan))
+        return GR_ERR_INVALID;
+    for (uint32_t i = 0; i < n_planes; ++i)
+        if (!finite3(planes + 4 * i) || !std::isfinite(planes[4 * i + 3])) return GR_ERR_INVALID;
+    gr::DevScene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    for (int i = 0; i < 3; ++i) {
+        sc.cam_o[i] = camera->origin[i];
+        sc.cam_f[i] = camera->forward[i];
+        sc.cam_r[i] = camera->right[i];
+        sc.cam_u[i] = camera->up[i];
+    }
+    sc.fov_tan = camera->fov_tan;
+    sc.width = width;
+    sc.height = height;
+    sc.n_planes = n_planes;
+    const size_t n = (size_t)width * height;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GR_CHECK(hipSetDevice(ctx->device));
+    float4 *d_planes = nullptr;
+    float *d_depth = nullptr, *d_normal = nullptr;
+    int32_t *d_facet = nullptr;
+    gr_status st = GR_OK;
+    if (hipMalloc(&d_planes, (n_planes ? n_planes : 1) * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&d_depth, n * sizeof(float)) != hipSuccess || hipMalloc(&d_normal, 3 * n * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d_facet, n * sizeof(int32_t)) != hipSuccess ||
+        (n_planes && hipMemcpyAsync(d_planes, planes, n_planes * sizeof(float4), hipMemcpyHostToDevice, ctx->stream) !=
+                         hipSuccess) ||
+        gr_launch_guide(sc, d_planes, d_depth, d_normal, d_facet, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(depth, d_depth, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(normal, d_normal, 3 * n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(facet, d_facet, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        st = GR_ERR_DEVICE;
+    if (d_planes) (void)hipFree(d_planes);
+    if (d_depth) (void)hipFree(d_depth);
+    if (d_normal) (void)hipFree(d_normal);
+    if (d_facet) (void)hipFree(d_facet);
+    return st;
+}
+
 gr_status gr_last_stats(gr_ctx *ctx, gr_stats *out) {
     if (!ctx || !out) return GR_ERR_INVALID;
     std::lock_guard<std::mutex> g(ctx->mu);

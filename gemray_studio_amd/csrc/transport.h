@@ -122,6 +122,9 @@ inline int scene_class(const DevScene &sc) {
 hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &args, hipStream_t stream);
 // Evaluates device function fn (GR_PROBE_*) on n items (9 floats in, 3 out each).
 hipError_t gr_launch_probe(int fn, const float *in, float *out, uint32_t n, hipStream_t stream);
+// Fills the primary-hit guide buffers of the frame (depth, normal xyz, facet per pixel).
+hipError_t gr_launch_guide(const gr::DevScene &sc, const float4 *planes, float *depth, float *normal, int32_t *facet,
+                           hipStream_t stream);
 // Fills cert[0, width * height) with the frame's camera-ray miss certificates.
 hipError_t gr_launch_cert(const gr::DevScene &sc, const float4 *planes, uint32_t *cert, hipStream_t stream);
 // Traces slices [0, n_slices) of one class (all with scene_class == cls) in one launch of

@@ -1434,6 +1434,59 @@ __global__ __launch_bounds__(kBlock) void cert_kernel(const DevScene sc, const f
     cert[pixel] = proof ? ((uint32_t)near_idx | ((uint32_t)far_idx << 16)) : 0xFFFFFFFFu;
 }
 
+// The denoiser's primary-hit guide buffers (apps/diagram-gui/src/bridge/guide_pass.rs:60-110):
+// per pixel, the camera ray through the pixel's corner (jitter 0, the ray cert_kernel traces)
+// against the stone (intersect_polyhedron, intersect.rs:9-78): the hit distance, the facet's
+// normal and index, or 1e6, a zero normal and -1 on a miss.  An unbounded side (no plane on
+// it) reports facet 0 with a zero normal, as HitRecord does.
+__global__ __launch_bounds__(kBlock) void guide_kernel(const DevScene sc, const float4 *__restrict__ planes_g,
+                                                       uint32_t n_pixels, float *__restrict__ depth,
+                                                       float *__restrict__ normal, int32_t *__restrict__ facet) {
+    __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
+    stage_planes(planes, cam_side, planes_g, sc);
+    __syncthreads();
+    const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;
+    if (pixel >= n_pixels) return;
+    const uint32_t x = pixel % sc.width, y = pixel / sc.width;
+    const float w = (float)sc.width, h = (float)sc.height;
+    const float aspect = w / h;
+    const float u = ((float)x / w - 0.5f) * 2.0f * aspect * sc.fov_tan;
+    const float v = (0.5f - (float)y / h) * 2.0f * sc.fov_tan;
+    const V3 rd = normalize(add(add(v3(sc.cam_f[0], sc.cam_f[1], sc.cam_f[2]),
+                                    mul(v3(sc.cam_r[0], sc.cam_r[1], sc.cam_r[2]), u)),
+                                mul(v3(sc.cam_u[0], sc.cam_u[1], sc.cam_u[2]), v)));
+    const V3 ro = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
+    float t_near = -1e30f, t_far = 1e30f;
+    int near_idx = -1, far_idx = -1;
+    bool outside = false;
+    clip_scan<true>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+    float t = 1.0e6f;
+    int idx = -1;
+    bool hit = false;
+    if (!outside && !(t_near > t_far)) {
+        if (t_near > 1e-4f) {
+            hit = true;
+            t = t_near;
+            idx = near_idx;
+        } else if (t_far > 1e-4f) {
+            hit = true;
+            t = t_far;
+            idx = far_idx;
+        }
+    }
+    V3 n = v3(0.0f, 0.0f, 0.0f);
+    if (hit && idx >= 0) {
+        const float4 p = planes[idx];
+        n = v3(p.x, p.y, p.z);
+    }
+    depth[pixel] = t;
+    normal[3 * (size_t)pixel + 0] = n.x;
+    normal[3 * (size_t)pixel + 1] = n.y;
+    normal[3 * (size_t)pixel + 2] = n.z;
+    facet[pixel] = hit ? (idx >= 0 ? idx : 0) : -1;
+}
+
 // Per-function probes: one device function of the transport path per lane, on
 // caller-given inputs (9 floats in, 3 out per item), for the per-function ULP tests.
 __global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__restrict__ in,
@@ -1724,6 +1777,14 @@ hipError_t gr_launch_cert(const gr::DevScene &sc, const float4 *planes, uint32_t
     const uint32_t n = sc.width * sc.height;
     hipLaunchKernelGGL(gr::cert_kernel, dim3((n + gr::kBlock - 1) / gr::kBlock), dim3(gr::kBlock), 0, stream, sc,
                        planes, n, cert);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_guide(const gr::DevScene &sc, const float4 *planes, float *depth, float *normal, int32_t *facet,
+                           hipStream_t stream) {
+    const uint32_t n = sc.width * sc.height;
+    hipLaunchKernelGGL(gr::guide_kernel, dim3((n + gr::kBlock - 1) / gr::kBlock), dim3(gr::kBlock), 0, stream, sc,
+                       planes, n, depth, normal, facet);
     return hipGetLastError();
 }
 

@@ -185,6 +185,16 @@ gr_status gr_trace_samples(gr_ctx *ctx, const gr_scene *scene, uint32_t pixel_be
  * from the origin, vertical field of view in degrees. */
 gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, gr_camera *out);
 
+/* Primary-hit guide buffers for the denoiser (generate_guide_buffers,
+ * apps/diagram-gui/src/bridge/guide_pass.rs:37-110): for each pixel (x, y) of a width x height
+ * frame, the camera ray Camera::generate_ray(x, y, width, height, 0, 0) against the planes
+ * (intersect_polyhedron, intersect.rs:9-78): depth[p] = hit distance, normal[3p..3p+2] = the
+ * hit facet's plane normal, facet[p] = its index; a miss gives 1e6, (0, 0, 0) and -1.  All
+ * three outputs are caller-owned host arrays of width * height entries; a zero-sized frame
+ * writes nothing. */
+gr_status gr_guide_buffers(gr_ctx *ctx, const gr_camera *camera, uint32_t width, uint32_t height, const float *planes,
+                           uint32_t n_planes, float *depth, float *normal, int32_t *facet);
+
 /* Per-function probe (no reference counterpart; for the per-function ULP tests): evaluates
  * one device function of the transport kernel on n items, 9 input floats and 3 output
  * floats per item, and copies the outputs to out (3 * n floats).

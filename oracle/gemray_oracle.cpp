@@ -1869,6 +1869,26 @@ extern "C" void or_uniaxial_eigen_polarizations(const float w[3], const float c[
     store3(ordinary_eigen_polarization(load3(w), load3(c)), out);
     store3(extraordinary_eigen_polarization(load3(w), load3(c)), out + 3);
 }
+// generate_guide_buffers (apps/diagram-gui/src/bridge/guide_pass.rs:37-110): per pixel the
+// ray Camera::generate_ray(x, y, w, h, 0, 0) against the planes (intersect_polyhedron,
+// intersect.rs:9-78): the hit distance, normal and facet, or 1e6, zero and -1 on a miss.
+extern "C" void or_guide_buffers(const or_camera *cam, uint32_t width, uint32_t height, const float *planes,
+                                 int n_planes, float *depth, float *normal, int32_t *facet) {
+    for (uint32_t y = 0; y < height; ++y) {
+        for (uint32_t x = 0; x < width; ++x) {
+            const size_t p = (size_t)y * width + x;
+            float o[3], d[3];
+            or_generate_ray(cam, (float)x, (float)y, (float)width, (float)height, 0.0f, 0.0f, o, d);
+            Hit h = intersect_planes(planes, n_planes, load3(o), load3(d));
+            depth[p] = h.hit ? h.t : 1.0e6f;
+            normal[3 * p + 0] = h.hit ? h.normal.x : 0.0f;
+            normal[3 * p + 1] = h.hit ? h.normal.y : 0.0f;
+            normal[3 * p + 2] = h.hit ? h.normal.z : 0.0f;
+            facet[p] = h.hit ? h.facet : -1;
+        }
+    }
+}
+
 // Scattering probes (scattering.rs:132-232, 428-447; raytracer/absorption.rs:174-214).
 extern "C" void or_sample_hg_batch(int n, const float *u1, const float *u2, float g, const float forward[3],
                                    float *out) {

@@ -135,6 +135,10 @@ void or_poynting_direction(const float k[3], const float e[3], float out[3]);
 void or_uniaxial_eigen_polarizations(const float w[3], const float c[3], float out[6] /* o, e */);
 float or_effective_pleochroic_alpha(const float alpha[3], int trichroic, const float axis[3], const float e[3],
                                     const float ea[3], const float eb[3], float dop);
+/* generate_guide_buffers (apps/diagram-gui/src/bridge/guide_pass.rs:37-110): the denoiser's
+ * primary-hit depth, normal (3 floats) and facet (-1 on a miss) per pixel. */
+void or_guide_buffers(const or_camera *cam, uint32_t width, uint32_t height, const float *planes, int n_planes,
+                      float *depth, float *normal, int32_t *facet);
 /* Henyey-Greenstein scattering probes (optics/raytracer/scattering.rs:132-232, 428-447;
  * raytracer/absorption.rs:174-214).  Stokes vectors travel as 8 x (I, Q, U, V). */
 void or_sample_hg_batch(int n, const float *u1, const float *u2, float g, const float forward[3], float *out);

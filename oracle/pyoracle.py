@@ -154,6 +154,8 @@ def _declare(L):
     L.or_uniaxial_eigen_polarizations.argtypes = [fp, fp, fp]
     L.or_effective_pleochroic_alpha.argtypes = [fp, ctypes.c_int, fp, fp, fp, fp, ctypes.c_float]
     L.or_effective_pleochroic_alpha.restype = ctypes.c_float
+    L.or_guide_buffers.argtypes = [ctypes.POINTER(OrCamera), u32, u32, fp, ctypes.c_int, fp, fp,
+                                   ctypes.POINTER(ctypes.c_int32)]
     L.or_sample_hg_batch.argtypes = [ctypes.c_int, fp, fp, ctypes.c_float, fp, fp]
     L.or_channel_absorption_alphas.argtypes = [ctypes.c_void_p, fp, fp, fp, fp, fp]
     L.or_maybe_scatter_or_extinguish.argtypes = [fp, ctypes.c_float, ctypes.c_float, fp, ctypes.c_float,
@@ -332,6 +334,19 @@ class Indicatrix:
         n = lib().or_biaxial_resolve_entry_mode(fptr(self.p), fptr(f32arr(incident)), fptr(f32arr(normal)),
                                                 cos_i, n_seed, int(want_slow), fptr(wave))
         return np.float32(n), wave
+
+
+def guide_buffers(camera: OrCamera, width, height, planes):
+    """generate_guide_buffers (apps/diagram-gui/src/bridge/guide_pass.rs:37-110): (depth,
+    normal (n, 3), facet int32) per pixel."""
+    p = np.ascontiguousarray(planes, dtype=np.float32)
+    n = width * height
+    depth = np.zeros(n, np.float32)
+    normal = np.zeros((n, 3), np.float32)
+    facet = np.zeros(n, np.int32)
+    lib().or_guide_buffers(ctypes.byref(camera), width, height, fptr(p), p.shape[0], fptr(depth), fptr(normal),
+                           facet.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return depth, normal, facet
 
 
 def poynting_direction(k, e) -> np.ndarray:
