@@ -258,6 +258,50 @@ def metrics_workload(gpu, args):
     return out
 
 
+def feature_workloads(gpu, scene, args):
+    """The material branches and passes beside the single job, each on the single-job frame
+    (oval brilliant, 2048x1024 HDR map, 1920x1080, depth 32) with one thing changed, timed as
+    one 16-spp launch after a warm-up: biaxial stones (Alexandrite, trichroic; Tanzanite),
+    Henyey-Greenstein scattering (Ruby with sigma_s 0.8, g 0.4), frosted near-vertical
+    (girdle) facets on the single-job sapphire; and the denoiser's guide pass over the frame
+    (gr_guide_buffers, host copies of its 41 MB of outputs included)."""
+    import dataclasses
+    from gemray_studio_amd import materials as M
+
+    spp = 16
+    girdle = (np.abs(scene.planes[:, 1]) < 0.2).astype(np.uint32)
+    variants = [
+        ("biaxial_alexandrite", dataclasses.replace(scene, material=M.by_name("Alexandrite"))),
+        ("biaxial_tanzanite", dataclasses.replace(scene, material=M.by_name("Tanzanite"))),
+        ("scattering_ruby", dataclasses.replace(scene, material=dataclasses.replace(
+            M.by_name("Ruby"), scattering_sigma_s=0.8, scattering_g=0.4))),
+        ("frosted_girdle_sapphire", dataclasses.replace(scene, facet_finishes=girdle)),
+    ]
+    out = {"workload": "single-job frame (1920x1080, depth 32) with the material or finishes changed, "
+                       f"one {spp}-spp launch each; guide pass over the same frame"}
+    film = gpu.film(scene.width, scene.height)
+    for name, sc in variants:
+        ds = gpu.upload(sc)
+        gpu.render(ds, 0, 2, film)
+        best = None
+        for k in range(2):
+            gpu.render(ds, 2 + k * spp, spp, film)
+            st = gpu.last_stats()
+            if best is None or st["total_ms"] < best["total_ms"]:
+                best = st
+        out[name] = {"segments_per_s": best["segments"] / (best["total_ms"] * 1e-3),
+                     "segments_per_path": best["segments"] / (scene.width * scene.height * spp),
+                     "launch_ms": best["total_ms"]}
+        print(f"bench: {name} {out[name]['segments_per_s'] / 1e9:.2f} G segments/s", file=sys.stderr, flush=True)
+    gpu.guide_buffers(scene.camera, scene.width, scene.height, scene.planes)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        gpu.guide_buffers(scene.camera, scene.width, scene.height, scene.planes)
+    gs = (time.perf_counter() - t0) / 5
+    out["guide_pass"] = {"pixels_per_s": scene.width * scene.height / gs, "call_ms": gs * 1e3}
+    return out
+
+
 def job_workloads(gpu, env_map, args):
     """Batched (all jobs submitted at once) and served (Poisson arrivals) throughput and
     latency of the job loop, each beside the plain per-job path on the same jobs."""
@@ -427,6 +471,7 @@ def main():
         out["metrics"] = metrics_workload(gpu, args)
         print(f"bench: metrics {out['metrics']['evaluations_per_s']:.0f} evaluations/s", file=sys.stderr,
               flush=True)
+        out["features"] = feature_workloads(gpu, scene, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, cam_args, args.spp, args.cpu_budget)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

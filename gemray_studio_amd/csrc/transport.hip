@@ -1033,11 +1033,18 @@ __device__ __forceinline__ void write_result(const PathState &ps, const DevScene
 // Occupancy target of the trace kernels (waves per SIMD); 0 leaves the register budget to
 // the compiler.  A tuning parameter chosen by measurement: on the single-job workload
 // 3 waves/SIMD (168 VGPRs) ran 1.36x faster than the compiler's 2 and 1.03x faster than 4.
+// The kernels built for biaxial stones (kAniso == 2) carry the indicatrix and take their
+// own target, GR_BIAX_WAVES_PER_EU: 3 as well, measured 8 % faster on the biaxial frames
+// than 2 waves/SIMD (256 VGPRs, no spills) despite ~100 spilled VGPRs.
 #ifndef GR_TRACE_WAVES_PER_EU
 #define GR_TRACE_WAVES_PER_EU 3
 #endif
+#ifndef GR_BIAX_WAVES_PER_EU
+#define GR_BIAX_WAVES_PER_EU 3
+#endif
 #if GR_TRACE_WAVES_PER_EU > 0
-#define GR_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_TRACE_WAVES_PER_EU, GR_TRACE_WAVES_PER_EU)))
+#define GR_WPE(kAniso) ((kAniso) == 2 ? GR_BIAX_WAVES_PER_EU : GR_TRACE_WAVES_PER_EU)
+#define GR_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_WPE(kAniso), GR_WPE(kAniso))))
 #else
 #define GR_TRACE_OCCUPANCY
 #endif
