@@ -30,7 +30,10 @@ __device__ __forceinline__ void clip_plane(float4 p, float side_cam, int i, V3 r
     const bool steep = fabsf(denom) > 1e-7f;
     const bool entering = denom < 0.0f;
     const float t_ext = entering ? t_near : t_far;
-    if (steep ? fmaf(t_ext, denom, side) > 0.0f : side > 0.0f) {
+    // One select and one compare rather than a branch per plane on steepness: the wave
+    // then branches only where a plane can change an extreme.
+    const float screen = steep ? fmaf(t_ext, denom, side) : side;
+    if (screen > 0.0f) {
         if (!steep) {
             outside = true;
         } else {
