@@ -30,24 +30,18 @@ __device__ __forceinline__ void clip_plane(float4 p, float side_cam, int i, V3 r
     const bool steep = fabsf(denom) > 1e-7f;
     const bool entering = denom < 0.0f;
     const float t_ext = entering ? t_near : t_far;
-    // One select and one compare rather than a branch per plane on steepness: the wave
-    // then branches only where a plane can change an extreme.
+    // Selects rather than branches on steepness and side: the wave branches only where a
+    // plane can change an extreme, and the update inside is branch-free too.
     const float screen = steep ? fmaf(t_ext, denom, side) : side;
-    if (screen > 0.0f) {
-        if (!steep) {
-            outside = true;
-        } else {
-            const float t = (-side) / denom;
-            if (entering) {
-                if (t > t_near) {
-                    t_near = t;
-                    near_idx = i;
-                }
-            } else if (t < t_far) {
-                t_far = t;
-                far_idx = i;
-            }
-        }
+    outside = outside || (screen > 0.0f && !steep);
+    if (screen > 0.0f && steep) {
+        const float t = (-side) / denom;
+        const bool upd_near = entering && t > t_near;
+        const bool upd_far = !entering && t < t_far;
+        t_near = upd_near ? t : t_near;
+        near_idx = upd_near ? i : near_idx;
+        t_far = upd_far ? t : t_far;
+        far_idx = upd_far ? i : far_idx;
     }
 }
 // Planes are read four at a time (the staged array is padded to a multiple of four with
