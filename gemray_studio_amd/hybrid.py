@@ -24,9 +24,11 @@ from typing import Callable
 import numpy as np
 
 from .backend import GpuBackend, SceneRef
-from ._lib import GrError
+from ._lib import (GR_DECLINE_BUDGET, GR_DECLINE_ENV, GR_DECLINE_MATERIAL, GR_DECLINE_NO_DEVICE,
+                   GrError)
 
 CpuTrace = Callable[[SceneRef, int, int], np.ndarray]
+_DECLINES = (GR_DECLINE_NO_DEVICE, GR_DECLINE_ENV, GR_DECLINE_MATERIAL, GR_DECLINE_BUDGET)
 
 
 @dataclass(frozen=True)
@@ -96,13 +98,16 @@ class HybridStats:
 def supports(gpu: GpuBackend, scene: SceneRef) -> bool:
     """Whether the device traces this scene (``GemMaterial::gpu_supported`` and the
     boundary's environment check): a zero-sample accumulate validates and declines exactly
-    as a real one would, and touches nothing."""
+    as a real one would, and touches nothing.  Only a decline (GR_DECLINE_*) means "render
+    on the CPU"; an invalid scene or a device failure is raised."""
     if not gpu.available:
         return False
     try:
         gpu.accumulate(scene, 0, 0, np.zeros((scene.width * scene.height, 3), np.float32))
-    except GrError:
-        return False
+    except GrError as e:
+        if e.status in _DECLINES:
+            return False
+        raise
     return True
 
 
