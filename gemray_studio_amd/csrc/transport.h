@@ -117,6 +117,14 @@ inline int scene_class(const DevScene &sc) {
            (rare_features(sc) ? 8 : 0) | (sc.biaxial ? 16 : 0);
 }
 
+// The launchers of one scene class (scene_class bits), defined in the per-class translation
+// units (trace_launch.h) and dispatched to by gr_launch_trace / gr_launch_trace_batch.
+template <int C>
+hipError_t launch_trace_class(const DevScene &sc, const TraceArgs &a, hipStream_t stream);
+template <int C>
+hipError_t launch_batch_class(const BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, uint32_t cont_blocks,
+                              hipStream_t stream);
+
 }  // namespace gr
 
 hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &args, hipStream_t stream);

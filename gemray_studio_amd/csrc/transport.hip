@@ -1,1416 +1,8 @@
-// transport.hip -- the gfx950 spectral transport kernels.
-//
-// The estimator is gemray's transport_main
-// (crates/gemray/src/renderer/shaders/spectral_transport.wgsl:672), whose CPU statement is
-// trace_spectral_ray (optics/raytracer/transport.rs:617-872).  It is split into
-//   init_path    -- draws, camera ray and per-path state for one (pixel, sample) tuple;
-//   bounce_step  -- one bounce-loop iteration: facet-plane clip plus the 8-channel event;
-//   finish_path  -- environment radiance at the escape, spectral MIS, CIE XYZ, white balance.
-// Two trace kernels drive them:
-//   trace_kernel        -- one lane per tuple, bounce loop then finish (the plain path);
-//   trace_wave_kernel   -- persistent wavefronts: a lane whose path ends takes the next
-//                          tuple of its wave's range (ballot + popcount compaction), and
-//                          the finished path is written as an escape record that
-//                          finish_kernel turns into XYZ in a separate, fully coherent pass.
-// Facet planes are staged in LDS once per workgroup; material, camera and environment
-// constants arrive as one by-value kernel argument; per-path state lives in VGPRs.
-// No MFMA: there is no dense contraction.
-#include "transport.h"
-
-#include <cstddef>
-
-#include "device_physics.h"
-#include "biaxial.h"
-#include "clip.h"
+// transport.hip -- the shared non-template kernels (miss certificates, guide pass, probes,
+// film reduction) and the dispatch of trace launches to the per-class translation units.
+#include "transport_kernels.h"
 
 namespace gr {
-
-// ---------------------------------------------------------------------------------
-// Environment radiance (transport.rs:67-90).
-// ---------------------------------------------------------------------------------
-
-// blackbody_spectrum (optics/raytracer/environment.rs:12-21) with the lambda-independent
-// 560 nm denominator precomputed on the host.
-__device__ __forceinline__ float blackbody(float lambda_nm, const DevScene &sc) {
-    float exp_val = expf(fminf(14388000.0f / (lambda_nm * sc.temp_k_clamped), 80.0f));
-    float denom = fmaxf(exp_val - 1.0f, 1e-6f);
-    float ratio = sc.denom_560 / denom;
-    return clampf(powi_cr<5>(560.0f / lambda_nm) * ratio, 0.01f, 20.0f);
-}
-
-// rgb_to_spectral_radiance (renderer/env_map_spectrum.rs:60-82).
-__device__ __forceinline__ float asym_gauss(float x, float mu, float lo, float hi) {
-    const bool below = x < mu;
-    float t = div_by_const(x - mu, below ? lo : hi, below ? 1.0f / lo : 1.0f / hi);
-    return exp_lobe(-0.5f * t * t);
-}
-__device__ __forceinline__ float rgb_to_spectral(V3 rgb, float l) {
-    float r = fmaxf(rgb.x, 0.0f), g = fmaxf(rgb.y, 0.0f), b = fmaxf(rgb.z, 0.0f);
-    return fmaf(r, asym_gauss(l, 615.0f, 45.0f, 65.0f),
-                fmaf(g, asym_gauss(l, 545.0f, 45.0f, 45.0f), b * asym_gauss(l, 465.0f, 40.0f, 45.0f)));
-}
-
-// EnvironmentMap::radiance_rgb (renderer/env_map.rs:231-319): lat-long bilinear lookup.
-__device__ __forceinline__ V3 map_rgb(V3 dir, const DevScene &sc) {
-    V3 d = normalize_or_zero(dir);
-    float theta = acosf(clampf(d.y, -1.0f, 1.0f));
-    float phi = atan2f(d.x, d.z);
-    float v = theta / kPi;
-    float u = rem_euclid(phi / (2.0f * kPi), 1.0f);
-    const int w = sc.map_w, h = sc.map_h;
-    const float *px = sc.map_rgb;
-    if (w == 1 && h == 1) return v3(px[0], px[1], px[2]);
-    float fx = fmaf(rem_euclid(u, 1.0f), (float)w, -0.5f);
-    float fy = fmaf(clampf(v, 0.0f, 1.0f), (float)h, -0.5f);
-    float x0 = floorf(fx), y0 = floorf(fy);
-    float tx = fx - x0, ty = fy - y0;
-    int xi = (int)x0, yi = (int)y0;
-    int x0i = xi % w;
-    if (x0i < 0) x0i += w;
-    int x1i = x0i + 1 == w ? 0 : x0i + 1;
-    int y0i = yi < 0 ? 0 : (yi > h - 1 ? h - 1 : yi);
-    int y1i = yi + 1 < 0 ? 0 : (yi + 1 > h - 1 ? h - 1 : yi + 1);
-    const float *p00 = px + 3 * (y0i * w + x0i);
-    const float *p10 = px + 3 * (y0i * w + x1i);
-    const float *p01 = px + 3 * (y1i * w + x0i);
-    const float *p11 = px + 3 * (y1i * w + x1i);
-    float out[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float top = fmaf(p10[c], tx, p00[c] * (1.0f - tx));
-        float bottom = fmaf(p11[c], tx, p01[c] * (1.0f - tx));
-        out[c] = fmaf(bottom, ty, top * (1.0f - ty));
-    }
-    return v3(out[0], out[1], out[2]);
-}
-
-// Column j of the absorption tensor's principal frame (a1, a2, c).
-__device__ __forceinline__ V3 afr(const DevScene &sc, int j) {
-    return v3(sc.abs_frame[j][0], sc.abs_frame[j][1], sc.abs_frame[j][2]);
-}
-
-__device__ __forceinline__ void fnv(uint32_t &h, uint32_t v) {
-    h ^= v;
-    h *= 16777619u;
-}
-
-// wrapped_hero_wavelengths (transport.rs:45-53) for channel k, from
-// hero_off = lambda_hero - 380.  offset = k * 50 + hero_off lies in [0, 750], so
-// rem_euclid(offset, 400) is offset or offset - 400, and that subtraction is exact
-// (Sterbenz): the same bits as the reference's fmod.
-__device__ __forceinline__ float comb_lambda(float hero_off, int k) {
-    float offset = fmaf((float)k, (780.0f - 380.0f) / 8.0f, hero_off);
-    return 380.0f + (offset >= 400.0f ? offset - 400.0f : offset);
-}
-
-// Per-lane event counters, two 16-bit counts per register: a lane counts at most
-// kWaveTuples paths x GR_MAX_BOUNCES segments (32768) before flush_events.
-constexpr int kEvWords = (GR_EV_COUNT + 1) / 2;
-__device__ __forceinline__ void ev_inc(uint32_t *ev, int i) { ev[i >> 1] += 1u << (16 * (i & 1)); }
-
-// The per-path dispersion and absorption caches (refraction.rs:127-174): 24 floats per
-// lane in LDS (32 in the kernels built for biaxial stones, whose trichroic tensor adds the
-// beta-axis absorption), channel-major so that a wave's accesses hit distinct banks.  They
-// are filled lazily and read a few times per bounce; keeping them out of VGPRs leaves
-// registers to the Stokes state.
-constexpr int kBlock = 256;
-#define LC_NO(k) lcache[(k) * kBlock + threadIdx.x]
-#define LC_AO(k) lcache[(8 + (k)) * kBlock + threadIdx.x]
-#define LC_AE(k) lcache[(16 + (k)) * kBlock + threadIdx.x]
-#define LC_AB(k) lcache[(24 + (k)) * kBlock + threadIdx.x]
-template <int kAniso>
-struct CacheFloats {
-    static constexpr int value = (kAniso == 2 ? 32 : 24) * kBlock;
-};
-
-// ---------------------------------------------------------------------------------
-// Per-path state.
-// ---------------------------------------------------------------------------------
-
-struct PathState {
-    V3 ray_o, ray_d, cur_k, prev_pn;
-    Stokes st[8];
-    float pdf[8];
-    float hero_off;
-    uint32_t seed, bounce, last_facet, cert;
-    uint32_t term_bounce, term_reason, fh, fate_h;
-    bool inside, is_extra, have_prev, have_n, have_abs, escaped;
-};
-
-// pixel_rotations + sample_draws (sampling.rs:232-271) with the sample-only radical
-// inverses (bases 3 and 5) taken from a host-built table, then Camera::generate_ray
-// (camera.rs:96-114) and the initial state of trace_spectral_ray_inner
-// (transport.rs:651-722).
-__device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_t sample_num,
-                                          float2 ri35, uint32_t cert, const DevScene &sc) {
-    const uint32_t x = pixel % sc.width, y = pixel / sc.width;
-    float rot_x = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_X_ROTATION_STREAM));
-    float rot_y = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_Y_ROTATION_STREAM));
-    float rot_h = low_discrepancy_base2(hash_u32(pixel ^ HERO_WAVELENGTH_ROTATION_STREAM));
-    ps.seed = hash_u32((pixel * 0x9e3779b9u) ^ (sample_num * 0x85ebca6bu));
-    float jx = cranley_patterson(low_discrepancy_base2(sample_num), rot_x) - 0.5f;
-    float jy = cranley_patterson(ri35.x, rot_y) - 0.5f;
-    float hero = cranley_patterson(ri35.y, rot_h);
-    float w = (float)sc.width, h = (float)sc.height;
-    float aspect = w / h;
-    float u = (((float)x + jx) / w - 0.5f) * 2.0f * aspect * sc.fov_tan;
-    float v = (0.5f - ((float)y + jy) / h) * 2.0f * sc.fov_tan;
-    ps.ray_d = normalize(add(add(v3(sc.cam_f[0], sc.cam_f[1], sc.cam_f[2]),
-                                 mul(v3(sc.cam_r[0], sc.cam_r[1], sc.cam_r[2]), u)),
-                             mul(v3(sc.cam_u[0], sc.cam_u[1], sc.cam_u[2]), v)));
-    ps.ray_o = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
-    ps.cur_k = ps.ray_d;
-    ps.prev_pn = v3(0.0f, 0.0f, 0.0f);
-    ps.hero_off = fmaf(hero, 780.0f - 380.0f, 380.0f) - 380.0f;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        ps.st[k] = Stokes{1.0f, 0.0f, 0.0f, 0.0f};
-        ps.pdf[k] = 1.0f;
-    }
-    ps.bounce = 0u;
-    ps.last_facet = 0u;
-    ps.cert = cert;
-    ps.term_bounce = sc.max_bounces;
-    ps.term_reason = 3u;
-    ps.fh = 2166136261u;
-    ps.fate_h = 2166136261u;
-    ps.inside = ps.is_extra = ps.have_prev = ps.have_n = ps.have_abs = ps.escaped = false;
-}
-
-
-// Miss certificate of a camera ray: planes i (entering) and j (exiting) whose exact t
-// values, formed with the scan's own operations, satisfy t_i > t_j.  The scan's entering
-// extreme is the maximum over entering planes, so at least t_i, and its exiting extreme at
-// most t_j: the scan reports a miss, which these two planes prove without the others.
-// cert packs i | j << 16 (from the pixel's central ray, see cert_kernel); ~0 = none.
-__device__ __forceinline__ bool camera_miss_certified(const float4 *__restrict__ planes,
-                                                      const float *__restrict__ cam_side, uint32_t cert, V3 rd) {
-    if (cert == 0xFFFFFFFFu) return false;
-    const uint32_t i = cert & 0xFFFFu, j = cert >> 16;
-    const float4 pi = planes[i], pj = planes[j];
-    const float di = (pi.x * rd.x + pi.y * rd.y) + pi.z * rd.z;
-    const float dj = (pj.x * rd.x + pj.y * rd.y) + pj.z * rd.z;
-    if (!(fabsf(di) > 1e-7f && di < 0.0f && fabsf(dj) > 1e-7f && dj > 0.0f)) return false;
-    const float ti = (-cam_side[i]) / di;
-    const float tj = (-cam_side[j]) / dj;
-    return ti > tj;
-}
-
-// spectral_absorption per eigen-axis (raytracer/absorption.rs:28-30, optics/absorption.rs:68-71)
-// into the lane's LDS caches, summed from -0.0 like Iterator::sum; the kernels built for
-// biaxial stones add the beta band set of a trichroic stone (refraction.rs:151-166; otherwise
-// their tensor is uniaxial, (ao, ao, ae)).  Called once per path (PathState::have_abs) by
-// the scatter step; the interior Beer-Lambert block spells the same loop out.
-template <bool kBiax>
-__device__ __forceinline__ void fill_absorption_cache(float hero_off, const DevScene &sc, float *__restrict__ lcache) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        float so = -0.0f, se = -0.0f;
-        const float lk = comb_lambda(hero_off, k);
-        for (int b = 0; b < sc.n_o_bands; ++b) {
-            float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
-            so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
-        }
-        for (int b = 0; b < sc.n_e_bands; ++b) {
-            float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
-            se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
-        }
-        LC_AO(k) = so;
-        LC_AE(k) = se;
-        if (kBiax) {
-            float sb = so;
-            if (sc.has_beta_bands) {
-                sb = -0.0f;
-                for (int b = 0; b < sc.n_beta_bands; ++b) {
-                    float t = (lk - sc.beta_bands[b][0]) / sc.beta_bands[b][1];
-                    sb = sb + sc.beta_bands[b][2] * expf(-0.5f * t * t);
-                }
-            }
-            LC_AB(k) = sb;
-        }
-    }
-}
-
-// channel_absorption_alphas (raytracer/absorption.rs:174-214): each channel's pleochroic
-// coefficient p alpha_pol + (1 - p) alpha_unpol for wave normal k and plane-of-incidence
-// normal s_axis, the eigenmodes being the hero's biaxial indicatrix's or the uniaxial
-// ordinary / extraordinary pair.  The channel-invariant parts of electric_field_direction
-// and of the quadratic forms (the s/p axes and their projections on the absorption frame
-// (a1, a2, c)) are formed once; per channel the form follows from (Q, U) (qf_channel).
-// Each coefficient is handed to use(c, alpha) as soon as it is formed, so that Beer-Lambert
-// consumes it at once instead of holding all eight (register pressure of the hot kernels).
-template <bool kBiax, class Use>
-__device__ __forceinline__ void channel_alphas(const PathState &ps, const DevScene &sc,
-                                               const float *__restrict__ lcache, V3 k, V3 s_axis, Use &&use) {
-    V3 ea, eb;
-    if (kBiax) {
-        const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
-        bx_eigen_polarizations(bx_of(LC_NO(0), sc.delta_beta_alpha, sc.birefringence_delta), fr, k, &ea, &eb);
-    } else {
-        const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
-        ea = ordinary_eigen_polarization(k, c_axis);
-        eb = extraordinary_eigen_polarization(k, c_axis);
-    }
-    V3 kk = normalize_or_zero(k);
-    V3 s_raw = sub(s_axis, mul(kk, dot(kk, s_axis)));
-    V3 s_hat = len2(s_raw) > 1e-8f ? normalize(s_raw) : arbitrary_perpendicular(kk);
-    V3 p_hat = cross(kk, s_hat);
-    V3 ea_l = v3(dot(afr(sc, 0), ea), dot(afr(sc, 1), ea), dot(afr(sc, 2), ea));
-    V3 eb_l = v3(dot(afr(sc, 0), eb), dot(afr(sc, 1), eb), dot(afr(sc, 2), eb));
-    V3 s_l = v3(dot(afr(sc, 0), s_hat), dot(afr(sc, 1), s_hat), dot(afr(sc, 2), s_hat));
-    V3 p_l = v3(dot(afr(sc, 0), p_hat), dot(afr(sc, 1), p_hat), dot(afr(sc, 2), p_hat));
-    if (kBiax) {
-        // The trichroic tensor diag(ao, ab, ae) (birefringence.rs:238-286): the same sums as
-        // below with the a1 and a2 axes weighted apart.
-        const QfInv3 qi = qf_invariants3(s_l, p_l);
-        const float u_x = (ea_l.x * ea_l.x + eb_l.x * eb_l.x) * 0.5f;
-        const float u_y = (ea_l.y * ea_l.y + eb_l.y * eb_l.y) * 0.5f;
-        const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const float ao = LC_AO(c), ab = LC_AB(c), ae = LC_AE(c);
-            const float qf_e = qf_channel3(qi, ps.st[c].q, ps.st[c].u, ao, ab, ae);
-            const float unpol = fmaf(ao, u_x, fmaf(ab, u_y, ae * u_z));
-            const float p = clampf(degree_of_polarization(ps.st[c]), 0.0f, 1.0f);
-            use(c, fmaf(p, qf_e - unpol, unpol));
-        }
-    } else {
-        const QfInv qi = qf_invariants(s_l, p_l);
-        // The unpolarized term midpoint(ea.A.ea, eb.A.eb) with its channel-invariant parts
-        // gathered: ao (|ea_xy|^2 + |eb_xy|^2) / 2 + ae (ea_z^2 + eb_z^2) / 2.
-        const float u_xy = ((ea_l.x * ea_l.x + ea_l.y * ea_l.y) + (eb_l.x * eb_l.x + eb_l.y * eb_l.y)) * 0.5f;
-        const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const float ao = LC_AO(c), ae = LC_AE(c);
-            const float qf_e = qf_channel(qi, ps.st[c].q, ps.st[c].u, ao, ae);
-            const float unpol = fmaf(ao, u_xy, ae * u_z);
-            const float p = clampf(degree_of_polarization(ps.st[c]), 0.0f, 1.0f);
-            use(c, fmaf(p, qf_e - unpol, unpol));
-        }
-    }
-}
-
-// One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
-// has ended (escape, Russian roulette, or the bounce cap).  kAniso: 0 isotropic, 1
-// uniaxial, 2 biaxial (an anisotropic stone with a biaxial indicatrix).
-template <int kAniso, bool kAbsorb, bool kFrost>
-__device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, const float4 *__restrict__ planes,
-                                            const float *__restrict__ cam_side, float *__restrict__ lcache,
-                                            uint32_t *ev) {
-    constexpr bool kBiax = kAniso == 2;
-    const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
-    const float dn = sc.birefringence_delta;
-    const uint32_t bounce = ps.bounce;
-    ev_inc(ev, GR_EV_SEGMENT);
-
-    // ---- convex facet-plane clip (simd/slab.rs:94-152, intersect.rs:93-132) ----
-    float t_near = -1e30f, t_far = 1e30f;
-    int near_idx = -1, far_idx = -1;
-    bool outside = false;
-    const V3 ro = ps.ray_o, rd = ps.ray_d;
-    if (bounce == 0u) {
-        // Camera rays share their origin: the origin's signed distances are staged per
-        // workgroup (same operations, same values).  Most background rays are proved to
-        // miss by their pixel's certificate.
-        if (camera_miss_certified(planes, cam_side, ps.cert, rd))
-            outside = true;
-        else
-            clip_scan<true>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
-    } else if (!ps.inside && exterior_miss(planes[ps.last_facet], ro, rd)) {
-        // The ray left the stone's surface outwards (a refraction out, or a reflection off
-        // the outside): it starts outside the facet it just met and moves away from it, so
-        // that facet alone makes the scan's far extreme negative and the scan a miss.
-        outside = true;
-    } else {
-        clip_scan<false>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
-    }
-    bool hit = false;
-    float hit_t = 0.0f;
-    int facet = 0;
-    if (!outside && !(t_near > t_far)) {
-        if (t_near > 1e-4f) {
-            hit = true;
-            hit_t = t_near;
-            facet = near_idx;
-        } else if (t_far > 1e-4f) {
-            hit = true;
-            hit_t = t_far;
-            facet = far_idx;
-        }
-    }
-    if (!hit) {
-        // Escape: the environment is sampled by finish_path (the path's only radiance
-        // contribution; transport.rs:307-318).
-        ev_inc(ev, GR_EV_MISS);
-        ps.escaped = true;
-        fnv(ps.fh, 0xFFFFu);
-        ps.term_bounce = bounce;
-        ps.term_reason = 0u;
-        return true;
-    }
-    ev_inc(ev, GR_EV_HIT);
-    if (!ps.have_n) {
-        // DispersionModel::evaluate (optics/dispersion.rs:12-37) for the comb.
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            float l_um = comb_lambda(ps.hero_off, k) * 1e-3f;
-            float l2 = l_um * l_um;
-            float n;
-            if (sc.disp_kind == 0) {
-                float n2 = 1.0f + (sc.disp[0] * l2) / (l2 - sc.disp[1]);
-                n = sqrtf(fmaxf(n2, 1.0f));
-            } else if (sc.disp_kind == 1) {
-                float n2 = 1.0f;
-                n2 += (sc.disp[0] * l2) / (l2 - sc.disp[3]);
-                n2 += (sc.disp[1] * l2) / (l2 - sc.disp[4]);
-                n2 += (sc.disp[2] * l2) / (l2 - sc.disp[5]);
-                n = sqrtf(fmaxf(n2, 1.0f));
-            } else {
-                float l4 = l2 * l2;
-                n = sc.disp[0] + (sc.disp[1] / l2) + (sc.disp[2] / l4);
-            }
-            LC_NO(k) = n;
-        }
-        ps.have_n = true;
-    }
-    // ---- volume scattering (try_scatter_step + maybe_scatter_or_extinguish,
-    //      scattering.rs:132-336; call site transport.rs:805-848) ----
-    // Inside a scattering stone, before the facet hit is processed: an exponential free path
-    // on the hero's extinction alpha_hero + sigma_s, the alphas taken in the previous plane
-    // of incidence (none after a scatter).  A scatter before the facet depolarizes and
-    // reweights every channel, turns the ray by the Henyey-Greenstein phase function, rolls
-    // the roulette past bounce 4 and ends the iteration; otherwise every channel takes its
-    // extinction over the segment relative to the hero's, in place of Beer-Lambert.  Traced
-    // by the kernels built for the rare features (kFrost).
-    const bool scattering = kFrost && !(sc.sigma_s <= 0.0f);
-    if (scattering && ps.inside) {
-        ev_inc(ev, GR_EV_SCATTER);
-        const float sigma_s = sc.sigma_s;
-        float alphas[8];
-        if (kAbsorb) {
-            if (!ps.have_abs) {
-                fill_absorption_cache<kBiax>(ps.hero_off, sc, lcache);
-                ps.have_abs = true;
-            }
-            channel_alphas<kBiax>(ps, sc, lcache, ps.cur_k, ps.have_prev ? ps.prev_pn : v3(0.0f, 0.0f, 0.0f),
-                                  [&](int c, float a) { alphas[c] = a; });
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) alphas[k] = 0.0f;  // no band: every coefficient is zero
-        }
-        const float sigma_t_hero = alphas[0] + sigma_s;
-        const float one_minus_u = fmaxf(1.0f - stream_draw(ps.seed, bounce, DISTANCE_SAMPLE_STREAM), 1e-7f);
-        const float t_free = -(logf(one_minus_u)) / sigma_t_hero;
-        const float hit_t_scaled = hit_t * sc.absorption_path_scale;
-        if (t_free < hit_t_scaled) {
-            const float pdf_hero = sigma_t_hero * one_minus_u;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float sigma_t_k = alphas[k] + sigma_s;
-                const float tr_k = exp_lobe(-(alphas[k] + sigma_s) * t_free);
-                const float weight = tr_k * sigma_s / pdf_hero;
-                ps.st[k] = Stokes{intensity(ps.st[k]) * weight, 0.0f, 0.0f, 0.0f};
-                ps.pdf[k] *= sigma_t_k * tr_k;
-            }
-            const float u1 = stream_draw(ps.seed, bounce, PHASE_DIR_U_STREAM);
-            const float u2 = stream_draw(ps.seed, bounce, PHASE_DIR_V_STREAM);
-            const V3 new_dir = sample_henyey_greenstein_direction(u1, u2, sc.scattering_g, rd);
-            ps.ray_o = add(ro, mul(rd, t_free / sc.absorption_path_scale));
-            ps.ray_d = new_dir;
-            ps.cur_k = new_dir;
-            ps.have_prev = false;
-            fnv(ps.fh, 0xFFFEu);
-            if (bounce > 4) {
-                // apply_russian_roulette (transport.rs:204-220); an ended path is
-                // PathTermination::ScatterAbsorbed.
-                ev_inc(ev, GR_EV_RR);
-                float max_i = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) max_i = fmaxf(max_i, intensity(ps.st[k]));
-                const float q = clampf(max_i, 0.05f, 1.0f);
-                if (stream_draw(ps.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) {
-                    ps.term_bounce = bounce;
-                    ps.term_reason = 1u;
-                    return true;
-                }
-                const float inv_q = 1.0f / q;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) ps.st[k] = scale(ps.st[k], inv_q);
-            }
-            ps.bounce = bounce + 1u;
-            return ps.bounce >= sc.max_bounces;
-        }
-        float survive[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) survive[k] = exp_lobe(-(alphas[k] + sigma_s) * hit_t_scaled);
-        const float survive_hero = fmaxf(survive[0], 1e-30f);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            ps.st[k] = scale(ps.st[k], survive[k] / survive_hero);
-            ps.pdf[k] *= survive[k];
-        }
-    }
-    // An unbounded side (no plane on it) reports facet 0 with a zero normal, as the
-    // reference's HitRecord does for a negative index (intersect.rs:105-126).
-    V3 normal = v3(0.0f, 0.0f, 0.0f);
-    if (facet >= 0) {
-        float4 hp = planes[facet];
-        normal = v3(hp.x, hp.y, hp.z);
-    } else {
-        facet = 0;
-    }
-    V3 hit_point = add(ro, mul(rd, hit_t));
-    // Facet edge rounding (transport.rs:850-857); off (the flat normal) for radius <= 0.
-    if (sc.edge_rounding_radius > 0.0f)
-        normal = shading_normal_near_edge(planes, sc.n_planes, hit_point, facet, normal, sc.edge_rounding_radius);
-    V3 cur_k = ps.cur_k;
-    const bool inside = ps.inside;
-
-    // ---- Stokes frame rotation into the new plane of incidence
-    //      (raytracer/absorption.rs:44-89) ----
-    V3 cpn = normalize_or_zero(cross(cur_k, normal));
-    if (ps.have_prev && len2(cpn) > 1e-6f && len2(ps.prev_pn) > 1e-6f) {
-        ev_inc(ev, GR_EV_ROTATE);
-        float c2, s2;
-        frame_rotation_cs(ps.prev_pn, cpn, cur_k, &c2, &s2);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) ps.st[k] = apply_rotation(ps.st[k], c2, s2);
-    }
-    ps.prev_pn = cpn;
-    ps.have_prev = true;
-
-    // ---- interior segment: pleochroic Beer-Lambert (raytracer/absorption.rs:115-216,
-    //      birefringence.rs:252-286, polarization.rs:172-192); a scattering stone's
-    //      extinction was taken by the scatter step above (transport.rs:586-606) ----
-    if (inside) {
-        normal = neg(normal);
-        if (!scattering) {
-            ev_inc(ev, GR_EV_ABSORB);
-            if (kAbsorb) {
-                // The cache fill is written out here rather than calling
-                // fill_absorption_cache: with the call, the compiler's allocation of this
-                // (hot) block spilled 178 VGPRs instead of 30 (same arithmetic).
-                if (!ps.have_abs) {
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        float so = -0.0f, se = -0.0f;
-                        const float lk = comb_lambda(ps.hero_off, k);
-                        for (int b = 0; b < sc.n_o_bands; ++b) {
-                            float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
-                            so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
-                        }
-                        for (int b = 0; b < sc.n_e_bands; ++b) {
-                            float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
-                            se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
-                        }
-                        LC_AO(k) = so;
-                        LC_AE(k) = se;
-                        if (kBiax) {
-                            float sb = so;
-                            if (sc.has_beta_bands) {
-                                sb = -0.0f;
-                                for (int b = 0; b < sc.n_beta_bands; ++b) {
-                                    float t = (lk - sc.beta_bands[b][0]) / sc.beta_bands[b][1];
-                                    sb = sb + sc.beta_bands[b][2] * expf(-0.5f * t * t);
-                                }
-                            }
-                            LC_AB(k) = sb;
-                        }
-                    }
-                    ps.have_abs = true;
-                }
-                const float spl = hit_t * sc.absorption_path_scale;
-                channel_alphas<kBiax>(ps, sc, lcache, cur_k, cpn,
-                                      [&](int c, float a) { ps.st[c] = scale(ps.st[c], exp_lobe(-a * spl)); });
-            }
-        }
-    }
-
-    // ---- per-bounce refraction geometry (refraction.rs:235-547) ----
-    float cos_i = clampf(dot(neg(cur_k), normal), 0.0f, 1.0f);
-    float sin_i = sqrtf(fmaxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
-    float one_m_c2 = fmaf(cos_i, -cos_i, 1.0f);
-    float n_med[8];
-    float n_biax_a[8];  // biaxial entry: each channel's mode-A (fast) index
-    if (kBiax) {
-        // hero_biaxial_wave_dirs + per_channel_biaxial_indices (refraction.rs:345-400): one
-        // wave normal per mode from the hero's indicatrix -- at an entry the two-step
-        // fixed point of resolve_entry_mode seeded with n_beta, inside the stone the
-        // tracked wave normal -- and each channel's own mode indices there.  Inside, the
-        // medium is mode A (fast) unless the path took mode B; at an entry, mode B.
-        const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
-        const float dba = sc.delta_beta_alpha;
-        if (inside) {
-            const bool use_a = !ps.is_extra;
-            const V3 l = bx_local(fr, normalize_or_zero(cur_k));
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                float ns, nf;
-                bx_wave_indices(bx_of(LC_NO(k), dba, dn), l, &ns, &nf);
-                n_med[k] = use_a ? nf : ns;
-                n_biax_a[k] = nf;
-            }
-        } else {
-            const Bx hero = bx_of(LC_NO(0), dba, dn);
-            const V3 dir_a = bx_entry_wave_dir(hero, fr, cur_k, normal, cos_i, one_m_c2, LC_NO(0), false);
-            const V3 dir_b = bx_entry_wave_dir(hero, fr, cur_k, normal, cos_i, one_m_c2, LC_NO(0), true);
-            const V3 la = bx_local(fr, normalize_or_zero(dir_a));
-            const V3 lb = bx_local(fr, normalize_or_zero(dir_b));
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const Bx bk = bx_of(LC_NO(k), dba, dn);
-                float ns, nf;
-                bx_wave_indices(bk, la, &ns, &nf);
-                n_biax_a[k] = nf;
-                bx_wave_indices(bk, lb, &ns, &nf);
-                n_med[k] = ns;
-            }
-        }
-    } else if (kAniso) {
-        const bool use_o = inside && !ps.is_extra;
-        if (use_o) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) n_med[k] = LC_NO(k);
-        } else {
-            // theta_c, the angle between the wave normal and the optic axis, enters only
-            // through its sine and cosine.  The reference forms theta_c = acos(|cos|) and
-            // takes sin/cos of it (refraction.rs:248-269, birefringence.rs:27-40); here they
-            // come from the cosine directly, sin = sqrt(1 - cos^2), shared by the 8 channels.
-            float ct = 1.0f, st = 0.0f;  // theta_c = 0
-            if (!inside) {
-                // theta_c_for_bounce's two-step fixed point.
-                const float n_seed = LC_NO(0);
-                const float n_e_seed = n_seed + dn;
-                float n_guess = n_seed;
-                for (int it = 0; it < 2; ++it) {
-                    float eta_g = 1.0f / n_guess;
-                    float sin2_g = eta_g * eta_g * one_m_c2;
-                    if (sin2_g > 1.0f) break;
-                    float cos_t_g = sqrtf(fmaxf(1.0f - sin2_g, 0.0f));
-                    V3 wave = normalize(add(mul(cur_k, eta_g), mul(normal, fmaf(eta_g, cos_i, -cos_t_g))));
-                    ct = fabsf(clampf(dot(wave, c_axis), -1.0f, 1.0f));
-                    st = sqrtf(fmaxf(fmaf(ct, -ct, 1.0f), 0.0f));
-                    n_guess = effective_extraordinary_index_sc(n_seed, n_e_seed, st, ct);
-                }
-            } else {
-                ct = fabsf(clampf(dot(cur_k, c_axis), -1.0f, 1.0f));
-                st = sqrtf(fmaxf(fmaf(ct, -ct, 1.0f), 0.0f));
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                n_med[k] = effective_extraordinary_index_sc(LC_NO(k), LC_NO(k) + dn, st, ct);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) n_med[k] = LC_NO(k);
-    }
-    float s2t_ch[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        float n1k = inside ? n_med[k] : 1.0f;
-        float n2k = inside ? 1.0f : n_med[k];
-        float etak = n1k / n2k;
-        s2t_ch[k] = etak * etak * one_m_c2;
-    }
-    const float n1 = inside ? n_med[0] : 1.0f, n2 = inside ? 1.0f : n_med[0];
-    const float sin2_t = s2t_ch[0];
-    const float n_o_hero = LC_NO(0), n_e_hero = LC_NO(0) + dn;
-
-    // ---- TIR / partial-reflect / refract dispatch (transport.rs:128-193) ----
-    V3 new_k, new_s;
-    bool new_inside = inside, has_update = false, update = false;
-    uint32_t event;
-    // Hero event: TIR when sin2_t > 1, else a Fresnel-weighted reflect/refract draw
-    // (transport.rs:158-176, refraction.rs:1036-1086).
-    const bool tir = sin2_t > 1.0f;
-    float r_unpol = 1.0f;
-    bool reflect = tir;
-    if (!tir) {
-        float cos_t = sqrtf(1.0f - sin2_t);
-        float r_s, r_p;
-        fresnel_rs_rp(n1, n2, cos_i, cos_t, &r_s, &r_p);
-        r_unpol = clampf(0.5f * fmaf(r_p, r_p, r_s * r_s), 1e-4f, 1.0f - 1e-4f);
-        reflect = stream_draw(ps.seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol;
-    }
-    if (kFrost && sc.n_frosted != 0u && sc.finishes[facet] == GR_FINISH_FROSTED) {
-        // apply_frosted_bounce (scattering.rs:522-590): the hero's TIR or Fresnel draw
-        // (above) picks a diffuse reflection or transmission, cosine-distributed about the
-        // facet normal; every channel is depolarized, its intensity divided by the hero's
-        // branch probability, and k' = S'.
-        const float u1 = stream_draw(ps.seed, bounce, FROSTED_DIR_U_STREAM);
-        const float u2 = stream_draw(ps.seed, bounce, FROSTED_DIR_V_STREAM);
-        if (reflect) {
-            if (tir) {
-                ev_inc(ev, GR_EV_TIR);
-                event = 4u;
-            } else {
-                ev_inc(ev, GR_EV_REFLECT);
-                event = 5u;
-            }
-            new_k = cosine_weighted_hemisphere(u1, u2, normal);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float i = intensity(ps.st[k]);
-                ps.st[k] = Stokes{tir ? i : i / r_unpol, 0.0f, 0.0f, 0.0f};
-                if (!tir) ps.pdf[k] *= r_unpol;
-            }
-        } else {
-            ev_inc(ev, GR_EV_REFRACT);
-            event = 6u;
-            new_k = cosine_weighted_hemisphere(u1, u2, neg(normal));
-            const bool entering = kAniso && !inside;
-            bool use_extra = ps.is_extra;
-            if (entering) {
-                ev_inc(ev, GR_EV_ENTRY_ANISO);
-                use_extra = stream_draw(ps.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
-            }
-            const float t_unpol = 1.0f - r_unpol;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                ps.st[k] = Stokes{intensity(ps.st[k]) / t_unpol, 0.0f, 0.0f, 0.0f};
-                ps.pdf[k] *= t_unpol;
-            }
-            new_inside = !inside;
-            has_update = entering;
-            update = use_extra;
-        }
-        new_s = new_k;
-    } else if (reflect) {
-        // apply_tir_bounce (refraction.rs:599-637) and apply_partial_reflect_bounce
-        // (:650-690) share their per-channel work: retardation for channels past their
-        // own critical angle, Fresnel reflection (and its pdf factor) otherwise.  The
-        // partial reflection then divides by the hero's r_unpol; TIR scales by 1.0, which
-        // is exact.
-        if (tir) {
-            ev_inc(ev, GR_EV_TIR);
-            event = 1u;
-        } else {
-            ev_inc(ev, GR_EV_REFLECT);
-            event = 2u;
-        }
-        const float inv_r = tir ? 1.0f : 1.0f / r_unpol;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const float n1k = inside ? n_med[k] : 1.0f, n2k = inside ? 1.0f : n_med[k];
-            if (s2t_ch[k] > 1.0f) {
-                float sd, cd;
-                tir_retardation_cs(n1k, cos_i, sin_i, &cd, &sd);
-                ps.st[k] = apply_retardation(ps.st[k], cd, sd);
-            } else {
-                float cos_t_k = sqrtf(fmaxf(1.0f - s2t_ch[k], 0.0f));
-                float rs, rp;
-                fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
-                ps.pdf[k] *= clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
-                ps.st[k] = apply_fresnel_reflection(ps.st[k], rs, rp);
-            }
-            ps.st[k] = scale(ps.st[k], inv_r);
-        }
-        new_k = sub(cur_k, mul(normal, 2.0f * dot(cur_k, normal)));
-        new_s = new_k;
-        if (kBiax && inside) {
-            // Both biaxial modes walk off (refraction.rs:574-576).
-            ev_inc(ev, GR_EV_WALKOFF);
-            const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
-            new_s = bx_mode_poynting_dir(bx_of(LC_NO(0), sc.delta_beta_alpha, dn), fr, new_k, ps.is_extra);
-        } else if (kAniso && inside && ps.is_extra) {
-            ev_inc(ev, GR_EV_WALKOFF);
-            new_s = extraordinary_poynting_dir(new_k, c_axis, n_o_hero, n_e_hero);
-        }
-    } else {
-        // apply_refract_bounce + apply_refract_channel (refraction.rs:748-1011).
-        ev_inc(ev, GR_EV_REFRACT);
-        event = 3u;
-        const float DIRECTION_MATCH_COS_TOL = 1.0f - 1e-6f;
-        bool entering = kAniso && !inside;
-        bool use_extra = ps.is_extra;
-        if (entering) {
-            ev_inc(ev, GR_EV_ENTRY_ANISO);
-            use_extra = stream_draw(ps.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
-        }
-        // At a biaxial entry the hero refracts with its own mode index (mode B: n_med, mode
-        // A: n_biax_a) and both modes walk off (refraction.rs:803-818).
-        const BxFrame fr{afr(sc, 0), afr(sc, 1), afr(sc, 2)};
-        const float dba = sc.delta_beta_alpha;
-        float n2_dir;
-        if (kBiax && entering)
-            n2_dir = use_extra ? n_med[0] : n_biax_a[0];
-        else
-            n2_dir = (entering && !use_extra) ? n_o_hero : n2;
-        float eta_dir = n1 / n2_dir;
-        float sin2_dir = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
-        float cos_t_dir = sqrtf(fmaxf(1.0f - sin2_dir, 0.0f));
-        V3 refr = normalize(add(mul(cur_k, eta_dir), mul(normal, fmaf(eta_dir, cos_i, -cos_t_dir))));
-        V3 final_dir = refr;
-        if (kBiax && entering) {
-            ev_inc(ev, GR_EV_WALKOFF);
-            final_dir = bx_mode_poynting_dir(bx_of(LC_NO(0), dba, dn), fr, refr, use_extra);
-        } else if (entering && use_extra) {
-            ev_inc(ev, GR_EV_WALKOFF);
-            final_dir = extraordinary_poynting_dir(refr, c_axis, n_o_hero, n_e_hero);
-        }
-        float inv_t = 1.0f / (1.0f - r_unpol);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            float n1k = inside ? n_med[k] : 1.0f;
-            float n2k = (entering && !use_extra) ? (kBiax ? n_biax_a[k] : LC_NO(k)) : (inside ? 1.0f : n_med[k]);
-            float q = n1k / n2k;
-            float s2k = q * q * one_m_c2;
-            bool alive = !(s2k > 1.0f);
-            float cos_t_k = sqrtf(fmaxf(1.0f - s2k, 0.0f));
-            if (alive) {
-                V3 dk = normalize(add(mul(cur_k, q), mul(normal, fmaf(q, cos_i, -cos_t_k))));
-                if (kBiax && entering)
-                    dk = bx_mode_poynting_dir(bx_of(LC_NO(k), dba, dn), fr, dk, use_extra);
-                else if (entering && use_extra)
-                    dk = extraordinary_poynting_dir(dk, c_axis, LC_NO(k), LC_NO(k) + dn);
-                alive = dot(dk, final_dir) >= DIRECTION_MATCH_COS_TOL;
-            }
-            if (alive) {
-                float num = 2.0f * n1k * cos_i;
-                float t_s = num / fmaf(n2k, cos_t_k, n1k * cos_i);
-                float t_p = num / fmaf(n1k, cos_t_k, n2k * cos_i);
-                ps.st[k] = scale(apply_fresnel_transmission(ps.st[k], n1k, n2k, cos_i, cos_t_k, t_s, t_p), inv_t);
-                float rs, rp;
-                fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
-                ps.pdf[k] *= 1.0f - clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
-            } else {
-                ps.st[k] = scale(ps.st[k], 0.0f);
-                ps.pdf[k] = 0.0f;
-            }
-        }
-        new_k = refr;
-        new_s = final_dir;
-        new_inside = !inside;
-        has_update = entering;
-        update = use_extra;
-    }
-    bool internal_reflection = inside && !has_update && new_inside == inside;
-    ps.ray_d = new_s;
-    ps.ray_o = add(hit_point, mul(new_s, 1e-4f));
-    ps.cur_k = new_k;
-    ps.inside = new_inside;
-    if (has_update) ps.is_extra = update;
-    if (kAniso && internal_reflection)
-        ps.is_extra = stream_draw(ps.seed, bounce, MODE_COUPLING_STREAM) < 0.5f;
-    ps.last_facet = (uint32_t)facet;
-    fnv(ps.fh, (uint32_t)facet);
-    fnv(ps.fh, event | (ps.is_extra ? 16u : 0u));
-    {
-        // Per-channel fates: past the channel's critical angle / pdf zero.
-        uint32_t fate = 0u;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            fate |= (s2t_ch[k] > 1.0f ? 1u : 0u) << k;
-            fate |= (ps.pdf[k] == 0.0f ? 1u : 0u) << (8 + k);
-        }
-        fnv(ps.fate_h, fate);
-    }
-
-    // ---- Russian roulette (transport.rs:204-220) ----
-    if (bounce > 4) {
-        ev_inc(ev, GR_EV_RR);
-        float max_i = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) max_i = fmaxf(max_i, intensity(ps.st[k]));
-        float q = clampf(max_i, 0.05f, 1.0f);
-        if (stream_draw(ps.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) {
-            ps.term_bounce = bounce;
-            ps.term_reason = 2u;
-            return true;
-        }
-        float inv_q = 1.0f / q;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) ps.st[k] = scale(ps.st[k], inv_q);
-    }
-    ps.bounce = bounce + 1u;
-    return ps.bounce >= sc.max_bounces;
-}
-
-// Environment radiance of the escape direction, spectral MIS, CIE XYZ and white balance
-// (transport.rs:67-90, color.rs:121-164, 215-251).  A path that never escaped carries
-// zero radiance: the reference's integration and white balance then produce exactly +0
-// in every component (every product is +-0 and every sum has a +0 operand), so that work
-// is skipped by the callers.
-template <int kEnv>
-__device__ __forceinline__ V3 finish_escaped(V3 dir, const float *inten, const float *pdf,
-                                             float hero_off, const DevScene &sc) {
-    float rad[8];
-    if (kEnv == 0) {
-        // sample_studio_environment_with_rig (environment.rs:280-321): the direction-only
-        // terms are formed once, the fma chain runs per channel.
-        V3 d = normalize(dir);
-        float bg = fmaxf(fmaf(0.012f, fmaf(d.y, 0.5f, 0.5f), 0.015f), 0.005f) * sc.exposure;
-        float terms[18];
-        bool act[18];
-        float kd = fmaxf(dot(d, v3(sc.key[0], sc.key[1], sc.key[2])), 0.0f);
-        act[0] = kd > 0.0f;
-        terms[0] = powi_cr<28>(kd) * 12.0f * sc.spot_mult * sc.exposure;
-        float fd = fmaxf(dot(d, v3(sc.fill[0], sc.fill[1], sc.fill[2])), 0.0f);
-        act[1] = fd > 0.0f;
-        terms[1] = powi_cr<18>(fd) * 4.5f * sc.exposure;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            float rd = fmaxf(dot(d, v3(sc.ring[r][0], sc.ring[r][1], sc.ring[r][2])), 0.0f);
-            act[2 + r] = rd > 0.96f;
-            // Used only when rd > 0.96, where div_by_const is the correctly rounded quotient.
-            float spark = div_by_const(rd - 0.96f, 0.04f, 1.0f / 0.04f);
-            terms[2 + r] = powi_cr<6>(spark) * 22.0f * sc.spot_mult * sc.exposure;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            float spec = blackbody(comb_lambda(hero_off, k), sc);
-            float r = bg * spec;
-#pragma unroll
-            for (int j = 0; j < 18; ++j)
-                if (act[j]) r = fmaf(terms[j], spec, r);
-            rad[k] = fmaf(inten[k], r, 0.0f);
-        }
-    } else {
-        V3 rgb = map_rgb(dir, sc);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) rad[k] = fmaf(inten[k], rgb_to_spectral(rgb, comb_lambda(hero_off, k)), 0.0f);
-    }
-    float sum_pdf = -0.0f;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) sum_pdf = sum_pdf + pdf[k];
-    float mis = sum_pdf <= 1e-12f ? 1.0f : (8.0f * pdf[0]) / sum_pdf;
-    const float norm_factor = (400.0f / 8.0f) / 106.856f;
-    V3 xyz = v3(0.0f, 0.0f, 0.0f);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        V3 c = cmf_x8_channel(comb_lambda(hero_off, k));
-        float w = rad[k] * mis * norm_factor;
-        xyz = add(xyz, mul(c, w));
-    }
-    V3 lms = v3(fmaf(0.8951f, xyz.x, fmaf(0.2664f, xyz.y, -0.1614f * xyz.z)),
-                fmaf(-0.7502f, xyz.x, fmaf(1.7135f, xyz.y, 0.0367f * xyz.z)),
-                fmaf(0.0389f, xyz.x, fmaf(-0.0685f, xyz.y, 1.0296f * xyz.z)));
-    lms = v3(lms.x * sc.wb[0], lms.y * sc.wb[1], lms.z * sc.wb[2]);
-    return v3(fmaf(0.986993f, lms.x, fmaf(-0.147054f, lms.y, 0.159963f * lms.z)),
-              fmaf(0.432305f, lms.x, fmaf(0.518360f, lms.y, 0.049291f * lms.z)),
-              fmaf(-0.008529f, lms.x, fmaf(0.040043f, lms.y, 0.968487f * lms.z)));
-}
-
-__device__ __forceinline__ uint32_t path_segments(const PathState &ps, const DevScene &sc) {
-    return ps.term_reason == 3u ? sc.max_bounces : ps.term_bounce + 1u;
-}
-
-// Continuation records (kContFields, transport.h): the complete state of a path still
-// running at the split bounce, field-major (field f of record r at cont[f * cap + r]).
-// Fields: 0 tuple index, 1-12 ray_o, ray_d, cur_k, prev_pn, 13-44 Stokes, 45-52 pdfs,
-// 53-76 the LDS caches (n_o, ao, ae), 77 hero offset, 78-84 seed, bounce, last facet,
-// termination bounce and reason and the two path hashes, 85 the mode flags, and in the
-// kernels built for biaxial stones (kBx) 86-93 the beta-axis absorption cache.
-
-template <bool kBx>
-__device__ __forceinline__ void save_path(const PathState &ps, uint32_t t, const float *__restrict__ lcache,
-                                          float *__restrict__ cont, uint32_t cap, uint32_t r) {
-    float *c = cont + r;
-    auto put = [&](int f, float v) { c[(size_t)f * cap] = v; };
-    auto putu = [&](int f, uint32_t v) { c[(size_t)f * cap] = __uint_as_float(v); };
-    putu(0, t);
-    const V3 v[4] = {ps.ray_o, ps.ray_d, ps.cur_k, ps.prev_pn};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        put(1 + 3 * j, v[j].x);
-        put(2 + 3 * j, v[j].y);
-        put(3 + 3 * j, v[j].z);
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        put(13 + 4 * k, ps.st[k].i);
-        put(14 + 4 * k, ps.st[k].q);
-        put(15 + 4 * k, ps.st[k].u);
-        put(16 + 4 * k, ps.st[k].v);
-        put(45 + k, ps.pdf[k]);
-        put(53 + k, LC_NO(k));
-        put(61 + k, LC_AO(k));
-        put(69 + k, LC_AE(k));
-    }
-    put(77, ps.hero_off);
-    putu(78, ps.seed);
-    putu(79, ps.bounce);
-    putu(80, ps.last_facet);
-    putu(81, ps.term_bounce);
-    putu(82, ps.term_reason);
-    putu(83, ps.fh);
-    putu(84, ps.fate_h);
-    putu(85, (ps.inside ? 1u : 0u) | (ps.is_extra ? 2u : 0u) | (ps.have_prev ? 4u : 0u) | (ps.have_n ? 8u : 0u) |
-                 (ps.have_abs ? 16u : 0u) | (ps.escaped ? 32u : 0u));
-    if (kBx) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) put(86 + k, LC_AB(k));
-    }
-}
-
-template <bool kBx>
-__device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__ lcache,
-                                              const float *__restrict__ cont, uint32_t cap, uint32_t r) {
-    const float *c = cont + r;
-    auto get = [&](int f) { return c[(size_t)f * cap]; };
-    auto getu = [&](int f) { return __float_as_uint(c[(size_t)f * cap]); };
-    ps.ray_o = v3(get(1), get(2), get(3));
-    ps.ray_d = v3(get(4), get(5), get(6));
-    ps.cur_k = v3(get(7), get(8), get(9));
-    ps.prev_pn = v3(get(10), get(11), get(12));
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        ps.st[k] = Stokes{get(13 + 4 * k), get(14 + 4 * k), get(15 + 4 * k), get(16 + 4 * k)};
-        ps.pdf[k] = get(45 + k);
-        LC_NO(k) = get(53 + k);
-        LC_AO(k) = get(61 + k);
-        LC_AE(k) = get(69 + k);
-    }
-    ps.hero_off = get(77);
-    ps.seed = getu(78);
-    ps.bounce = getu(79);
-    ps.last_facet = getu(80);
-    ps.cert = 0xFFFFFFFFu;  // used at bounce 0 only
-    ps.term_bounce = getu(81);
-    ps.term_reason = getu(82);
-    ps.fh = getu(83);
-    ps.fate_h = getu(84);
-    const uint32_t fl = getu(85);
-    ps.inside = fl & 1u;
-    ps.is_extra = fl & 2u;
-    ps.have_prev = fl & 4u;
-    ps.have_n = fl & 8u;
-    ps.have_abs = fl & 16u;
-    ps.escaped = fl & 32u;
-    if (kBx) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) LC_AB(k) = get(86 + k);
-    }
-    return getu(0);
-}
-
-// Reserves a continuation record for every calling lane (one atomic per wave) and saves
-// the path into it; false when the records have run out (the path then carries on).
-// Called by all live lanes of a wave together: in the plain and continuation kernels
-// every live lane reaches a given bounce count in the same loop iteration.
-template <bool kBx>
-__device__ __forceinline__ bool park(const PathState &ps, uint32_t t, const float *__restrict__ lcache,
-                                     float *__restrict__ cont, uint32_t cap, uint32_t *__restrict__ count) {
-    const uint64_t live = __ballot(1);
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)live) - 1u;
-    uint32_t base = 0u;
-    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(live));
-    base = __shfl(base, (int)leader, 64);
-    const uint32_t r = base + (uint32_t)__popcll(live & ((1ull << lane) - 1ull));
-    if (r >= cap) return false;
-    save_path<kBx>(ps, t, lcache, cont, cap, r);
-    return true;
-}
-
-// XYZ (and, with kInfo, the per-path record) of a finished path into tuple t's slot.
-template <int kEnv, bool kInfo>
-__device__ __forceinline__ void write_result(const PathState &ps, const DevScene &sc, uint32_t t,
-                                             float *__restrict__ xyz_out, uint32_t *__restrict__ info_out) {
-    V3 xyz = v3(0.0f, 0.0f, 0.0f);
-    if (ps.escaped) {
-        float inten[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) inten[k] = intensity(ps.st[k]);
-        xyz = finish_escaped<kEnv>(ps.ray_d, inten, ps.pdf, ps.hero_off, sc);
-    }
-    xyz_out[3 * (size_t)t + 0] = xyz.x;
-    xyz_out[3 * (size_t)t + 1] = xyz.y;
-    xyz_out[3 * (size_t)t + 2] = xyz.z;
-    if (kInfo) {
-        info_out[4 * (size_t)t + 0] = path_segments(ps, sc);
-        info_out[4 * (size_t)t + 1] = ps.term_reason;
-        info_out[4 * (size_t)t + 2] = ps.fh;
-        info_out[4 * (size_t)t + 3] = ps.fate_h;
-    }
-}
-
-// ---------------------------------------------------------------------------------
-// Kernels.
-// ---------------------------------------------------------------------------------
-
-// Occupancy target of the trace kernels (waves per SIMD); 0 leaves the register budget to
-// the compiler.  A tuning parameter chosen by measurement: on the single-job workload
-// 3 waves/SIMD (168 VGPRs) ran 1.36x faster than the compiler's 2 and 1.03x faster than 4.
-// The kernels built for biaxial stones (kAniso == 2) carry the indicatrix and take their
-// own target, GR_BIAX_WAVES_PER_EU: 3 as well, measured 8 % faster on the biaxial frames
-// than 2 waves/SIMD (256 VGPRs, no spills) despite ~100 spilled VGPRs.
-#ifndef GR_TRACE_WAVES_PER_EU
-#define GR_TRACE_WAVES_PER_EU 3
-#endif
-#ifndef GR_BIAX_WAVES_PER_EU
-#define GR_BIAX_WAVES_PER_EU 3
-#endif
-#if GR_TRACE_WAVES_PER_EU > 0
-#define GR_WPE(kAniso) ((kAniso) == 2 ? GR_BIAX_WAVES_PER_EU : GR_TRACE_WAVES_PER_EU)
-#define GR_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_WPE(kAniso), GR_WPE(kAniso))))
-#else
-#define GR_TRACE_OCCUPANCY
-#endif
-
-// Tuples per wavefront of trace_wave_kernel: 64 lanes x this many rounds.
-#ifndef GR_WAVE_ROUNDS
-#define GR_WAVE_ROUNDS 8
-#endif
-constexpr uint32_t kWaveTuples = 64u * GR_WAVE_ROUNDS;
-
-// Facet planes into LDS, with the camera origin's signed distance to each
-// (p.w + n . origin, the clip's operations), padded to a multiple of four with inert
-// planes (normal 0, d = -1) for the scan's four-plane reads.
-__device__ __forceinline__ void stage_planes(float4 *lds, float *cam_side, const float4 *__restrict__ g,
-                                             const DevScene &sc) {
-    const V3 o = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
-    const uint32_t n4 = (sc.n_planes + 3u) & ~3u;
-    for (uint32_t i = threadIdx.x; i < n4; i += kBlock) {
-        const float4 p = i < sc.n_planes ? g[i] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-        lds[i] = p;
-        cam_side[i] = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
-    }
-}
-
-// Per-class counters: wave-level sum, one LDS atomic per wave, one global atomic per
-// counter per workgroup.
-__device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block_ev,
-                                             unsigned long long *stats) {
-#pragma unroll
-    for (int i = 0; i < GR_EV_COUNT; ++i) {
-        uint32_t v = (ev[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&block_ev[i], v);
-    }
-    __syncthreads();
-    if (threadIdx.x < GR_EV_COUNT && block_ev[threadIdx.x])
-        atomicAdd(&stats[threadIdx.x], (unsigned long long)block_ev[threadIdx.x]);
-}
-
-// The plain path: one lane per (pixel, sample) tuple t of the launch, pixel =
-// pixel_begin + t / spp, sample = sample_offset + t % spp.  Writes the tuple's XYZ (and,
-// with kInfo, segments, termination and the two path hashes).
-//
-// Path-length split: a wave runs until its longest path ends, and stone paths are short
-// with a long tail (on the oval-brilliant workload 69 % end after 3 segments, but most
-// 64-sample waves hold one path that runs much longer).  With split_bounce > 0, a path
-// still running after split_bounce segments is parked as a continuation record (one
-// atomic per wave reserves the records) and cont_kernel continues the parked paths in
-// fully populated waves, parking again at each further multiple of split_bounce for
-// kContLevels levels.  Parking moves state, not arithmetic: every path's result is
-// bit-identical to an unsplit run.  When the records run out, the lane carries on.
-template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
-    const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
-    const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp,
-    uint32_t n_tuples, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
-    unsigned long long *__restrict__ stats, uint32_t split_bounce, float *__restrict__ cont, uint32_t cont_cap,
-    uint32_t *__restrict__ cont_count) {
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[CacheFloats<kAniso>::value];
-    __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes(planes, cam_side, planes_g, sc);
-    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
-    __syncthreads();
-
-    uint32_t ev[kEvWords];
-#pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
-
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t < n_tuples) {
-        const uint32_t local_pixel = t / spp;
-        const uint32_t s = t - local_pixel * spp;
-        PathState ps;
-        init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
-        bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == split_bounce && park<kAniso == 2>(ps, t, lcache, cont, cont_cap, cont_count)) {
-                parked = true;
-                break;
-            }
-        }
-        if (!parked) write_result<kEnv, kInfo>(ps, sc, t, xyz_out, info_out);
-    }
-    flush_events(ev, block_ev, stats);
-}
-
-// Continues the paths the previous launch parked: record r per lane, for r < *cont_count
-// (capped at cont_cap); workgroups past the count return at once.  With split_bounce > 0
-// the paths still running at that bounce are parked again, into next / next_count, for
-// the next level.
-template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
-    const DevScene sc, const float4 *__restrict__ planes_g, const float *__restrict__ cont, uint32_t cont_cap,
-    const uint32_t *__restrict__ cont_count, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
-    unsigned long long *__restrict__ stats, uint32_t split_bounce, float *__restrict__ next,
-    uint32_t *__restrict__ next_count) {
-    const uint32_t count = min(*cont_count, cont_cap);
-    if (blockIdx.x * kBlock >= count) return;
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[CacheFloats<kAniso>::value];
-    __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes(planes, cam_side, planes_g, sc);
-    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
-    __syncthreads();
-
-    uint32_t ev[kEvWords];
-#pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
-
-    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r < count) {
-        PathState ps;
-        const uint32_t t = load_path<kAniso == 2>(ps, lcache, cont, cont_cap, r);
-        bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == split_bounce && park<kAniso == 2>(ps, t, lcache, next, cont_cap, next_count)) {
-                parked = true;
-                break;
-            }
-        }
-        if (!parked) write_result<kEnv, kInfo>(ps, sc, t, xyz_out, info_out);
-    }
-    flush_events(ev, block_ev, stats);
-}
-
-// The batched kernels' view of BatchSlice: the same layout with every pointer typed by the
-// memory it addresses.  The scene block is constant for the launch (address space 4), so its
-// uniform fields are read with scalar loads exactly as the plain kernel reads its by-value
-// scene from the kernel arguments; the buffers are global memory (address space 1), so the
-// kernels issue global rather than flat loads and stores.  Through generic pointers loaded
-// from the slice array the compiler could prove neither and fell back to per-lane flat
-// accesses.
-#define GR_AS_CONST __attribute__((address_space(4)))
-#define GR_AS_GLOBAL __attribute__((address_space(1)))
-struct BatchSliceView {
-    const GR_AS_CONST DevScene *scene;
-    const GR_AS_CONST float4 *planes;
-    const GR_AS_CONST float *ri35;  // float2 pairs
-    const GR_AS_CONST uint32_t *cert;
-    GR_AS_GLOBAL float *xyz;
-    GR_AS_GLOBAL float *film;
-    GR_AS_GLOBAL unsigned long long *stats;
-    uint32_t pixel_begin, n_pixels, sample_offset, spp;
-    uint32_t trace_block_begin;
-    uint32_t reduce_block_begin;
-    uint32_t split_bounce;
-    uint32_t cont_cap;
-    uint32_t cont_block_begin;
-    GR_AS_GLOBAL float *cont[2];
-    GR_AS_GLOBAL uint32_t *cont_count;
-};
-static_assert(sizeof(BatchSliceView) == sizeof(BatchSlice), "BatchSliceView mirrors BatchSlice");
-static_assert(offsetof(BatchSliceView, xyz) == offsetof(BatchSlice, xyz), "BatchSliceView mirrors BatchSlice");
-static_assert(offsetof(BatchSliceView, pixel_begin) == offsetof(BatchSlice, pixel_begin),
-              "BatchSliceView mirrors BatchSlice");
-static_assert(offsetof(BatchSliceView, split_bounce) == offsetof(BatchSlice, split_bounce),
-              "BatchSliceView mirrors BatchSlice");
-static_assert(offsetof(BatchSliceView, cont) == offsetof(BatchSlice, cont), "BatchSliceView mirrors BatchSlice");
-static_assert(offsetof(BatchSliceView, cont_count) == offsetof(BatchSlice, cont_count),
-              "BatchSliceView mirrors BatchSlice");
-
-// Index of the slice that owns workgroup b: the last slice whose first block is <= b
-// (slices are in block order and the first starts at block 0).
-enum SliceBlocks { kTraceBlocks = 0, kReduceBlocks = 1, kContBlocks = 2 };
-__device__ __forceinline__ uint32_t find_slice(const BatchSliceView *__restrict__ slices, uint32_t n, uint32_t b,
-                                               int which) {
-    uint32_t lo = 0u, hi = n;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t first = which == kTraceBlocks    ? slices[mid].trace_block_begin
-                               : which == kReduceBlocks ? slices[mid].reduce_block_begin
-                                                        : slices[mid].cont_block_begin;
-        if (first <= b) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// The batched path: one launch traces the pieces of many jobs of one material class (the
-// reference's per-class pipeline specialisation, frame.rs:514-524).  Every workgroup
-// belongs to one slice, stages that job's planes and reads that job's scene block; the
-// per-lane work is the plain kernel's, so each tuple's result is bit-identical to a plain
-// launch of the same job.
-template <int kAniso, bool kAbsorb, int kEnv, bool kFrost>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
-    const BatchSliceView *__restrict__ slices, uint32_t n_slices) {
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[CacheFloats<kAniso>::value];
-    __shared__ uint32_t block_ev[GR_EV_COUNT];
-    const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kTraceBlocks)];
-    const DevScene &sc = *(const DevScene *)J.scene;
-    stage_planes(planes, cam_side, (const float4 *)J.planes, sc);
-    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
-    __syncthreads();
-
-    uint32_t ev[kEvWords];
-#pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
-
-    const uint32_t spp = J.spp;
-    const uint32_t t = (blockIdx.x - J.trace_block_begin) * kBlock + threadIdx.x;
-    if (t < J.n_pixels * spp) {
-        const uint32_t local_pixel = t / spp;
-        const uint32_t s = t - local_pixel * spp;
-        PathState ps;
-        init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, make_float2(J.ri35[2 * s], J.ri35[2 * s + 1]),
-                  J.cert[J.pixel_begin + local_pixel], sc);
-        bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == J.split_bounce &&
-                park<kAniso == 2>(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
-                parked = true;
-                break;
-            }
-        }
-        if (!parked) write_result<kEnv, false>(ps, sc, t, (float *)J.xyz, nullptr);
-    }
-    flush_events(ev, block_ev, (unsigned long long *)J.stats);
-}
-
-// Continuation level `level` (1..kContLevels) of a batched class launch: each workgroup
-// belongs to one slice (binary search of cont_block_begin) and continues that slice's
-// records of the previous level, parking again for the next level except at the last.
-template <int kAniso, bool kAbsorb, int kEnv, bool kFrost>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
-    const BatchSliceView *__restrict__ slices, uint32_t n_slices, int level) {
-    const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kContBlocks)];
-    const uint32_t count = min(J.cont_count[level - 1], J.cont_cap);
-    const uint32_t r = (blockIdx.x - J.cont_block_begin) * kBlock + threadIdx.x;
-    if ((blockIdx.x - J.cont_block_begin) * kBlock >= count) return;
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[CacheFloats<kAniso>::value];
-    __shared__ uint32_t block_ev[GR_EV_COUNT];
-    const DevScene &sc = *(const DevScene *)J.scene;
-    stage_planes(planes, cam_side, (const float4 *)J.planes, sc);
-    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
-    __syncthreads();
-
-    uint32_t ev[kEvWords];
-#pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
-
-    if (r < count) {
-        PathState ps;
-        const uint32_t t = load_path<kAniso == 2>(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
-        const uint32_t next_split = level < kContLevels ? (uint32_t)(level + 1) * J.split_bounce : 0u;
-        bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == next_split &&
-                park<kAniso == 2>(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
-                parked = true;
-                break;
-            }
-        }
-        if (!parked) write_result<kEnv, false>(ps, sc, t, (float *)J.xyz, nullptr);
-    }
-    flush_events(ev, block_ev, (unsigned long long *)J.stats);
-}
-
-// Escape records: field-major (record field f of tuple t at rec[f * cap + t]).
-// Fields: 0-2 escape direction, 3-10 channel intensities, 11-18 channel pdfs,
-// 19 hero offset (negative: the path never escaped).
-constexpr int kRecFields = 20;
-
-// Persistent wavefronts.  Wave w owns tuples [w * kWaveTuples, (w+1) * kWaveTuples) of
-// the launch.  Every lane starts on one tuple; whenever lanes finish their paths, the
-// wave hands them the next unstarted tuples of its range in lane order (ballot +
-// popcount), so lanes stay busy until the range is exhausted instead of idling behind the
-// wave's longest path.  Each finished path writes its escape record; every loop
-// iteration advances every busy lane by one bounce, and every path ends within
-// max_bounces iterations, so the loop ends after at most kWaveTuples * max_bounces
-// iterations.
-template <int kAniso, bool kAbsorb, bool kInfo>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
-    const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
-    const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, uint32_t rec_cap,
-    float *__restrict__ rec, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    __shared__ float lcache[CacheFloats<kAniso>::value];
-    __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes(planes, cam_side, planes_g, sc);
-    if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
-    __syncthreads();
-
-    uint32_t ev[kEvWords];
-#pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
-
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    const uint32_t begin = wave * kWaveTuples;
-    const uint32_t end = min(begin + kWaveTuples, n_tuples);
-    const uint64_t lanes_below = (1ull << lane) - 1ull;
-
-    PathState ps;
-    uint32_t t = begin + lane;
-    bool busy = t < end;
-    uint32_t next = begin + 64u;  // wave-uniform
-    if (busy) {
-        const uint32_t lp = t / spp, s = t - lp * spp;
-        init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], cert[pixel_begin + lp], sc);
-    }
-    while (__ballot(busy) != 0ull) {
-        if (busy && bounce_step<kAniso, kAbsorb, true>(ps, sc, planes, cam_side, lcache, ev)) {
-            // Path ended: escape record (and per-tuple info).
-            float *r = rec + t;
-            r[0 * (size_t)rec_cap] = ps.ray_d.x;
-            r[1 * (size_t)rec_cap] = ps.ray_d.y;
-            r[2 * (size_t)rec_cap] = ps.ray_d.z;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                r[(3 + k) * (size_t)rec_cap] = intensity(ps.st[k]);
-                r[(11 + k) * (size_t)rec_cap] = ps.pdf[k];
-            }
-            r[19 * (size_t)rec_cap] = ps.escaped ? ps.hero_off : -1.0f;
-            if (kInfo) {
-                info_out[4 * (size_t)t + 0] = path_segments(ps, sc);
-                info_out[4 * (size_t)t + 1] = ps.term_reason;
-                info_out[4 * (size_t)t + 2] = ps.fh;
-                info_out[4 * (size_t)t + 3] = ps.fate_h;
-            }
-            busy = false;
-        }
-        const uint64_t idle = __ballot(!busy);
-        const uint32_t slot = next + (uint32_t)__popcll(idle & lanes_below);
-        next += (uint32_t)__popcll(idle);
-        if (!busy && slot < end) {
-            t = slot;
-            const uint32_t lp = t / spp, s = t - lp * spp;
-            init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], cert[pixel_begin + lp], sc);
-            busy = true;
-        }
-    }
-    flush_events(ev, block_ev, stats);
-}
-
-// Escape records -> per-tuple XYZ (the finish of trace_spectral_ray_inner).
-template <int kEnv>
-__global__ __launch_bounds__(kBlock) void finish_kernel(const DevScene sc, uint32_t n_tuples, uint32_t rec_cap,
-                                                        const float *__restrict__ rec,
-                                                        float *__restrict__ xyz_out) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n_tuples) return;
-    const float *r = rec + t;
-    const float hero_off = r[19 * (size_t)rec_cap];
-    V3 xyz = v3(0.0f, 0.0f, 0.0f);
-    if (hero_off >= 0.0f) {
-        float inten[8], pdf[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            inten[k] = r[(3 + k) * (size_t)rec_cap];
-            pdf[k] = r[(11 + k) * (size_t)rec_cap];
-        }
-        V3 dir = v3(r[0], r[(size_t)rec_cap], r[2 * (size_t)rec_cap]);
-        xyz = finish_escaped<kEnv>(dir, inten, pdf, hero_off, sc);
-    }
-    xyz_out[3 * (size_t)t + 0] = xyz.x;
-    xyz_out[3 * (size_t)t + 1] = xyz.y;
-    xyz_out[3 * (size_t)t + 2] = xyz.z;
-}
 
 // Per-pixel camera-ray miss certificates: the scan of the ray through the pixel's centre
 // (jitter 0); when it misses because its entering extreme exceeds its exiting one, the two
@@ -1619,149 +211,28 @@ __global__ __launch_bounds__(kBlock) void film_reduce_batch_kernel(const BatchSl
 // Launchers.
 // ---------------------------------------------------------------------------------
 
-namespace {
-
-template <int A, bool B, int E, bool F, bool I>
-hipError_t launch_plain_i(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
-    dim3 grid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
-    const bool split = a.split_bounce != 0 && a.cont != nullptr && a.cont_cap != 0;
-    if (split) {
-        hipError_t e = hipMemsetAsync(a.cont_count, 0, (gr::kContLevels + 1) * sizeof(uint32_t), stream);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((gr::trace_kernel<A, B, E, F, I>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes, a.ri35,
-                       a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats,
-                       split ? a.split_bounce : 0u, a.cont, a.cont_cap, a.cont_count);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !split) return e;
-    // Level k (k = 1..kContLevels) continues the records of level k-1 and, except the
-    // last, parks again at (k + 1) * split_bounce; records alternate between the two
-    // halves of cont.
-    dim3 cgrid((a.cont_cap + gr::kBlock - 1) / gr::kBlock);
-    for (int k = 1; k <= gr::kContLevels; ++k) {
-        float *in = a.cont + (size_t)((k - 1) & 1) * gr::kContFields * a.cont_cap;
-        float *out = a.cont + (size_t)(k & 1) * gr::kContFields * a.cont_cap;
-        const uint32_t next_split = k < gr::kContLevels ? (uint32_t)(k + 1) * a.split_bounce : 0u;
-        hipLaunchKernelGGL((gr::cont_kernel<A, B, E, F, I>), cgrid, dim3(gr::kBlock), 0, stream, sc, a.planes, in,
-                           a.cont_cap, a.cont_count + (k - 1), a.xyz, a.info, a.stats, next_split, out,
-                           a.cont_count + k);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
-}
-
-template <int A, bool B, int E>
-hipError_t launch_plain(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
-    // Scenes with frosted facets take the kernels built with the frosted bounce; the others
-    // keep kernels without it (its code raises the register pressure of every bounce).
-    if (gr::rare_features(sc))
-        return a.info ? launch_plain_i<A, B, E, true, true>(sc, a, stream)
-                      : launch_plain_i<A, B, E, true, false>(sc, a, stream);
-    return a.info ? launch_plain_i<A, B, E, false, true>(sc, a, stream)
-                  : launch_plain_i<A, B, E, false, false>(sc, a, stream);
-}
-
-template <int A, bool B, int E>
-hipError_t launch_wave(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
-    const uint32_t waves = (a.n_tuples + gr::kWaveTuples - 1) / gr::kWaveTuples;
-    dim3 grid((waves + gr::kBlock / 64 - 1) / (gr::kBlock / 64));
-    if (a.info)
-        hipLaunchKernelGGL((gr::trace_wave_kernel<A, B, true>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
-                           a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec,
-                           a.info, a.stats);
-    else
-        hipLaunchKernelGGL((gr::trace_wave_kernel<A, B, false>), grid, dim3(gr::kBlock), 0, stream, sc, a.planes,
-                           a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec,
-                           a.info, a.stats);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    dim3 fgrid((a.n_tuples + gr::kBlock - 1) / gr::kBlock);
-    hipEvent_t mark = a.trace_done;
-    if (mark) {
-        e = hipEventRecord(mark, stream);
-        if (e != hipSuccess) return e;
-    }
-    if (E == 0)
-        hipLaunchKernelGGL((gr::finish_kernel<0>), fgrid, dim3(gr::kBlock), 0, stream, sc, a.n_tuples, a.rec_cap,
-                           a.rec, a.xyz);
-    else
-        hipLaunchKernelGGL((gr::finish_kernel<1>), fgrid, dim3(gr::kBlock), 0, stream, sc, a.n_tuples, a.rec_cap,
-                           a.rec, a.xyz);
-    return hipGetLastError();
-}
-
-template <int A, bool B, int E>
-hipError_t launch_one(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
-    return a.wavefront ? launch_wave<A, B, E>(sc, a, stream) : launch_plain<A, B, E>(sc, a, stream);
-}
-
-}  // namespace
+#define GR_CLASS_CASES(X)                                                                                    \
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(17) X(19) X(21) \
+        X(23) X(25) X(27) X(29) X(31)
 
 hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &a, hipStream_t stream) {
-    const bool an = sc.anisotropic != 0, ab = sc.has_absorption != 0;
-    const int e = sc.env_kind;
-    if (an && sc.biaxial) {
-        if (ab) return e == 0 ? launch_one<2, true, 0>(sc, a, stream) : launch_one<2, true, 1>(sc, a, stream);
-        return e == 0 ? launch_one<2, false, 0>(sc, a, stream) : launch_one<2, false, 1>(sc, a, stream);
+    switch (gr::scene_class(sc)) {
+#define GR_TRACE_CASE(C) \
+    case C: return gr::launch_trace_class<C>(sc, a, stream);
+        GR_CLASS_CASES(GR_TRACE_CASE)
+#undef GR_TRACE_CASE
+        default: return hipErrorInvalidValue;
     }
-    if (an && ab) return e == 0 ? launch_one<true, true, 0>(sc, a, stream) : launch_one<true, true, 1>(sc, a, stream);
-    if (an && !ab) return e == 0 ? launch_one<true, false, 0>(sc, a, stream) : launch_one<true, false, 1>(sc, a, stream);
-    if (!an && ab) return e == 0 ? launch_one<false, true, 0>(sc, a, stream) : launch_one<false, true, 1>(sc, a, stream);
-    return e == 0 ? launch_one<false, false, 0>(sc, a, stream) : launch_one<false, false, 1>(sc, a, stream);
 }
-
-namespace {
-template <int C>
-hipError_t launch_batch_class(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
-                              uint32_t cont_blocks, hipStream_t stream) {
-    constexpr int kA = (C & 16) ? 2 : (C & 1);
-    constexpr bool kB = (C & 2) != 0, kF = (C & 8) != 0;
-    constexpr int kE = (C & 4) ? 1 : 0;
-    const auto *view = reinterpret_cast<const gr::BatchSliceView *>(slices);
-    hipLaunchKernelGGL((gr::trace_batch_kernel<kA, kB, kE, kF>), dim3(n_blocks), dim3(gr::kBlock), 0, stream, view,
-                       n_slices);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || cont_blocks == 0) return e;
-    for (int level = 1; level <= gr::kContLevels; ++level) {
-        hipLaunchKernelGGL((gr::cont_batch_kernel<kA, kB, kE, kF>), dim3(cont_blocks), dim3(gr::kBlock), 0, stream,
-                           view, n_slices, level);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
-}
-}  // namespace
 
 hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
                                  uint32_t cont_blocks, int cls, hipStream_t stream) {
     if (n_slices == 0 || n_blocks == 0) return hipSuccess;
     switch (cls) {
-        case 0: return launch_batch_class<0>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 1: return launch_batch_class<1>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 2: return launch_batch_class<2>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 3: return launch_batch_class<3>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 4: return launch_batch_class<4>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 5: return launch_batch_class<5>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 6: return launch_batch_class<6>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 7: return launch_batch_class<7>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 8: return launch_batch_class<8>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 9: return launch_batch_class<9>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 10: return launch_batch_class<10>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 11: return launch_batch_class<11>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 12: return launch_batch_class<12>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 13: return launch_batch_class<13>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 14: return launch_batch_class<14>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 15: return launch_batch_class<15>(slices, n_slices, n_blocks, cont_blocks, stream);
-        // Biaxial classes (bit 4, always with bit 0).
-        case 17: return launch_batch_class<17>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 19: return launch_batch_class<19>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 21: return launch_batch_class<21>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 23: return launch_batch_class<23>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 25: return launch_batch_class<25>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 27: return launch_batch_class<27>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 29: return launch_batch_class<29>(slices, n_slices, n_blocks, cont_blocks, stream);
-        case 31: return launch_batch_class<31>(slices, n_slices, n_blocks, cont_blocks, stream);
+#define GR_BATCH_CASE(C) \
+    case C: return gr::launch_batch_class<C>(slices, n_slices, n_blocks, cont_blocks, stream);
+        GR_CLASS_CASES(GR_BATCH_CASE)
+#undef GR_BATCH_CASE
         default: return hipErrorInvalidValue;
     }
 }

@@ -55,7 +55,7 @@ int main(void) {
 
 def lobe_triples():
     triples = set()
-    for name in ("device_physics.h", "transport.hip"):
+    for name in ("device_physics.h", "transport_kernels.h"):
         text = open(os.path.join(CSRC, name)).read()
         for fn in ("cmf_lobe", "asym_gauss"):
             for m in re.finditer(fn + r"\(l,\s*([0-9.]+)f,\s*([0-9.]+)f,\s*([0-9.]+)f\)", text):
@@ -109,7 +109,7 @@ int main(void) {
 
 def test_ring_light_spark_division_is_correctly_rounded_where_used(tmp_path):
     # finish_escaped: spark = (rd - 0.96) / 0.04 is used only for rd in (0.96, 1].
-    text = open(os.path.join(CSRC, "transport.hip")).read()
+    text = open(os.path.join(CSRC, "transport_kernels.h")).read()
     assert "div_by_const(rd - 0.96f, 0.04f, 1.0f / 0.04f)" in text
     c = tmp_path / "spark.c"
     c.write_text(SPARK_C)
