@@ -27,83 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "8-λ path segments/sec on 1 MI355X; p99 job latency, batched vs plain"
+
+from gemray_studio_amd.workloads import (batched_jobs, cpu_config_scene, served_jobs, single_job_scene,  # noqa: E402
+                                         stress_scene)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-
-
-def single_job_scene(width=1920, height=1080):
-    from gemray_studio_amd import backend as B
-    from gemray_studio_amd import scenes
-
-    material = scenes.pleochroic_sapphire(seed=2)
-    planes = scenes.oval_brilliant()
-    env = scenes.procedural_env(seed=2, width=2048, height=1024)
-    cam = B.camera_new(0.35, 0.45, 5.0, 28.0)
-    scene = B.SceneRef(camera=cam, width=width, height=height, planes=planes, material=material,
-                       max_bounces=32, environment=B.HdrMap(env))
-    return scene, (0.35, 0.45, 5.0, 28.0)
-
-
-ISOTROPIC = ["Diamond", "Cubic Zirconia", "Spinel"]
-UNIAXIAL = ["Sapphire", "Ruby", "Emerald", "Zircon", "Quartz", "Tourmaline", "Synthetic Moissanite"]
-PRESETS = ["Daylight", "Incandescent", "RingLights", "DarkSpotlight"]
-
-
-def batched_jobs(seed=3, n_jobs=64, poses=None):
-    """Batched workload: n_jobs seeded convex stones with 17-181 planes (cut angles jittered
-    +-3 deg), an isotropic/uniaxial material mix (uniaxial ones on a seeded optic axis),
-    square frames of log-uniform 128^2..1024^2 pixels, log-uniform 64..1024 spp, studio
-    lighting with a random preset, depth 16."""
-    from dataclasses import replace
-
-    from gemray_studio_amd import backend as B
-    from gemray_studio_amd import materials as M
-    from gemray_studio_amd import scenes
-
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for i in range(n_jobs):
-        n_planes = int(rng.integers(17, 182))
-        planes = scenes.random_convex_stone(seed * 1000 + i, n_planes, 3.0)
-        if rng.random() < 0.5:
-            mat = M.by_name(ISOTROPIC[int(rng.integers(len(ISOTROPIC)))])
-        else:
-            mat = M.by_name(UNIAXIAL[int(rng.integers(len(UNIAXIAL)))])
-            mat = replace(mat, c_axis=scenes.seeded_unit_vector(seed * 1000 + i))
-        side = int(round(float(np.exp(rng.uniform(np.log(128), np.log(1024))))))
-        spp = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1024))))))
-        cam_yaw, cam_pitch = float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.2, 0.6))
-        cam = B.camera_new(cam_yaw, cam_pitch, 5.0, float(rng.uniform(16.0, 28.0)))
-        env = B.Studio(PRESETS[int(rng.integers(len(PRESETS)))], 1.0, float(rng.uniform(0, 2 * np.pi)),
-                       float(rng.uniform(0.2, 1.0)))
-        jobs.append((B.SceneRef(cam, side, side, planes, mat, 16, env), 0, spp))
-        if poses is not None:
-            poses.append((cam_yaw, cam_pitch, env.light_yaw, env.light_pitch))
-    return jobs
-
-
-def served_jobs(env_map, seed=4, n_jobs=40):
-    """Served workload: the batched workload's per-job distribution with the single-job
-    workload's material and environment family (a seeded pleochroic sapphire, the 2048x1024
-    procedural HDR map, depth 32), plus 10 % of jobs at 1920x1080 x 512 spp on the 69-plane
-    oval brilliant."""
-    from gemray_studio_amd import backend as B
-    from gemray_studio_amd import scenes
-
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for i in range(n_jobs):
-        mat = scenes.pleochroic_sapphire(seed * 1000 + i)
-        cam = B.camera_new(float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.2, 0.6)), 5.0,
-                           float(rng.uniform(16.0, 28.0)))
-        if rng.random() < 0.1:
-            jobs.append((B.SceneRef(cam, 1920, 1080, scenes.oval_brilliant(), mat, 32, B.HdrMap(env_map)), 0, 512))
-            continue
-        planes = scenes.random_convex_stone(seed * 1000 + i, int(rng.integers(17, 182)), 3.0)
-        side = int(round(float(np.exp(rng.uniform(np.log(128), np.log(1024))))))
-        spp = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1024))))))
-        jobs.append((B.SceneRef(cam, side, side, planes, mat, 32, B.HdrMap(env_map)), 0, spp))
-    return jobs
 
 
 def run_plain(gpu, jobs):
@@ -415,9 +343,11 @@ def main():
     dscene.free()
 
     m = scene.material
-    flops = op_counts.algorithmic_flops(events, scene.planes.shape[0], paths, m.is_anisotropic,
-                                        bool(m.o_bands or m.e_bands), False,
-                                        n_bands=len(m.o_bands) + len(m.e_bands))
+    model = (events, scene.planes.shape[0], paths, m.is_anisotropic, bool(m.o_bands or m.e_bands), False)
+    n_bands = len(m.o_bands) + len(m.e_bands)
+    flops = op_counts.algorithmic_flops(*model, n_bands=n_bands)
+    flops_uniform = op_counts.algorithmic_flops_uniform(*model, n_bands=n_bands)
+    background = events["first_miss"]
     # One launch = one chunk's trace kernel plus, in wavefront mode, its finish kernel
     # (the escape radiance pass); the op-count model covers both, so both are charged.
     avg_launch_s = (trace_ms + finish_ms) / 1e3 / launches
@@ -442,13 +372,19 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "single job: pleochroic uniaxial sapphire, 69-plane oval brilliant, 2048x1024 "
-                        f"procedural HDR env, {args.width}x{args.height} x {args.spp} spp per step, depth 32",
+                        f"procedural HDR env, {args.width}x{args.height} x {args.spp} spp per step, depth 32, "
+                        f"framed with the reference's test camera {tuple(cam_args)} (yaw, pitch, distance, fov)",
+            "camera": list(cam_args),
             "width": args.width, "height": args.height, "spp_per_step": args.spp,
             "trace_mode": args.trace_mode,
             "max_bounces": scene.max_bounces, "n_planes": int(scene.planes.shape[0]),
             "paths": paths, "segments": segments,
             "segments_per_path": segments / max(paths, 1),
             "paths_per_s": paths / elapsed,
+            # Paths whose camera ray misses the stone (one segment each, device-counted), and
+            # the throughput of the segments that meet it.
+            "background_path_fraction": background / max(paths, 1),
+            "stone_segments_per_s": (segments - background) / elapsed,
         },
         "roofline": {
             "bound": "valu",
@@ -464,6 +400,10 @@ def main():
             "avg_finish_kernel_ms": finish_ms / launches,
             "launches": launches,
             "algorithmic_flops_per_launch": flops / launches,
+            # The same events charged as if no shortcut ran (every segment the full clip,
+            # every path the full setup and finish): an upper bound on the credited work.
+            "frac_if_every_segment_ran_the_full_clip": flops_uniform / launches / avg_launch_s / 1e12
+            / PEAK_FP32_TFLOPS,
         },
     }
     if not args.no_jobs:

@@ -362,8 +362,11 @@ class GpuBackend:
         """Evaluates device function fn (_lib.GR_PROBE_*) on rows of up to 9 float32 inputs;
         returns (n, 3) float32 outputs (a test probe of the kernel's own functions)."""
         self._require()
-        x = np.zeros((len(inputs), 9), np.float32)
-        x[:, :inputs.shape[1]] = inputs
+        # Rows travel as raw 32-bit words: uint32 inputs (the integer probes) keep their bits.
+        inputs = np.asarray(inputs)
+        bits = inputs.astype(np.uint32) if inputs.dtype.kind in "ui" else inputs.astype(np.float32).view(np.uint32)
+        x = np.zeros((len(inputs), 9), np.uint32)
+        x[:, :inputs.shape[1]] = bits
         out = np.zeros((len(inputs), 3), np.float32)
         check(self._L.gr_probe(self._ctx, fn, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(x),
                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "gr_probe")

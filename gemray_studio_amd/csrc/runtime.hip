@@ -439,6 +439,7 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     if (chunk_pixels > n_pixels) chunk_pixels = n_pixels;
     uint64_t cap = chunk_pixels * spp;
     const bool wavefront = ctx->trace_mode == GR_TRACE_WAVEFRONT;
+    const uint32_t split_bounce = ctx->split_bounce.load(std::memory_order_relaxed);
     gr_status st = ensure_xyz(ctx, cap);
     if (st != GR_OK) return st;
     if (info_host) {
@@ -448,7 +449,7 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     if (wavefront) {
         st = ensure_rec(ctx, cap);
         if (st != GR_OK) return st;
-    } else if (ctx->split_bounce) {
+    } else if (split_bounce) {
         st = ensure_cont(ctx, cap);
         if (st != GR_OK) return st;
     }
@@ -476,7 +477,7 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
         args.rec = ctx->rec;
         args.rec_cap = (uint32_t)ctx->rec_cap;
         args.trace_done = wavefront ? ctx->ev_k1 : nullptr;
-        args.split_bounce = wavefront ? 0u : ctx->split_bounce;
+        args.split_bounce = wavefront ? 0u : split_bounce;
         args.cont = ctx->cont;
         args.cont_cap = ctx->cont_cap;
         args.cont_count = ctx->cont_count;
@@ -533,7 +534,13 @@ gr_status gr_open(gr_ctx **out, const gr_options *opts) {
     ctx->chunk_tuples = kDefaultChunkTuples;
     if (opts) ctx->hbm_budget = opts->hbm_budget_bytes;
     if (opts && opts->chunk_tuples) ctx->chunk_tuples = opts->chunk_tuples < kMaxChunkTuples ? opts->chunk_tuples : kMaxChunkTuples;
-    std::snprintf(ctx->label, sizeof(ctx->label), "%s (%s, HIP)", prop.name, prop.gcnArchName);
+    // Some drivers report no marketing name: the label then names the device by ordinal and
+    // compute-unit count, so it is never empty (gpu_backend.rs:146-153 shows the adapter).
+    if (prop.name[0] != '\0')
+        std::snprintf(ctx->label, sizeof(ctx->label), "%s (%s, HIP)", prop.name, prop.gcnArchName);
+    else
+        std::snprintf(ctx->label, sizeof(ctx->label), "HIP device %d, %d CUs (%s, HIP)", dev,
+                      prop.multiProcessorCount, prop.gcnArchName);
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev_begin) != hipSuccess || hipEventCreate(&ctx->ev_end) != hipSuccess ||
         hipEventCreate(&ctx->ev_k0) != hipSuccess || hipEventCreate(&ctx->ev_k1) != hipSuccess ||
@@ -619,6 +626,19 @@ gr_status gr_camera_new(float yaw, float pitch, float distance, float fov_deg, g
 
 gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out) {
     if (!ctx || !in || !out || fn < 0 || fn >= GR_PROBE_COUNT || n == 0 || n > (1u << 24)) return GR_ERR_INVALID;
+    // The sample-draw probe takes its radical inverses from the host's sample table, exactly
+    // as a render does (ensure_ri35).
+    std::vector<float> staged;
+    if (fn == GR_PROBE_SAMPLE_DRAWS) {
+        staged.assign(in, in + n * 9);
+        for (size_t i = 0; i < n; ++i) {
+            uint32_t s;
+            std::memcpy(&s, &staged[9 * i + 1], sizeof(s));
+            staged[9 * i + 2] = h_radical_inverse(s, 3);
+            staged[9 * i + 3] = h_radical_inverse(s, 5);
+        }
+        in = staged.data();
+    }
     std::lock_guard<std::mutex> g(ctx->mu);
     GR_CHECK(hipSetDevice(ctx->device));
     float *d_in = nullptr, *d_out = nullptr;

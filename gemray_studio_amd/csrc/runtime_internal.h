@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -44,7 +45,9 @@ struct gr_ctx {
     uint32_t ri35_cap = 0, ri35_offset = 0, ri35_n = 0;
     std::vector<float2> ri35_host;
     unsigned long long *stats = nullptr;
-    uint32_t split_bounce = 6;  // plain mode: park paths running after this many segments
+    // Plain mode: park paths running after this many segments.  Atomic: the job loop's
+    // scheduler thread reads it without the context mutex, which a render holds throughout.
+    std::atomic<uint32_t> split_bounce{6};
     float *cont = nullptr;      // continuation records, kContFields x cont_cap
     uint32_t cont_cap = 0;
     uint32_t *cont_count = nullptr;

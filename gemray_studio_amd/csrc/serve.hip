@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -30,6 +31,7 @@
 #include <vector>
 
 #include "../../include/gemray_hip.h"
+#include "progressive.h"
 #include "runtime_internal.h"
 #include "transport.h"
 
@@ -70,8 +72,7 @@ struct Job {
     bool progressive = false, fixed_batches = false;
     uint32_t subbatch_ms = 0, produced = 0, subbatches = 0;
     Clock::time_point t_batch;
-    std::vector<float> pending;
-    uint32_t pending_first = 0, pending_samples = 0;
+    PendingDelta pending;
     bool delta_taken = false;
     uint64_t n_pixels = 0;
     uint64_t footprint = 0;
@@ -94,7 +95,9 @@ struct Job {
     Clock::time_point t_submit, t_admit, t_done;
     // Deadline policy: submit time (ms from the loop's start) + slack x estimated service.
     double deadline_ms = 0.0;
-    bool cancel_req = false;
+    // Set by gr_job_cancel (under the loop's mutex), read by the scheduler thread also
+    // outside it.
+    std::atomic<bool> cancel_req{false};
 };
 
 struct Piece {
@@ -183,10 +186,14 @@ uint64_t job_footprint(const Job &j) {
     return b;
 }
 
-void free_job_device(Job &j) {
+// Device memory of the loop is stream-ordered (hipMallocAsync / hipFreeAsync on the control
+// stream): a job's buffers are freed only after its last tick completed and its film was read
+// back, and a stream-ordered free does not wait for the device, so retiring a job never
+// stalls the tick still running on the other slot (hipFree would synchronize the device).
+void free_job_device(ServeState &S, Job &j) {
     for (void *p : {(void *)j.d_scene, (void *)j.d_planes, (void *)j.d_finishes, (void *)j.d_map, (void *)j.d_ri35,
                     (void *)j.d_cert, (void *)j.d_film, (void *)j.d_stats})
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, S.stream);
     j.d_scene = nullptr;
     j.d_planes = nullptr;
     j.d_finishes = nullptr;
@@ -198,14 +205,14 @@ void free_job_device(Job &j) {
 }
 
 gr_status upload_job(ServeState &S, Job &j) {
-    SV_CHECK(hipMalloc(&j.d_scene, sizeof(DevScene)));
-    SV_CHECK(hipMalloc(&j.d_planes, j.planes.size() * sizeof(float)));
-    if (!j.finishes.empty()) SV_CHECK(hipMalloc(&j.d_finishes, j.finishes.size() * sizeof(uint32_t)));
-    if (!j.map.empty()) SV_CHECK(hipMalloc(&j.d_map, j.map.size() * sizeof(float)));
-    SV_CHECK(hipMalloc(&j.d_ri35, (size_t)j.total_spp * sizeof(float2)));
-    SV_CHECK(hipMalloc(&j.d_cert, j.n_pixels * sizeof(uint32_t)));
-    SV_CHECK(hipMalloc(&j.d_film, j.n_pixels * 3 * sizeof(float)));
-    SV_CHECK(hipMalloc(&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long)));
+    SV_CHECK(hipMallocAsync((void **)&j.d_scene, sizeof(DevScene), S.stream));
+    SV_CHECK(hipMallocAsync((void **)&j.d_planes, j.planes.size() * sizeof(float), S.stream));
+    if (!j.finishes.empty()) SV_CHECK(hipMallocAsync((void **)&j.d_finishes, j.finishes.size() * sizeof(uint32_t), S.stream));
+    if (!j.map.empty()) SV_CHECK(hipMallocAsync((void **)&j.d_map, j.map.size() * sizeof(float), S.stream));
+    SV_CHECK(hipMallocAsync((void **)&j.d_ri35, (size_t)j.total_spp * sizeof(float2), S.stream));
+    SV_CHECK(hipMallocAsync((void **)&j.d_cert, j.n_pixels * sizeof(uint32_t), S.stream));
+    SV_CHECK(hipMallocAsync((void **)&j.d_film, j.n_pixels * 3 * sizeof(float), S.stream));
+    SV_CHECK(hipMallocAsync((void **)&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long), S.stream));
     DevScene sc = j.host_sc;
     sc.map_rgb = j.d_map;
     sc.finishes = j.d_finishes;
@@ -242,30 +249,20 @@ gr_status read_batch(ServeState &S, Job &j, std::vector<float> &out, bool last) 
     return GR_OK;
 }
 
-// next_batch_size (apps/gemray-worker/src/stream_emit/sizing.rs:21-30): scales the previous
-// sub-batch toward the target duration, growing at most x4 and never returning 0.
-uint32_t next_batch_size(uint32_t prev, double elapsed_ms, uint32_t target_ms) {
-    const uint64_t grow64 = std::max<uint64_t>(std::min<uint64_t>((uint64_t)prev * 4u, 0xFFFFFFFFull), 1u);
-    if (!(elapsed_ms > 0.0)) return (uint32_t)grow64;
-    const double scaled = (double)prev * ((double)target_ms / elapsed_ms);
-    const double c = std::min(std::max(scaled, 1.0), (double)grow64);
-    return (uint32_t)std::round(c);
-}
-
 gr_status ensure_staging(TickSlot &T, uint64_t tuples, uint32_t slices) {
     if (tuples > T.staging_cap) {
-        if (T.staging) SV_CHECK(hipFree(T.staging));
+        if (T.staging) SV_CHECK(hipFreeAsync(T.staging, T.stream));
         T.staging = nullptr;
         T.staging_cap = 0;
-        SV_CHECK(hipMalloc(&T.staging, tuples * 3 * sizeof(float)));
+        SV_CHECK(hipMallocAsync((void **)&T.staging, tuples * 3 * sizeof(float), T.stream));
         T.staging_cap = tuples;
     }
     if (slices > T.slices_cap) {
-        if (T.d_slices) SV_CHECK(hipFree(T.d_slices));
+        if (T.d_slices) SV_CHECK(hipFreeAsync(T.d_slices, T.stream));
         T.d_slices = nullptr;
         T.slices_cap = 0;
         uint32_t cap = std::max<uint32_t>(slices, 64);
-        SV_CHECK(hipMalloc(&T.d_slices, cap * sizeof(BatchSlice)));
+        SV_CHECK(hipMallocAsync((void **)&T.d_slices, cap * sizeof(BatchSlice), T.stream));
         T.slices_cap = cap;
     }
     return GR_OK;
@@ -273,18 +270,18 @@ gr_status ensure_staging(TickSlot &T, uint64_t tuples, uint32_t slices) {
 
 gr_status ensure_cont(TickSlot &T, uint64_t floats, uint32_t slices) {
     if (floats > T.cont_cap) {
-        if (T.cont) SV_CHECK(hipFree(T.cont));
+        if (T.cont) SV_CHECK(hipFreeAsync(T.cont, T.stream));
         T.cont = nullptr;
         T.cont_cap = 0;
-        SV_CHECK(hipMalloc(&T.cont, floats * sizeof(float)));
+        SV_CHECK(hipMallocAsync((void **)&T.cont, floats * sizeof(float), T.stream));
         T.cont_cap = floats;
     }
     if (slices > T.counts_cap) {
-        if (T.cont_counts) SV_CHECK(hipFree(T.cont_counts));
+        if (T.cont_counts) SV_CHECK(hipFreeAsync(T.cont_counts, T.stream));
         T.cont_counts = nullptr;
         T.counts_cap = 0;
         uint32_t cap = std::max<uint32_t>(slices, 64);
-        SV_CHECK(hipMalloc(&T.cont_counts, (size_t)cap * (kContLevels + 1) * sizeof(uint32_t)));
+        SV_CHECK(hipMallocAsync((void **)&T.cont_counts, (size_t)cap * (kContLevels + 1) * sizeof(uint32_t), T.stream));
         T.counts_cap = cap;
     }
     return GR_OK;
@@ -451,7 +448,7 @@ void percentiles(std::vector<double> v, gr_serve_metrics *m) {
 }
 
 void retire(ServeState &S, Job &j, int state, gr_status err) {
-    free_job_device(j);
+    free_job_device(S, j);
     S.in_use -= j.footprint;
     j.state = state;
     j.error = err;
@@ -524,13 +521,7 @@ void finish_jobs(ServeState &S, std::vector<Job *> &finished) {
             // PendingDelta::add and the running total (tracer.rs:93-103): the first
             // sub-batch since a take is copied, later ones are added in sample order.
             const std::vector<float> &b = batch[i];
-            if (j.pending_samples == 0) {
-                j.pending = b;
-                j.pending_first = j.sample_offset;
-            } else {
-                for (size_t k = 0; k < b.size(); ++k) j.pending[k] += b[k];
-            }
-            j.pending_samples += j.spp;
+            j.pending.add(j.sample_offset, j.spp, b.data());
             for (size_t k = 0; k < b.size(); ++k) j.film[k] += b[k];
         }
         j.produced += j.spp;
@@ -619,7 +610,7 @@ void loop(ServeState *Sp) {
             for (auto &j : S.active)
                 if (!j->cancel_req && j->next_pixel < j->n_pixels) schedulable.push_back(j.get());
             tick_tuples = S.tick_tuples;
-            split_bounce = S.ctx->split_bounce;
+            split_bounce = S.ctx->split_bounce.load(std::memory_order_relaxed);
             policy = S.policy;
         }
         TickSlot &next = S.slot[S.ticks_launched % kTickSlots];
@@ -661,10 +652,9 @@ void loop(ServeState *Sp) {
 void release_device(ServeState &S) {
     for (TickSlot &T : S.slot) {
         if (T.stream) (void)hipStreamSynchronize(T.stream);
-        if (T.staging) (void)hipFree(T.staging);
-        if (T.d_slices) (void)hipFree(T.d_slices);
-        if (T.cont) (void)hipFree(T.cont);
-        if (T.cont_counts) (void)hipFree(T.cont_counts);
+        for (void *p : {(void *)T.staging, (void *)T.d_slices, (void *)T.cont, (void *)T.cont_counts})
+            if (p) (void)hipFreeAsync(p, T.stream);
+        if (T.stream) (void)hipStreamSynchronize(T.stream);
         if (T.ev0) (void)hipEventDestroy(T.ev0);
         if (T.ev1) (void)hipEventDestroy(T.ev1);
         if (T.stream) (void)hipStreamDestroy(T.stream);
@@ -738,7 +728,8 @@ void serve_shutdown(gr_ctx *ctx) {
     (void)hipStreamSynchronize(S->stream);
     for (TickSlot &T : S->slot)
         if (T.stream) (void)hipStreamSynchronize(T.stream);
-    for (auto &j : S->active) free_job_device(*j);
+    for (auto &j : S->active) free_job_device(*S, *j);
+    (void)hipStreamSynchronize(S->stream);
     release_device(*S);
     delete S;
 }
@@ -778,6 +769,7 @@ gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) 
         j->fixed_batches = desc->subbatch_fixed != 0;
         j->spp = std::min(std::max(desc->subbatch_samples, 1u), j->total_spp);
         j->film.assign(j->n_pixels * 3, 0.0f);
+        j->pending.resize(j->n_pixels * 3);
     }
     j->footprint = gr::job_footprint(*j);
     j->t_submit = gr::Clock::now();
@@ -895,11 +887,7 @@ gr_status gr_job_take_delta(gr_ctx *ctx, uint64_t job_id, uint32_t *first_sample
     *first_sample = j->first_sample;
     *samples = 0;
     if (j->progressive) {
-        if (j->pending_samples == 0) return GR_OK;
-        std::memcpy(delta_xyz, j->pending.data(), j->pending.size() * sizeof(float));
-        *first_sample = j->pending_first;
-        *samples = j->pending_samples;
-        j->pending_samples = 0;
+        (void)j->pending.take(first_sample, samples, delta_xyz);
         return GR_OK;
     }
     // One batch: the whole film once the job is done.

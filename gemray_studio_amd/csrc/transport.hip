@@ -158,6 +158,32 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__re
             o[2] = r.z;
             break;
         }
+        case GR_PROBE_SAMPLE_DRAWS: {
+            // a = (pixel, sample_num as u32 bits, ri3, ri5 from the host's sample table).
+            uint32_t seed;
+            sample_draws(__float_as_uint(a[0]), __float_as_uint(a[1]), make_float2(a[2], a[3]), &seed, &o[0], &o[1],
+                         &o[2]);
+            break;
+        }
+        case GR_PROBE_SEED: {
+            // a = (pixel, sample_num, x as u32 bits).
+            uint32_t seed;
+            float jx, jy, hero;
+            sample_draws(__float_as_uint(a[0]), __float_as_uint(a[1]), make_float2(0.0f, 0.0f), &seed, &jx, &jy, &hero);
+            o[0] = __uint_as_float(seed);
+            o[1] = __uint_as_float(hash_u32(__float_as_uint(a[2])));
+            o[2] = __uint_as_float(__builtin_bitreverse32(__float_as_uint(a[2])));
+            break;
+        }
+        case GR_PROBE_STREAM_DRAW: {
+            // a = (seed, bounce, salt0, salt1, salt2 as u32 bits).
+            const uint32_t seed = __float_as_uint(a[0]), bounce = __float_as_uint(a[1]);
+            for (int k = 0; k < 3; ++k) o[k] = stream_draw(seed, bounce, __float_as_uint(a[2 + k]));
+            break;
+        }
+        case GR_PROBE_RGB_SPECTRAL:
+            o[0] = rgb_to_spectral(v3(a[0], a[1], a[2]), a[3]);
+            break;
         default:
             break;
     }

@@ -145,16 +145,21 @@ struct PathState {
 // inverses (bases 3 and 5) taken from a host-built table, then Camera::generate_ray
 // (camera.rs:96-114) and the initial state of trace_spectral_ray_inner
 // (transport.rs:651-722).
+__device__ __forceinline__ void sample_draws(uint32_t pixel, uint32_t sample_num, float2 ri35, uint32_t *seed,
+                                             float *jx, float *jy, float *hero) {
+    const float rot_x = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_X_ROTATION_STREAM));
+    const float rot_y = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_Y_ROTATION_STREAM));
+    const float rot_h = low_discrepancy_base2(hash_u32(pixel ^ HERO_WAVELENGTH_ROTATION_STREAM));
+    *seed = hash_u32((pixel * 0x9e3779b9u) ^ (sample_num * 0x85ebca6bu));
+    *jx = cranley_patterson(low_discrepancy_base2(sample_num), rot_x) - 0.5f;
+    *jy = cranley_patterson(ri35.x, rot_y) - 0.5f;
+    *hero = cranley_patterson(ri35.y, rot_h);
+}
 __device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_t sample_num,
                                           float2 ri35, uint32_t cert, const DevScene &sc) {
     const uint32_t x = pixel % sc.width, y = pixel / sc.width;
-    float rot_x = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_X_ROTATION_STREAM));
-    float rot_y = low_discrepancy_base2(hash_u32(pixel ^ PIXEL_JITTER_Y_ROTATION_STREAM));
-    float rot_h = low_discrepancy_base2(hash_u32(pixel ^ HERO_WAVELENGTH_ROTATION_STREAM));
-    ps.seed = hash_u32((pixel * 0x9e3779b9u) ^ (sample_num * 0x85ebca6bu));
-    float jx = cranley_patterson(low_discrepancy_base2(sample_num), rot_x) - 0.5f;
-    float jy = cranley_patterson(ri35.x, rot_y) - 0.5f;
-    float hero = cranley_patterson(ri35.y, rot_h);
+    float jx, jy, hero;
+    sample_draws(pixel, sample_num, ri35, &ps.seed, &jx, &jy, &hero);
     float w = (float)sc.width, h = (float)sc.height;
     float aspect = w / h;
     float u = (((float)x + jx) / w - 0.5f) * 2.0f * aspect * sc.fov_tan;
@@ -317,14 +322,16 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         // Camera rays share their origin: the origin's signed distances are staged per
         // workgroup (same operations, same values).  Most background rays are proved to
         // miss by their pixel's certificate.
-        if (camera_miss_certified(planes, cam_side, ps.cert, rd))
+        if (camera_miss_certified(planes, cam_side, ps.cert, rd)) {
+            ev_inc(ev, GR_EV_CERT_MISS);
             outside = true;
-        else
+        } else
             clip_scan<true>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
     } else if (!ps.inside && exterior_miss(planes[ps.last_facet], ro, rd)) {
         // The ray left the stone's surface outwards (a refraction out, or a reflection off
         // the outside): it starts outside the facet it just met and moves away from it, so
         // that facet alone makes the scan's far extreme negative and the scan a miss.
+        ev_inc(ev, GR_EV_EXTERIOR_MISS);
         outside = true;
     } else {
         clip_scan<false>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
@@ -347,6 +354,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         // Escape: the environment is sampled by finish_path (the path's only radiance
         // contribution; transport.rs:307-318).
         ev_inc(ev, GR_EV_MISS);
+        if (bounce == 0u) ev_inc(ev, GR_EV_FIRST_MISS);
         ps.escaped = true;
         fnv(ps.fh, 0xFFFFu);
         ps.term_bounce = bounce;
@@ -394,6 +402,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         float alphas[8];
         if (kAbsorb) {
             if (!ps.have_abs) {
+                ev_inc(ev, GR_EV_ABSORPTION_FILL);
                 fill_absorption_cache<kBiax>(ps.hero_off, sc, lcache);
                 ps.have_abs = true;
             }
@@ -496,6 +505,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 // fill_absorption_cache: with the call, the compiler's allocation of this
                 // (hot) block spilled 178 VGPRs instead of 30 (same arithmetic).
                 if (!ps.have_abs) {
+                    ev_inc(ev, GR_EV_ABSORPTION_FILL);
 #pragma unroll
                     for (int k = 0; k < 8; ++k) {
                         float so = -0.0f, se = -0.0f;

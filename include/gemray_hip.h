@@ -122,10 +122,15 @@ typedef struct {
     const gr_env *env;
 } gr_scene;
 
-/* Per-class event counters of the last render (device-side, exact). */
+/* Per-class event counters of the last render (device-side, exact).  The first twelve are
+ * the reference tracer's event classes; the last four count work the kernels skip or do once
+ * per path: camera rays proved to miss by their pixel's certificate (two planes instead of
+ * the full clip), exterior misses (one plane), paths whose first segment misses the stone
+ * (no dispersion evaluation), and absorption-cache fills (the band sums, once per path). */
 enum {
     GR_EV_SEGMENT = 0, GR_EV_MISS, GR_EV_HIT, GR_EV_ROTATE, GR_EV_ABSORB, GR_EV_TIR,
-    GR_EV_REFLECT, GR_EV_REFRACT, GR_EV_ENTRY_ANISO, GR_EV_WALKOFF, GR_EV_RR, GR_EV_SCATTER, GR_EV_COUNT
+    GR_EV_REFLECT, GR_EV_REFRACT, GR_EV_ENTRY_ANISO, GR_EV_WALKOFF, GR_EV_RR, GR_EV_SCATTER,
+    GR_EV_CERT_MISS, GR_EV_EXTERIOR_MISS, GR_EV_FIRST_MISS, GR_EV_ABSORPTION_FILL, GR_EV_COUNT
 };
 typedef struct {
     uint64_t events[GR_EV_COUNT]; /* events[GR_EV_SEGMENT] = 8-lambda path segments     */
@@ -210,11 +215,21 @@ gr_status gr_guide_buffers(gr_ctx *ctx, const gr_camera *camera, uint32_t width,
  * w.xyz, gamma_axis.xyz) -- the principal frame is BiaxialIndicatrix::from_gamma_axis's:
  *   GR_PROBE_BX_WAVE_INDICES  out (n_slow, n_fast)             wave_indices(w)
  *   GR_PROBE_BX_EIGEN_SLOW/_FAST      out (D.xyz)              eigen_polarizations(w).0 / .1
- *   GR_PROBE_BX_POYNTING_SLOW/_FAST   out (S.xyz)              mode_poynting_dir(w, slow / fast) */
+ *   GR_PROBE_BX_POYNTING_SLOW/_FAST   out (S.xyz)              mode_poynting_dir(w, slow / fast)
+ * The integer sampling streams (optics/raytracer/sampling.rs:12-271), u32 values passed and
+ * returned as their bit patterns in the float arrays:
+ *   GR_PROBE_SAMPLE_DRAWS     in (pixel, sample_num)             out (jx, jy, hero) of sample_draws,
+ *                             the radical inverses (bases 3, 5) taken from the host's sample table
+ *   GR_PROBE_SEED             in (pixel, sample_num, x)          out (path seed, hash_u32(x),
+ *                                                                 x with its bits reversed)
+ *   GR_PROBE_STREAM_DRAW      in (seed, bounce, salt0..2)        out (the three per-bounce draws)
+ *   GR_PROBE_RGB_SPECTRAL     in (r, g, b, lambda_nm)            out (rgb_to_spectral_radiance,
+ *                                                                 renderer/env_map_spectrum.rs:60-82) */
 enum { GR_PROBE_NEFF = 0, GR_PROBE_POYNTING = 1, GR_PROBE_TIR_RETARDATION = 2, GR_PROBE_FRAME_ROTATION = 3,
        GR_PROBE_FIELD_QFORM = 4, GR_PROBE_EXP = 5, GR_PROBE_CMF = 6, GR_PROBE_ACOS = 7,
        GR_PROBE_BX_WAVE_INDICES = 8, GR_PROBE_BX_EIGEN_SLOW = 9, GR_PROBE_BX_EIGEN_FAST = 10,
-       GR_PROBE_BX_POYNTING_SLOW = 11, GR_PROBE_BX_POYNTING_FAST = 12, GR_PROBE_COUNT = 13 };
+       GR_PROBE_BX_POYNTING_SLOW = 11, GR_PROBE_BX_POYNTING_FAST = 12, GR_PROBE_SAMPLE_DRAWS = 13,
+       GR_PROBE_SEED = 14, GR_PROBE_STREAM_DRAW = 15, GR_PROBE_RGB_SPECTRAL = 16, GR_PROBE_COUNT = 17 };
 gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out);
 
 /* Statistics of the last gr_accumulate / gr_trace_samples / gr_render call. */

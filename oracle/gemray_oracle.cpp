@@ -1538,6 +1538,44 @@ extern "C" void or_trace_spectral_ray(const float origin[3], const float dir[3],
 // ---------------------------------------------------------------------------------
 
 extern "C" uint32_t or_hash_u32(uint32_t x) { return hash_u32(x); }
+// The per-bounce draw of one stream (sampling.rs:45-65), the base-2 van der Corput value
+// (:139-147) and the Cranley-Patterson rotation (:189-197), for the integer-exact probes.
+extern "C" float or_stream_draw(uint32_t seed, uint32_t bounce, uint32_t stream) {
+    return stream_draw(seed, bounce, stream);
+}
+extern "C" float or_low_discrepancy_base2(uint32_t n) { return low_discrepancy_base2(n); }
+extern "C" float or_cranley_patterson_rotate(float x, float offset) { return cranley_patterson_rotate(x, offset); }
+// The RNG record of renderer/gpu/rng_check.rs:1-35 for n (pixel, sample, bounce) tuples, as
+// 32-bit patterns, out[(8 + n_salts) * i + k]: the path seed, the three pixel-rotation hashes
+// hash_u32(pixel ^ ROTATION_STREAM), the sample's reversed bits, the stratified draws
+// (jx, jy, hero) of sample_draws, then stream_draw(seed, bounce, salt) for each salt.
+extern "C" void or_rng_records(size_t n, const uint32_t *pixel, const uint32_t *sample, const uint32_t *bounce,
+                               const uint32_t *salts, int n_salts, uint32_t *out) {
+    auto bits = [](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, sizeof(u));
+        return u;
+    };
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t *o = out + (size_t)(8 + n_salts) * i;
+        float rot[3];
+        pixel_rotations(pixel[i], rot);
+        const Draws d = sample_draws(pixel[i], sample[i], rot);
+        o[0] = d.seed;
+        o[1] = hash_u32(pixel[i] ^ PIXEL_JITTER_X_ROTATION_STREAM);
+        o[2] = hash_u32(pixel[i] ^ PIXEL_JITTER_Y_ROTATION_STREAM);
+        o[3] = hash_u32(pixel[i] ^ HERO_WAVELENGTH_ROTATION_STREAM);
+        o[4] = reverse_bits(sample[i]);
+        o[5] = bits(d.jx);
+        o[6] = bits(d.jy);
+        o[7] = bits(d.hero);
+        for (int k = 0; k < n_salts; ++k) o[8 + k] = bits(stream_draw(d.seed, bounce[i], salts[k]));
+    }
+}
+// rgb_to_spectral_radiance (renderer/env_map_spectrum.rs:60-82).
+extern "C" float or_rgb_to_spectral_radiance(const float rgb[3], float lambda_nm) {
+    return rgb_to_spectral(rgb, lambda_nm);
+}
 extern "C" void or_pixel_rotations(uint32_t pixel, float out[3]) { pixel_rotations(pixel, out); }
 extern "C" void or_sample_draws(uint32_t pixel, uint32_t sample_num, uint32_t *seed,
                                 float out[3]) {

@@ -13,6 +13,7 @@
 #ifndef GEMRAY_ORACLE_H
 #define GEMRAY_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -87,6 +88,12 @@ typedef struct {
 } or_path_info;
 
 uint32_t or_hash_u32(uint32_t x);
+float or_stream_draw(uint32_t seed, uint32_t bounce, uint32_t stream);
+float or_low_discrepancy_base2(uint32_t n);
+float or_cranley_patterson_rotate(float x, float offset);
+float or_rgb_to_spectral_radiance(const float rgb[3], float lambda_nm);
+void or_rng_records(size_t n, const uint32_t *pixel, const uint32_t *sample, const uint32_t *bounce,
+                    const uint32_t *salts, int n_salts, uint32_t *out);
 void or_pixel_rotations(uint32_t pixel, float out[3]);
 void or_sample_draws(uint32_t pixel, uint32_t sample_num, uint32_t *seed, float out[3]);
 float or_radical_inverse(uint32_t n, uint32_t base);

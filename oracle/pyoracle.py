@@ -102,6 +102,17 @@ def _declare(L):
     u32 = ctypes.c_uint32
     L.or_hash_u32.argtypes = [u32]
     L.or_hash_u32.restype = u32
+    L.or_stream_draw.argtypes = [u32, u32, u32]
+    L.or_stream_draw.restype = ctypes.c_float
+    L.or_low_discrepancy_base2.argtypes = [u32]
+    L.or_low_discrepancy_base2.restype = ctypes.c_float
+    L.or_cranley_patterson_rotate.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.or_cranley_patterson_rotate.restype = ctypes.c_float
+    L.or_rgb_to_spectral_radiance.argtypes = [fp, ctypes.c_float]
+    L.or_rgb_to_spectral_radiance.restype = ctypes.c_float
+    u32p = ctypes.POINTER(u32)
+    L.or_rng_records.argtypes = [ctypes.c_size_t, u32p, u32p, u32p, u32p, ctypes.c_int, u32p]
+    L.or_rng_records.restype = None
     L.or_sample_draws.argtypes = [u32, u32, ctypes.POINTER(u32), fp]
     L.or_radical_inverse.argtypes = [u32, u32]
     L.or_radical_inverse.restype = ctypes.c_float
@@ -428,3 +439,23 @@ def render(camera: OrCamera, width, height, planes, material_struct, max_bounces
         info.ctypes.data if info is not None else None, ev,
     )
     return accum, xyz, info, int(segs)
+
+
+def rng_records(pixel, sample, bounce, salts) -> np.ndarray:
+    """or_rng_records: (n, 8 + len(salts)) uint32 -- seed, the three pixel-rotation hashes,
+    the sample's reversed bits, the bit patterns of jx, jy, hero and of each salt's
+    per-bounce draw (renderer/gpu/rng_check.rs:1-35)."""
+    p = np.ascontiguousarray(pixel, np.uint32)
+    s = np.ascontiguousarray(sample, np.uint32)
+    b = np.ascontiguousarray(bounce, np.uint32)
+    sl = np.ascontiguousarray(salts, np.uint32)
+    out = np.zeros((len(p), 8 + len(sl)), np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    lib().or_rng_records(len(p), p.ctypes.data_as(u32p), s.ctypes.data_as(u32p), b.ctypes.data_as(u32p),
+                         sl.ctypes.data_as(u32p), len(sl), out.ctypes.data_as(u32p))
+    return out
+
+
+def rgb_to_spectral_radiance(rgb, lambda_nm) -> float:
+    """rgb_to_spectral_radiance (renderer/env_map_spectrum.rs:60-82)."""
+    return lib().or_rgb_to_spectral_radiance(fptr(f32arr(rgb)), float(lambda_nm))

@@ -18,8 +18,8 @@ through
 import numpy as np
 import pytest
 
-import bench
 from gemray_studio_amd import backend as B
+from gemray_studio_amd import workloads
 from gemray_studio_amd import scenes
 from scene_util import Case
 
@@ -42,15 +42,15 @@ def gpu():
 
 
 def single_job_case():
-    scene, cam = bench.single_job_scene()
+    scene, cam = workloads.single_job_scene()
     return Case("single-job", scene.material, scene.planes, scene.environment, scene.width, scene.height,
                 scene.max_bounces, camera=cam)
 
 
 def stress_case():
-    return Case("stress", scenes.stress_material(), scenes.step_cut(),
-                B.HdrMap(scenes.procedural_env(seed=2, width=2048, height=1024)), 4096, 4096, 64,
-                camera=(0.35, 0.28, 4.0, 24.0))
+    scene, cam = workloads.stress_scene()
+    return Case("stress", scene.material, scene.planes, scene.environment, scene.width, scene.height,
+                scene.max_bounces, camera=cam)
 
 
 def window_parity(gpu, case, rows, spp):

@@ -179,7 +179,7 @@ def test_per_path_parity_with_oracle(gpu, srb, idx):
     assert stats["segments"] == int(info_g[:, 0].sum())
     assert int(info_g[same, 0].sum()) == int(seg_o[same].sum())
     if frac == 1.0:
-        gpu_ev = [stats["events"][n] for n in B._lib.EV_NAMES]
+        gpu_ev = [stats["events"][n] for n in po.OR_EV_NAMES]
         assert gpu_ev == [int(v) for v in ev]
 
 
@@ -322,22 +322,48 @@ def test_lossless_scattering_furnace(gpu, srb):
     assert np.all(np.abs((mg - mo) / se) <= 4.0), (mg, mo, se)
 
 
-def test_statistical_image_comparison(gpu, srb):
+def _t3_cases(srb):
+    env = B.HdrMap(scenes.procedural_env(5, 256, 128))
+    return [
+        Case("t3-pleochroic-oval", scenes.pleochroic_sapphire(7), scenes.oval_brilliant(), env, 32, 32, 32),
+        # The biaxial and scattering stones, whose per-path tiers allow a rare matched path to
+        # move by up to 1e-2 near an optic axis: T3 shows that no bias hides behind them.
+        Case("t3-alexandrite", M.by_name("Alexandrite"), srb, env, 32, 32, 16),
+        Case("t3-tanzanite-oval", dataclasses.replace(M.by_name("Tanzanite"), absorption_path_scale=0.6),
+             scenes.oval_brilliant(), env, 32, 32, 32),
+        Case("t3-ruby-scattering", dataclasses.replace(M.by_name("Ruby"), scattering_sigma_s=0.8, scattering_g=-0.3,
+                                                       absorption_path_scale=0.7), srb, env, 32, 32, 16),
+        Case("t3-tanzanite-scattering-oval", dataclasses.replace(M.by_name("Tanzanite"), scattering_sigma_s=2.0,
+                                                                 scattering_g=0.6),
+             scenes.oval_brilliant(), env, 32, 32, 24),
+    ]
+
+
+@pytest.mark.parametrize("idx", range(5), ids=["pleochroic-oval", "alexandrite", "tanzanite-oval", "ruby-scattering",
+                                               "tanzanite-scattering-oval"])
+def test_statistical_image_comparison(gpu, srb, idx):
     """T3 on disjoint samples: 32x32 px, 256 spp per engine; per-pixel Welch z; pass if
-    |z| > 3 pixels <= max(5 x binomial expectation, 10) (image_comparison.rs:52-58)."""
-    pleo = scenes.pleochroic_sapphire(7)
-    case = Case("t3", pleo, scenes.oval_brilliant(), B.HdrMap(scenes.procedural_env(5, 256, 128)), 32, 32, 32)
+    |z| > 3 pixels <= max(5 x binomial expectation, 10) (image_comparison.rs:52-58); and the
+    frame's mean X, Y, Z within 4 standard errors of each other (an unbiased estimator)."""
+    case = _t3_cases(srb)[idx]
     spp = 256
     n_pix = case.width * case.height
+    print(f"[t3] {case.name}", flush=True)
     xyz_g, _ = gpu.trace_samples(case.product_scene(), 0, n_pix, 0, spp, with_info=False)
     _, xyz_o, _, _ = case.oracle_render(spp, spp, per_sample=True)  # disjoint sample range
-    g = xyz_g.astype(np.float64).reshape(n_pix, spp, 3)[:, :, 1]
-    o = xyz_o.astype(np.float64).reshape(n_pix, spp, 3)[:, :, 1]
+    g3 = xyz_g.astype(np.float64).reshape(n_pix, spp, 3)
+    o3 = xyz_o.astype(np.float64).reshape(n_pix, spp, 3)
+    g, o = g3[:, :, 1], o3[:, :, 1]
     se = np.sqrt(g.var(1, ddof=1) / spp + o.var(1, ddof=1) / spp)
     z = np.where(se > 0, (g.mean(1) - o.mean(1)) / np.maximum(se, 1e-30), 0.0)
     n_bad = int((np.abs(z) > 3.0).sum())
     expected = n_pix * 0.0027
-    assert n_bad <= max(5 * expected, 10), n_bad
+    assert n_bad <= max(5 * expected, 10), (case.name, n_bad)
+    gm, om = g3.reshape(-1, 3), o3.reshape(-1, 3)
+    se_tot = np.sqrt(gm.var(0, ddof=1) / len(gm) + om.var(0, ddof=1) / len(om))
+    z_tot = (gm.mean(0) - om.mean(0)) / se_tot
+    print(f"[t3] {case.name}: {n_bad} pixels |z| > 3, frame z {np.round(z_tot, 2).tolist()}", flush=True)
+    assert np.all(np.abs(z_tot) <= 4.0), (case.name, z_tot)
 
 
 def test_declines_leave_film_untouched(gpu, srb):
