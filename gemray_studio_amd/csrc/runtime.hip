@@ -425,9 +425,109 @@ void free_scene(gr_scene_dev *s) {
     s->cert = nullptr;
 }
 
+// The device buffers one traced call uses besides the scene and the film: per-tuple XYZ
+// staging (xyz_cap tuples), per-tuple info, escape records (wavefront mode), continuation
+// records (two halves of kContFields x cont_cap floats) and their counters, the call's
+// sample table (radical inverses of its samples) and the event counters.
+struct Workspace {
+    float *xyz = nullptr;
+    uint64_t xyz_cap = 0;
+    uint32_t *info = nullptr;
+    float *rec = nullptr;
+    uint64_t rec_cap = 0;
+    float *cont = nullptr;
+    uint32_t cont_cap = 0;
+    uint32_t *cont_count = nullptr;
+    const float2 *ri35 = nullptr;
+    unsigned long long *stats = nullptr;
+};
+
 // Traces pixels [pixel_begin, pixel_begin + n_pixels) x samples [sample_offset, +spp) in
-// launches of at most chunk_tuples tuples.  With film != NULL each pixel's ordered sum is
-// added into film; with xyz_host/info_host != NULL per-tuple outputs are copied out.
+// launches of at most chunk_pixels pixels, with the buffers of w.  With film != NULL each
+// pixel's ordered sum is added into film; with xyz_host/info_host != NULL per-tuple outputs
+// are copied out.  Chunks are queued ahead: the host waits for a chunk's timing events only
+// when their slot of the ring comes round again, or at the end.
+gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, uint32_t n_pixels,
+                       uint32_t sample_offset, uint32_t spp, float *film, float *xyz_host, uint32_t *info_host,
+                       const Workspace &w, uint64_t chunk_pixels, bool wavefront, uint32_t split_bounce) {
+    GR_CHECK(hipMemsetAsync(w.stats, 0, GR_EV_COUNT * sizeof(unsigned long long), ctx->stream));
+    GR_CHECK(hipEventRecord(ctx->ev_begin, ctx->stream));
+    double kernel_ms = 0.0, finish_ms = 0.0;
+    uint32_t queued = 0;  // chunks whose timing events have not been read
+    auto read_slot = [&](uint32_t k) -> gr_status {
+        const gr_ctx::ChunkEvents &e = ctx->chunk_ev[k % gr_ctx::kChunkRing];
+        GR_CHECK(hipEventSynchronize(e.k2));
+        float ms = 0.0f, fms = 0.0f;
+        GR_CHECK(hipEventElapsedTime(&ms, e.k0, e.k1));
+        GR_CHECK(hipEventElapsedTime(&fms, e.k1, e.k2));
+        kernel_ms += ms;
+        finish_ms += fms;
+        return GR_OK;
+    };
+    uint32_t chunk = 0;
+    for (uint64_t done = 0; done < n_pixels; done += chunk_pixels, ++chunk) {
+        uint32_t np = (uint32_t)((n_pixels - done) < chunk_pixels ? (n_pixels - done) : chunk_pixels);
+        uint32_t first = pixel_begin + (uint32_t)done;
+        uint32_t n_tuples = np * spp;
+        if (chunk >= gr_ctx::kChunkRing) {
+            gr_status st = read_slot(chunk - gr_ctx::kChunkRing);
+            if (st != GR_OK) return st;
+            --queued;
+        }
+        const gr_ctx::ChunkEvents &e = ctx->chunk_ev[chunk % gr_ctx::kChunkRing];
+        gr::TraceArgs args{};
+        args.planes = s->planes;
+        args.ri35 = w.ri35;
+        args.cert = s->cert;
+        args.pixel_begin = first;
+        args.sample_offset = sample_offset;
+        args.spp = spp;
+        args.n_tuples = n_tuples;
+        args.xyz = w.xyz;
+        args.info = info_host ? w.info : nullptr;
+        args.stats = w.stats;
+        args.wavefront = wavefront;
+        args.rec = w.rec;
+        args.rec_cap = (uint32_t)w.rec_cap;
+        args.trace_done = wavefront ? e.k1 : nullptr;
+        args.split_bounce = wavefront ? 0u : split_bounce;
+        args.cont = w.cont;
+        args.cont_cap = w.cont_cap;
+        args.cont_count = w.cont_count;
+        GR_CHECK(hipEventRecord(e.k0, ctx->stream));
+        GR_CHECK(gr_launch_trace(s->sc, args, ctx->stream));
+        if (!wavefront) GR_CHECK(hipEventRecord(e.k1, ctx->stream));
+        GR_CHECK(hipEventRecord(e.k2, ctx->stream));
+        if (film) GR_CHECK(gr_launch_film_reduce(w.xyz, np, spp, first, film, ctx->stream));
+        if (xyz_host)
+            GR_CHECK(hipMemcpyAsync(xyz_host + 3 * done * spp, w.xyz, (size_t)n_tuples * 3 * sizeof(float),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        if (info_host)
+            GR_CHECK(hipMemcpyAsync(info_host + 4 * done * spp, w.info, (size_t)n_tuples * 4 * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        ++queued;
+        ctx->last.launches++;
+    }
+    for (uint32_t k = chunk - queued; k < chunk; ++k) {
+        gr_status st = read_slot(k);
+        if (st != GR_OK) return st;
+    }
+    GR_CHECK(hipEventRecord(ctx->ev_end, ctx->stream));
+    GR_CHECK(hipStreamSynchronize(ctx->stream));
+    float total = 0.0f;
+    GR_CHECK(hipEventElapsedTime(&total, ctx->ev_begin, ctx->ev_end));
+    unsigned long long ev[GR_EV_COUNT];
+    GR_CHECK(hipMemcpy(ev, w.stats, sizeof(ev), hipMemcpyDeviceToHost));
+    for (int i = 0; i < GR_EV_COUNT; ++i) ctx->last.events[i] = ev[i];
+    ctx->last.paths = (uint64_t)n_pixels * spp;
+    ctx->last.trace_kernel_ms = kernel_ms;
+    ctx->last.finish_kernel_ms = finish_ms;
+    ctx->last.total_ms = total;
+    return GR_OK;
+}
+
+// Traces with the context's own buffers, grown on demand (gr_render, gr_accumulate,
+// gr_trace_samples): launches of at most chunk_tuples tuples.
 gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, uint32_t n_pixels,
                     uint32_t sample_offset, uint32_t spp, float *film, float *xyz_host,
                     uint32_t *info_host) {
@@ -455,63 +555,27 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     }
     st = ensure_ri35(ctx, sample_offset, spp);
     if (st != GR_OK) return st;
-    GR_CHECK(hipMemsetAsync(ctx->stats, 0, GR_EV_COUNT * sizeof(unsigned long long), ctx->stream));
-    GR_CHECK(hipEventRecord(ctx->ev_begin, ctx->stream));
-    double kernel_ms = 0.0, finish_ms = 0.0;
-    for (uint64_t done = 0; done < n_pixels; done += chunk_pixels) {
-        uint32_t np = (uint32_t)((n_pixels - done) < chunk_pixels ? (n_pixels - done) : chunk_pixels);
-        uint32_t first = pixel_begin + (uint32_t)done;
-        uint32_t n_tuples = np * spp;
-        gr::TraceArgs args{};
-        args.planes = s->planes;
-        args.ri35 = ctx->ri35;
-        args.cert = s->cert;
-        args.pixel_begin = first;
-        args.sample_offset = sample_offset;
-        args.spp = spp;
-        args.n_tuples = n_tuples;
-        args.xyz = ctx->xyz;
-        args.info = info_host ? ctx->info : nullptr;
-        args.stats = ctx->stats;
-        args.wavefront = wavefront;
-        args.rec = ctx->rec;
-        args.rec_cap = (uint32_t)ctx->rec_cap;
-        args.trace_done = wavefront ? ctx->ev_k1 : nullptr;
-        args.split_bounce = wavefront ? 0u : split_bounce;
-        args.cont = ctx->cont;
-        args.cont_cap = ctx->cont_cap;
-        args.cont_count = ctx->cont_count;
-        GR_CHECK(hipEventRecord(ctx->ev_k0, ctx->stream));
-        GR_CHECK(gr_launch_trace(s->sc, args, ctx->stream));
-        if (!wavefront) GR_CHECK(hipEventRecord(ctx->ev_k1, ctx->stream));
-        GR_CHECK(hipEventRecord(ctx->ev_k2, ctx->stream));
-        if (film) GR_CHECK(gr_launch_film_reduce(ctx->xyz, np, spp, first, film, ctx->stream));
-        if (xyz_host)
-            GR_CHECK(hipMemcpyAsync(xyz_host + 3 * done * spp, ctx->xyz, (size_t)n_tuples * 3 * sizeof(float),
-                                    hipMemcpyDeviceToHost, ctx->stream));
-        if (info_host)
-            GR_CHECK(hipMemcpyAsync(info_host + 4 * done * spp, ctx->info, (size_t)n_tuples * 4 * sizeof(uint32_t),
-                                    hipMemcpyDeviceToHost, ctx->stream));
-        GR_CHECK(hipEventSynchronize(ctx->ev_k2));
-        float ms = 0.0f, fms = 0.0f;
-        GR_CHECK(hipEventElapsedTime(&ms, ctx->ev_k0, ctx->ev_k1));
-        GR_CHECK(hipEventElapsedTime(&fms, ctx->ev_k1, ctx->ev_k2));
-        kernel_ms += ms;
-        finish_ms += fms;
-        ctx->last.launches++;
-    }
-    GR_CHECK(hipEventRecord(ctx->ev_end, ctx->stream));
-    GR_CHECK(hipStreamSynchronize(ctx->stream));
-    float total = 0.0f;
-    GR_CHECK(hipEventElapsedTime(&total, ctx->ev_begin, ctx->ev_end));
-    unsigned long long ev[GR_EV_COUNT];
-    GR_CHECK(hipMemcpy(ev, ctx->stats, sizeof(ev), hipMemcpyDeviceToHost));
-    for (int i = 0; i < GR_EV_COUNT; ++i) ctx->last.events[i] = ev[i];
-    ctx->last.paths = (uint64_t)n_pixels * spp;
-    ctx->last.trace_kernel_ms = kernel_ms;
-    ctx->last.finish_kernel_ms = finish_ms;
-    ctx->last.total_ms = total;
-    return GR_OK;
+    Workspace w;
+    w.xyz = ctx->xyz;
+    w.xyz_cap = ctx->xyz_cap;
+    w.info = ctx->info;
+    w.rec = ctx->rec;
+    w.rec_cap = ctx->rec_cap;
+    w.cont = ctx->cont;
+    w.cont_cap = ctx->cont_cap;
+    w.cont_count = ctx->cont_count;
+    w.ri35 = ctx->ri35;
+    w.stats = ctx->stats;
+    return trace_chunks(ctx, s, pixel_begin, n_pixels, sample_offset, spp, film, xyz_host, info_host, w,
+                        chunk_pixels, wavefront, split_bounce);
+}
+
+bool create_chunk_events(gr_ctx *ctx) {
+    for (gr_ctx::ChunkEvents &c : ctx->chunk_ev)
+        if (hipEventCreate(&c.k0) != hipSuccess || hipEventCreate(&c.k1) != hipSuccess ||
+            hipEventCreate(&c.k2) != hipSuccess)
+            return false;
+    return true;
 }
 
 }  // namespace
@@ -541,10 +605,10 @@ gr_status gr_open(gr_ctx **out, const gr_options *opts) {
     else
         std::snprintf(ctx->label, sizeof(ctx->label), "HIP device %d, %d CUs (%s, HIP)", dev,
                       prop.multiProcessorCount, prop.gcnArchName);
-    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+    if (hipSetDevice(dev) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev_begin) != hipSuccess || hipEventCreate(&ctx->ev_end) != hipSuccess ||
-        hipEventCreate(&ctx->ev_k0) != hipSuccess || hipEventCreate(&ctx->ev_k1) != hipSuccess ||
-        hipEventCreate(&ctx->ev_k2) != hipSuccess ||
+        !create_chunk_events(ctx) ||
         hipMalloc(&ctx->stats, GR_EV_COUNT * sizeof(unsigned long long)) != hipSuccess) {
         gr_close(ctx);
         return GR_ERR_DEVICE;
@@ -566,8 +630,11 @@ void gr_close(gr_ctx *ctx) {
     if (ctx->cont_count) (void)hipFree(ctx->cont_count);
     if (ctx->mx_buf) (void)hipFree(ctx->mx_buf);
     if (ctx->stats) (void)hipFree(ctx->stats);
-    for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_k0, ctx->ev_k1, ctx->ev_k2})
+    for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end})
         if (e) (void)hipEventDestroy(e);
+    for (gr_ctx::ChunkEvents &c : ctx->chunk_ev)
+        for (hipEvent_t e : {c.k0, c.k1, c.k2})
+            if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

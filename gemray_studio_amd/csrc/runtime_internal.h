@@ -31,7 +31,13 @@ void serve_shutdown(gr_ctx *ctx);
 struct gr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_k2 = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    // Timing events of the chunks in flight (trace start, trace end, finish end): a call
+    // queues up to kChunkRing chunks ahead of the one whose times it reads.
+    static constexpr uint32_t kChunkRing = 4;
+    struct ChunkEvents {
+        hipEvent_t k0 = nullptr, k1 = nullptr, k2 = nullptr;
+    } chunk_ev[kChunkRing];
     std::mutex mu;
     uint64_t chunk_tuples = 0;
     int trace_mode = GR_TRACE_PLAIN;

@@ -143,5 +143,7 @@ hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices
 // Adds each slice's per-pixel ordered sample sums into its job's film.
 hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
                                        hipStream_t stream);
+// Fills ri35[0, n) with the radical inverses (bases 3, 5) of samples offset + i.
+hipError_t gr_launch_ri35(uint32_t offset, uint32_t n, float2 *ri35, hipStream_t stream);
 hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t spp,
                                  uint32_t pixel_begin, float *film, hipStream_t stream);

@@ -189,6 +189,15 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__re
     }
 }
 
+// The sample table of a call on the device: ri35[i] = the radical inverses (bases 3 and 5) of
+// sample offset + i (radical_inverse_base, sampling.rs:170-181), the operations the host's
+// table takes.
+__global__ __launch_bounds__(kBlock) void ri35_kernel(uint32_t offset, uint32_t n, float2 *__restrict__ ri35) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    ri35[i] = make_float2(radical_inverse(offset + i, 3u), radical_inverse(offset + i, 5u));
+}
+
 // Each pixel's spp tuples summed in sample order and ADDED into the film
 // (frame.rs:799-814: sum = 0; sum += xyz[s] for s in order; accum[pixel] += sum).
 __global__ __launch_bounds__(kBlock) void film_reduce_kernel(const float *__restrict__ xyz,
@@ -297,5 +306,12 @@ hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t s
     dim3 grid((n_pixels + gr::kBlock - 1) / gr::kBlock);
     hipLaunchKernelGGL(gr::film_reduce_kernel, grid, dim3(gr::kBlock), 0, stream, xyz, n_pixels, spp,
                        pixel_begin, film);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_ri35(uint32_t offset, uint32_t n, float2 *ri35, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gr::ri35_kernel, dim3((n + gr::kBlock - 1) / gr::kBlock), dim3(gr::kBlock), 0, stream, offset,
+                       n, ri35);
     return hipGetLastError();
 }
