@@ -570,6 +570,21 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
                         chunk_pixels, wavefront, split_bounce);
 }
 
+// gr_render_noalloc's workspace, one fixed device array per GPU (56 MB): XYZ staging for a
+// launch of kNoAllocTuples tuples, continuation records for an eighth of them (a path that
+// finds the records full carries on in its launch, so results do not depend on the
+// capacity), a sample table for GR_MAX_SAMPLES_PER_CALL samples, the event counters and the
+// continuation counters, each section 256-byte aligned.
+constexpr uint64_t kNoAllocTuples = 1ull << 19;
+constexpr uint32_t kNoAllocContCap = (uint32_t)(kNoAllocTuples / 8);
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kScratchXyz = 0;
+constexpr size_t kScratchCont = kScratchXyz + align256(kNoAllocTuples * 3 * sizeof(float));
+constexpr size_t kScratchRi35 = kScratchCont + align256(2ull * kNoAllocContCap * gr::kContFields * sizeof(float));
+constexpr size_t kScratchStats = kScratchRi35 + align256((size_t)GR_MAX_SAMPLES_PER_CALL * sizeof(float2));
+constexpr size_t kScratchCounts = kScratchStats + align256(GR_EV_COUNT * sizeof(unsigned long long));
+constexpr size_t kScratchBytes = kScratchCounts + align256((gr::kContLevels + 1) * sizeof(uint32_t));
+
 bool create_chunk_events(gr_ctx *ctx) {
     for (gr_ctx::ChunkEvents &c : ctx->chunk_ev)
         if (hipEventCreate(&c.k0) != hipSuccess || hipEventCreate(&c.k1) != hipSuccess ||
@@ -579,6 +594,10 @@ bool create_chunk_events(gr_ctx *ctx) {
 }
 
 }  // namespace
+
+// The host-side name of this array is a stand-in, not its device address: host code uses the
+// address hipGetSymbolAddress returns for the current device (gr_ctx::scratch) only.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 extern "C" {
 
@@ -606,6 +625,7 @@ gr_status gr_open(gr_ctx **out, const gr_options *opts) {
         std::snprintf(ctx->label, sizeof(ctx->label), "HIP device %d, %d CUs (%s, HIP)", dev,
                       prop.multiProcessorCount, prop.gcnArchName);
     if (hipSetDevice(dev) != hipSuccess ||
+        hipGetSymbolAddress((void **)&ctx->scratch, HIP_SYMBOL(scratch)) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev_begin) != hipSuccess || hipEventCreate(&ctx->ev_end) != hipSuccess ||
         !create_chunk_events(ctx) ||
@@ -861,6 +881,38 @@ gr_status gr_render(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offs
     std::lock_guard<std::mutex> g(ctx->mu);
     GR_CHECK(hipSetDevice(ctx->device));
     return run_trace(ctx, scene, 0, film->width * film->height, sample_offset, spp, film->d, nullptr, nullptr);
+}
+
+gr_status gr_render_noalloc(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset, uint32_t spp,
+                            gr_film *film) {
+    if (!ctx || !scene || !film) return GR_ERR_INVALID;
+    if (film->width != scene->sc.width || film->height != scene->sc.height) return GR_ERR_INVALID;
+    if (spp > GR_MAX_SAMPLES_PER_CALL) {
+        std::fprintf(stderr, "gemray_hip: gr_render_noalloc: %u samples per call exceed its sample table (%u)\n",
+                     spp, GR_MAX_SAMPLES_PER_CALL);
+        return GR_ERR_INVALID;
+    }
+    if (!ctx->scratch) return GR_ERR_DEVICE;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GR_CHECK(hipSetDevice(ctx->device));
+    std::memset(&ctx->last, 0, sizeof(ctx->last));
+    ctx->last.n_planes = scene->sc.n_planes;
+    const uint32_t n_pixels = film->width * film->height;
+    if (spp == 0) return GR_OK;
+    uint64_t chunk_pixels = kNoAllocTuples / spp;
+    if (chunk_pixels > n_pixels) chunk_pixels = n_pixels;
+    Workspace w;
+    w.xyz = reinterpret_cast<float *>(ctx->scratch + kScratchXyz);
+    w.xyz_cap = kNoAllocTuples;
+    w.cont = reinterpret_cast<float *>(ctx->scratch + kScratchCont);
+    w.cont_cap = kNoAllocContCap;
+    w.cont_count = reinterpret_cast<uint32_t *>(ctx->scratch + kScratchCounts);
+    float2 *ri35 = reinterpret_cast<float2 *>(ctx->scratch + kScratchRi35);
+    w.ri35 = ri35;
+    w.stats = reinterpret_cast<unsigned long long *>(ctx->scratch + kScratchStats);
+    GR_CHECK(gr_launch_ri35(sample_offset, spp, ri35, ctx->stream));
+    return trace_chunks(ctx, scene, 0, n_pixels, sample_offset, spp, film->d, nullptr, nullptr, w, chunk_pixels,
+                        false, ctx->split_bounce.load(std::memory_order_relaxed));
 }
 
 gr_status gr_accumulate(gr_ctx *ctx, const gr_scene *scene, uint32_t sample_offset, uint32_t spp,

@@ -57,6 +57,9 @@ struct gr_ctx {
     float *cont = nullptr;      // continuation records, kContFields x cont_cap
     uint32_t cont_cap = 0;
     uint32_t *cont_count = nullptr;
+    // This device's copy of the library's fixed scratch array (gr_render_noalloc), from
+    // hipGetSymbolAddress at gr_open with the device current.
+    unsigned char *scratch = nullptr;
     char *mx_buf = nullptr;     // gr_optical_metrics: planes, poses, per-ray records, results
     size_t mx_cap = 0;          // bytes
     gr_stats last{};

@@ -253,6 +253,17 @@ gr_status gr_film_download(gr_ctx *ctx, gr_film *film, float *xyz_out, size_t n_
 /* Adds samples [sample_offset, sample_offset + spp) of every pixel into film. */
 gr_status gr_render(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset,
                     uint32_t spp, gr_film *film);
+/* gr_render without any allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The per-tuple staging, continuation records, sample
+ * table and counters live in one fixed 56 MB device array per GPU (its address taken once
+ * per device at gr_open), so every call traces in launches of at most 2^19 tuples and
+ * calls no hipMalloc, hipFree or allocating library routine; spp above
+ * GR_MAX_SAMPLES_PER_CALL is rejected (GR_ERR_INVALID).  Results are bit-identical to
+ * gr_render's.  Calls on one context are serialized; calls that overlap on one GPU from
+ * different contexts share the array and race. */
+gr_status gr_render_noalloc(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset,
+                            uint32_t spp, gr_film *film);
 
 /* ---- Gemological metrics ------------------------------------------------------------
  * evaluate_gem_optical_metrics (crates/gemray/src/color/metrics.rs:1082-1248): brilliance,
