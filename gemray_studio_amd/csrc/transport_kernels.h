@@ -138,8 +138,15 @@ struct PathState {
     float hero_off;
     uint32_t seed, bounce, last_facet, cert;
     uint32_t term_bounce, term_reason, fh, fate_h;
+    // Channels zeroed by a refraction whose direction they did not share (bit k): their
+    // Stokes vector is +-0 and their pdf 0, and every later operation of the estimator keeps
+    // them so, so their per-channel updates are skipped -- when no lane of the wave needs a
+    // channel, the wave skips its work.  The hero (channel 0) is always traced.
+    uint32_t dead;
     bool inside, is_extra, have_prev, have_n, have_abs, escaped;
 };
+
+#define GR_LIVE(ps, k) ((k) == 0 || !(((ps).dead >> (k)) & 1u))
 
 // pixel_rotations + sample_draws (sampling.rs:232-271) with the sample-only radical
 // inverses (bases 3 and 5) taken from a host-built table, then Camera::generate_ray
@@ -183,6 +190,7 @@ __device__ __forceinline__ void init_path(PathState &ps, uint32_t pixel, uint32_
     ps.term_reason = 3u;
     ps.fh = 2166136261u;
     ps.fate_h = 2166136261u;
+    ps.dead = 0u;
     ps.inside = ps.is_extra = ps.have_prev = ps.have_n = ps.have_abs = ps.escaped = false;
 }
 
@@ -277,6 +285,7 @@ __device__ __forceinline__ void channel_alphas(const PathState &ps, const DevSce
         const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
+            if (!GR_LIVE(ps, c)) continue;
             const float ao = LC_AO(c), ab = LC_AB(c), ae = LC_AE(c);
             const float qf_e = qf_channel3(qi, ps.st[c].q, ps.st[c].u, ao, ab, ae);
             const float unpol = fmaf(ao, u_x, fmaf(ab, u_y, ae * u_z));
@@ -291,6 +300,7 @@ __device__ __forceinline__ void channel_alphas(const PathState &ps, const DevSce
         const float u_z = (ea_l.z * ea_l.z + eb_l.z * eb_l.z) * 0.5f;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
+            if (!GR_LIVE(ps, c)) continue;
             const float ao = LC_AO(c), ae = LC_AE(c);
             const float qf_e = qf_channel(qi, ps.st[c].q, ps.st[c].u, ao, ae);
             const float unpol = fmaf(ao, u_xy, ae * u_z);
@@ -314,7 +324,13 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     ev_inc(ev, GR_EV_SEGMENT);
 
     // ---- convex facet-plane clip (simd/slab.rs:94-152, intersect.rs:93-132) ----
-    float t_near = -1e30f, t_far = 1e30f;
+    // The entering extreme starts at the hit threshold 1e-4 instead of -1e30.  The scan's
+    // outcome reads t_near only through t_near > 1e-4 (then it is the hit, with its facet)
+    // and t_near > t_far; planes whose exact t is at most 1e-4 can change neither (if no
+    // entering plane exceeds 1e-4 the hit is t_far > 1e-4 or a miss, whatever t_near is), so
+    // the screen skips them: a ray inside the stone, whose entering planes all lie behind it,
+    // takes no division for them.  Same hit, t and facet as the reference's scan.
+    float t_near = 1e-4f, t_far = 1e30f;
     int near_idx = -1, far_idx = -1;
     bool outside = false;
     const V3 ro = ps.ray_o, rd = ps.ray_d;
@@ -400,6 +416,8 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         ev_inc(ev, GR_EV_SCATTER);
         const float sigma_s = sc.sigma_s;
         float alphas[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) alphas[k] = 0.0f;  // a dead channel's stays unused
         if (kAbsorb) {
             if (!ps.have_abs) {
                 ev_inc(ev, GR_EV_ABSORPTION_FILL);
@@ -408,10 +426,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             }
             channel_alphas<kBiax>(ps, sc, lcache, ps.cur_k, ps.have_prev ? ps.prev_pn : v3(0.0f, 0.0f, 0.0f),
                                   [&](int c, float a) { alphas[c] = a; });
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) alphas[k] = 0.0f;  // no band: every coefficient is zero
-        }
+        }  // no band: every coefficient is zero
         const float sigma_t_hero = alphas[0] + sigma_s;
         const float one_minus_u = fmaxf(1.0f - stream_draw(ps.seed, bounce, DISTANCE_SAMPLE_STREAM), 1e-7f);
         const float t_free = -(logf(one_minus_u)) / sigma_t_hero;
@@ -420,6 +435,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             const float pdf_hero = sigma_t_hero * one_minus_u;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
+                if (!GR_LIVE(ps, k)) continue;
                 const float sigma_t_k = alphas[k] + sigma_s;
                 const float tr_k = exp_lobe(-(alphas[k] + sigma_s) * t_free);
                 const float weight = tr_k * sigma_s / pdf_hero;
@@ -449,17 +465,19 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 }
                 const float inv_q = 1.0f / q;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) ps.st[k] = scale(ps.st[k], inv_q);
+                for (int k = 0; k < 8; ++k)
+                    if (GR_LIVE(ps, k)) ps.st[k] = scale(ps.st[k], inv_q);
             }
             ps.bounce = bounce + 1u;
             return ps.bounce >= sc.max_bounces;
         }
         float survive[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) survive[k] = exp_lobe(-(alphas[k] + sigma_s) * hit_t_scaled);
+        for (int k = 0; k < 8; ++k) survive[k] = GR_LIVE(ps, k) ? exp_lobe(-(alphas[k] + sigma_s) * hit_t_scaled) : 0.0f;
         const float survive_hero = fmaxf(survive[0], 1e-30f);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
+            if (!GR_LIVE(ps, k)) continue;
             ps.st[k] = scale(ps.st[k], survive[k] / survive_hero);
             ps.pdf[k] *= survive[k];
         }
@@ -488,7 +506,8 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         float c2, s2;
         frame_rotation_cs(ps.prev_pn, cpn, cur_k, &c2, &s2);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) ps.st[k] = apply_rotation(ps.st[k], c2, s2);
+        for (int k = 0; k < 8; ++k)
+            if (GR_LIVE(ps, k)) ps.st[k] = apply_rotation(ps.st[k], c2, s2);
     }
     ps.prev_pn = cpn;
     ps.have_prev = true;
@@ -665,6 +684,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             new_k = cosine_weighted_hemisphere(u1, u2, normal);
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
+                if (!GR_LIVE(ps, k)) continue;
                 const float i = intensity(ps.st[k]);
                 ps.st[k] = Stokes{tir ? i : i / r_unpol, 0.0f, 0.0f, 0.0f};
                 if (!tir) ps.pdf[k] *= r_unpol;
@@ -682,6 +702,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             const float t_unpol = 1.0f - r_unpol;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
+                if (!GR_LIVE(ps, k)) continue;
                 ps.st[k] = Stokes{intensity(ps.st[k]) / t_unpol, 0.0f, 0.0f, 0.0f};
                 ps.pdf[k] *= t_unpol;
             }
@@ -706,6 +727,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         const float inv_r = tir ? 1.0f : 1.0f / r_unpol;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
+            if (!GR_LIVE(ps, k)) continue;
             const float n1k = inside ? n_med[k] : 1.0f, n2k = inside ? 1.0f : n_med[k];
             if (s2t_ch[k] > 1.0f) {
                 float sd, cd;
@@ -766,6 +788,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         float inv_t = 1.0f / (1.0f - r_unpol);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
+            if (!GR_LIVE(ps, k)) continue;
             float n1k = inside ? n_med[k] : 1.0f;
             float n2k = (entering && !use_extra) ? (kBiax ? n_biax_a[k] : LC_NO(k)) : (inside ? 1.0f : n_med[k]);
             float q = n1k / n2k;
@@ -791,6 +814,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             } else {
                 ps.st[k] = scale(ps.st[k], 0.0f);
                 ps.pdf[k] = 0.0f;
+                ps.dead |= 1u << k;
             }
         }
         new_k = refr;
@@ -835,7 +859,8 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         }
         float inv_q = 1.0f / q;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) ps.st[k] = scale(ps.st[k], inv_q);
+        for (int k = 0; k < 8; ++k)
+            if (GR_LIVE(ps, k)) ps.st[k] = scale(ps.st[k], inv_q);
     }
     ps.bounce = bounce + 1u;
     return ps.bounce >= sc.max_bounces;
@@ -914,7 +939,8 @@ __device__ __forceinline__ uint32_t path_segments(const PathState &ps, const Dev
 // running at the split bounce, field-major (field f of record r at cont[f * cap + r]).
 // Fields: 0 tuple index, 1-12 ray_o, ray_d, cur_k, prev_pn, 13-44 Stokes, 45-52 pdfs,
 // 53-76 the LDS caches (n_o, ao, ae), 77 hero offset, 78-84 seed, bounce, last facet,
-// termination bounce and reason and the two path hashes, 85 the mode flags, and in the
+// termination bounce and reason and the two path hashes, 85 the mode flags (the dead-channel
+// mask in bits 8-15), and in the
 // kernels built for biaxial stones (kBx) 86-93 the beta-axis absorption cache.
 
 template <bool kBx>
@@ -951,7 +977,7 @@ __device__ __forceinline__ void save_path(const PathState &ps, uint32_t t, const
     putu(83, ps.fh);
     putu(84, ps.fate_h);
     putu(85, (ps.inside ? 1u : 0u) | (ps.is_extra ? 2u : 0u) | (ps.have_prev ? 4u : 0u) | (ps.have_n ? 8u : 0u) |
-                 (ps.have_abs ? 16u : 0u) | (ps.escaped ? 32u : 0u));
+                 (ps.have_abs ? 16u : 0u) | (ps.escaped ? 32u : 0u) | (ps.dead << 8));
     if (kBx) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) put(86 + k, LC_AB(k));
@@ -992,6 +1018,7 @@ __device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__
     ps.have_n = fl & 8u;
     ps.have_abs = fl & 16u;
     ps.escaped = fl & 32u;
+    ps.dead = (fl >> 8) & 0xFFu;
     if (kBx) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) LC_AB(k) = get(86 + k);

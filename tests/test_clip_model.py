@@ -38,12 +38,13 @@ def dot_rows(p, v):
     return ((p[..., 0] * v[..., 0] + p[..., 1] * v[..., 1]) + p[..., 2] * v[..., 2]).astype(F)
 
 
-def model_scan(planes, ro, rd, rng=None):
-    """clip_scan<false> for a batch of rays (ro, rd: (R, 3) float32).  The screen's fma
-    sign is evaluated in float64, where t_ext * denom is exact and adding side keeps the
-    exact sum's sign."""
+def model_scan(planes, ro, rd, rng=None, near_start=-1e30):
+    """clip_scan<false> for a batch of rays (ro, rd: (R, 3) float32), the entering extreme
+    starting at near_start (-1e30 as the reference; the transport kernel starts it at the
+    1e-4 hit threshold).  The screen's fma sign is evaluated in float64, where
+    t_ext * denom is exact and adding side keeps the exact sum's sign."""
     R = ro.shape[0]
-    t_near = np.full(R, F(-1e30))
+    t_near = np.full(R, F(near_start))
     t_far = np.full(R, F(1e30))
     near_idx = np.full(R, -1)
     far_idx = np.full(R, -1)
@@ -136,6 +137,38 @@ def test_screened_scan_selects_the_oracle_planes():
     assert total == 15000
     # The screen is worth having: most planes skip the division.
     assert divs < 0.25 * planes_seen
+
+
+def test_entering_extreme_floored_at_the_hit_threshold_selects_the_oracle_planes():
+    """The transport kernel starts t_near at 1e-4: entering planes whose exact t is at most
+    the hit threshold are screened out.  Same hit, t bits and facet as the oracle on exterior
+    rays, interior rays and rays leaving a facet (origin = hit point + 1e-4 along the new
+    direction, as the tracer continues a path), and interior scans take fewer divisions."""
+    rng = np.random.default_rng(5)
+    total = 0
+    for name, planes in STONES:
+        planes = with_near_duplicates(np.ascontiguousarray(planes, F), rng)
+        o, d = random_rays(rng, 1000)
+        hit0, t0, f0, _ = model_scan(planes, o, d)
+        # Continue the exterior hits from their facet: reflected or straight on, inside.
+        ok = hit0
+        p_hit = (o[ok] + d[ok] * t0[ok][:, None]).astype(F)
+        d2 = random_rays(rng, int(ok.sum()))[1]
+        o2 = (p_hit + d2 * F(1e-4)).astype(F)
+        inner = (rng.normal(size=(1000, 3)) * 0.15).astype(F)
+        o = np.concatenate([o, o2, inner]).astype(F)
+        d = np.concatenate([d, d2, random_rays(rng, 1000)[1]]).astype(F)
+        hit, t, facet, nd = model_scan(planes, o, d, near_start=1e-4)
+        _, _, _, nd_ref = model_scan(planes, o, d)
+        assert nd < nd_ref, name
+        for k in range(len(o)):
+            h_ref, t_ref, f_ref = oracle_intersect(planes, o[k], d[k])
+            assert hit[k] == h_ref, (name, k)
+            if h_ref:
+                assert t[k].view(np.uint32) == t_ref.view(np.uint32), (name, k)
+                assert facet[k] == f_ref, (name, k)
+        total += len(o)
+    assert total > 10000
 
 
 def test_exterior_shortcut_only_fires_on_misses():
