@@ -277,7 +277,54 @@ def job_workloads(gpu, env_map, args):
         "plain_segments_per_s": s_plain_segs / s_plain_s,
         "plain_fifo_wall_s": float(arrivals[-1] + plain_lat[-1]),
     }
-    return batched, served
+    previews = preview_workload(gpu, env_map, args)
+    return batched, served, previews
+
+
+def preview_workload(gpu, env_map, args):
+    """Interactive previews (workloads.preview_jobs): many small frames at few samples that
+    share one device-resident map, submitted all at once (batched) and as a Poisson stream at
+    1.5x the plain path's service rate (served), beside the plain per-job path (gr_accumulate
+    per job, the same device-resident map)."""
+    from gemray_studio_amd import backend as B
+    from gemray_studio_amd.workloads import preview_jobs
+
+    dev = gpu.upload_env_map(env_map)
+    try:
+        jobs = preview_jobs(B.HdrMap(env_map, device=dev), n_jobs=args.preview_jobs)
+        paths = sum(sc.width * sc.height * spp for sc, _, spp in jobs)
+        run_plain(gpu, jobs[:8])  # first-call setup outside the measurement
+        plain_s, service, plain_segs = run_plain(gpu, jobs)
+        print(f"bench: previews, {len(jobs)} jobs, plain per-job path {plain_s:.2f} s", file=sys.stderr, flush=True)
+        b_wall, b_lat, b_segs, b_m = run_batched(gpu, jobs)
+        if b_segs != plain_segs:
+            raise RuntimeError("preview jobs: batched and plain paths traced different work")
+        rate = 1.5 * len(jobs) / plain_s
+        rng = np.random.default_rng(7)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(jobs)))
+        arrivals -= arrivals[0]
+        s_wall, s_lat, s_segs, s_m = run_batched(gpu, jobs, arrivals=arrivals)
+        plain_lat = fifo_latencies(arrivals, service)
+        print(f"bench: previews, job loop {b_wall:.2f} s batched, {s_wall:.2f} s served", file=sys.stderr,
+              flush=True)
+    finally:
+        dev.release()
+    return {
+        "workload": f"{len(jobs)} preview jobs: 96^2-256^2 px, 4-32 spp (log-uniform), seeded convex stones "
+                    "(17-181 planes), seeded pleochroic sapphires, one device-resident 2048x1024 HDR map, depth 32",
+        "jobs": len(jobs), "paths": paths, "segments": b_segs,
+        "plain_s": plain_s, "plain_segments_per_s": plain_segs / plain_s,
+        "plain_jobs_per_s": len(jobs) / plain_s,
+        "batched_s": b_wall, "batched_segments_per_s": b_segs / b_wall, "batched_speedup": plain_s / b_wall,
+        "batched_latency_p99_ms": b_m["latency_p99_ms"],
+        "served_arrival_rate_per_s": rate, "served_wall_s": s_wall,
+        "served_segments_per_s": s_segs / s_wall,
+        "served_latency_p50_ms": 1e3 * pct(s_lat, 0.5), "served_latency_p99_ms": 1e3 * pct(s_lat, 0.99),
+        "plain_fifo_latency_p50_ms": 1e3 * pct(plain_lat, 0.5),
+        "plain_fifo_latency_p99_ms": 1e3 * pct(plain_lat, 0.99),
+        "plain_fifo_wall_s": float(arrivals[-1] + plain_lat[-1]),
+        "served_throughput_over_plain": (s_segs / s_wall) / (plain_segs / float(arrivals[-1] + plain_lat[-1])),
+    }
 
 
 def main():
@@ -295,6 +342,7 @@ def main():
     ap.add_argument("--batched-jobs", type=int, default=64)
     ap.add_argument("--served-jobs", type=int, default=40)
     ap.add_argument("--arrival-rate", type=float, default=20.0, help="served jobs per second (Poisson)")
+    ap.add_argument("--preview-jobs", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -407,7 +455,7 @@ def main():
         },
     }
     if not args.no_jobs:
-        out["batched"], out["served"] = job_workloads(gpu, scene.environment.rgb, args)
+        out["batched"], out["served"], out["previews"] = job_workloads(gpu, scene.environment.rgb, args)
         out["metrics"] = metrics_workload(gpu, args)
         print(f"bench: metrics {out['metrics']['evaluations_per_s']:.0f} evaluations/s", file=sys.stderr,
               flush=True)

@@ -61,7 +61,8 @@ class Env(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("temp_k", ctypes.c_float), ("spot_mult", ctypes.c_float),
                 ("exposure", ctypes.c_float), ("light_yaw", ctypes.c_float),
                 ("light_pitch", ctypes.c_float), ("map_w", ctypes.c_int32),
-                ("map_h", ctypes.c_int32), ("map_rgb", ctypes.POINTER(ctypes.c_float))]
+                ("map_h", ctypes.c_int32), ("map_rgb", ctypes.POINTER(ctypes.c_float)),
+                ("map", ctypes.c_void_p)]
 
 
 class Scene(ctypes.Structure):
@@ -119,7 +120,7 @@ class ServeMetrics(ctypes.Structure):
 EXPORTS = [
     "gr_open", "gr_close", "gr_adapter_label", "gr_set_chunk_tuples", "gr_accumulate",
     "gr_trace_samples", "gr_last_stats", "gr_scene_upload", "gr_scene_free", "gr_film_create",
-    "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_render_noalloc", "gr_camera_new",
+    "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_render_noalloc", "gr_camera_new", "gr_env_map_upload", "gr_env_map_release",
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
     "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_serve_set_policy",
@@ -176,6 +177,10 @@ def load():
     L.gr_film_download.restype = ctypes.c_int
     L.gr_render.argtypes = [vp, vp, u32, u32, vp]
     L.gr_render.restype = ctypes.c_int
+    L.gr_env_map_upload.argtypes = [vp, fp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(vp)]
+    L.gr_env_map_upload.restype = ctypes.c_int
+    L.gr_env_map_release.argtypes = [vp, vp]
+    L.gr_env_map_release.restype = None
     L.gr_render_noalloc.argtypes = [vp, vp, u32, u32, vp]
     L.gr_render_noalloc.restype = ctypes.c_int
     L.gr_camera_new.argtypes = [ctypes.c_float] * 4 + [ctypes.POINTER(Camera)]

@@ -47,9 +47,12 @@ class Studio:
 
 @dataclass(frozen=True)
 class HdrMap:
-    """``EnvironmentSource::HdrMap`` -- an ``EnvironmentMap`` of shape (H, W, 3) f32."""
+    """``EnvironmentSource::HdrMap`` -- an ``EnvironmentMap`` of shape (H, W, 3) f32.  With
+    ``device`` (GpuBackend.upload_env_map) the calls reference that HBM copy instead of
+    uploading ``rgb``."""
 
     rgb: np.ndarray
+    device: Optional["EnvMapDevice"] = None
 
     @staticmethod
     def uniform(l0: float) -> "HdrMap":
@@ -104,6 +107,8 @@ class _Packed:
             self.env.kind = _lib.GR_ENV_MAP
             self.env.map_h, self.env.map_w = self.map.shape[0], self.map.shape[1]
             self.env.map_rgb = self.map.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            if env.device is not None:
+                self.env.map = env.device.handle
         else:
             raise TypeError(f"unknown environment {env!r}")
         self.finishes = None
@@ -136,6 +141,25 @@ class DeviceScene:
     def free(self):
         if self.handle:
             self._backend._L.gr_scene_free(self._backend._ctx, self.handle)
+            self.handle = None
+
+
+class EnvMapDevice:
+    """A device-resident environment map (gr_env_map_upload); release() drops the caller's
+    reference (scenes and jobs that use it hold their own)."""
+
+    def __init__(self, backend: "GpuBackend", rgb: np.ndarray):
+        self._backend = backend
+        self.rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        h = ctypes.c_void_p()
+        check(backend._L.gr_env_map_upload(backend._ctx, self.rgb.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                           self.rgb.shape[1], self.rgb.shape[0], ctypes.byref(h)),
+              "gr_env_map_upload")
+        self.handle = h
+
+    def release(self):
+        if self.handle:
+            self._backend._L.gr_env_map_release(self._backend._ctx, self.handle)
             self.handle = None
 
 
@@ -261,6 +285,11 @@ class GpuBackend:
     def upload(self, scene: SceneRef) -> DeviceScene:
         self._require()
         return DeviceScene(self, scene)
+
+    def upload_env_map(self, rgb: np.ndarray) -> "EnvMapDevice":
+        """Uploads an environment map once; HdrMap(rgb, device=...) scenes reference it."""
+        self._require()
+        return EnvMapDevice(self, rgb)
 
     def film(self, width: int, height: int) -> Film:
         self._require()

@@ -167,8 +167,12 @@ gr_status validate_scene(const gr_scene *s) {
             !std::isfinite(e->light_yaw) || !std::isfinite(e->light_pitch))
             return GR_ERR_INVALID;
     } else if (e->kind == GR_ENV_MAP) {
-        if (e->map_w <= 0 || e->map_h <= 0 || !e->map_rgb) return GR_ERR_INVALID;
-        if ((uint64_t)e->map_w * (uint64_t)e->map_h > (1ull << 28)) return GR_ERR_INVALID;
+        if (e->map) {
+            if (!e->map->d || e->map->w <= 0 || e->map->h <= 0) return GR_ERR_INVALID;
+        } else {
+            if (e->map_w <= 0 || e->map_h <= 0 || !e->map_rgb) return GR_ERR_INVALID;
+            if ((uint64_t)e->map_w * (uint64_t)e->map_h > (1ull << 28)) return GR_ERR_INVALID;
+        }
     } else {
         return GR_DECLINE_ENV;
     }
@@ -277,13 +281,26 @@ void fill_dev_scene(const gr_scene *s, DevScene *d) {
     } else {
         // HdrMap: no white balance (environment.rs:238-243), LMS round trip with scale 1.
         d->wb[0] = d->wb[1] = d->wb[2] = 1.0f;
-        d->map_w = e->map_w;
-        d->map_h = e->map_h;
+        d->map_w = e->map ? e->map->w : e->map_w;
+        d->map_h = e->map ? e->map->h : e->map_h;
+        if (e->map) d->map_rgb = e->map->d;
     }
 }
 
 // radical_inverse_base (optics/raytracer/sampling.rs:170-181), the same operations as the
 // device's, evaluated once per call for its sample range.
+void env_map_retain(const gr_env_map *m) { const_cast<gr_env_map *>(m)->refs.fetch_add(1); }
+void env_map_release(const gr_env_map *m, hipStream_t stream) {
+    gr_env_map *mm = const_cast<gr_env_map *>(m);
+    if (mm->refs.fetch_sub(1) == 1) {
+        // No work that reads the texels is outstanding: the scenes and jobs holding
+        // references have completed their work before dropping them.  A stream-ordered free
+        // does not wait for the device (the job loop's other tick keeps running).
+        (void)hipFreeAsync(mm->d, stream);
+        delete mm;
+    }
+}
+
 float h_radical_inverse(uint32_t n, uint32_t base) {
     uint32_t remaining = n;
     float val = 0.0f;
@@ -307,7 +324,8 @@ struct gr_scene_dev {
     gr::DevScene sc;
     float4 *planes = nullptr;
     uint32_t *finishes = nullptr;  // only when some facet is frosted
-    float *map = nullptr;
+    float *map = nullptr;             // the scene's own copy of its map, or
+    const gr_env_map *env_map = nullptr;  // the device-resident map it references
     uint32_t *cert = nullptr;  // per-pixel camera-ray miss certificates
 };
 
@@ -401,7 +419,11 @@ gr_status upload_scene(gr_ctx *ctx, const gr_scene *s, gr_scene_dev *out) {
                                 ctx->stream));
         out->sc.finishes = out->finishes;
     }
-    if (s->env->kind == GR_ENV_MAP) {
+    if (s->env->kind == GR_ENV_MAP && s->env->map) {
+        gr::env_map_retain(s->env->map);
+        out->env_map = s->env->map;
+        out->sc.map_rgb = s->env->map->d;
+    } else if (s->env->kind == GR_ENV_MAP) {
         size_t bytes = (size_t)s->env->map_w * s->env->map_h * 3 * sizeof(float);
         GR_CHECK(hipMalloc(&out->map, bytes));
         GR_CHECK(hipMemcpyAsync(out->map, s->env->map_rgb, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -418,10 +440,12 @@ void free_scene(gr_scene_dev *s) {
     if (s->planes) (void)hipFree(s->planes);
     if (s->finishes) (void)hipFree(s->finishes);
     if (s->map) (void)hipFree(s->map);
+    if (s->env_map) gr::env_map_release(s->env_map, nullptr);
     if (s->cert) (void)hipFree(s->cert);
     s->planes = nullptr;
     s->finishes = nullptr;
     s->map = nullptr;
+    s->env_map = nullptr;
     s->cert = nullptr;
 }
 
@@ -787,6 +811,35 @@ gr_status gr_guide_buffers(gr_ctx *ctx, const gr_camera *camera, uint32_t width,
     if (d_normal) (void)hipFree(d_normal);
     if (d_facet) (void)hipFree(d_facet);
     return st;
+}
+
+gr_status gr_env_map_upload(gr_ctx *ctx, const float *rgb, int32_t w, int32_t h, gr_env_map **out) {
+    if (!ctx || !rgb || !out || w <= 0 || h <= 0 || (uint64_t)w * (uint64_t)h > (1ull << 28)) return GR_ERR_INVALID;
+    *out = nullptr;
+    const size_t n = (size_t)w * h * 3;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(rgb[i])) return GR_ERR_INVALID;
+    gr_env_map *m = new (std::nothrow) gr_env_map();
+    if (!m) return GR_ERR_DEVICE;
+    m->w = w;
+    m->h = h;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess ||
+        hipMallocAsync((void **)&m->d, n * sizeof(float), ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(m->d, rgb, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        if (m->d) (void)hipFreeAsync(m->d, ctx->stream);
+        delete m;
+        return GR_ERR_DEVICE;
+    }
+    *out = m;
+    return GR_OK;
+}
+
+void gr_env_map_release(gr_ctx *ctx, gr_env_map *map) {
+    if (!ctx || !map) return;
+    (void)hipSetDevice(ctx->device);
+    gr::env_map_release(map, ctx->stream);
 }
 
 gr_status gr_last_stats(gr_ctx *ctx, gr_stats *out) {

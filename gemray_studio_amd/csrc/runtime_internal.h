@@ -12,9 +12,21 @@
 #include "../../include/gemray_hip.h"
 #include "transport.h"
 
+// A device-resident environment map with its reference count (gr_env_map_upload).
+struct gr_env_map {
+    float *d = nullptr;
+    int32_t w = 0, h = 0;
+    std::atomic<int> refs{1};
+};
+
 namespace gr {
 
 struct ServeState;
+
+// References to device-resident environment maps: a scene or job that shows one holds a
+// reference; the last release frees the texels, stream-ordered on `stream`.
+void env_map_retain(const gr_env_map *m);
+void env_map_release(const gr_env_map *m, hipStream_t stream);
 
 // validate_scene (apps/gemray-worker/src/validate/mod.rs:74-192) plus the decline rules of
 // frame.rs:664-700 for the branches this build does not implement.

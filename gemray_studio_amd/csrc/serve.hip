@@ -59,7 +59,8 @@ struct Job {
     // Host copies of the request (inputs are borrowed only during gr_job_submit).
     std::vector<float> planes;
     std::vector<uint32_t> finishes;  // only when some facet is frosted
-    std::vector<float> map;
+    std::vector<float> map;              // a map the request passed as host texels, or
+    const gr_env_map *env_map = nullptr;  // the device-resident map it references
     DevScene host_sc{};
     int cls = 0;
     uint32_t width = 0, height = 0;
@@ -179,7 +180,7 @@ uint64_t job_footprint(const Job &j) {
     uint64_t b = j.n_pixels * 3 * sizeof(float);             // film
     b += j.planes.size() * sizeof(float);                    // planes
     b += j.finishes.size() * sizeof(uint32_t);               // facet finishes
-    b += j.map.size() * sizeof(float);                       // environment map
+    b += j.map.size() * sizeof(float);                       // environment map (a shared one is the map's own)
     b += (uint64_t)j.total_spp * sizeof(float2);             // sample table
     b += j.n_pixels * sizeof(uint32_t);                      // miss certificates
     b += sizeof(DevScene) + GR_EV_COUNT * sizeof(unsigned long long);
@@ -190,10 +191,16 @@ uint64_t job_footprint(const Job &j) {
 // stream): a job's buffers are freed only after its last tick completed and its film was read
 // back, and a stream-ordered free does not wait for the device, so retiring a job never
 // stalls the tick still running on the other slot (hipFree would synchronize the device).
+void drop_env_map(ServeState &S, Job &j) {
+    if (j.env_map) env_map_release(j.env_map, S.stream);
+    j.env_map = nullptr;
+}
+
 void free_job_device(ServeState &S, Job &j) {
     for (void *p : {(void *)j.d_scene, (void *)j.d_planes, (void *)j.d_finishes, (void *)j.d_map, (void *)j.d_ri35,
                     (void *)j.d_cert, (void *)j.d_film, (void *)j.d_stats})
         if (p) (void)hipFreeAsync(p, S.stream);
+    drop_env_map(S, j);
     j.d_scene = nullptr;
     j.d_planes = nullptr;
     j.d_finishes = nullptr;
@@ -214,7 +221,7 @@ gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMallocAsync((void **)&j.d_film, j.n_pixels * 3 * sizeof(float), S.stream));
     SV_CHECK(hipMallocAsync((void **)&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long), S.stream));
     DevScene sc = j.host_sc;
-    sc.map_rgb = j.d_map;
+    sc.map_rgb = j.env_map ? j.env_map->d : j.d_map;
     sc.finishes = j.d_finishes;
     std::vector<float2> ri(j.total_spp);
     for (uint32_t i = 0; i < j.total_spp; ++i)
@@ -558,6 +565,7 @@ void loop(ServeState *Sp) {
             // pieces is in flight.
             for (auto it = S.queue.begin(); it != S.queue.end();) {
                 if ((*it)->cancel_req) {
+                    drop_env_map(S, **it);
                     (*it)->state = GR_JOB_CANCELLED;
                     (*it)->t_done = Clock::now();
                     S.m.cancelled++;
@@ -729,6 +737,7 @@ void serve_shutdown(gr_ctx *ctx) {
     for (TickSlot &T : S->slot)
         if (T.stream) (void)hipStreamSynchronize(T.stream);
     for (auto &j : S->active) free_job_device(*S, *j);
+    for (auto &j : S->queue) drop_env_map(*S, *j);
     (void)hipStreamSynchronize(S->stream);
     release_device(*S);
     delete S;
@@ -756,8 +765,12 @@ gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) 
     j->cls = gr::scene_class(j->host_sc);
     j->planes.assign(s->planes, s->planes + 4 * (size_t)s->n_planes);
     if (j->host_sc.n_frosted) j->finishes.assign(s->finishes, s->finishes + s->n_planes);
-    if (s->env->kind == GR_ENV_MAP)
+    if (s->env->kind == GR_ENV_MAP && s->env->map) {
+        gr::env_map_retain(s->env->map);
+        j->env_map = s->env->map;
+    } else if (s->env->kind == GR_ENV_MAP) {
         j->map.assign(s->env->map_rgb, s->env->map_rgb + (size_t)s->env->map_w * s->env->map_h * 3);
+    }
     j->width = s->width;
     j->height = s->height;
     j->n_pixels = (uint64_t)s->width * s->height;
@@ -776,6 +789,7 @@ gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) 
     std::lock_guard<std::mutex> g(S->mu);
     if (j->footprint > S->budget) {
         S->m.declined++;
+        gr::drop_env_map(*S, *j);
         return GR_DECLINE_BUDGET;
     }
     const double rate = S->rate_ms > 0.0 ? (double)S->rate_tuples / S->rate_ms : gr::kDefaultRate;

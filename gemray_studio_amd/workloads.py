@@ -11,7 +11,9 @@ GPU tests (so a test runs exactly the jobs the benchmark times):
   cpu_config_scene  -- the reference CPU tracer's configuration (Diamond, parametric
                        brilliant, uniform furnace environment, 256x256, depth 16);
   stress_scene      -- the stress configuration (233-plane step cut, Abbe 20, n_e - n_o
-                       0.17, 4096x4096, depth 64).
+                       0.17, 4096x4096, depth 64);
+  preview_jobs      -- interactive previews: many small frames at few samples sharing one
+                       device-resident map (the served mix at the sizes a viewer requests).
 """
 import numpy as np
 
@@ -133,3 +135,27 @@ def served_arrivals(n_jobs, rate_per_s=20.0, seed=5):
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, size=n_jobs))
     return arrivals - arrivals[0]
+
+
+def preview_jobs(env, seed=6, n_jobs=400, cams=None):
+    """Interactive previews: n_jobs square frames of log-uniform 96^2..256^2 pixels at
+    log-uniform 4..32 spp, seeded convex stones of 17-181 planes (+-3 deg) in the single-job
+    material family (a seeded pleochroic sapphire), lit by one shared environment (env: an
+    HdrMap, normally device-resident), depth 32 -- the requests a viewer sends while the user
+    turns the stone."""
+    from gemray_studio_amd import backend as B
+    from gemray_studio_amd import scenes
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_jobs):
+        mat = scenes.pleochroic_sapphire(seed * 1000 + i)
+        cam_args = (float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.2, 0.6)), 5.0,
+                    float(rng.uniform(16.0, 28.0)))
+        planes = scenes.random_convex_stone(seed * 1000 + i, int(rng.integers(17, 182)), 3.0)
+        side = int(round(float(np.exp(rng.uniform(np.log(96), np.log(256))))))
+        spp = int(round(float(np.exp(rng.uniform(np.log(4), np.log(32))))))
+        jobs.append((B.SceneRef(B.camera_new(*cam_args), side, side, planes, mat, 32, env), 0, spp))
+        if cams is not None:
+            cams.append(cam_args)
+    return jobs

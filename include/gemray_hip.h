@@ -98,6 +98,11 @@ typedef struct {
  * (scattering.rs:522-590) instead of reflecting and refracting specularly. */
 enum { GR_FINISH_POLISHED = 0, GR_FINISH_FROSTED = 1 };
 
+/* An EnvironmentMap resident in HBM (gr_env_map_upload): the reference borrows one map for
+ * every render that shows it (EnvironmentSource::HdrMap(&EnvironmentMap),
+ * environment.rs:188-196), so a map shared by many scenes and jobs is uploaded once. */
+typedef struct gr_env_map gr_env_map;
+
 /* EnvironmentSource (optics/raytracer/environment.rs:188-196). */
 enum { GR_ENV_STUDIO = 0, GR_ENV_MAP = 1 };
 typedef struct {
@@ -105,9 +110,12 @@ typedef struct {
     /* Studio: LightingPreset::params() (temp_k, spot_mult) plus exposure/yaw/pitch. */
     float temp_k, spot_mult, exposure, light_yaw, light_pitch;
     /* Map: an EnvironmentMap, map_w x map_h RGB f32 texels, row-major (a 1x1 map is the
-     * uniform furnace environment). */
+     * uniform furnace environment), copied in by every call that uploads the scene ... */
     int32_t map_w, map_h;
     const float *map_rgb;
+    /* ... or, when map is not NULL, that device-resident map (map_w, map_h and map_rgb are
+     * then ignored): scenes and jobs hold a reference to it until they are freed. */
+    const gr_env_map *map;
 } gr_env;
 
 /* GpuSceneRef (renderer/gpu_backend.rs:85-99). */
@@ -231,6 +239,12 @@ enum { GR_PROBE_NEFF = 0, GR_PROBE_POYNTING = 1, GR_PROBE_TIR_RETARDATION = 2, G
        GR_PROBE_BX_POYNTING_SLOW = 11, GR_PROBE_BX_POYNTING_FAST = 12, GR_PROBE_SAMPLE_DRAWS = 13,
        GR_PROBE_SEED = 14, GR_PROBE_STREAM_DRAW = 15, GR_PROBE_RGB_SPECTRAL = 16, GR_PROBE_COUNT = 17 };
 gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out);
+
+/* Uploads a w x h RGB f32 lat-long map (EnvironmentMap::new, renderer/env_map.rs) to HBM;
+ * *out holds one reference, which gr_env_map_release drops.  The device copy is freed when
+ * the last reference (the caller's, a device scene's or a job's) is dropped. */
+gr_status gr_env_map_upload(gr_ctx *ctx, const float *rgb, int32_t w, int32_t h, gr_env_map **out);
+void gr_env_map_release(gr_ctx *ctx, gr_env_map *map);
 
 /* Statistics of the last gr_accumulate / gr_trace_samples / gr_render call. */
 gr_status gr_last_stats(gr_ctx *ctx, gr_stats *out);

@@ -36,7 +36,7 @@ def test_struct_sizes_match_header_layout():
     # gr_material: 1 + 6 + 1 + 1 + 3 + 2 + 24 + 24 + 1 + 3 + 2 + 24 + 1 four-byte fields.
     assert ctypes.sizeof(_lib.MaterialStruct) == 4 * 93
     assert ctypes.sizeof(_lib.Camera) == 4 * 13
-    assert ctypes.sizeof(_lib.Env) == 4 * 8 + 8
+    assert ctypes.sizeof(_lib.Env) == 4 * 8 + 8 + 8
     assert ctypes.sizeof(_lib.Stats) == 8 * 16 + 8 + 8 + 8 + 8 + 8 + 8
 
 

@@ -360,3 +360,29 @@ def test_progressive_cancel_keeps_finished_subbatches(gpu):
     assert np.array_equal(acc.view(np.uint32), d0.view(np.uint32))
     gpu.release(jid)
     assert gpu.metrics()["hbm_in_use"] == 0
+
+
+def test_device_env_map_shared_by_scenes_and_jobs(gpu):
+    """A map uploaded once (gr_env_map_upload) gives the same films as the host texels, in
+    plain calls and in the job loop, and stays alive while jobs reference it after the
+    caller's release."""
+    env_rgb = scenes.procedural_env(6, 256, 128)
+    dev = gpu.upload_env_map(env_rgb)
+    cam = B.camera_new(0.35, 0.28, 5.0, 18.0)
+    jobs = []
+    for k, (w, h, spp) in enumerate([(40, 32, 8), (24, 24, 16), (33, 17, 5)]):
+        planes = scenes.random_convex_stone(30 + k, 20 + 40 * k)
+        mat = scenes.pleochroic_sapphire(40 + k)
+        jobs.append((B.SceneRef(cam, w, h, planes, mat, 32, B.HdrMap(env_rgb)),
+                     B.SceneRef(cam, w, h, planes, mat, 32, B.HdrMap(env_rgb, device=dev)), 2 * k, spp))
+    for host, shared, first, spp in jobs:
+        a, b = _accumulate(gpu, host, first, spp), _accumulate(gpu, shared, first, spp)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    ids = [gpu.submit(shared, first, spp) for _, shared, first, spp in jobs]
+    dev.release()  # the jobs keep their own references
+    for jid, (host, shared, first, spp) in zip(ids, jobs):
+        assert gpu.wait(jid, 60000) == _lib.GR_OK
+        film = np.zeros((shared.width * shared.height, 3), np.float32)
+        gpu.job_film(jid, film)
+        gpu.release(jid)
+        assert np.array_equal(film.view(np.uint32), _accumulate(gpu, host, first, spp).view(np.uint32))
