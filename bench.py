@@ -284,7 +284,7 @@ def job_workloads(gpu, env_map, args):
 def preview_workload(gpu, env_map, args):
     """Interactive previews (workloads.preview_jobs): many small frames at few samples that
     share one device-resident map, submitted all at once (batched) and as a Poisson stream at
-    1.5x the plain path's service rate (served), beside the plain per-job path (gr_accumulate
+    3x the plain path's service rate (served), beside the plain per-job path (gr_accumulate
     per job, the same device-resident map)."""
     from gemray_studio_amd import backend as B
     from gemray_studio_amd.workloads import preview_jobs
@@ -299,7 +299,7 @@ def preview_workload(gpu, env_map, args):
         b_wall, b_lat, b_segs, b_m = run_batched(gpu, jobs)
         if b_segs != plain_segs:
             raise RuntimeError("preview jobs: batched and plain paths traced different work")
-        rate = 1.5 * len(jobs) / plain_s
+        rate = 3.0 * len(jobs) / plain_s
         rng = np.random.default_rng(7)
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(jobs)))
         arrivals -= arrivals[0]

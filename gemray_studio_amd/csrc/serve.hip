@@ -62,6 +62,9 @@ struct Job {
     std::vector<float> map;              // a map the request passed as host texels, or
     const gr_env_map *env_map = nullptr;  // the device-resident map it references
     DevScene host_sc{};
+    // Host staging of the admission copies, kept until the control stream has run them.
+    DevScene upload_sc{};
+    std::vector<float2> upload_ri;
     int cls = 0;
     uint32_t width = 0, height = 0;
     // The requested range, and the sub-batch being traced: pieces cover samples
@@ -211,6 +214,9 @@ void free_job_device(ServeState &S, Job &j) {
     j.d_stats = nullptr;
 }
 
+// Queues the admission of a job on the control stream: device buffers, input copies and
+// the miss certificates.  The host staging lives in the job until the stream has run the
+// copies (the loop synchronizes once for every job it admits at a time).
 gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMallocAsync((void **)&j.d_scene, sizeof(DevScene), S.stream));
     SV_CHECK(hipMallocAsync((void **)&j.d_planes, j.planes.size() * sizeof(float), S.stream));
@@ -220,10 +226,12 @@ gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMallocAsync((void **)&j.d_cert, j.n_pixels * sizeof(uint32_t), S.stream));
     SV_CHECK(hipMallocAsync((void **)&j.d_film, j.n_pixels * 3 * sizeof(float), S.stream));
     SV_CHECK(hipMallocAsync((void **)&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long), S.stream));
-    DevScene sc = j.host_sc;
+    DevScene &sc = j.upload_sc;
+    sc = j.host_sc;
     sc.map_rgb = j.env_map ? j.env_map->d : j.d_map;
     sc.finishes = j.d_finishes;
-    std::vector<float2> ri(j.total_spp);
+    std::vector<float2> &ri = j.upload_ri;
+    ri.resize(j.total_spp);
     for (uint32_t i = 0; i < j.total_spp; ++i)
         ri[i] = make_float2(h_radical_inverse(j.first_sample + i, 3), h_radical_inverse(j.first_sample + i, 5));
     SV_CHECK(hipMemcpyAsync(j.d_scene, &sc, sizeof(DevScene), hipMemcpyHostToDevice, S.stream));
@@ -239,20 +247,24 @@ gr_status upload_job(ServeState &S, Job &j) {
     SV_CHECK(hipMemsetAsync(j.d_film, 0, j.n_pixels * 3 * sizeof(float), S.stream));
     SV_CHECK(hipMemsetAsync(j.d_stats, 0, GR_EV_COUNT * sizeof(unsigned long long), S.stream));
     SV_CHECK(gr_launch_cert(sc, j.d_planes, j.d_cert, S.stream));
-    SV_CHECK(hipStreamSynchronize(S.stream));
-    // The host copy of the environment map is no longer needed.
-    std::vector<float>().swap(j.map);
     return GR_OK;
 }
 
-// Reads back the film of a job's finished (sub-)batch into `out` and, after its last
-// batch, the job's counters; before a further sub-batch the device film is cleared.
+// After the control stream has run a job's admission: its host staging and host copy of
+// the environment map are no longer needed.
+void uploaded(Job &j) {
+    std::vector<float>().swap(j.map);
+    std::vector<float2>().swap(j.upload_ri);
+}
+
+// Queues the read-back of a job's finished (sub-)batch film into `out` and, after its last
+// batch, of the job's counters; before a further sub-batch the device film is cleared.  The
+// caller synchronizes the control stream once for all the jobs it reads back.
 gr_status read_batch(ServeState &S, Job &j, std::vector<float> &out, bool last) {
     out.resize(j.n_pixels * 3);
     SV_CHECK(hipMemcpyAsync(out.data(), j.d_film, j.n_pixels * 3 * sizeof(float), hipMemcpyDeviceToHost, S.stream));
     if (last) SV_CHECK(hipMemcpyAsync(j.events, j.d_stats, sizeof(j.events), hipMemcpyDeviceToHost, S.stream));
     if (!last) SV_CHECK(hipMemsetAsync(j.d_film, 0, j.n_pixels * 3 * sizeof(float), S.stream));
-    SV_CHECK(hipStreamSynchronize(S.stream));
     return GR_OK;
 }
 
@@ -515,6 +527,8 @@ void finish_jobs(ServeState &S, std::vector<Job *> &finished) {
         const bool last = j.produced + j.spp >= j.total_spp;
         fin[i] = read_batch(S, j, j.progressive ? batch[i] : j.film, last);
     }
+    if (hipStreamSynchronize(S.stream) != hipSuccess)
+        for (gr_status &f : fin) f = GR_ERR_DEVICE;
     const Clock::time_point now = Clock::now();
     std::lock_guard<std::mutex> g(S.mu);
     for (size_t i = 0; i < finished.size(); ++i) {
@@ -590,8 +604,14 @@ void loop(ServeState *Sp) {
             }
             S.cv_done.notify_all();
         }
-        for (auto &j : admit) {
-            gr_status st = upload_job(S, *j);
+        std::vector<gr_status> ust(admit.size(), GR_OK);
+        for (size_t i = 0; i < admit.size(); ++i) ust[i] = upload_job(S, *admit[i]);
+        if (!admit.empty() && hipStreamSynchronize(S.stream) != hipSuccess)
+            for (gr_status &u : ust) u = GR_ERR_DEVICE;
+        for (size_t i = 0; i < admit.size(); ++i) {
+            auto &j = admit[i];
+            uploaded(*j);
+            const gr_status st = ust[i];
             std::lock_guard<std::mutex> g(S.mu);
             if (st != GR_OK) {
                 retire(S, *j, GR_JOB_FAILED, st);
