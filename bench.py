@@ -339,6 +339,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--trace-mode", choices=["plain", "wavefront"], default="plain")
     ap.add_argument("--no-jobs", action="store_true", help="skip the batched and served workloads")
+    ap.add_argument("--no-autotune", action="store_true", help="time the context's default configuration")
     ap.add_argument("--batched-jobs", type=int, default=64)
     ap.add_argument("--served-jobs", type=int, default=40)
     ap.add_argument("--arrival-rate", type=float, default=20.0, help="served jobs per second (Poisson)")
@@ -362,6 +363,13 @@ def main():
     gpu.set_trace_mode(args.trace_mode)
     dscene = gpu.upload(scene)
     film = gpu.film(scene.width, scene.height)
+    tuned = None
+    if not args.no_autotune:
+        # The autotuner's choice for this scene class and size, picked from probe renders
+        # before the timed region; the timed steps then use it (gr_set_autotune).
+        tuned = gpu.autotune(dscene, args.spp)
+        gpu.set_autotune(True)
+        print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         gpu.render(dscene, i * args.spp, args.spp, film)
         print(f"bench: warmup {i} {gpu.last_stats()['total_ms']:.0f} ms", file=sys.stderr, flush=True)
@@ -389,6 +397,7 @@ def main():
     elapsed = time.perf_counter() - t0
     film.free()
     dscene.free()
+    gpu.set_autotune(False)
 
     m = scene.material
     model = (events, scene.planes.shape[0], paths, m.is_anisotropic, bool(m.o_bands or m.e_bands), False)
@@ -425,6 +434,7 @@ def main():
             "camera": list(cam_args),
             "width": args.width, "height": args.height, "spp_per_step": args.spp,
             "trace_mode": args.trace_mode,
+            "autotuned": tuned,
             "max_bounces": scene.max_bounces, "n_planes": int(scene.planes.shape[0]),
             "paths": paths, "segments": segments,
             "segments_per_path": segments / max(paths, 1),

@@ -116,11 +116,18 @@ class ServeMetrics(ctypes.Structure):
         ("rate_tuples_per_ms", ctypes.c_double)]
 
 
+class TuneChoice(ctypes.Structure):
+    _fields_ = [("waves_per_eu", ctypes.c_uint32), ("split_bounce", ctypes.c_uint32), ("cont_div", ctypes.c_uint32),
+                ("trace_mode", ctypes.c_uint32), ("chunk_tuples", ctypes.c_uint64), ("probe_ms", ctypes.c_double),
+                ("probes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
 # Every symbol include/gemray_hip.h declares.
 EXPORTS = [
     "gr_open", "gr_close", "gr_adapter_label", "gr_set_chunk_tuples", "gr_accumulate",
     "gr_trace_samples", "gr_last_stats", "gr_scene_upload", "gr_scene_free", "gr_film_create",
     "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_render_noalloc", "gr_camera_new", "gr_env_map_upload", "gr_env_map_release",
+    "gr_autotune", "gr_set_autotune", "gr_render_tuned",
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
     "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_serve_set_policy",
@@ -177,6 +184,12 @@ def load():
     L.gr_film_download.restype = ctypes.c_int
     L.gr_render.argtypes = [vp, vp, u32, u32, vp]
     L.gr_render.restype = ctypes.c_int
+    L.gr_autotune.argtypes = [vp, vp, u32, ctypes.POINTER(TuneChoice)]
+    L.gr_autotune.restype = ctypes.c_int
+    L.gr_set_autotune.argtypes = [vp, ctypes.c_int]
+    L.gr_set_autotune.restype = ctypes.c_int
+    L.gr_render_tuned.argtypes = [vp, vp, u32, u32, vp, ctypes.POINTER(TuneChoice)]
+    L.gr_render_tuned.restype = ctypes.c_int
     L.gr_env_map_upload.argtypes = [vp, fp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(vp)]
     L.gr_env_map_upload.restype = ctypes.c_int
     L.gr_env_map_release.argtypes = [vp, vp]

@@ -286,6 +286,31 @@ class GpuBackend:
         self._require()
         return DeviceScene(self, scene)
 
+    def autotune(self, scene: DeviceScene, spp: int) -> dict:
+        """gr_autotune: times probe renders of the scene over the trace configurations and
+        caches the fastest for its class and size; returns it."""
+        self._require()
+        c = _lib.TuneChoice()
+        with self._mu:
+            st = self._L.gr_autotune(self._ctx, scene.handle, spp, ctypes.byref(c))
+        check(st, "gr_autotune")
+        return {name: getattr(c, name) for name, _ in _lib.TuneChoice._fields_ if name != "reserved"}
+
+    def set_autotune(self, enabled: bool) -> None:
+        self._require()
+        check(self._L.gr_set_autotune(self._ctx, 1 if enabled else 0), "gr_set_autotune")
+
+    def render_tuned(self, scene: DeviceScene, sample_offset: int, spp: int, film: Film, choice: dict) -> None:
+        """gr_render_tuned: gr_render with the given configuration (keys of gr_tune_choice)."""
+        self._require()
+        c = _lib.TuneChoice()
+        for name, value in choice.items():
+            if name not in ("probe_ms", "probes"):
+                setattr(c, name, value)
+        with self._mu:
+            st = self._L.gr_render_tuned(self._ctx, scene.handle, sample_offset, spp, film.handle, ctypes.byref(c))
+        check(st, "gr_render_tuned")
+
     def upload_env_map(self, rgb: np.ndarray) -> "EnvMapDevice":
         """Uploads an environment map once; HdrMap(rgb, device=...) scenes reference it."""
         self._require()

@@ -10,12 +10,14 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <new>
 #include <vector>
 
 #include "../../include/gemray_hip.h"
+#include "autotune.h"
 #include "runtime_internal.h"
 #include "transport.h"
 
@@ -376,8 +378,8 @@ gr_status ensure_rec(gr_ctx *ctx, uint64_t tuples) {
 
 // Two halves of continuation records for a quarter of a launch's tuples each (paths
 // beyond that carry on in the launch that would have parked them).
-gr_status ensure_cont(gr_ctx *ctx, uint64_t tuples) {
-    const uint64_t cap = (tuples + 3) / 4;
+gr_status ensure_cont(gr_ctx *ctx, uint64_t tuples, uint32_t div) {
+    const uint64_t cap = (tuples + div - 1) / div;
     if (cap <= ctx->cont_cap) return GR_OK;
     if (ctx->cont) GR_CHECK(hipFree(ctx->cont));
     ctx->cont = nullptr;
@@ -473,7 +475,8 @@ struct Workspace {
 // when their slot of the ring comes round again, or at the end.
 gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, uint32_t n_pixels,
                        uint32_t sample_offset, uint32_t spp, float *film, float *xyz_host, uint32_t *info_host,
-                       const Workspace &w, uint64_t chunk_pixels, bool wavefront, uint32_t split_bounce) {
+                       const Workspace &w, uint64_t chunk_pixels, bool wavefront, uint32_t split_bounce,
+                       uint32_t waves_per_eu) {
     GR_CHECK(hipMemsetAsync(w.stats, 0, GR_EV_COUNT * sizeof(unsigned long long), ctx->stream));
     GR_CHECK(hipEventRecord(ctx->ev_begin, ctx->stream));
     double kernel_ms = 0.0, finish_ms = 0.0;
@@ -518,6 +521,7 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
         args.cont = w.cont;
         args.cont_cap = w.cont_cap;
         args.cont_count = w.cont_count;
+        args.waves_per_eu = waves_per_eu;
         GR_CHECK(hipEventRecord(e.k0, ctx->stream));
         GR_CHECK(gr_launch_trace(s->sc, args, ctx->stream));
         if (!wavefront) GR_CHECK(hipEventRecord(e.k1, ctx->stream));
@@ -552,18 +556,30 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
 
 // Traces with the context's own buffers, grown on demand (gr_render, gr_accumulate,
 // gr_trace_samples): launches of at most chunk_tuples tuples.
+// The context's own settings as a configuration (occupancy variant 0 = each class's default).
+gr::TuneConfig context_config(gr_ctx *ctx) {
+    gr::TuneConfig c;
+    c.waves_per_eu = 0;
+    c.split_bounce = ctx->split_bounce.load(std::memory_order_relaxed);
+    c.cont_div = 4;
+    c.trace_mode = (uint32_t)ctx->trace_mode;
+    c.chunk_tuples = ctx->chunk_tuples;
+    return c;
+}
+
 gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, uint32_t n_pixels,
                     uint32_t sample_offset, uint32_t spp, float *film, float *xyz_host,
-                    uint32_t *info_host) {
+                    uint32_t *info_host, const gr::TuneConfig *cfg = nullptr) {
     std::memset(&ctx->last, 0, sizeof(ctx->last));
     ctx->last.n_planes = s->sc.n_planes;
     if (spp == 0 || n_pixels == 0) return GR_OK;
-    uint64_t chunk_pixels = ctx->chunk_tuples / spp;
+    const gr::TuneConfig c = cfg ? *cfg : context_config(ctx);
+    uint64_t chunk_pixels = c.chunk_tuples / spp;
     if (chunk_pixels == 0) chunk_pixels = 1;
     if (chunk_pixels > n_pixels) chunk_pixels = n_pixels;
     uint64_t cap = chunk_pixels * spp;
-    const bool wavefront = ctx->trace_mode == GR_TRACE_WAVEFRONT;
-    const uint32_t split_bounce = ctx->split_bounce.load(std::memory_order_relaxed);
+    const bool wavefront = c.trace_mode == GR_TRACE_WAVEFRONT;
+    const uint32_t split_bounce = c.split_bounce;
     gr_status st = ensure_xyz(ctx, cap);
     if (st != GR_OK) return st;
     if (info_host) {
@@ -574,7 +590,7 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
         st = ensure_rec(ctx, cap);
         if (st != GR_OK) return st;
     } else if (split_bounce) {
-        st = ensure_cont(ctx, cap);
+        st = ensure_cont(ctx, cap, c.cont_div);
         if (st != GR_OK) return st;
     }
     st = ensure_ri35(ctx, sample_offset, spp);
@@ -586,12 +602,68 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     w.rec = ctx->rec;
     w.rec_cap = ctx->rec_cap;
     w.cont = ctx->cont;
-    w.cont_cap = ctx->cont_cap;
+    w.cont_cap = (uint32_t)std::min<uint64_t>(ctx->cont_cap, (cap + c.cont_div - 1) / c.cont_div);
     w.cont_count = ctx->cont_count;
     w.ri35 = ctx->ri35;
     w.stats = ctx->stats;
     return trace_chunks(ctx, s, pixel_begin, n_pixels, sample_offset, spp, film, xyz_host, info_host, w,
-                        chunk_pixels, wavefront, split_bounce);
+                        chunk_pixels, wavefront, split_bounce, c.waves_per_eu);
+}
+
+// Probe launches of the autotuner: the scene's frame at as many samples as make about this
+// many tuples (at most the call's own).
+constexpr uint64_t kTuneProbeTuples = 1ull << 25;
+
+gr::TuneConfig from_choice(const gr_tune_choice &c) {
+    gr::TuneConfig t;
+    t.waves_per_eu = c.waves_per_eu;
+    t.split_bounce = c.split_bounce;
+    t.cont_div = c.cont_div;
+    t.trace_mode = c.trace_mode;
+    t.chunk_tuples = c.chunk_tuples;
+    return t;
+}
+void to_choice(const gr::TuneConfig &t, double ms, int probes, gr_tune_choice *c) {
+    std::memset(c, 0, sizeof(*c));
+    c->waves_per_eu = t.waves_per_eu;
+    c->split_bounce = t.split_bounce;
+    c->cont_div = t.cont_div;
+    c->trace_mode = t.trace_mode;
+    c->chunk_tuples = t.chunk_tuples;
+    c->probe_ms = ms;
+    c->probes = (uint32_t)probes;
+}
+bool valid_choice(const gr_tune_choice &c) {
+    return c.waves_per_eu >= 2 && c.waves_per_eu <= 4 && c.split_bounce <= GR_MAX_BOUNCES && c.cont_div >= 1 &&
+           c.cont_div <= 64 && (c.trace_mode == GR_TRACE_PLAIN || c.trace_mode == GR_TRACE_WAVEFRONT) &&
+           c.chunk_tuples >= 1 && c.chunk_tuples <= kMaxChunkTuples;
+}
+
+// Tunes the configuration for renders of scene at spp samples per pixel (coordinate descent
+// over probe renders into a scratch film, autotune.h) and caches it under the scene's
+// class and size.  ctx->mu is held.
+gr_status tune(gr_ctx *ctx, const gr_scene_dev *s, uint32_t spp, gr::TuneConfig *out, double *ms, int *probes) {
+    const uint32_t n_pixels = s->sc.width * s->sc.height;
+    const uint64_t want = std::max<uint64_t>(1, kTuneProbeTuples / n_pixels);
+    const uint32_t probe_spp = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(spp, 1), want);
+    float *film = nullptr;
+    GR_CHECK(hipMalloc(&film, (size_t)n_pixels * 3 * sizeof(float)));
+    gr::TuneConfig start;
+    start.chunk_tuples = ctx->chunk_tuples;
+    start.split_bounce = ctx->split_bounce.load(std::memory_order_relaxed);
+    gr_status st = GR_OK;
+    auto time = [&](const gr::TuneConfig &c) {
+        const gr_status r = run_trace(ctx, s, 0, n_pixels, 0, probe_spp, film, nullptr, nullptr, &c);
+        if (r != GR_OK) st = r;
+        return r == GR_OK ? ctx->last.total_ms : 1e30;
+    };
+    (void)time(start);  // first-launch costs outside the comparison
+    const gr::TuneSpace space;
+    *out = gr::coordinate_descent(start, space, 0.01, 2, time, ms, probes);
+    (void)hipFree(film);
+    if (st != GR_OK) return st;
+    ctx->tuned[gr::tune_key(gr::scene_class(s->sc), s->sc.n_planes, (uint64_t)n_pixels * spp)] = *out;
+    return GR_OK;
 }
 
 // gr_render_noalloc's workspace, one fixed device array per GPU (56 MB): XYZ staging for a
@@ -933,7 +1005,53 @@ gr_status gr_render(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offs
     if (spp > GR_MAX_SAMPLES_PER_CALL) return GR_ERR_INVALID;
     std::lock_guard<std::mutex> g(ctx->mu);
     GR_CHECK(hipSetDevice(ctx->device));
-    return run_trace(ctx, scene, 0, film->width * film->height, sample_offset, spp, film->d, nullptr, nullptr);
+    const uint32_t n_pixels = film->width * film->height;
+    if (ctx->autotune && spp > 0) {
+        const uint64_t key = gr::tune_key(gr::scene_class(scene->sc), scene->sc.n_planes, (uint64_t)n_pixels * spp);
+        auto it = ctx->tuned.find(key);
+        gr::TuneConfig cfg;
+        if (it != ctx->tuned.end()) {
+            cfg = it->second;
+        } else {
+            double ms = 0.0;
+            int probes = 0;
+            gr_status st = tune(ctx, scene, spp, &cfg, &ms, &probes);
+            if (st != GR_OK) return st;
+        }
+        return run_trace(ctx, scene, 0, n_pixels, sample_offset, spp, film->d, nullptr, nullptr, &cfg);
+    }
+    return run_trace(ctx, scene, 0, n_pixels, sample_offset, spp, film->d, nullptr, nullptr);
+}
+
+gr_status gr_autotune(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t spp, gr_tune_choice *out) {
+    if (!ctx || !scene || !out || spp == 0 || spp > GR_MAX_SAMPLES_PER_CALL) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GR_CHECK(hipSetDevice(ctx->device));
+    gr::TuneConfig cfg;
+    double ms = 0.0;
+    int probes = 0;
+    gr_status st = tune(ctx, scene, spp, &cfg, &ms, &probes);
+    if (st != GR_OK) return st;
+    to_choice(cfg, ms, probes, out);
+    return GR_OK;
+}
+
+gr_status gr_set_autotune(gr_ctx *ctx, int enabled) {
+    if (!ctx) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->autotune = enabled != 0;
+    return GR_OK;
+}
+
+gr_status gr_render_tuned(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset, uint32_t spp, gr_film *film,
+                          const gr_tune_choice *choice) {
+    if (!ctx || !scene || !film || !choice || !valid_choice(*choice)) return GR_ERR_INVALID;
+    if (film->width != scene->sc.width || film->height != scene->sc.height) return GR_ERR_INVALID;
+    if (spp > GR_MAX_SAMPLES_PER_CALL) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GR_CHECK(hipSetDevice(ctx->device));
+    const gr::TuneConfig cfg = from_choice(*choice);
+    return run_trace(ctx, scene, 0, film->width * film->height, sample_offset, spp, film->d, nullptr, nullptr, &cfg);
 }
 
 gr_status gr_render_noalloc(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset, uint32_t spp,
@@ -965,7 +1083,7 @@ gr_status gr_render_noalloc(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sam
     w.stats = reinterpret_cast<unsigned long long *>(ctx->scratch + kScratchStats);
     GR_CHECK(gr_launch_ri35(sample_offset, spp, ri35, ctx->stream));
     return trace_chunks(ctx, scene, 0, n_pixels, sample_offset, spp, film->d, nullptr, nullptr, w, chunk_pixels,
-                        false, ctx->split_bounce.load(std::memory_order_relaxed));
+                        false, ctx->split_bounce.load(std::memory_order_relaxed), 0u);
 }
 
 gr_status gr_accumulate(gr_ctx *ctx, const gr_scene *scene, uint32_t sample_offset, uint32_t spp,

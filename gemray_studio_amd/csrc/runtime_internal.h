@@ -6,8 +6,11 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <map>
 #include <mutex>
 #include <vector>
+
+#include "autotune.h"
 
 #include "../../include/gemray_hip.h"
 #include "transport.h"
@@ -75,6 +78,10 @@ struct gr_ctx {
     char *mx_buf = nullptr;     // gr_optical_metrics: planes, poses, per-ray records, results
     size_t mx_cap = 0;          // bytes
     gr_stats last{};
+    // Autotuning: when on, gr_render and gr_accumulate trace with the configuration tuned
+    // for the scene's class and size, tuning it on first use.
+    bool autotune = false;
+    std::map<uint64_t, gr::TuneConfig> tuned;
     char label[256] = {0};
     uint64_t hbm_budget = 0;  // admission budget of the job loop; 0 = 3/4 of free HBM
     std::mutex serve_mu;      // guards creation of serve

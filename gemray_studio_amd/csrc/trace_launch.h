@@ -8,7 +8,7 @@
 namespace gr {
 namespace detail {
 
-template <int A, bool B, int E, bool F, bool I>
+template <int A, bool B, int E, bool F, bool I, int W>
 hipError_t launch_plain_i(const DevScene &sc, const TraceArgs &a, hipStream_t stream) {
     dim3 grid((a.n_tuples + kBlock - 1) / kBlock);
     const bool split = a.split_bounce != 0 && a.cont != nullptr && a.cont_cap != 0;
@@ -16,7 +16,7 @@ hipError_t launch_plain_i(const DevScene &sc, const TraceArgs &a, hipStream_t st
         hipError_t e = hipMemsetAsync(a.cont_count, 0, (kContLevels + 1) * sizeof(uint32_t), stream);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((trace_kernel<A, B, E, F, I>), grid, dim3(kBlock), 0, stream, sc, a.planes, a.ri35,
+    hipLaunchKernelGGL((trace_kernel<A, B, E, F, I, W>), grid, dim3(kBlock), 0, stream, sc, a.planes, a.ri35,
                        a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats,
                        split ? a.split_bounce : 0u, a.cont, a.cont_cap, a.cont_count);
     hipError_t e = hipGetLastError();
@@ -29,7 +29,7 @@ hipError_t launch_plain_i(const DevScene &sc, const TraceArgs &a, hipStream_t st
         float *in = a.cont + (size_t)((k - 1) & 1) * kContFields * a.cont_cap;
         float *out = a.cont + (size_t)(k & 1) * kContFields * a.cont_cap;
         const uint32_t next_split = k < kContLevels ? (uint32_t)(k + 1) * a.split_bounce : 0u;
-        hipLaunchKernelGGL((cont_kernel<A, B, E, F, I>), cgrid, dim3(kBlock), 0, stream, sc, a.planes, in,
+        hipLaunchKernelGGL((cont_kernel<A, B, E, F, I, W>), cgrid, dim3(kBlock), 0, stream, sc, a.planes, in,
                            a.cont_cap, a.cont_count + (k - 1), a.xyz, a.info, a.stats, next_split, out,
                            a.cont_count + k);
         e = hipGetLastError();
@@ -97,9 +97,16 @@ hipError_t launch_trace_class(const DevScene &sc, const TraceArgs &a, hipStream_
     constexpr int kA = (C & 16) ? 2 : (C & 1);
     constexpr bool kB = (C & 2) != 0, kF = (C & 8) != 0;
     constexpr int kE = (C & 4) ? 1 : 0;
+    constexpr int kW = GR_WPE(kA);
     if (a.wavefront) return detail::launch_wave<kA, kB, kE>(sc, a, stream);
-    return a.info ? detail::launch_plain_i<kA, kB, kE, kF, true>(sc, a, stream)
-                  : detail::launch_plain_i<kA, kB, kE, kF, false>(sc, a, stream);
+    // The per-path record kernels (parity tests) are built at the default occupancy only.
+    if (a.info) return detail::launch_plain_i<kA, kB, kE, kF, true, kW>(sc, a, stream);
+    switch (a.waves_per_eu) {
+        case 2: return detail::launch_plain_i<kA, kB, kE, kF, false, 2>(sc, a, stream);
+        case 3: return detail::launch_plain_i<kA, kB, kE, kF, false, 3>(sc, a, stream);
+        case 4: return detail::launch_plain_i<kA, kB, kE, kF, false, 4>(sc, a, stream);
+        default: return detail::launch_plain_i<kA, kB, kE, kF, false, kW>(sc, a, stream);
+    }
 }
 
 template <int C>

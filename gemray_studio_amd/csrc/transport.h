@@ -78,6 +78,8 @@ struct TraceArgs {
     float *cont;
     uint32_t cont_cap;
     uint32_t *cont_count;
+    // Occupancy variant of the plain trace kernels (2, 3 or 4 waves per SIMD; 0 = default).
+    uint32_t waves_per_eu;
 };
 
 // One piece of one job in a batched launch: pixels [pixel_begin, pixel_begin + n_pixels)

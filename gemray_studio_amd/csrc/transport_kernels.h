@@ -1071,24 +1071,21 @@ __device__ __forceinline__ void write_result(const PathState &ps, const DevScene
 // Kernels.
 // ---------------------------------------------------------------------------------
 
-// Occupancy target of the trace kernels (waves per SIMD); 0 leaves the register budget to
-// the compiler.  A tuning parameter chosen by measurement: on the single-job workload
-// 3 waves/SIMD (168 VGPRs) ran 1.36x faster than the compiler's 2 and 1.03x faster than 4.
-// The kernels built for biaxial stones (kAniso == 2) carry the indicatrix and take their
-// own target, GR_BIAX_WAVES_PER_EU: 3 as well, measured 8 % faster on the biaxial frames
-// than 2 waves/SIMD (256 VGPRs, no spills) despite ~100 spilled VGPRs.
+// Occupancy target of the trace kernels (waves per SIMD), which sets their register budget
+// (2: 256 VGPRs, 3: 168, 4: 128).  The plain trace and continuation kernels are built at 2,
+// 3 and 4 waves/SIMD and the autotuner picks one per scene class and plane count; the
+// default, chosen by measurement on the single-job workload, is 3 (25 % faster than 2 or 4
+// there), for the biaxial kernels too (8 % faster than 2 despite ~100 spilled VGPRs).  The
+// batched and persistent kernels take the default.
 #ifndef GR_TRACE_WAVES_PER_EU
 #define GR_TRACE_WAVES_PER_EU 3
 #endif
 #ifndef GR_BIAX_WAVES_PER_EU
 #define GR_BIAX_WAVES_PER_EU 3
 #endif
-#if GR_TRACE_WAVES_PER_EU > 0
 #define GR_WPE(kAniso) ((kAniso) == 2 ? GR_BIAX_WAVES_PER_EU : GR_TRACE_WAVES_PER_EU)
 #define GR_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_WPE(kAniso), GR_WPE(kAniso))))
-#else
-#define GR_TRACE_OCCUPANCY
-#endif
+#define GR_OCCUPANCY(w) __attribute__((amdgpu_waves_per_eu((w), (w))))
 
 // Tuples per wavefront of trace_wave_kernel: 64 lanes x this many rounds.
 #ifndef GR_WAVE_ROUNDS
@@ -1137,8 +1134,8 @@ __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block
 // fully populated waves, parking again at each further multiple of split_bounce for
 // kContLevels levels.  Parking moves state, not arithmetic: every path's result is
 // bit-identical to an unsplit run.  When the records run out, the lane carries on.
-template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
+template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo, int kW>
+__global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
     const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp,
     uint32_t n_tuples, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
@@ -1178,8 +1175,8 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_kernel(
 // (capped at cont_cap); workgroups past the count return at once.  With split_bounce > 0
 // the paths still running at that bounce are parked again, into next / next_count, for
 // the next level.
-template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_kernel(
+template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo, int kW>
+__global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float *__restrict__ cont, uint32_t cont_cap,
     const uint32_t *__restrict__ cont_count, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
     unsigned long long *__restrict__ stats, uint32_t split_bounce, float *__restrict__ next,

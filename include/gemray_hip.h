@@ -279,6 +279,35 @@ gr_status gr_render(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offs
 gr_status gr_render_noalloc(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset,
                             uint32_t spp, gr_film *film);
 
+/* ---- Autotuning (no reference counterpart: the reference fixes its workgroup size,
+ * spectral_transport.wgsl:672, and its chunk budget, frame.rs:192) -----------------------
+ * A configuration of the plain trace path.  Every configuration gives bit-identical films;
+ * they differ only in speed. */
+typedef struct {
+    uint32_t waves_per_eu;  /* occupancy variant of the trace kernels: 2, 3 or 4 waves per SIMD */
+    uint32_t split_bounce;  /* paths running after this many segments are parked (0: never)    */
+    uint32_t cont_div;      /* continuation records for 1 / cont_div of a launch's tuples     */
+    uint32_t trace_mode;    /* GR_TRACE_PLAIN or GR_TRACE_WAVEFRONT                            */
+    uint64_t chunk_tuples;  /* tuples per launch                                               */
+    double probe_ms;        /* gr_autotune: the chosen configuration's probe time              */
+    uint32_t probes;        /* gr_autotune: configurations timed                               */
+    uint32_t reserved;
+} gr_tune_choice;
+/* Times probe renders of the scene (its frame at up to spp samples, about 2^25 tuples) over
+ * the configurations by coordinate descent (occupancy, split bounce, continuation capacity,
+ * launch size, trace mode; a change must win by more than 1 %) and caches the fastest under
+ * the scene's class (material and environment kernels), plane-count bucket and launch-size
+ * bucket; writes it to *out. */
+gr_status gr_autotune(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t spp, gr_tune_choice *out);
+/* With autotuning on, gr_render (and so gr_accumulate) traces with the configuration cached
+ * for the scene's class and size, tuning it on first use; off (the default), with the
+ * context's settings (gr_set_chunk_tuples, gr_set_trace_mode, gr_set_split_bounce). */
+gr_status gr_set_autotune(gr_ctx *ctx, int enabled);
+/* gr_render with a given configuration (GR_ERR_INVALID outside the ranges above). */
+gr_status gr_render_tuned(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sample_offset, uint32_t spp,
+                          gr_film *film, const gr_tune_choice *choice);
+
+
 /* ---- Gemological metrics ------------------------------------------------------------
  * evaluate_gem_optical_metrics (crates/gemray/src/color/metrics.rs:1082-1248): brilliance,
  * fire, scintillation, windowing and extinction of a cut stone seen from a camera pose

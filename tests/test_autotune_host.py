@@ -1,0 +1,79 @@
+"""The autotuner's selection rule (gemray_studio_amd/csrc/autotune.h, coordinate_descent),
+host C++ only, compiled with g++ against synthetic timings (CPU only): the choice is the
+coordinate-wise fastest configuration, a change must win by more than the margin, every
+configuration is timed once, and the same timings always give the same choice."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "gemray_studio_amd", "csrc")
+
+DRIVER = r"""
+#include <cstdio>
+#include <cstdlib>
+#include "autotune.h"
+using gr::TuneConfig;
+// A synthetic cost surface: separable in the knobs, minimum at (4 waves, split 8, cont 2,
+// chunk 2^24, plain), plus an interaction between occupancy and split.
+static int calls = 0;
+static double cost(const TuneConfig &c) {
+    ++calls;
+    double t = 100.0;
+    t += c.waves_per_eu == 4 ? -10.0 : (c.waves_per_eu == 2 ? 5.0 : 0.0);
+    t += c.split_bounce == 8 ? -6.0 : (c.split_bounce == 0 ? 20.0 : 0.0);
+    t += c.cont_div == 2 ? -3.0 : 0.0;
+    t += c.chunk_tuples == (1ull << 24) ? -2.0 : 0.0;
+    t += c.trace_mode == 1 ? 30.0 : 0.0;
+    if (c.waves_per_eu == 4 && c.split_bounce == 10) t -= 7.0;  // found on the second knob
+    return t;
+}
+int main(int argc, char **argv) {
+    const double margin = argc > 1 ? std::atof(argv[1]) : 0.01;
+    gr::TuneSpace space;
+    double best_ms = 0;
+    int n = 0;
+    TuneConfig a = gr::coordinate_descent(TuneConfig{}, space, margin, 2, cost, &best_ms, &n);
+    const int calls_a = calls;
+    TuneConfig b = gr::coordinate_descent(TuneConfig{}, space, margin, 2, cost, nullptr, nullptr);
+    std::printf("%u %u %u %u %llu %.3f %d %d %d\n", a.waves_per_eu, a.split_bounce, a.cont_div, a.trace_mode,
+                (unsigned long long)a.chunk_tuples, best_ms, n, calls_a, (int)(a == b));
+    // A margin wider than every gain keeps the start.
+    TuneConfig c = gr::coordinate_descent(TuneConfig{}, space, 0.5, 2, cost);
+    std::printf("%u %u %u %u %llu\n", c.waves_per_eu, c.split_bounce, c.cont_div, c.trace_mode,
+                (unsigned long long)c.chunk_tuples);
+    std::printf("%llx %llx\n", (unsigned long long)gr::tune_key(7, 69, 33177600ull),
+                (unsigned long long)gr::tune_key(7, 64, 33177600ull));
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    d = tmp_path_factory.mktemp("autotune")
+    (d / "driver.cpp").write_text(DRIVER)
+    exe = d / "driver"
+    subprocess.run(["g++", "-O2", "-std=c++17", f"-I{CSRC}", "-o", str(exe), str(d / "driver.cpp")], check=True)
+    return exe
+
+
+def test_coordinate_descent_picks_the_fastest_and_is_deterministic(driver):
+    out = subprocess.run([str(driver)], check=True, capture_output=True, text=True).stdout.split("\n")
+    w, split, cont, mode, chunk, best, n, calls, same = out[0].split()
+    # Occupancy 4 first; then split 10 (-7 with occupancy 4) beats split 8 (-6); cont 2; 2^24.
+    assert (int(w), int(split), int(cont), int(mode), int(chunk)) == (4, 10, 2, 0, 1 << 24)
+    assert float(best) == pytest.approx(100 - 10 - 7 - 3 - 2)
+    # Each distinct configuration is timed once: 3 + 6 + 2 + 2 + 1 on the first pass, and the
+    # second pass only re-meets configurations already timed except the new neighbours.
+    assert int(n) == int(calls) and int(n) <= 2 * 14
+    assert same == "1"
+    assert out[1].split() == ["3", "6", "4", "0", str(1 << 25)]
+
+
+def test_tune_key_buckets(driver):
+    out = subprocess.run([str(driver)], check=True, capture_output=True, text=True).stdout.split("\n")
+    k69, k64 = (int(x, 16) for x in out[2].split())
+    assert k69 & 0xFF == 7 and (k69 >> 8) & 0xFF == 7 and (k64 >> 8) & 0xFF == 6
+    assert (k69 >> 16) == 25  # 1920 x 1080 x 16 tuples -> 2^25 bucket
