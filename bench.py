@@ -88,8 +88,8 @@ def pct(v, q):
     return v[min(len(v), max(1, int(np.ceil(q * len(v))))) - 1]
 
 
-def cpu_baseline(scene, cam_args, spp_total, budget_s):
-    """Oracle on a centred pixel window of the same frame; window grown until the run
+def cpu_baseline(scene, cam_args, spp, budget_s):
+    """Oracle on row windows of the same frame (spp samples per pass), grown until the run
     takes about budget_s seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -100,7 +100,6 @@ def cpu_baseline(scene, cam_args, spp_total, budget_s):
     env = po.map_env(scene.environment.rgb)
     mat = M.to_struct(scene.material)
     w, h = scene.width, scene.height
-    spp = 16
     rows = 2
     total_segs = 0
     total_paths = 0
@@ -327,6 +326,38 @@ def preview_workload(gpu, env_map, args):
     }
 
 
+def config_workload(gpu, scene, cam_args, spp, steps, cpu_budget_s, cpu_spp, no_cpu, label):
+    """One more configuration, timed like the single job (device-resident scene and film,
+    steps of spp samples per pixel), with the oracle on a bounded window of the same frame
+    (row windows from the frame centre, cpu_spp samples, about cpu_budget_s seconds)."""
+    dscene = gpu.upload(scene)
+    film = gpu.film(scene.width, scene.height)
+    try:
+        gpu.render(dscene, 0, min(spp, 4), film)  # first-launch costs
+        segs = paths = 0
+        ms = 0.0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            gpu.render(dscene, (k + 1) * spp, spp, film)
+            st = gpu.last_stats()
+            segs += st["segments"]
+            paths += st["paths"]
+            ms += st["total_ms"]
+        wall = time.perf_counter() - t0
+    finally:
+        film.free()
+        dscene.free()
+    out = {"workload": label, "width": scene.width, "height": scene.height, "spp_per_step": spp, "steps": steps,
+           "n_planes": int(scene.planes.shape[0]), "max_bounces": scene.max_bounces,
+           "segments_per_s": segs / wall, "paths_per_s": paths / wall, "segments_per_path": segs / max(paths, 1),
+           "ms_per_step": wall * 1e3 / steps, "device_ms_per_step": ms / steps}
+    print(f"bench: {label}: {out['segments_per_s'] / 1e9:.2f} G segments/s", file=sys.stderr, flush=True)
+    if not no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, cam_args, cpu_spp, cpu_budget_s)
+        out["gpu_over_cpu"] = out["segments_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -470,8 +501,18 @@ def main():
         print(f"bench: metrics {out['metrics']['evaluations_per_s']:.0f} evaluations/s", file=sys.stderr,
               flush=True)
         out["features"] = feature_workloads(gpu, scene, args)
+        c1, c1_cam = cpu_config_scene()
+        out["cpu_configuration"] = config_workload(
+            gpu, c1, c1_cam, 16, 5, args.cpu_budget, 16, args.no_cpu,
+            "the CPU tracer's configuration: Diamond on the parametric round brilliant (57 facets + 16 girdle "
+            "planes), uniform furnace environment, 256x256 x 16 spp, depth 16")
+        c5, c5_cam = stress_scene()
+        out["stress"] = config_workload(
+            gpu, c5, c5_cam, 16, 1, args.cpu_budget, 4, args.no_cpu,
+            "stress: Abbe-20, n_e - n_o = 0.17 stone on the 233-plane step cut, 2048x1024 HDR map, 4096x4096 "
+            "x 16 spp per step, depth 64")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, cam_args, args.spp, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(scene, cam_args, 16, args.cpu_budget)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 

@@ -119,6 +119,8 @@ void or_shading_normal_near_edge(const float *planes, int n_planes, const float 
                                  const float hit_normal[3], float rounding_radius, float out[3]);
 int or_intersect(const float *planes, int n_planes, const float origin[3], const float dir[3],
                  float *t, float normal[3], int *facet);
+int or_intersect_scalar(const float *planes, int n_planes, const float origin[3], const float dir[3],
+                 float *t, float normal[3], int *facet);
 float or_effective_extraordinary_index(float n_o, float n_e, float theta);
 float or_walk_off_angle(float n_o, float n_e, float theta);
 float or_neff_from_cos(float n_o, float n_e, float c);

@@ -137,6 +137,8 @@ def _declare(L):
     L.or_generate_ray.argtypes = [ctypes.POINTER(OrCamera)] + [ctypes.c_float] * 6 + [fp, fp]
     L.or_intersect.argtypes = [fp, ctypes.c_int, fp, fp, fp, fp, ctypes.POINTER(ctypes.c_int)]
     L.or_intersect.restype = ctypes.c_int
+    L.or_intersect_scalar.argtypes = L.or_intersect.argtypes
+    L.or_intersect_scalar.restype = ctypes.c_int
     L.or_effective_extraordinary_index.argtypes = [ctypes.c_float] * 3
     L.or_effective_extraordinary_index.restype = ctypes.c_float
     L.or_walk_off_angle.argtypes = [ctypes.c_float] * 3

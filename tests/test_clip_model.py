@@ -274,3 +274,26 @@ def test_padding_planes_are_inert():
         b = model_scan(padded, o, d, np.random.default_rng(9))
         for x, y in zip(a[:3], b[:3]):
             assert np.array_equal(x, y), name
+
+
+def test_vector_scan_of_the_oracle_matches_its_scalar_scan():
+    """The oracle's tracer scans eight planes at a time (AVX2, as the reference's CPU tracer
+    does, simd/slab.rs:186-270); it must select exactly what the one-plane-at-a-time scan
+    selects, ties and near-parallel planes included."""
+    rng = np.random.default_rng(9)
+    for name, planes in STONES:
+        planes = with_near_duplicates(np.ascontiguousarray(planes, F), rng, n=16)
+        # A near-parallel plane (|denom| <= 1e-7 for rays along +x) in some stones.
+        o, d = random_rays(rng, 1500)
+        inner = (rng.normal(size=(600, 3)) * 0.15).astype(F)
+        o = np.concatenate([o, inner]).astype(F)
+        d = np.concatenate([d, random_rays(rng, 600)[1]]).astype(F)
+        for k in range(len(o)):
+            t1, n1, f1 = np.zeros(1, F), np.zeros(3, F), ctypes.c_int()
+            t2, n2, f2 = np.zeros(1, F), np.zeros(3, F), ctypes.c_int()
+            args = (po.fptr(planes), len(planes), po.fptr(po.f32arr(o[k])), po.fptr(po.f32arr(d[k])))
+            h1 = L.or_intersect(*args, po.fptr(t1), po.fptr(n1), ctypes.byref(f1))
+            h2 = L.or_intersect_scalar(*args, po.fptr(t2), po.fptr(n2), ctypes.byref(f2))
+            assert h1 == h2, (name, k)
+            if h1:
+                assert t1.view(np.uint32)[0] == t2.view(np.uint32)[0] and f1.value == f2.value, (name, k)
