@@ -503,12 +503,12 @@ def main():
         out["features"] = feature_workloads(gpu, scene, args)
         c1, c1_cam = cpu_config_scene()
         out["cpu_configuration"] = config_workload(
-            gpu, c1, c1_cam, 16, 5, args.cpu_budget, 16, args.no_cpu,
+            gpu, c1, c1_cam, 16, 5, args.cpu_budget / 2, 16, args.no_cpu,
             "the CPU tracer's configuration: Diamond on the parametric round brilliant (57 facets + 16 girdle "
             "planes), uniform furnace environment, 256x256 x 16 spp, depth 16")
         c5, c5_cam = stress_scene()
         out["stress"] = config_workload(
-            gpu, c5, c5_cam, 16, 1, args.cpu_budget, 4, args.no_cpu,
+            gpu, c5, c5_cam, 16, 1, args.cpu_budget / 2, 4, args.no_cpu,
             "stress: Abbe-20, n_e - n_o = 0.17 stone on the 233-plane step cut, 2048x1024 HDR map, 4096x4096 "
             "x 16 spp per step, depth 64")
     if not args.no_cpu:
