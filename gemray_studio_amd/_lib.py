@@ -104,6 +104,11 @@ class GemMetrics(ctypes.Structure):
         "brilliance_pct", "fire_index", "scintillation_pct", "windowing_pct", "extinction_pct")]
 
 
+class MetricsStone(ctypes.Structure):
+    _fields_ = [("planes", ctypes.POINTER(ctypes.c_float)), ("n_planes", ctypes.c_uint32),
+                ("material", ctypes.c_void_p), ("poses", ctypes.POINTER(MetricsPose)), ("n_poses", ctypes.c_uint32)]
+
+
 class ServeMetrics(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "submitted", "admitted", "completed", "cancelled", "failed", "declined", "queued", "running",
@@ -127,7 +132,7 @@ EXPORTS = [
     "gr_open", "gr_close", "gr_adapter_label", "gr_set_chunk_tuples", "gr_accumulate",
     "gr_trace_samples", "gr_last_stats", "gr_scene_upload", "gr_scene_free", "gr_film_create",
     "gr_film_clear", "gr_film_free", "gr_film_download", "gr_render", "gr_render_noalloc", "gr_camera_new", "gr_env_map_upload", "gr_env_map_release",
-    "gr_autotune", "gr_set_autotune", "gr_render_tuned",
+    "gr_autotune", "gr_set_autotune", "gr_render_tuned", "gr_optical_metrics_batch",
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
     "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_serve_set_policy",
@@ -184,6 +189,8 @@ def load():
     L.gr_film_download.restype = ctypes.c_int
     L.gr_render.argtypes = [vp, vp, u32, u32, vp]
     L.gr_render.restype = ctypes.c_int
+    L.gr_optical_metrics_batch.argtypes = [vp, ctypes.POINTER(MetricsStone), u32, ctypes.POINTER(GemMetrics)]
+    L.gr_optical_metrics_batch.restype = ctypes.c_int
     L.gr_autotune.argtypes = [vp, vp, u32, ctypes.POINTER(TuneChoice)]
     L.gr_autotune.restype = ctypes.c_int
     L.gr_set_autotune.argtypes = [vp, ctypes.c_int]

@@ -461,6 +461,37 @@ class GpuBackend:
                                          ctypes.byref(m), arr, len(poses), out), "gr_optical_metrics")
         return [{n: float(getattr(o, n)) for n, _ in _lib.GemMetrics._fields_} for o in out]
 
+    def optical_metrics_batch(self, stones) -> list:
+        """gr_optical_metrics_batch: stones is a sequence of (planes, material, poses); every
+        pose of every stone in one launch.  Returns one list of per-pose dicts per stone."""
+        self._require()
+        keep, arr = [], (_lib.MetricsStone * len(stones))()
+        total = 0
+        for s, (planes, material, poses) in enumerate(stones):
+            p = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+            poses = list(poses)
+            pa = (_lib.MetricsPose * len(poses))()
+            for i, q in enumerate(poses):
+                pa[i].cam_yaw, pa[i].cam_pitch, pa[i].light_yaw, pa[i].light_pitch = (float(v) for v in q)
+            m = to_struct(material)
+            keep += [p, pa, m]
+            arr[s].planes = p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            arr[s].n_planes = p.shape[0]
+            arr[s].material = ctypes.cast(ctypes.byref(m), ctypes.c_void_p)
+            arr[s].poses = pa
+            arr[s].n_poses = len(poses)
+            total += len(poses)
+        out = (_lib.GemMetrics * total)()
+        with self._mu:
+            st = self._L.gr_optical_metrics_batch(self._ctx, arr, len(stones), out)
+        check(st, "gr_optical_metrics_batch")
+        res, k = [], 0
+        for _, _, poses in stones:
+            n = len(list(poses))
+            res.append([{f: float(getattr(o, f)) for f, _ in _lib.GemMetrics._fields_} for o in out[k:k + n]])
+            k += n
+        return res
+
     def gem_optical_metrics(self, planes, material, cam_yaw, cam_pitch, light_yaw, light_pitch) -> dict:
         """One pose of ``optical_metrics`` (the reference's evaluate_gem_optical_metrics)."""
         return self.optical_metrics(planes, material, [(cam_yaw, cam_pitch, light_yaw, light_pitch)])[0]

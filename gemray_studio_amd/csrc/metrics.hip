@@ -42,6 +42,7 @@ enum : uint8_t { MX_NONE = 0, MX_BLOCKED = 1, MX_WINDOWED = 2, MX_EXTINCT = 3, M
 // and fill directions and sin(light pitch), and the (forward, right, up) bases of the pose
 // and of its five temporal yaw offsets.
 struct MetricsPose {
+    uint32_t plane_off, n_planes;  // the pose's stone: planes [plane_off, plane_off + n_planes)
     float nd, nf, nc, sin_lp;
     float key[3], fill[3];
     float basis[1 + kMxSamples][9];
@@ -169,18 +170,20 @@ __device__ __forceinline__ float mx_grid(int i) { return fmaf(((float)i + 0.5f) 
 // One lane per ray.  Lane item i < kMxItems is aperture ray (cell i / 5, sample i % 5)
 // (classify_aperture_sample, metrics.rs:519-771); kMxItems <= i < 2 kMxItems is the
 // temporal ray (cell, yaw offset) (cell_returned_at_yaw_offset, :309-334).
-__global__ __launch_bounds__(kMxBlock) void metrics_items_kernel(const float4 *__restrict__ planes_g, uint32_t n_planes,
+__global__ __launch_bounds__(kMxBlock) void metrics_items_kernel(const float4 *__restrict__ planes_all,
                                                                  const MetricsPose *__restrict__ poses, uint8_t *cls,
                                                                  float2 *fire, uint8_t *temporal) {
     __shared__ float4 planes[GR_MAX_PLANES];
     const uint32_t pose_i = blockIdx.x / kMxBlocksPerPose;
     const uint32_t item = (blockIdx.x % kMxBlocksPerPose) * kMxBlock + threadIdx.x;
+    const MetricsPose &p = poses[pose_i];
+    const uint32_t n_planes = p.n_planes;
+    const float4 *__restrict__ planes_g = planes_all + p.plane_off;
     const uint32_t n4 = (n_planes + 3u) & ~3u;
     for (uint32_t i = threadIdx.x; i < n4; i += kMxBlock)
         planes[i] = i < n_planes ? planes_g[i] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     __syncthreads();
     if (item >= 2u * kMxItems) return;
-    const MetricsPose &p = poses[pose_i];
     const bool aperture = item < (uint32_t)kMxItems;
     const uint32_t j = aperture ? item : item - kMxItems;
     const int cell = (int)(j / kMxSamples), k = (int)(j % kMxSamples);
@@ -394,10 +397,10 @@ bool finite4(const float *v) {
 
 }  // namespace
 
-hipError_t gr_launch_metrics(const float4 *planes, uint32_t n_planes, const gr::MetricsPose *poses, uint32_t n_poses,
-                             uint8_t *cls, float2 *fire, uint8_t *temporal, float *out, hipStream_t stream) {
+hipError_t gr_launch_metrics(const float4 *planes, const gr::MetricsPose *poses, uint32_t n_poses, uint8_t *cls,
+                             float2 *fire, uint8_t *temporal, float *out, hipStream_t stream) {
     hipLaunchKernelGGL(gr::metrics_items_kernel, dim3(n_poses * gr::kMxBlocksPerPose), dim3(gr::kMxBlock), 0, stream,
-                       planes, n_planes, poses, cls, fire, temporal);
+                       planes, poses, cls, fire, temporal);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(gr::metrics_reduce_kernel, dim3((n_poses + 63u) / 64u), dim3(64), 0, stream, cls, fire,
@@ -405,38 +408,64 @@ hipError_t gr_launch_metrics(const float4 *planes, uint32_t n_planes, const gr::
     return hipGetLastError();
 }
 
-extern "C" {
-
-gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes, const gr_material *material,
-                             const gr_metrics_pose *poses, uint32_t n_poses, gr_gem_metrics *out) {
-    if (!ctx || !material || !poses || !out || n_poses == 0 || n_poses > gr::kMxMaxPoses) return GR_ERR_INVALID;
-    if (n_planes > GR_MAX_PLANES || (n_planes > 0 && !planes)) return GR_ERR_INVALID;
-    for (uint32_t i = 0; i < n_planes; ++i) {
-        const float *p = planes + 4 * i;
-        if (!finite4(p)) return GR_ERR_INVALID;
-        const double len = std::sqrt((double)p[0] * p[0] + (double)p[1] * p[1] + (double)p[2] * p[2]);
-        if (len <= 1e-6) return GR_ERR_INVALID;
+// Every (stone, pose) of the call in one launch: the stones' planes concatenated, each pose
+// pointing at its stone's.  Stones without facets get the reference's placeholders.
+static gr_status optical_metrics(gr_ctx *ctx, const gr_metrics_stone *stones, uint32_t n_stones, gr_gem_metrics *out) {
+    if (!ctx || !stones || !out || n_stones == 0) return GR_ERR_INVALID;
+    uint64_t total_poses = 0, total_planes = 0;
+    for (uint32_t s = 0; s < n_stones; ++s) {
+        const gr_metrics_stone &st = stones[s];
+        if (!st.material || !st.poses || st.n_poses == 0) return GR_ERR_INVALID;
+        if (st.n_planes > GR_MAX_PLANES || (st.n_planes > 0 && !st.planes)) return GR_ERR_INVALID;
+        for (uint32_t i = 0; i < st.n_planes; ++i) {
+            const float *p = st.planes + 4 * i;
+            if (!finite4(p)) return GR_ERR_INVALID;
+            const double len = std::sqrt((double)p[0] * p[0] + (double)p[1] * p[1] + (double)p[2] * p[2]);
+            if (len <= 1e-6) return GR_ERR_INVALID;
+        }
+        if (st.material->disp_kind < 0 || st.material->disp_kind > 2) return GR_ERR_INVALID;
+        for (uint32_t i = 0; i < st.n_poses; ++i)
+            if (!finite4(&st.poses[i].cam_yaw)) return GR_ERR_INVALID;
+        for (float l : {486.1f, 589.3f, 656.3f}) {
+            const float n = h_dispersion(st.material, l);
+            if (!std::isfinite(n) || n < 1.0f || n > 6.0f) return GR_ERR_INVALID;
+        }
+        total_poses += st.n_poses;
+        if (st.n_planes > 0) total_planes += st.n_planes;
     }
-    if (material->disp_kind < 0 || material->disp_kind > 2) return GR_ERR_INVALID;
-    for (uint32_t i = 0; i < n_poses; ++i)
-        if (!finite4(&poses[i].cam_yaw)) return GR_ERR_INVALID;
-    for (float l : {486.1f, 589.3f, 656.3f}) {
-        const float n = h_dispersion(material, l);
-        if (!std::isfinite(n) || n < 1.0f || n > 6.0f) return GR_ERR_INVALID;
+    if (total_poses > gr::kMxMaxPoses) return GR_ERR_INVALID;
+    std::vector<gr::MetricsPose> host;
+    std::vector<float> planes;
+    std::vector<uint32_t> slot(total_poses, 0xFFFFFFFFu);  // launched pose of each output, or none
+    host.reserve(total_poses);
+    planes.reserve(4 * total_planes);
+    uint64_t o = 0;
+    for (uint32_t s = 0; s < n_stones; ++s) {
+        const gr_metrics_stone &st = stones[s];
+        const uint32_t off = (uint32_t)(planes.size() / 4);
+        if (st.n_planes > 0) planes.insert(planes.end(), st.planes, st.planes + 4 * (size_t)st.n_planes);
+        for (uint32_t i = 0; i < st.n_poses; ++i, ++o) {
+            // No facets: the reference's placeholder values (metrics.rs:1090-1098).
+            if (st.n_planes == 0) {
+                out[o] = gr_gem_metrics{85.0f, 25.0f, 75.0f, 5.0f, 5.0f};
+                continue;
+            }
+            gr::MetricsPose p;
+            h_metrics_pose(st.material, st.poses[i], &p);
+            p.plane_off = off;
+            p.n_planes = st.n_planes;
+            slot[o] = (uint32_t)host.size();
+            host.push_back(p);
+        }
     }
-    // No facets: the reference's placeholder values (metrics.rs:1090-1098).
-    if (n_planes == 0) {
-        for (uint32_t i = 0; i < n_poses; ++i) out[i] = gr_gem_metrics{85.0f, 25.0f, 75.0f, 5.0f, 5.0f};
-        return GR_OK;
-    }
-    std::vector<gr::MetricsPose> host(n_poses);
-    for (uint32_t i = 0; i < n_poses; ++i) h_metrics_pose(material, poses[i], &host[i]);
+    const uint32_t n_poses = (uint32_t)host.size();
+    if (n_poses == 0) return GR_OK;
 
     std::lock_guard<std::mutex> g(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return GR_ERR_DEVICE;
     // One buffer, kept by the context between calls: planes, poses, per-ray records, results.
     const size_t items = (size_t)n_poses * gr::kMxItems;
-    const size_t off_poses = ((size_t)n_planes * sizeof(float4) + 255) & ~(size_t)255;
+    const size_t off_poses = ((planes.size() / 4) * sizeof(float4) + 255) & ~(size_t)255;
     const size_t off_fire = (off_poses + n_poses * sizeof(gr::MetricsPose) + 255) & ~(size_t)255;
     const size_t off_cls = off_fire + items * sizeof(float2);
     const size_t off_temporal = off_cls + items;
@@ -452,11 +481,11 @@ gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes
     char *buf = ctx->mx_buf;
     std::vector<float> res((size_t)n_poses * 5);
     gr_status st = GR_OK;
-    if (hipMemcpyAsync(buf, planes, (size_t)n_planes * sizeof(float4), hipMemcpyHostToDevice, ctx->stream) !=
+    if (hipMemcpyAsync(buf, planes.data(), planes.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream) !=
             hipSuccess ||
         hipMemcpyAsync(buf + off_poses, host.data(), n_poses * sizeof(gr::MetricsPose), hipMemcpyHostToDevice,
                        ctx->stream) != hipSuccess ||
-        gr_launch_metrics(reinterpret_cast<const float4 *>(buf), n_planes,
+        gr_launch_metrics(reinterpret_cast<const float4 *>(buf),
                           reinterpret_cast<const gr::MetricsPose *>(buf + off_poses), n_poses,
                           reinterpret_cast<uint8_t *>(buf + off_cls), reinterpret_cast<float2 *>(buf + off_fire),
                           reinterpret_cast<uint8_t *>(buf + off_temporal), reinterpret_cast<float *>(buf + off_out),
@@ -466,9 +495,25 @@ gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = GR_ERR_DEVICE;
     if (st != GR_OK) return st;
-    for (uint32_t i = 0; i < n_poses; ++i)
-        out[i] = gr_gem_metrics{res[5 * i + 0], res[5 * i + 1], res[5 * i + 2], res[5 * i + 3], res[5 * i + 4]};
+    for (uint64_t i = 0; i < total_poses; ++i) {
+        const uint32_t k = slot[i];
+        if (k != 0xFFFFFFFFu)
+            out[i] = gr_gem_metrics{res[5 * k + 0], res[5 * k + 1], res[5 * k + 2], res[5 * k + 3], res[5 * k + 4]};
+    }
     return GR_OK;
+}
+
+extern "C" {
+
+gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes, const gr_material *material,
+                             const gr_metrics_pose *poses, uint32_t n_poses, gr_gem_metrics *out) {
+    const gr_metrics_stone stone{planes, n_planes, material, poses, n_poses};
+    return optical_metrics(ctx, &stone, 1, out);
+}
+
+gr_status gr_optical_metrics_batch(gr_ctx *ctx, const gr_metrics_stone *stones, uint32_t n_stones,
+                                   gr_gem_metrics *out) {
+    return optical_metrics(ctx, stones, n_stones, out);
 }
 
 }  // extern "C"

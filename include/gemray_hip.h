@@ -328,6 +328,18 @@ typedef struct {
 
 gr_status gr_optical_metrics(gr_ctx *ctx, const float *planes, uint32_t n_planes, const gr_material *material,
                              const gr_metrics_pose *poses, uint32_t n_poses, gr_gem_metrics *out);
+/* gr_optical_metrics for many stones in one launch (the batched jobs' metrics, reduced on the
+ * device together): every pose of every stone is evaluated by one items launch and one
+ * reduction; out receives the stones' results in order, n_poses of each. */
+typedef struct {
+    const float *planes;
+    uint32_t n_planes;
+    const gr_material *material;
+    const gr_metrics_pose *poses;
+    uint32_t n_poses;
+} gr_metrics_stone;
+gr_status gr_optical_metrics_batch(gr_ctx *ctx, const gr_metrics_stone *stones, uint32_t n_stones,
+                                   gr_gem_metrics *out);
 
 /* ---- Batched and served rendering: the device-resident job loop ---------------------
  * The reference serves one RenderRequest{request_id, scene, first_sample, samples} at a

@@ -121,9 +121,12 @@ def test_batched_workload(gpu):
                     sc.height, sc.max_bounces, camera=cams[k])
         window_tiers(gpu, case, sc.height // 2, 4)
 
-    # Gemological metrics of every stone at its job's camera pose, bit for bit.
+    # Gemological metrics of every stone at its job's camera pose, bit for bit, all 64 stones
+    # reduced on the device in one launch (and equal to one call per stone).
+    batch = gpu.optical_metrics_batch([(sc.planes, sc.material, [pose]) for (sc, _, _), pose in zip(jobs, poses)])
     for k, ((sc, _, _), pose) in enumerate(zip(jobs, poses)):
         dev = gpu.optical_metrics(sc.planes, sc.material, [pose])[0]
+        assert batch[k][0] == dev, k
         ref = po.gem_optical_metrics(sc.planes, M.to_struct(sc.material), *pose)
         for i, name in enumerate(po.METRIC_NAMES):
             assert np.float32(dev[name]) == ref[i], (k, name, dev[name], ref[i])
