@@ -312,8 +312,11 @@ __device__ __forceinline__ void channel_alphas(const PathState &ps, const DevSce
 
 // One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
 // has ended (escape, Russian roulette, or the bounce cap).  kAniso: 0 isotropic, 1
-// uniaxial, 2 biaxial (an anisotropic stone with a biaxial indicatrix).
-template <int kAniso, bool kAbsorb, bool kFrost>
+// uniaxial, 2 biaxial (an anisotropic stone with a biaxial indicatrix).  kInfo: keep the
+// two path hashes of the per-tuple record (facet/event sequence and per-channel fates);
+// the kernels that write no record skip them, and with them the dead channels' refraction
+// indices, which only the fate hash reads.
+template <int kAniso, bool kAbsorb, bool kFrost, bool kInfo>
 __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, const float4 *__restrict__ planes,
                                             const float *__restrict__ cam_side, float *__restrict__ lcache,
                                             uint32_t *ev) {
@@ -372,7 +375,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         ev_inc(ev, GR_EV_MISS);
         if (bounce == 0u) ev_inc(ev, GR_EV_FIRST_MISS);
         ps.escaped = true;
-        fnv(ps.fh, 0xFFFFu);
+        if (kInfo) fnv(ps.fh, 0xFFFFu);
         ps.term_bounce = bounce;
         ps.term_reason = 0u;
         return true;
@@ -449,7 +452,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             ps.ray_d = new_dir;
             ps.cur_k = new_dir;
             ps.have_prev = false;
-            fnv(ps.fh, 0xFFFEu);
+            if (kInfo) fnv(ps.fh, 0xFFFEu);
             if (bounce > 4) {
                 // apply_russian_roulette (transport.rs:204-220); an ended path is
                 // PathTermination::ScatterAbsorbed.
@@ -579,6 +582,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             const V3 l = bx_local(fr, normalize_or_zero(cur_k));
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
+                if (!kInfo && !GR_LIVE(ps, k)) continue;
                 float ns, nf;
                 bx_wave_indices(bx_of(LC_NO(k), dba, dn), l, &ns, &nf);
                 n_med[k] = use_a ? nf : ns;
@@ -592,6 +596,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             const V3 lb = bx_local(fr, normalize_or_zero(dir_b));
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
+                if (!kInfo && !GR_LIVE(ps, k)) continue;
                 const Bx bk = bx_of(LC_NO(k), dba, dn);
                 float ns, nf;
                 bx_wave_indices(bk, la, &ns, &nf);
@@ -631,16 +636,21 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 st = sqrtf(fmaxf(fmaf(ct, -ct, 1.0f), 0.0f));
             }
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
+            for (int k = 0; k < 8; ++k) {
+                if (!kInfo && !GR_LIVE(ps, k)) continue;
                 n_med[k] = effective_extraordinary_index_sc(LC_NO(k), LC_NO(k) + dn, st, ct);
+            }
         }
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) n_med[k] = LC_NO(k);
     }
+    // A dead channel's index and sin^2(theta_t) are read only by the fate hash (the
+    // reflection and refraction loops skip the channel).
     float s2t_ch[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
+        if (!kInfo && !GR_LIVE(ps, k)) continue;
         float n1k = inside ? n_med[k] : 1.0f;
         float n2k = inside ? 1.0f : n_med[k];
         float etak = n1k / n2k;
@@ -832,9 +842,9 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     if (kAniso && internal_reflection)
         ps.is_extra = stream_draw(ps.seed, bounce, MODE_COUPLING_STREAM) < 0.5f;
     ps.last_facet = (uint32_t)facet;
-    fnv(ps.fh, (uint32_t)facet);
-    fnv(ps.fh, event | (ps.is_extra ? 16u : 0u));
-    {
+    if (kInfo) {
+        fnv(ps.fh, (uint32_t)facet);
+        fnv(ps.fh, event | (ps.is_extra ? 16u : 0u));
         // Per-channel fates: past the channel's critical angle / pdf zero.
         uint32_t fate = 0u;
 #pragma unroll
@@ -915,11 +925,15 @@ __device__ __forceinline__ V3 finish_escaped(V3 dir, const float *inten, const f
     for (int k = 0; k < 8; ++k) sum_pdf = sum_pdf + pdf[k];
     float mis = sum_pdf <= 1e-12f ? 1.0f : (8.0f * pdf[0]) / sum_pdf;
     const float norm_factor = (400.0f / 8.0f) / 106.856f;
+    // A channel whose weight is +-0 (a dead channel: zero intensity) adds +-0 to each
+    // component: the CIE lobes are finite, and a sum that starts at +0 never holds -0, so
+    // x + (+-0) == x bit for bit.  Its colour-matching evaluation is skipped.
     V3 xyz = v3(0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
+        const float w = rad[k] * mis * norm_factor;
+        if (w == 0.0f) continue;
         V3 c = cmf_x8_channel(comb_lambda(hero_off, k));
-        float w = rad[k] * mis * norm_factor;
         xyz = add(xyz, mul(c, w));
     }
     V3 lms = v3(fmaf(0.8951f, xyz.x, fmaf(0.2664f, xyz.y, -0.1614f * xyz.z)),
@@ -1160,7 +1174,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
         PathState ps;
         init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
         bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
+        while (!bounce_step<kAniso, kAbsorb, kFrost, kInfo>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == split_bounce && park<kAniso == 2>(ps, t, lcache, cont, cont_cap, cont_count)) {
                 parked = true;
                 break;
@@ -1200,7 +1214,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
         PathState ps;
         const uint32_t t = load_path<kAniso == 2>(ps, lcache, cont, cont_cap, r);
         bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
+        while (!bounce_step<kAniso, kAbsorb, kFrost, kInfo>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == split_bounce && park<kAniso == 2>(ps, t, lcache, next, cont_cap, next_count)) {
                 parked = true;
                 break;
@@ -1295,7 +1309,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, make_float2(J.ri35[2 * s], J.ri35[2 * s + 1]),
                   J.cert[J.pixel_begin + local_pixel], sc);
         bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
+        while (!bounce_step<kAniso, kAbsorb, kFrost, false>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == J.split_bounce &&
                 park<kAniso == 2>(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
                 parked = true;
@@ -1335,7 +1349,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
         const uint32_t t = load_path<kAniso == 2>(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
         const uint32_t next_split = level < kContLevels ? (uint32_t)(level + 1) * J.split_bounce : 0u;
         bool parked = false;
-        while (!bounce_step<kAniso, kAbsorb, kFrost>(ps, sc, planes, cam_side, lcache, ev)) {
+        while (!bounce_step<kAniso, kAbsorb, kFrost, false>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == next_split &&
                 park<kAniso == 2>(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
                 parked = true;
@@ -1392,7 +1406,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
         init_path(ps, pixel_begin + lp, sample_offset + s, ri35[s], cert[pixel_begin + lp], sc);
     }
     while (__ballot(busy) != 0ull) {
-        if (busy && bounce_step<kAniso, kAbsorb, true>(ps, sc, planes, cam_side, lcache, ev)) {
+        if (busy && bounce_step<kAniso, kAbsorb, true, kInfo>(ps, sc, planes, cam_side, lcache, ev)) {
             // Path ended: escape record (and per-tuple info).
             float *r = rec + t;
             r[0 * (size_t)rec_cap] = ps.ray_d.x;
