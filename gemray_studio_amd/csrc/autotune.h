@@ -31,8 +31,8 @@ struct TuneConfig {
 // The values each knob is searched over, the current value first.
 struct TuneSpace {
     std::vector<uint32_t> waves_per_eu{3, 2, 4};
-    std::vector<uint32_t> split_bounce{6, 4, 5, 7, 8, 10, 0};
-    std::vector<uint32_t> cont_div{4, 2, 8};
+    std::vector<uint32_t> split_bounce{6, 3, 4, 5, 7, 8, 10, 0};
+    std::vector<uint32_t> cont_div{4, 2, 1, 8};
     std::vector<uint64_t> chunk_tuples{1ull << 25, 1ull << 24, 1ull << 23};
     std::vector<uint32_t> trace_mode{0, 1};
 };

@@ -956,8 +956,16 @@ __device__ __forceinline__ uint32_t path_segments(const PathState &ps, const Dev
 // termination bounce and reason and the two path hashes, 85 the mode flags (the dead-channel
 // mask in bits 8-15), and in the
 // kernels built for biaxial stones (kBx) 86-93 the beta-axis absorption cache.
+//
+// A dead channel's fields are neither written nor read: its Stokes vector is +-0 and its
+// pdf +0, and every later use of them gives the same result for +0 (the roulette's maximum
+// intensity is clamped up to 0.05, the radiance fma(+-0, r, +0) is +0 and a +-0 weight adds
+// nothing), so a continued path reloads them as +0; its absorption caches are read by no
+// later step, and its dispersion index only by the fate hash (kInfo).  A path parked deep
+// in the stone, whose companions have mostly died at the entry refraction, moves about half
+// of its record.
 
-template <bool kBx>
+template <bool kBx, bool kInfo>
 __device__ __forceinline__ void save_path(const PathState &ps, uint32_t t, const float *__restrict__ lcache,
                                           float *__restrict__ cont, uint32_t cap, uint32_t r) {
     float *c = cont + r;
@@ -973,32 +981,33 @@ __device__ __forceinline__ void save_path(const PathState &ps, uint32_t t, const
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
+        const bool live = GR_LIVE(ps, k);
+        if (kInfo || live) put(53 + k, LC_NO(k));
+        if (!live) continue;
         put(13 + 4 * k, ps.st[k].i);
         put(14 + 4 * k, ps.st[k].q);
         put(15 + 4 * k, ps.st[k].u);
         put(16 + 4 * k, ps.st[k].v);
         put(45 + k, ps.pdf[k]);
-        put(53 + k, LC_NO(k));
         put(61 + k, LC_AO(k));
         put(69 + k, LC_AE(k));
+        if (kBx) put(86 + k, LC_AB(k));
     }
     put(77, ps.hero_off);
     putu(78, ps.seed);
     putu(79, ps.bounce);
     putu(80, ps.last_facet);
-    putu(81, ps.term_bounce);
-    putu(82, ps.term_reason);
-    putu(83, ps.fh);
-    putu(84, ps.fate_h);
+    if (kInfo) {  // read only by the per-path record
+        putu(81, ps.term_bounce);
+        putu(82, ps.term_reason);
+        putu(83, ps.fh);
+        putu(84, ps.fate_h);
+    }
     putu(85, (ps.inside ? 1u : 0u) | (ps.is_extra ? 2u : 0u) | (ps.have_prev ? 4u : 0u) | (ps.have_n ? 8u : 0u) |
                  (ps.have_abs ? 16u : 0u) | (ps.escaped ? 32u : 0u) | (ps.dead << 8));
-    if (kBx) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) put(86 + k, LC_AB(k));
-    }
 }
 
-template <bool kBx>
+template <bool kBx, bool kInfo>
 __device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__ lcache,
                                               const float *__restrict__ cont, uint32_t cap, uint32_t r) {
     const float *c = cont + r;
@@ -1008,35 +1017,44 @@ __device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__
     ps.ray_d = v3(get(4), get(5), get(6));
     ps.cur_k = v3(get(7), get(8), get(9));
     ps.prev_pn = v3(get(10), get(11), get(12));
+    const uint32_t fl = getu(85);
+    ps.dead = (fl >> 8) & 0xFFu;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
+        const bool live = GR_LIVE(ps, k);
+        if (kInfo || live) LC_NO(k) = get(53 + k);
+        if (!live) {
+            ps.st[k] = Stokes{0.0f, 0.0f, 0.0f, 0.0f};
+            ps.pdf[k] = 0.0f;
+            continue;
+        }
         ps.st[k] = Stokes{get(13 + 4 * k), get(14 + 4 * k), get(15 + 4 * k), get(16 + 4 * k)};
         ps.pdf[k] = get(45 + k);
-        LC_NO(k) = get(53 + k);
         LC_AO(k) = get(61 + k);
         LC_AE(k) = get(69 + k);
+        if (kBx) LC_AB(k) = get(86 + k);
     }
     ps.hero_off = get(77);
     ps.seed = getu(78);
     ps.bounce = getu(79);
     ps.last_facet = getu(80);
     ps.cert = 0xFFFFFFFFu;  // used at bounce 0 only
-    ps.term_bounce = getu(81);
-    ps.term_reason = getu(82);
-    ps.fh = getu(83);
-    ps.fate_h = getu(84);
-    const uint32_t fl = getu(85);
+    if (kInfo) {
+        ps.term_bounce = getu(81);
+        ps.term_reason = getu(82);
+        ps.fh = getu(83);
+        ps.fate_h = getu(84);
+    } else {
+        ps.term_bounce = 0u;
+        ps.term_reason = 3u;
+        ps.fh = ps.fate_h = 0u;
+    }
     ps.inside = fl & 1u;
     ps.is_extra = fl & 2u;
     ps.have_prev = fl & 4u;
     ps.have_n = fl & 8u;
     ps.have_abs = fl & 16u;
     ps.escaped = fl & 32u;
-    ps.dead = (fl >> 8) & 0xFFu;
-    if (kBx) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) LC_AB(k) = get(86 + k);
-    }
     return getu(0);
 }
 
@@ -1044,7 +1062,7 @@ __device__ __forceinline__ uint32_t load_path(PathState &ps, float *__restrict__
 // the path into it; false when the records have run out (the path then carries on).
 // Called by all live lanes of a wave together: in the plain and continuation kernels
 // every live lane reaches a given bounce count in the same loop iteration.
-template <bool kBx>
+template <bool kBx, bool kInfo>
 __device__ __forceinline__ bool park(const PathState &ps, uint32_t t, const float *__restrict__ lcache,
                                      float *__restrict__ cont, uint32_t cap, uint32_t *__restrict__ count) {
     const uint64_t live = __ballot(1);
@@ -1055,7 +1073,7 @@ __device__ __forceinline__ bool park(const PathState &ps, uint32_t t, const floa
     base = __shfl(base, (int)leader, 64);
     const uint32_t r = base + (uint32_t)__popcll(live & ((1ull << lane) - 1ull));
     if (r >= cap) return false;
-    save_path<kBx>(ps, t, lcache, cont, cap, r);
+    save_path<kBx, kInfo>(ps, t, lcache, cont, cap, r);
     return true;
 }
 
@@ -1175,7 +1193,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
         init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb, kFrost, kInfo>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == split_bounce && park<kAniso == 2>(ps, t, lcache, cont, cont_cap, cont_count)) {
+            if (ps.bounce == split_bounce && park<kAniso == 2, kInfo>(ps, t, lcache, cont, cont_cap, cont_count)) {
                 parked = true;
                 break;
             }
@@ -1212,10 +1230,10 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r < count) {
         PathState ps;
-        const uint32_t t = load_path<kAniso == 2>(ps, lcache, cont, cont_cap, r);
+        const uint32_t t = load_path<kAniso == 2, kInfo>(ps, lcache, cont, cont_cap, r);
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb, kFrost, kInfo>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == split_bounce && park<kAniso == 2>(ps, t, lcache, next, cont_cap, next_count)) {
+            if (ps.bounce == split_bounce && park<kAniso == 2, kInfo>(ps, t, lcache, next, cont_cap, next_count)) {
                 parked = true;
                 break;
             }
@@ -1311,7 +1329,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb, kFrost, false>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == J.split_bounce &&
-                park<kAniso == 2>(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
+                park<kAniso == 2, false>(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
                 parked = true;
                 break;
             }
@@ -1346,12 +1364,12 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
 
     if (r < count) {
         PathState ps;
-        const uint32_t t = load_path<kAniso == 2>(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
+        const uint32_t t = load_path<kAniso == 2, false>(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
         const uint32_t next_split = level < kContLevels ? (uint32_t)(level + 1) * J.split_bounce : 0u;
         bool parked = false;
         while (!bounce_step<kAniso, kAbsorb, kFrost, false>(ps, sc, planes, cam_side, lcache, ev)) {
             if (ps.bounce == next_split &&
-                park<kAniso == 2>(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
+                park<kAniso == 2, false>(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
                 parked = true;
                 break;
             }

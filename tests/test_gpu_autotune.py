@@ -20,8 +20,9 @@ def gpu():
 
 
 BASE = {"waves_per_eu": 3, "split_bounce": 6, "cont_div": 4, "trace_mode": 0, "chunk_tuples": 1 << 25}
-VARIANTS = ([dict(BASE, waves_per_eu=w) for w in (2, 4)] + [dict(BASE, split_bounce=s) for s in (0, 4, 5, 7, 8, 10)]
-            + [dict(BASE, cont_div=d) for d in (2, 8, 64)] + [dict(BASE, chunk_tuples=c) for c in (1 << 14, 1 << 17)]
+VARIANTS = ([dict(BASE, waves_per_eu=w) for w in (2, 4)] + [dict(BASE, split_bounce=s) for s in (0, 3, 4, 5, 7, 8, 10)]
+            + [dict(BASE, cont_div=d) for d in (1, 2, 8, 64)] + [dict(BASE, chunk_tuples=c) for c in (1 << 14, 1 << 17)]
+            + [dict(BASE, split_bounce=3, cont_div=1)]
             + [dict(BASE, trace_mode=1), dict(BASE, waves_per_eu=4, split_bounce=3, cont_div=8, chunk_tuples=1 << 15)])
 
 
