@@ -52,6 +52,25 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// One device block for the jobs admitted together: their scene blocks, planes, facet
+// finishes, sample tables and certificate table, then their counters, certificates and
+// films.  An admission round makes one allocation, one upload (from pinned staging), one
+// clear and one certificate launch for all of its jobs instead of a dozen calls per job; the
+// block is freed, stream-ordered, when the last of its jobs retires.
+struct Arena {
+    void *d = nullptr;
+    hipStream_t stream = nullptr;
+    ~Arena() {
+        if (d) (void)hipFreeAsync(d, stream);
+    }
+};
+
+// A job whose outputs exceed this gets a block of its own, so that a long job does not
+// hold the memory of the short jobs admitted with it.
+constexpr uint64_t kOwnArenaBytes = 32ull << 20;
+// Films up to this size are read back through pinned staging.
+constexpr size_t kStagedFilmBytes = 2ull << 20;
+
 struct Job {
     uint64_t id = 0;
     int state = GR_JOB_QUEUED;
@@ -62,9 +81,6 @@ struct Job {
     std::vector<float> map;              // a map the request passed as host texels, or
     const gr_env_map *env_map = nullptr;  // the device-resident map it references
     DevScene host_sc{};
-    // Host staging of the admission copies, kept until the control stream has run them.
-    DevScene upload_sc{};
-    std::vector<float2> upload_ri;
     int cls = 0;
     uint32_t width = 0, height = 0;
     // The requested range, and the sub-batch being traced: pieces cover samples
@@ -80,7 +96,9 @@ struct Job {
     bool delta_taken = false;
     uint64_t n_pixels = 0;
     uint64_t footprint = 0;
-    // Device copies (from admission to completion).
+    // Device copies (from admission to completion): in the admission round's block, except
+    // a map the job brought as host texels.
+    std::shared_ptr<Arena> arena;
     DevScene *d_scene = nullptr;
     float4 *d_planes = nullptr;
     uint32_t *d_finishes = nullptr;
@@ -159,6 +177,12 @@ struct ServeState {
     // other runs.
     hipStream_t stream = nullptr;
     TickSlot slot[kTickSlots];
+    // Pinned host staging of the control stream's uploads and read-backs (reused once the
+    // stream has been synchronized, which the loop does after every admission and read-back
+    // round).
+    unsigned char *h_stage = nullptr, *h_back = nullptr;
+    size_t stage_cap = 0, back_cap = 0;
+    std::vector<unsigned char *> retired_pinned;
     uint64_t ticks_launched = 0, ticks_done = 0;
     size_t rr = 0;  // round-robin start of the next tick
     // Metrics since the last reset.
@@ -200,9 +224,8 @@ void drop_env_map(ServeState &S, Job &j) {
 }
 
 void free_job_device(ServeState &S, Job &j) {
-    for (void *p : {(void *)j.d_scene, (void *)j.d_planes, (void *)j.d_finishes, (void *)j.d_map, (void *)j.d_ri35,
-                    (void *)j.d_cert, (void *)j.d_film, (void *)j.d_stats})
-        if (p) (void)hipFreeAsync(p, S.stream);
+    if (j.d_map) (void)hipFreeAsync(j.d_map, S.stream);
+    j.arena.reset();
     drop_env_map(S, j);
     j.d_scene = nullptr;
     j.d_planes = nullptr;
@@ -214,59 +237,137 @@ void free_job_device(ServeState &S, Job &j) {
     j.d_stats = nullptr;
 }
 
-// Queues the admission of a job on the control stream: device buffers, input copies and
-// the miss certificates.  The host staging lives in the job until the stream has run the
-// copies (the loop synchronizes once for every job it admits at a time).
-gr_status upload_job(ServeState &S, Job &j) {
-    SV_CHECK(hipMallocAsync((void **)&j.d_scene, sizeof(DevScene), S.stream));
-    SV_CHECK(hipMallocAsync((void **)&j.d_planes, j.planes.size() * sizeof(float), S.stream));
-    if (!j.finishes.empty()) SV_CHECK(hipMallocAsync((void **)&j.d_finishes, j.finishes.size() * sizeof(uint32_t), S.stream));
-    if (!j.map.empty()) SV_CHECK(hipMallocAsync((void **)&j.d_map, j.map.size() * sizeof(float), S.stream));
-    SV_CHECK(hipMallocAsync((void **)&j.d_ri35, (size_t)j.total_spp * sizeof(float2), S.stream));
-    SV_CHECK(hipMallocAsync((void **)&j.d_cert, j.n_pixels * sizeof(uint32_t), S.stream));
-    SV_CHECK(hipMallocAsync((void **)&j.d_film, j.n_pixels * 3 * sizeof(float), S.stream));
-    SV_CHECK(hipMallocAsync((void **)&j.d_stats, GR_EV_COUNT * sizeof(unsigned long long), S.stream));
-    DevScene &sc = j.upload_sc;
-    sc = j.host_sc;
-    sc.map_rgb = j.env_map ? j.env_map->d : j.d_map;
-    sc.finishes = j.d_finishes;
-    std::vector<float2> &ri = j.upload_ri;
-    ri.resize(j.total_spp);
-    for (uint32_t i = 0; i < j.total_spp; ++i)
-        ri[i] = make_float2(h_radical_inverse(j.first_sample + i, 3), h_radical_inverse(j.first_sample + i, 5));
-    SV_CHECK(hipMemcpyAsync(j.d_scene, &sc, sizeof(DevScene), hipMemcpyHostToDevice, S.stream));
-    SV_CHECK(hipMemcpyAsync(j.d_planes, j.planes.data(), j.planes.size() * sizeof(float), hipMemcpyHostToDevice,
-                            S.stream));
-    if (!j.finishes.empty())
-        SV_CHECK(hipMemcpyAsync(j.d_finishes, j.finishes.data(), j.finishes.size() * sizeof(uint32_t),
-                                hipMemcpyHostToDevice, S.stream));
-    if (!j.map.empty())
-        SV_CHECK(hipMemcpyAsync(j.d_map, j.map.data(), j.map.size() * sizeof(float), hipMemcpyHostToDevice,
-                                S.stream));
-    SV_CHECK(hipMemcpyAsync(j.d_ri35, ri.data(), ri.size() * sizeof(float2), hipMemcpyHostToDevice, S.stream));
-    SV_CHECK(hipMemsetAsync(j.d_film, 0, j.n_pixels * 3 * sizeof(float), S.stream));
-    SV_CHECK(hipMemsetAsync(j.d_stats, 0, GR_EV_COUNT * sizeof(unsigned long long), S.stream));
-    SV_CHECK(gr_launch_cert(sc, j.d_planes, j.d_cert, S.stream));
+// Grows a pinned host buffer to at least `want` bytes, geometrically from `min_bytes`.
+// Freeing pinned memory waits for the device, so an outgrown buffer is kept in `retired`
+// until the loop's resources are released (with doubling, at most the final size more).
+// Called only while no copy from or into the buffer is outstanding.
+gr_status reserve_pinned(unsigned char *&buf, size_t &cap, size_t want, size_t min_bytes,
+                         std::vector<unsigned char *> &retired) {
+    if (want <= cap) return GR_OK;
+    const size_t c = std::max({want, 2 * cap, min_bytes});
+    if (buf) retired.push_back(buf);
+    buf = nullptr;
+    cap = 0;
+    SV_CHECK(hipHostMalloc((void **)&buf, c, hipHostMallocDefault));
+    cap = c;
     return GR_OK;
 }
 
-// After the control stream has run a job's admission: its host staging and host copy of
-// the environment map are no longer needed.
-void uploaded(Job &j) {
-    std::vector<float>().swap(j.map);
-    std::vector<float2>().swap(j.upload_ri);
+constexpr size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Output bytes of a job in its block: counters, certificates, film.
+size_t job_out_bytes(const Job &j) {
+    return align_up(GR_EV_COUNT * sizeof(unsigned long long)) + align_up(j.n_pixels * sizeof(uint32_t)) +
+           align_up(j.n_pixels * 3 * sizeof(float));
+}
+// Input bytes of a job in its block: scene block, planes, finishes, sample table.
+size_t job_in_bytes(const Job &j) {
+    return align_up(sizeof(DevScene)) + align_up(j.planes.size() * sizeof(float)) +
+           align_up(j.finishes.size() * sizeof(uint32_t)) + align_up((size_t)j.total_spp * sizeof(float2));
 }
 
-// Queues the read-back of a job's finished (sub-)batch film into `out` and, after its last
-// batch, of the job's counters; before a further sub-batch the device film is cleared.  The
-// caller synchronizes the control stream once for all the jobs it reads back.
-gr_status read_batch(ServeState &S, Job &j, std::vector<float> &out, bool last) {
-    out.resize(j.n_pixels * 3);
-    SV_CHECK(hipMemcpyAsync(out.data(), j.d_film, j.n_pixels * 3 * sizeof(float), hipMemcpyDeviceToHost, S.stream));
-    if (last) SV_CHECK(hipMemcpyAsync(j.events, j.d_stats, sizeof(j.events), hipMemcpyDeviceToHost, S.stream));
-    if (!last) SV_CHECK(hipMemsetAsync(j.d_film, 0, j.n_pixels * 3 * sizeof(float), S.stream));
-    return GR_OK;
+// Queues the admission of `jobs` on the control stream: the jobs' blocks (the small jobs
+// share one, each large one has its own), the upload of their inputs from pinned staging,
+// the clear of their counters and films and one certificate launch per block; a map brought
+// as host texels is allocated and copied per job.  st[i] receives job i's status.  The
+// caller synchronizes the control stream once for the whole round.
+void admit_round(ServeState &S, const std::vector<std::shared_ptr<Job>> &jobs, std::vector<gr_status> &st) {
+    std::vector<std::vector<size_t>> groups(1);
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        if (job_out_bytes(*jobs[i]) > kOwnArenaBytes) groups.push_back({i});
+        else groups[0].push_back(i);
+    }
+    // Pinned staging for every block's inputs and certificate table.
+    std::vector<size_t> in_bytes(groups.size(), 0), stage_off(groups.size(), 0);
+    size_t stage_total = 0;
+    for (size_t g = 0; g < groups.size(); ++g) {
+        for (size_t i : groups[g]) in_bytes[g] += job_in_bytes(*jobs[i]);
+        in_bytes[g] += align_up(groups[g].size() * sizeof(CertSlice));
+        stage_off[g] = stage_total;
+        stage_total += in_bytes[g];
+    }
+    const gr_status rs = reserve_pinned(S.h_stage, S.stage_cap, stage_total, (size_t)4 << 20, S.retired_pinned);
+    for (size_t g = 0; g < groups.size(); ++g) {
+        if (groups[g].empty()) continue;
+        auto fail = [&](gr_status e) {
+            for (size_t i : groups[g]) st[i] = e;
+        };
+        if (rs != GR_OK) {
+            fail(rs);
+            continue;
+        }
+        size_t out_bytes = 0;
+        for (size_t i : groups[g]) out_bytes += job_out_bytes(*jobs[i]);
+        auto arena = std::make_shared<Arena>();
+        arena->stream = S.stream;
+        if (hipMallocAsync(&arena->d, in_bytes[g] + out_bytes, S.stream) != hipSuccess) {
+            arena->d = nullptr;
+            fail(GR_ERR_DEVICE);
+            continue;
+        }
+        unsigned char *dev = (unsigned char *)arena->d;
+        unsigned char *host = S.h_stage + stage_off[g];
+        size_t in_off = 0, out_off = in_bytes[g];
+        std::vector<CertSlice> certs;
+        uint32_t blocks = 0;
+        gr_status gst = GR_OK;
+        for (size_t i : groups[g]) {
+            Job &j = *jobs[i];
+            j.arena = arena;
+            auto take_in = [&](size_t bytes) {
+                const size_t o = in_off;
+                in_off += align_up(bytes);
+                return o;
+            };
+            auto take_out = [&](size_t bytes) {
+                const size_t o = out_off;
+                out_off += align_up(bytes);
+                return o;
+            };
+            const size_t o_scene = take_in(sizeof(DevScene));
+            const size_t o_planes = take_in(j.planes.size() * sizeof(float));
+            const size_t o_fin = take_in(j.finishes.size() * sizeof(uint32_t));
+            const size_t o_ri = take_in((size_t)j.total_spp * sizeof(float2));
+            j.d_scene = (DevScene *)(dev + o_scene);
+            j.d_planes = (float4 *)(dev + o_planes);
+            j.d_finishes = j.finishes.empty() ? nullptr : (uint32_t *)(dev + o_fin);
+            j.d_ri35 = (float2 *)(dev + o_ri);
+            j.d_stats = (unsigned long long *)(dev + take_out(GR_EV_COUNT * sizeof(unsigned long long)));
+            j.d_cert = (uint32_t *)(dev + take_out(j.n_pixels * sizeof(uint32_t)));
+            j.d_film = (float *)(dev + take_out(j.n_pixels * 3 * sizeof(float)));
+            if (!j.map.empty()) {
+                if (hipMallocAsync((void **)&j.d_map, j.map.size() * sizeof(float), S.stream) != hipSuccess ||
+                    hipMemcpyAsync(j.d_map, j.map.data(), j.map.size() * sizeof(float), hipMemcpyHostToDevice,
+                                   S.stream) != hipSuccess)
+                    gst = GR_ERR_DEVICE;
+            }
+            DevScene sc = j.host_sc;
+            sc.map_rgb = j.env_map ? j.env_map->d : j.d_map;
+            sc.finishes = j.d_finishes;
+            std::memcpy(host + o_scene, &sc, sizeof(DevScene));
+            std::memcpy(host + o_planes, j.planes.data(), j.planes.size() * sizeof(float));
+            if (!j.finishes.empty()) std::memcpy(host + o_fin, j.finishes.data(), j.finishes.size() * sizeof(uint32_t));
+            float2 *ri = (float2 *)(host + o_ri);
+            for (uint32_t k = 0; k < j.total_spp; ++k)
+                ri[k] = make_float2(h_radical_inverse(j.first_sample + k, 3), h_radical_inverse(j.first_sample + k, 5));
+            certs.push_back(CertSlice{j.d_scene, j.d_planes, j.d_cert, (uint32_t)j.n_pixels, blocks});
+            blocks += (uint32_t)((j.n_pixels + kBlockTuples - 1) / kBlockTuples);
+        }
+        const size_t o_table = in_off;
+        std::memcpy(host + o_table, certs.data(), certs.size() * sizeof(CertSlice));
+        if (gst == GR_OK &&
+            (hipMemcpyAsync(dev, host, in_bytes[g], hipMemcpyHostToDevice, S.stream) != hipSuccess ||
+             hipMemsetAsync(dev + in_bytes[g], 0, out_bytes, S.stream) != hipSuccess ||
+             gr_launch_cert_batch((const CertSlice *)(dev + o_table), (uint32_t)certs.size(), blocks, S.stream) !=
+                 hipSuccess))
+            gst = GR_ERR_DEVICE;
+        if (gst != GR_OK) fail(gst);
+    }
 }
+
+// After the control stream has run a job's admission: its host copy of the environment
+// map is no longer needed.
+void uploaded(Job &j) { std::vector<float>().swap(j.map); }
 
 gr_status ensure_staging(TickSlot &T, uint64_t tuples, uint32_t slices) {
     if (tuples > T.staging_cap) {
@@ -520,15 +621,55 @@ gr_status complete_tick(ServeState &S, TickSlot &T, std::vector<Job *> &finished
 // leaving it idle; until retire() marks a job done no API call reads its film.
 void finish_jobs(ServeState &S, std::vector<Job *> &finished) {
     if (finished.empty()) return;
-    std::vector<gr_status> fin(finished.size());
+    // Read-back round, with one synchronization: every finished (sub-)batch film -- a small
+    // one through pinned staging (a pageable copy costs more per call than its bytes), a
+    // large one straight into the job's host film -- and after a job's last batch its
+    // counters; before a further sub-batch the device film is cleared.
+    std::vector<gr_status> fin(finished.size(), GR_OK);
     std::vector<std::vector<float>> batch(finished.size());
+    std::vector<size_t> off(finished.size());
+    std::vector<bool> last(finished.size()), staged(finished.size());
+    size_t total = 0;
+    for (size_t i = 0; i < finished.size(); ++i) {
+        const size_t film_bytes = finished[i]->n_pixels * 3 * sizeof(float);
+        staged[i] = film_bytes <= kStagedFilmBytes;
+        off[i] = total;
+        total += (staged[i] ? align_up(film_bytes) : 0) + align_up(sizeof(finished[i]->events));
+    }
+    const gr_status rs = reserve_pinned(S.h_back, S.back_cap, total, (size_t)64 << 20, S.retired_pinned);
     for (size_t i = 0; i < finished.size(); ++i) {
         Job &j = *finished[i];
-        const bool last = j.produced + j.spp >= j.total_spp;
-        fin[i] = read_batch(S, j, j.progressive ? batch[i] : j.film, last);
+        last[i] = j.produced + j.spp >= j.total_spp;
+        if (rs != GR_OK) {
+            fin[i] = rs;
+            continue;
+        }
+        const size_t film_bytes = j.n_pixels * 3 * sizeof(float);
+        std::vector<float> &dst = j.progressive ? batch[i] : j.film;
+        if (!staged[i]) dst.resize(j.n_pixels * 3);
+        unsigned char *h = S.h_back + off[i];
+        void *film_dst = staged[i] ? (void *)h : (void *)dst.data();
+        unsigned char *ev_dst = h + (staged[i] ? align_up(film_bytes) : 0);
+        if (hipMemcpyAsync(film_dst, j.d_film, film_bytes, hipMemcpyDeviceToHost, S.stream) != hipSuccess ||
+            (last[i] && hipMemcpyAsync(ev_dst, j.d_stats, sizeof(j.events), hipMemcpyDeviceToHost, S.stream) !=
+                            hipSuccess) ||
+            (!last[i] && hipMemsetAsync(j.d_film, 0, film_bytes, S.stream) != hipSuccess))
+            fin[i] = GR_ERR_DEVICE;
     }
     if (hipStreamSynchronize(S.stream) != hipSuccess)
         for (gr_status &f : fin) f = GR_ERR_DEVICE;
+    for (size_t i = 0; i < finished.size(); ++i) {
+        if (fin[i] != GR_OK) continue;
+        Job &j = *finished[i];
+        const size_t film_bytes = j.n_pixels * 3 * sizeof(float);
+        if (staged[i]) {
+            std::vector<float> &dst = j.progressive ? batch[i] : j.film;
+            dst.resize(j.n_pixels * 3);
+            std::memcpy(dst.data(), S.h_back + off[i], film_bytes);
+        }
+        if (last[i])
+            std::memcpy(j.events, S.h_back + off[i] + (staged[i] ? align_up(film_bytes) : 0), sizeof(j.events));
+    }
     const Clock::time_point now = Clock::now();
     std::lock_guard<std::mutex> g(S.mu);
     for (size_t i = 0; i < finished.size(); ++i) {
@@ -605,7 +746,7 @@ void loop(ServeState *Sp) {
             S.cv_done.notify_all();
         }
         std::vector<gr_status> ust(admit.size(), GR_OK);
-        for (size_t i = 0; i < admit.size(); ++i) ust[i] = upload_job(S, *admit[i]);
+        if (!admit.empty()) admit_round(S, admit, ust);
         if (!admit.empty() && hipStreamSynchronize(S.stream) != hipSuccess)
             for (gr_status &u : ust) u = GR_ERR_DEVICE;
         for (size_t i = 0; i < admit.size(); ++i) {
@@ -690,6 +831,13 @@ void release_device(ServeState &S) {
     }
     if (S.stream) (void)hipStreamDestroy(S.stream);
     S.stream = nullptr;
+    S.retired_pinned.push_back(S.h_stage);
+    S.retired_pinned.push_back(S.h_back);
+    for (unsigned char *p : S.retired_pinned)
+        if (p) (void)hipHostFree(p);
+    S.retired_pinned.clear();
+    S.h_stage = S.h_back = nullptr;
+    S.stage_cap = S.back_cap = 0;
 }
 
 // Creates the loop on first use.

@@ -108,6 +108,16 @@ struct BatchSlice {
     uint32_t *cont_count;
 };
 
+// One job's frame in a batched certificate launch: workgroups [block_begin, block_begin +
+// ceil(n_pixels / kBlock)) fill cert[0, n_pixels) from the scene block and planes.
+struct CertSlice {
+    const DevScene *scene;  // device
+    const float4 *planes;   // device, scene->n_planes
+    uint32_t *cert;         // device, n_pixels
+    uint32_t n_pixels;
+    uint32_t block_begin;
+};
+
 // Whether a scene needs the kernels built for the rare features (kFrost): some facet
 // frosted, or a scattering material.
 inline bool rare_features(const DevScene &sc) { return sc.n_frosted != 0 || sc.sigma_s > 0.0f; }
@@ -137,6 +147,9 @@ hipError_t gr_launch_guide(const gr::DevScene &sc, const float4 *planes, float *
                            hipStream_t stream);
 // Fills cert[0, width * height) with the frame's camera-ray miss certificates.
 hipError_t gr_launch_cert(const gr::DevScene &sc, const float4 *planes, uint32_t *cert, hipStream_t stream);
+// Fills the certificates of slices [0, n_slices) (block_begin ascending, the first 0) in one
+// launch of n_blocks workgroups; bit-identical to one gr_launch_cert per slice.
+hipError_t gr_launch_cert_batch(const gr::CertSlice *slices, uint32_t n_slices, uint32_t n_blocks, hipStream_t stream);
 // Traces slices [0, n_slices) of one class (all with scene_class == cls) in one launch of
 // n_blocks workgroups, then, when the slices split paths, kContLevels continuation
 // launches of cont_blocks workgroups.

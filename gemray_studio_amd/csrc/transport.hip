@@ -8,14 +8,8 @@ namespace gr {
 // (jitter 0); when it misses because its entering extreme exceeds its exiting one, the two
 // extreme planes are the pixel's certificate, which every sample of the pixel then checks
 // exactly for its own ray (camera_miss_certified).
-__global__ __launch_bounds__(kBlock) void cert_kernel(const DevScene sc, const float4 *__restrict__ planes_g,
-                                                      uint32_t n_pixels, uint32_t *__restrict__ cert) {
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
-    stage_planes(planes, cam_side, planes_g, sc);
-    __syncthreads();
-    const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;
-    if (pixel >= n_pixels) return;
+__device__ __forceinline__ uint32_t pixel_cert(const DevScene &sc, const float4 *__restrict__ planes,
+                                               const float *__restrict__ cam_side, uint32_t pixel) {
     const uint32_t x = pixel % sc.width, y = pixel / sc.width;
     const float w = (float)sc.width, h = (float)sc.height;
     const float aspect = w / h;
@@ -30,7 +24,36 @@ __global__ __launch_bounds__(kBlock) void cert_kernel(const DevScene sc, const f
     bool outside = false;
     clip_scan<true>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
     const bool proof = !outside && t_near > t_far && near_idx >= 0 && far_idx >= 0;
-    cert[pixel] = proof ? ((uint32_t)near_idx | ((uint32_t)far_idx << 16)) : 0xFFFFFFFFu;
+    return proof ? ((uint32_t)near_idx | ((uint32_t)far_idx << 16)) : 0xFFFFFFFFu;
+}
+__global__ __launch_bounds__(kBlock) void cert_kernel(const DevScene sc, const float4 *__restrict__ planes_g,
+                                                      uint32_t n_pixels, uint32_t *__restrict__ cert) {
+    __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
+    stage_planes(planes, cam_side, planes_g, sc);
+    __syncthreads();
+    const uint32_t pixel = blockIdx.x * kBlock + threadIdx.x;
+    if (pixel >= n_pixels) return;
+    cert[pixel] = pixel_cert(sc, planes, cam_side, pixel);
+}
+// The certificates of many jobs' frames in one launch (the job loop's admission round):
+// workgroup b belongs to the last slice whose first block is <= b.
+__global__ __launch_bounds__(kBlock) void cert_batch_kernel(const CertSlice *__restrict__ slices, uint32_t n_slices) {
+    uint32_t lo = 0u, hi = n_slices;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (slices[mid].block_begin <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const CertSlice &J = slices[lo];
+    const DevScene &sc = *J.scene;
+    __shared__ float4 planes[GR_MAX_PLANES];
+    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
+    stage_planes(planes, cam_side, J.planes, sc);
+    __syncthreads();
+    const uint32_t pixel = (blockIdx.x - J.block_begin) * kBlock + threadIdx.x;
+    if (pixel >= J.n_pixels) return;
+    J.cert[pixel] = pixel_cert(sc, planes, cam_side, pixel);
 }
 
 // The denoiser's primary-hit guide buffers (apps/diagram-gui/src/bridge/guide_pass.rs:60-110):
@@ -290,6 +313,12 @@ hipError_t gr_launch_cert(const gr::DevScene &sc, const float4 *planes, uint32_t
     const uint32_t n = sc.width * sc.height;
     hipLaunchKernelGGL(gr::cert_kernel, dim3((n + gr::kBlock - 1) / gr::kBlock), dim3(gr::kBlock), 0, stream, sc,
                        planes, n, cert);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_cert_batch(const gr::CertSlice *slices, uint32_t n_slices, uint32_t n_blocks, hipStream_t stream) {
+    if (n_slices == 0 || n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(gr::cert_batch_kernel, dim3(n_blocks), dim3(gr::kBlock), 0, stream, slices, n_slices);
     return hipGetLastError();
 }
 
