@@ -669,9 +669,12 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     const bool tir = sin2_t > 1.0f;
     float r_unpol = 1.0f;
     bool reflect = tir;
+    // The hero's Fresnel amplitudes (set when !tir).  The channel loops below form channel
+    // 0's from the same operands (n1, n2, cos_i and cos_t = sqrt(1 - sin2_t), whose fmaxf
+    // with 0 is a no-op here), so they take these instead.
+    float r_s = 0.0f, r_p = 0.0f;
     if (!tir) {
         float cos_t = sqrtf(1.0f - sin2_t);
-        float r_s, r_p;
         fresnel_rs_rp(n1, n2, cos_i, cos_t, &r_s, &r_p);
         r_unpol = clampf(0.5f * fmaf(r_p, r_p, r_s * r_s), 1e-4f, 1.0f - 1e-4f);
         reflect = stream_draw(ps.seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol;
@@ -744,9 +747,11 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 tir_retardation_cs(n1k, cos_i, sin_i, &cd, &sd);
                 ps.st[k] = apply_retardation(ps.st[k], cd, sd);
             } else {
-                float cos_t_k = sqrtf(fmaxf(1.0f - s2t_ch[k], 0.0f));
-                float rs, rp;
-                fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
+                float rs = r_s, rp = r_p;  // channel 0 (not past its critical angle: !tir)
+                if (k != 0) {
+                    float cos_t_k = sqrtf(fmaxf(1.0f - s2t_ch[k], 0.0f));
+                    fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
+                }
                 ps.pdf[k] *= clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
                 ps.st[k] = apply_fresnel_reflection(ps.st[k], rs, rp);
             }
@@ -796,30 +801,43 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             final_dir = extraordinary_poynting_dir(refr, c_axis, n_o_hero, n_e_hero);
         }
         float inv_t = 1.0f / (1.0f - r_unpol);
+        // Channel 0's medium indices are the hero's (n1, n2_dir), so its q, cos_t_k and
+        // refracted (and walked-off) direction are eta_dir, cos_t_dir and final_dir, formed
+        // above by the same operations (its s2k never exceeds 1 here, so the fminf is a
+        // no-op); unless it enters in the ordinary mode (n2k = n_o instead of n2), its
+        // Fresnel amplitudes are the hero event's r_s, r_p.
+        const bool hero_rs = !(entering && !use_extra);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (!GR_LIVE(ps, k)) continue;
             float n1k = inside ? n_med[k] : 1.0f;
             float n2k = (entering && !use_extra) ? (kBiax ? n_biax_a[k] : LC_NO(k)) : (inside ? 1.0f : n_med[k]);
-            float q = n1k / n2k;
-            float s2k = q * q * one_m_c2;
-            bool alive = !(s2k > 1.0f);
-            float cos_t_k = sqrtf(fmaxf(1.0f - s2k, 0.0f));
-            if (alive) {
-                V3 dk = normalize(add(mul(cur_k, q), mul(normal, fmaf(q, cos_i, -cos_t_k))));
-                if (kBiax && entering)
-                    dk = bx_mode_poynting_dir(bx_of(LC_NO(k), dba, dn), fr, dk, use_extra);
-                else if (entering && use_extra)
-                    dk = extraordinary_poynting_dir(dk, c_axis, LC_NO(k), LC_NO(k) + dn);
-                alive = dot(dk, final_dir) >= DIRECTION_MATCH_COS_TOL;
+            float cos_t_k;
+            bool alive;
+            if (k == 0) {
+                cos_t_k = cos_t_dir;
+                alive = dot(final_dir, final_dir) >= DIRECTION_MATCH_COS_TOL;
+            } else {
+                float q = n1k / n2k;
+                float s2k = q * q * one_m_c2;
+                alive = !(s2k > 1.0f);
+                cos_t_k = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+                if (alive) {
+                    V3 dk = normalize(add(mul(cur_k, q), mul(normal, fmaf(q, cos_i, -cos_t_k))));
+                    if (kBiax && entering)
+                        dk = bx_mode_poynting_dir(bx_of(LC_NO(k), dba, dn), fr, dk, use_extra);
+                    else if (entering && use_extra)
+                        dk = extraordinary_poynting_dir(dk, c_axis, LC_NO(k), LC_NO(k) + dn);
+                    alive = dot(dk, final_dir) >= DIRECTION_MATCH_COS_TOL;
+                }
             }
             if (alive) {
                 float num = 2.0f * n1k * cos_i;
                 float t_s = num / fmaf(n2k, cos_t_k, n1k * cos_i);
                 float t_p = num / fmaf(n1k, cos_t_k, n2k * cos_i);
                 ps.st[k] = scale(apply_fresnel_transmission(ps.st[k], n1k, n2k, cos_i, cos_t_k, t_s, t_p), inv_t);
-                float rs, rp;
-                fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
+                float rs = r_s, rp = r_p;
+                if (k != 0 || !hero_rs) fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
                 ps.pdf[k] *= 1.0f - clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
             } else {
                 ps.st[k] = scale(ps.st[k], 0.0f);
