@@ -386,6 +386,28 @@ def test_declines_leave_film_untouched(gpu, srb):
         gpu.try_accumulate(B.SceneRef(material=M.by_name("Diamond"), **base), 0, 2, np.zeros((5, 3), np.float32))
 
 
+@pytest.mark.parametrize("idx", range(len(CASE_IDS)), ids=CASE_IDS)
+def test_kernels_with_and_without_path_records_agree(gpu, srb, idx):
+    """The kernels that write per-path records (trace_samples' info) and those that do not
+    (films, trace_samples without info) are separate builds: the latter skip the path
+    hashes and the dead channels' indices, which only the records read.  Every tuple's XYZ
+    is the same bit for bit, and the film is the in-order sum of those tuples."""
+    case = _cases(srb)[idx]
+    scene = case.product_scene()
+    n_pix = case.width * case.height
+    spp = 32
+    xyz_r, _ = gpu.trace_samples(scene, 0, n_pix, 7, spp)
+    xyz_n, _ = gpu.trace_samples(scene, 0, n_pix, 7, spp, with_info=False)
+    assert np.array_equal(xyz_r.view(np.uint32), xyz_n.view(np.uint32)), case.name
+    film = np.zeros((n_pix, 3), np.float32)
+    assert gpu.try_accumulate(scene, 7, spp, film)
+    want = np.zeros((n_pix, 3), np.float32)
+    per = xyz_r.reshape(n_pix, spp, 3)
+    for k in range(spp):  # sequential f32 sum in sample order (frame.rs:806-812)
+        want = (want + per[:, k, :]).astype(np.float32)
+    assert np.array_equal(film.view(np.uint32), want.view(np.uint32)), case.name
+
+
 @pytest.mark.parametrize("idx", [0, 2, 8, 10, 17, 21], ids=["diamond", "sapphire", "pleochroic-oval", "stress",
                                                             "tanzanite-oval", "ruby-scattering"])
 def test_plain_and_wavefront_kernels_agree_bit_for_bit(gpu, srb, idx):
