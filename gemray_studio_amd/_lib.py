@@ -135,7 +135,7 @@ EXPORTS = [
     "gr_autotune", "gr_set_autotune", "gr_render_tuned", "gr_optical_metrics_batch",
     "gr_set_trace_mode", "gr_job_submit", "gr_job_poll", "gr_job_wait", "gr_job_cancel",
     "gr_job_film", "gr_job_info", "gr_job_release", "gr_metrics", "gr_metrics_reset",
-    "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_serve_set_policy",
+    "gr_serve_set_tick_tuples", "gr_probe", "gr_set_split_bounce", "gr_set_hero_tables", "gr_serve_set_policy",
     "gr_optical_metrics", "gr_job_take_delta", "gr_guide_buffers",
 ]
 GR_SERVE_EQUAL_SHARE, GR_SERVE_DEADLINE = 0, 1
@@ -224,6 +224,8 @@ def load():
     L.gr_serve_set_policy.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     L.gr_set_split_bounce.argtypes = [vp, ctypes.c_uint32]
     L.gr_set_split_bounce.restype = ctypes.c_int
+    L.gr_set_hero_tables.argtypes = [vp, ctypes.c_int]
+    L.gr_set_hero_tables.restype = ctypes.c_int
     L.gr_optical_metrics.argtypes = [vp, fp, ctypes.c_uint32, ctypes.POINTER(MaterialStruct),
                                      ctypes.POINTER(MetricsPose), ctypes.c_uint32, ctypes.POINTER(GemMetrics)]
     L.gr_optical_metrics.restype = ctypes.c_int

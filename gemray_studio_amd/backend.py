@@ -519,6 +519,13 @@ class GpuBackend:
         self._require()
         check(self._L.gr_set_split_bounce(self._ctx, split_bounce), "gr_set_split_bounce")
 
+    def set_hero_tables(self, mode: int) -> None:
+        """0: evaluate every path's spectral functions; 1 (default): read the hero tables,
+        building a scene's material table for traces of >= 2^26 tuples; 2: also for smaller
+        traces.  Results are bit-identical in every mode."""
+        self._require()
+        check(self._L.gr_set_hero_tables(self._ctx, int(mode)), "gr_set_hero_tables")
+
     def set_chunk_tuples(self, tuples: int) -> None:
         self._require()
         check(self._L.gr_set_chunk_tuples(self._ctx, tuples), "gr_set_chunk_tuples")

@@ -329,6 +329,9 @@ struct gr_scene_dev {
     float *map = nullptr;             // the scene's own copy of its map, or
     const gr_env_map *env_map = nullptr;  // the device-resident map it references
     uint32_t *cert = nullptr;  // per-pixel camera-ray miss certificates
+    // The material's hero table, built by the first trace large enough to repay it (an
+    // absorbing scene only); the scene is otherwise read-only to traces.
+    mutable float *mat_tab = nullptr;
 };
 
 struct gr_film {
@@ -444,11 +447,48 @@ void free_scene(gr_scene_dev *s) {
     if (s->map) (void)hipFree(s->map);
     if (s->env_map) gr::env_map_release(s->env_map, nullptr);
     if (s->cert) (void)hipFree(s->cert);
+    if (s->mat_tab) (void)hipFree(s->mat_tab);
+    s->mat_tab = nullptr;
     s->planes = nullptr;
     s->finishes = nullptr;
     s->map = nullptr;
     s->env_map = nullptr;
     s->cert = nullptr;
+}
+
+// A material hero table costs about as much to fill as the lazy fills of kHeroEntries stone
+// paths (each evaluates the eight channels once); a trace of fewer tuples than this never
+// builds one.
+constexpr uint64_t kMatTableMinTuples = 1ull << 26;
+
+// Builds scene s's material hero table, stream-ordered before the traces that read it, when
+// the scene absorbs, the context reads hero tables and a trace of `tuples` tuples repays it.
+// Not having one (no memory) is not an error: the kernels then evaluate per path.
+gr_status ensure_mat_table(gr_ctx *ctx, const gr_scene_dev *s, uint64_t tuples) {
+    const int mode = ctx->hero_tables.load();
+    if (s->mat_tab || !s->sc.has_absorption || mode == 0 || (mode == 1 && tuples < kMatTableMinTuples)) return GR_OK;
+    float *tab = nullptr;
+    const size_t bytes = (size_t)gr::kHeroEntries * gr::mat_floats(s->sc.biaxial != 0) * sizeof(float);
+    if (hipMalloc(&tab, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return GR_OK;
+    }
+    if (gr_launch_mat_table(s->sc, tab, ctx->stream) != hipSuccess) {
+        (void)hipFree(tab);
+        return GR_ERR_DEVICE;
+    }
+    s->mat_tab = tab;
+    return GR_OK;
+}
+
+// The scene block a trace launch takes: the scene's own, with the hero tables the context
+// reads.
+gr::DevScene launch_scene(gr_ctx *ctx, const gr_scene_dev *s) {
+    gr::DevScene sc = s->sc;
+    const bool tables = ctx->hero_tables.load() != 0;
+    sc.spec_tab = tables ? ctx->spec_tab : nullptr;
+    sc.mat_tab = tables ? s->mat_tab : nullptr;
+    return sc;
 }
 
 // The device buffers one traced call uses besides the scene and the film: per-tuple XYZ
@@ -479,6 +519,7 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
                        uint32_t waves_per_eu) {
     GR_CHECK(hipMemsetAsync(w.stats, 0, GR_EV_COUNT * sizeof(unsigned long long), ctx->stream));
     GR_CHECK(hipEventRecord(ctx->ev_begin, ctx->stream));
+    const gr::DevScene sc = launch_scene(ctx, s);
     double kernel_ms = 0.0, finish_ms = 0.0;
     uint32_t queued = 0;  // chunks whose timing events have not been read
     auto read_slot = [&](uint32_t k) -> gr_status {
@@ -523,7 +564,7 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
         args.cont_count = w.cont_count;
         args.waves_per_eu = waves_per_eu;
         GR_CHECK(hipEventRecord(e.k0, ctx->stream));
-        GR_CHECK(gr_launch_trace(s->sc, args, ctx->stream));
+        GR_CHECK(gr_launch_trace(sc, args, ctx->stream));
         if (!wavefront) GR_CHECK(hipEventRecord(e.k1, ctx->stream));
         GR_CHECK(hipEventRecord(e.k2, ctx->stream));
         if (film) GR_CHECK(gr_launch_film_reduce(w.xyz, np, spp, first, film, ctx->stream));
@@ -594,6 +635,8 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
         if (st != GR_OK) return st;
     }
     st = ensure_ri35(ctx, sample_offset, spp);
+    if (st != GR_OK) return st;
+    st = ensure_mat_table(ctx, s, (uint64_t)n_pixels * spp);
     if (st != GR_OK) return st;
     Workspace w;
     w.xyz = ctx->xyz;
@@ -729,6 +772,16 @@ gr_status gr_open(gr_ctx **out, const gr_options *opts) {
         gr_close(ctx);
         return GR_ERR_DEVICE;
     }
+    // The spectral hero table (1.7 GB): without the memory for it the kernels evaluate the
+    // finish per path instead.
+    if (hipMalloc(&ctx->spec_tab, (size_t)gr::kHeroEntries * gr::kSpecFloats * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->spec_tab = nullptr;
+    } else if (gr_launch_spec_table(ctx->spec_tab, ctx->stream) != hipSuccess ||
+               hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        gr_close(ctx);
+        return GR_ERR_DEVICE;
+    }
     *out = ctx;
     return GR_OK;
 }
@@ -746,6 +799,7 @@ void gr_close(gr_ctx *ctx) {
     if (ctx->cont_count) (void)hipFree(ctx->cont_count);
     if (ctx->mx_buf) (void)hipFree(ctx->mx_buf);
     if (ctx->stats) (void)hipFree(ctx->stats);
+    if (ctx->spec_tab) (void)hipFree(ctx->spec_tab);
     for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end})
         if (e) (void)hipEventDestroy(e);
     for (gr_ctx::ChunkEvents &c : ctx->chunk_ev)
@@ -773,6 +827,13 @@ gr_status gr_set_split_bounce(gr_ctx *ctx, uint32_t split_bounce) {
     if (!ctx || split_bounce > GR_MAX_BOUNCES) return GR_ERR_INVALID;
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->split_bounce = split_bounce;
+    return GR_OK;
+}
+
+gr_status gr_set_hero_tables(gr_ctx *ctx, int mode) {
+    if (!ctx || mode < 0 || mode > 2) return GR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->hero_tables.store(mode);
     return GR_OK;
 }
 

@@ -66,6 +66,12 @@ struct gr_ctx {
     uint32_t ri35_cap = 0, ri35_offset = 0, ri35_n = 0;
     std::vector<float2> ri35_host;
     unsigned long long *stats = nullptr;
+    // Hero tables (transport.h): the spectral table, built at gr_open (null when its memory
+    // could not be had: the kernels then evaluate per path), and the gr_set_hero_tables mode
+    // (0 off, 1 on, 2 on with a material table for every absorbing scene's trace; results
+    // are the same bits in every mode).
+    float *spec_tab = nullptr;
+    std::atomic<int> hero_tables{1};
     // Plain mode: park paths running after this many segments.  Atomic: the job loop's
     // scheduler thread reads it without the context mutex, which a render holds throughout.
     std::atomic<uint32_t> split_bounce{6};

@@ -344,6 +344,7 @@ void admit_round(ServeState &S, const std::vector<std::shared_ptr<Job>> &jobs, s
             DevScene sc = j.host_sc;
             sc.map_rgb = j.env_map ? j.env_map->d : j.d_map;
             sc.finishes = j.d_finishes;
+            sc.spec_tab = S.ctx->hero_tables.load() != 0 ? S.ctx->spec_tab : nullptr;
             std::memcpy(host + o_scene, &sc, sizeof(DevScene));
             std::memcpy(host + o_planes, j.planes.data(), j.planes.size() * sizeof(float));
             if (!j.finishes.empty()) std::memcpy(host + o_fin, j.finishes.data(), j.finishes.size() * sizeof(uint32_t));

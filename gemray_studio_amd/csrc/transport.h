@@ -49,7 +49,29 @@ struct DevScene {
     float wb[3];
     int32_t map_w, map_h;
     const float *map_rgb;     // device pointer
+    // Hero tables (device, or null: the kernels then evaluate per path): the colour-matching
+    // values and map-spectrum lobes of every comb channel (kSpecFloats per entry, shared by
+    // every scene), and this material's dispersion indices and absorption spectra
+    // (mat_floats per entry; absorbing scenes only).
+    const float *spec_tab;
+    const float *mat_tab;
 };
+
+// Hero tables.  Every per-path quantity that depends on the path only through its comb
+// wavelengths -- the colour-matching values and map-spectrum lobes of the finish, and the
+// material's dispersion indices and absorption spectra -- is a function of the hero offset
+// hero_off = lambda_hero - 380 that init_path forms as fma(hero, 400, 380) - 380.  That
+// difference is exact (Sterbenz), so hero offsets lie on lambda_hero's float grid: multiples
+// of 2^-15 in [0, 132) (lambda_hero in [380, 512)) and of 2^-14 in [132, 400], kHeroEntries
+// values in all.  Entry e of a hero table holds the eight channels' values at offset
+// hero_key(e), evaluated by the same device functions the kernels evaluate per path, so a
+// lookup returns the same bits, and one path's lookups touch one contiguous entry.
+constexpr uint32_t kHeroLow = 132u << 15;                      // entries below offset 132
+constexpr uint32_t kHeroEntries = kHeroLow + (268u << 14) + 1u;  // through offset 400
+constexpr int kSpecFloats = 48;  // per channel k: CMF x, y, z at 3k; map lobes r, g, b at 24 + 3k
+// Material table: per channel k, the dispersion index at k, the ordinary and extraordinary
+// absorption sums at 8 + k and 16 + k, and (biaxial stones) the beta-axis sum at 24 + k.
+inline constexpr int mat_floats(bool biaxial) { return biaxial ? 32 : 24; }
 
 // Floats per continuation record (the parked state of one path), and the number of
 // continuation launches after the plain trace launch (paths are parked again at each
@@ -158,6 +180,10 @@ hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices
 // Adds each slice's per-pixel ordered sample sums into its job's film.
 hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
                                        hipStream_t stream);
+// Fills the spectral hero table (kHeroEntries x kSpecFloats floats at tab).
+hipError_t gr_launch_spec_table(float *tab, hipStream_t stream);
+// Fills scene sc's material hero table (kHeroEntries x mat_floats(sc.biaxial) floats at tab).
+hipError_t gr_launch_mat_table(const gr::DevScene &sc, float *tab, hipStream_t stream);
 // Fills ri35[0, n) with the radical inverses (bases 3, 5) of samples offset + i.
 hipError_t gr_launch_ri35(uint32_t offset, uint32_t n, float2 *ri35, hipStream_t stream);
 hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t spp,

@@ -212,6 +212,49 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__re
     }
 }
 
+// The spectral hero table (transport.h): entry e holds, for hero offset hero_key(e), each comb
+// channel's CIE colour-matching values (cmf_x8_channel) and map-spectrum lobes (rgb_lobes),
+// the calls finish_escaped makes per path.
+__global__ __launch_bounds__(kBlock) void spec_table_kernel(float *__restrict__ tab) {
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= kHeroEntries) return;
+    const float h = hero_key(e);
+    float *out = tab + (size_t)e * kSpecFloats;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float l = comb_lambda(h, k);
+        const V3 c = cmf_x8_channel(l);
+        const V3 lobes = rgb_lobes(l);
+        out[3 * k] = c.x;
+        out[3 * k + 1] = c.y;
+        out[3 * k + 2] = c.z;
+        out[24 + 3 * k] = lobes.x;
+        out[25 + 3 * k] = lobes.y;
+        out[26 + 3 * k] = lobes.z;
+    }
+}
+
+// A material's hero table (transport.h): entry e holds, for hero offset hero_key(e), each comb
+// channel's dispersion index and absorption sums, as the trace kernels' lazy fills evaluate
+// them (dispersion_index, band_sum; a biaxial stone without a beta band set repeats the
+// ordinary sum).
+template <bool kBiax>
+__global__ __launch_bounds__(kBlock) void mat_table_kernel(const DevScene sc, float *__restrict__ tab) {
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= kHeroEntries) return;
+    const float h = hero_key(e);
+    float *out = tab + (size_t)e * mat_floats(kBiax);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float l = comb_lambda(h, k);
+        const float so = band_sum<0>(l, sc);
+        out[k] = dispersion_index(l, sc);
+        out[8 + k] = so;
+        out[16 + k] = band_sum<1>(l, sc);
+        if (kBiax) out[24 + k] = sc.has_beta_bands ? band_sum<2>(l, sc) : so;
+    }
+}
+
 // The sample table of a call on the device: ri35[i] = the radical inverses (bases 3 and 5) of
 // sample offset + i (radical_inverse_base, sampling.rs:170-181), the operations the host's
 // table takes.
@@ -335,6 +378,21 @@ hipError_t gr_launch_film_reduce(const float *xyz, uint32_t n_pixels, uint32_t s
     dim3 grid((n_pixels + gr::kBlock - 1) / gr::kBlock);
     hipLaunchKernelGGL(gr::film_reduce_kernel, grid, dim3(gr::kBlock), 0, stream, xyz, n_pixels, spp,
                        pixel_begin, film);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_spec_table(float *tab, hipStream_t stream) {
+    hipLaunchKernelGGL(gr::spec_table_kernel, dim3((gr::kHeroEntries + gr::kBlock - 1) / gr::kBlock), dim3(gr::kBlock),
+                       0, stream, tab);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_mat_table(const gr::DevScene &sc, float *tab, hipStream_t stream) {
+    const dim3 grid((gr::kHeroEntries + gr::kBlock - 1) / gr::kBlock);
+    if (sc.biaxial)
+        hipLaunchKernelGGL(gr::mat_table_kernel<true>, grid, dim3(gr::kBlock), 0, stream, sc, tab);
+    else
+        hipLaunchKernelGGL(gr::mat_table_kernel<false>, grid, dim3(gr::kBlock), 0, stream, sc, tab);
     return hipGetLastError();
 }
 

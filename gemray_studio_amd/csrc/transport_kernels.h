@@ -48,11 +48,16 @@ __device__ __forceinline__ float asym_gauss(float x, float mu, float lo, float h
     float t = div_by_const(x - mu, below ? lo : hi, below ? 1.0f / lo : 1.0f / hi);
     return exp_lobe(-0.5f * t * t);
 }
-__device__ __forceinline__ float rgb_to_spectral(V3 rgb, float l) {
-    float r = fmaxf(rgb.x, 0.0f), g = fmaxf(rgb.y, 0.0f), b = fmaxf(rgb.z, 0.0f);
-    return fmaf(r, asym_gauss(l, 615.0f, 45.0f, 65.0f),
-                fmaf(g, asym_gauss(l, 545.0f, 45.0f, 45.0f), b * asym_gauss(l, 465.0f, 40.0f, 45.0f)));
+// The red, green and blue lobes at l, and their weighted sum.
+__device__ __forceinline__ V3 rgb_lobes(float l) {
+    return v3(asym_gauss(l, 615.0f, 45.0f, 65.0f), asym_gauss(l, 545.0f, 45.0f, 45.0f),
+              asym_gauss(l, 465.0f, 40.0f, 45.0f));
 }
+__device__ __forceinline__ float rgb_to_spectral_lobes(V3 rgb, V3 lobes) {
+    float r = fmaxf(rgb.x, 0.0f), g = fmaxf(rgb.y, 0.0f), b = fmaxf(rgb.z, 0.0f);
+    return fmaf(r, lobes.x, fmaf(g, lobes.y, b * lobes.z));
+}
+__device__ __forceinline__ float rgb_to_spectral(V3 rgb, float l) { return rgb_to_spectral_lobes(rgb, rgb_lobes(l)); }
 
 // EnvironmentMap::radiance_rgb (renderer/env_map.rs:231-319): lat-long bilinear lookup.
 __device__ __forceinline__ V3 map_rgb(V3 dir, const DevScene &sc) {
@@ -105,6 +110,53 @@ __device__ __forceinline__ void fnv(uint32_t &h, uint32_t v) {
 __device__ __forceinline__ float comb_lambda(float hero_off, int k) {
     float offset = fmaf((float)k, (780.0f - 380.0f) / 8.0f, hero_off);
     return 380.0f + (offset >= 400.0f ? offset - 400.0f : offset);
+}
+
+// The hero offset of entry e of a hero table (transport.h), and the entry of hero offset h:
+// kHeroEntries unless h is a key (every offset init_path forms is one; the check costs a
+// few instructions once per path and keeps any other value on the direct evaluation).
+__device__ __forceinline__ float hero_key(uint32_t e) {
+    return e < kHeroLow ? (float)e * 0x1p-15f : 132.0f + (float)(e - kHeroLow) * 0x1p-14f;
+}
+__device__ __forceinline__ uint32_t hero_entry(float h) {
+    if (!(h >= 0.0f && h <= 400.0f)) return kHeroEntries;
+    const uint32_t e = h < 132.0f ? (uint32_t)(h * 32768.0f) : kHeroLow + (uint32_t)((h - 132.0f) * 16384.0f);
+    return hero_key(e) == h ? e : kHeroEntries;
+}
+
+// DispersionModel::evaluate (optics/dispersion.rs:12-37) at comb wavelength lk (nm).
+__device__ __forceinline__ float dispersion_index(float lk, const DevScene &sc) {
+    float l_um = lk * 1e-3f;
+    float l2 = l_um * l_um;
+    if (sc.disp_kind == 0) {
+        float n2 = 1.0f + (sc.disp[0] * l2) / (l2 - sc.disp[1]);
+        return sqrtf(fmaxf(n2, 1.0f));
+    } else if (sc.disp_kind == 1) {
+        float n2 = 1.0f;
+        n2 += (sc.disp[0] * l2) / (l2 - sc.disp[3]);
+        n2 += (sc.disp[1] * l2) / (l2 - sc.disp[4]);
+        n2 += (sc.disp[2] * l2) / (l2 - sc.disp[5]);
+        return sqrtf(fmaxf(n2, 1.0f));
+    }
+    float l4 = l2 * l2;
+    return sc.disp[0] + (sc.disp[1] / l2) + (sc.disp[2] / l4);
+}
+
+// spectral_absorption (raytracer/absorption.rs:28-30, optics/absorption.rs:68-71) of band
+// set kSet (0 ordinary, 1 extraordinary, 2 beta axis) at lk, summed from -0.0 like
+// Iterator::sum.
+template <int kSet>
+__device__ __forceinline__ float band_sum(float lk, const DevScene &sc) {
+    const int n = kSet == 0 ? sc.n_o_bands : kSet == 1 ? sc.n_e_bands : sc.n_beta_bands;
+    float s = -0.0f;
+    for (int b = 0; b < n; ++b) {
+        const float c = kSet == 0 ? sc.o_bands[b][0] : kSet == 1 ? sc.e_bands[b][0] : sc.beta_bands[b][0];
+        const float w = kSet == 0 ? sc.o_bands[b][1] : kSet == 1 ? sc.e_bands[b][1] : sc.beta_bands[b][1];
+        const float p = kSet == 0 ? sc.o_bands[b][2] : kSet == 1 ? sc.e_bands[b][2] : sc.beta_bands[b][2];
+        float t = (lk - c) / w;
+        s = s + p * expf(-0.5f * t * t);
+    }
+    return s;
 }
 
 // Per-lane event counters, two 16-bit counts per register: a lane counts at most
@@ -218,33 +270,27 @@ __device__ __forceinline__ bool camera_miss_certified(const float4 *__restrict__
 // biaxial stones add the beta band set of a trichroic stone (refraction.rs:151-166; otherwise
 // their tensor is uniaxial, (ao, ao, ae)).  Called once per path (PathState::have_abs) by
 // the scatter step; the interior Beer-Lambert block spells the same loop out.
+// With the scene's material table, the sums are read from the path's entry.
 template <bool kBiax>
 __device__ __forceinline__ void fill_absorption_cache(float hero_off, const DevScene &sc, float *__restrict__ lcache) {
+    const uint32_t e = sc.mat_tab ? hero_entry(hero_off) : kHeroEntries;
+    if (e < kHeroEntries) {
+        const float *m = sc.mat_tab + (size_t)e * mat_floats(kBiax);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            LC_AO(k) = m[8 + k];
+            LC_AE(k) = m[16 + k];
+            if (kBiax) LC_AB(k) = m[24 + k];
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        float so = -0.0f, se = -0.0f;
         const float lk = comb_lambda(hero_off, k);
-        for (int b = 0; b < sc.n_o_bands; ++b) {
-            float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
-            so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
-        }
-        for (int b = 0; b < sc.n_e_bands; ++b) {
-            float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
-            se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
-        }
+        const float so = band_sum<0>(lk, sc);
         LC_AO(k) = so;
-        LC_AE(k) = se;
-        if (kBiax) {
-            float sb = so;
-            if (sc.has_beta_bands) {
-                sb = -0.0f;
-                for (int b = 0; b < sc.n_beta_bands; ++b) {
-                    float t = (lk - sc.beta_bands[b][0]) / sc.beta_bands[b][1];
-                    sb = sb + sc.beta_bands[b][2] * expf(-0.5f * t * t);
-                }
-            }
-            LC_AB(k) = sb;
-        }
+        LC_AE(k) = band_sum<1>(lk, sc);
+        if (kBiax) LC_AB(k) = sc.has_beta_bands ? band_sum<2>(lk, sc) : so;
     }
 }
 
@@ -382,26 +428,15 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     }
     ev_inc(ev, GR_EV_HIT);
     if (!ps.have_n) {
-        // DispersionModel::evaluate (optics/dispersion.rs:12-37) for the comb.
+        // The comb's dispersion indices: the material table's entry, or evaluated.
+        const uint32_t e = sc.mat_tab ? hero_entry(ps.hero_off) : kHeroEntries;
+        if (e < kHeroEntries) {
+            const float *m = sc.mat_tab + (size_t)e * mat_floats(kBiax);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            float l_um = comb_lambda(ps.hero_off, k) * 1e-3f;
-            float l2 = l_um * l_um;
-            float n;
-            if (sc.disp_kind == 0) {
-                float n2 = 1.0f + (sc.disp[0] * l2) / (l2 - sc.disp[1]);
-                n = sqrtf(fmaxf(n2, 1.0f));
-            } else if (sc.disp_kind == 1) {
-                float n2 = 1.0f;
-                n2 += (sc.disp[0] * l2) / (l2 - sc.disp[3]);
-                n2 += (sc.disp[1] * l2) / (l2 - sc.disp[4]);
-                n2 += (sc.disp[2] * l2) / (l2 - sc.disp[5]);
-                n = sqrtf(fmaxf(n2, 1.0f));
-            } else {
-                float l4 = l2 * l2;
-                n = sc.disp[0] + (sc.disp[1] / l2) + (sc.disp[2] / l4);
-            }
-            LC_NO(k) = n;
+            for (int k = 0; k < 8; ++k) LC_NO(k) = m[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) LC_NO(k) = dispersion_index(comb_lambda(ps.hero_off, k), sc);
         }
         ps.have_n = true;
     }
@@ -528,32 +563,7 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 // (hot) block spilled 178 VGPRs instead of 30 (same arithmetic).
                 if (!ps.have_abs) {
                     ev_inc(ev, GR_EV_ABSORPTION_FILL);
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        float so = -0.0f, se = -0.0f;
-                        const float lk = comb_lambda(ps.hero_off, k);
-                        for (int b = 0; b < sc.n_o_bands; ++b) {
-                            float t = (lk - sc.o_bands[b][0]) / sc.o_bands[b][1];
-                            so = so + sc.o_bands[b][2] * expf(-0.5f * t * t);
-                        }
-                        for (int b = 0; b < sc.n_e_bands; ++b) {
-                            float t = (lk - sc.e_bands[b][0]) / sc.e_bands[b][1];
-                            se = se + sc.e_bands[b][2] * expf(-0.5f * t * t);
-                        }
-                        LC_AO(k) = so;
-                        LC_AE(k) = se;
-                        if (kBiax) {
-                            float sb = so;
-                            if (sc.has_beta_bands) {
-                                sb = -0.0f;
-                                for (int b = 0; b < sc.n_beta_bands; ++b) {
-                                    float t = (lk - sc.beta_bands[b][0]) / sc.beta_bands[b][1];
-                                    sb = sb + sc.beta_bands[b][2] * expf(-0.5f * t * t);
-                                }
-                            }
-                            LC_AB(k) = sb;
-                        }
-                    }
+                    fill_absorption_cache<kBiax>(ps.hero_off, sc, lcache);
                     ps.have_abs = true;
                 }
                 const float spl = hit_t * sc.absorption_path_scale;
@@ -903,6 +913,9 @@ template <int kEnv>
 __device__ __forceinline__ V3 finish_escaped(V3 dir, const float *inten, const float *pdf,
                                              float hero_off, const DevScene &sc) {
     float rad[8];
+    // The spectral table's entry for this hero offset (kHeroEntries: evaluate instead).
+    const uint32_t he = sc.spec_tab ? hero_entry(hero_off) : kHeroEntries;
+    const float *spec = sc.spec_tab + (size_t)(he < kHeroEntries ? he : 0u) * kSpecFloats;
     if (kEnv == 0) {
         // sample_studio_environment_with_rig (environment.rs:280-321): the direction-only
         // terms are formed once, the fma chain runs per channel.
@@ -935,8 +948,16 @@ __device__ __forceinline__ V3 finish_escaped(V3 dir, const float *inten, const f
         }
     } else {
         V3 rgb = map_rgb(dir, sc);
+        if (he < kHeroEntries) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) rad[k] = fmaf(inten[k], rgb_to_spectral(rgb, comb_lambda(hero_off, k)), 0.0f);
+            for (int k = 0; k < 8; ++k) {
+                const V3 lobes = v3(spec[24 + 3 * k], spec[25 + 3 * k], spec[26 + 3 * k]);
+                rad[k] = fmaf(inten[k], rgb_to_spectral_lobes(rgb, lobes), 0.0f);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) rad[k] = fmaf(inten[k], rgb_to_spectral(rgb, comb_lambda(hero_off, k)), 0.0f);
+        }
     }
     float sum_pdf = -0.0f;
 #pragma unroll
@@ -947,12 +968,20 @@ __device__ __forceinline__ V3 finish_escaped(V3 dir, const float *inten, const f
     // component: the CIE lobes are finite, and a sum that starts at +0 never holds -0, so
     // x + (+-0) == x bit for bit.  Its colour-matching evaluation is skipped.
     V3 xyz = v3(0.0f, 0.0f, 0.0f);
+    if (he < kHeroEntries) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float w = rad[k] * mis * norm_factor;
-        if (w == 0.0f) continue;
-        V3 c = cmf_x8_channel(comb_lambda(hero_off, k));
-        xyz = add(xyz, mul(c, w));
+        for (int k = 0; k < 8; ++k) {
+            const float w = rad[k] * mis * norm_factor;
+            if (w == 0.0f) continue;
+            xyz = add(xyz, mul(v3(spec[3 * k], spec[3 * k + 1], spec[3 * k + 2]), w));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float w = rad[k] * mis * norm_factor;
+            if (w == 0.0f) continue;
+            xyz = add(xyz, mul(cmf_x8_channel(comb_lambda(hero_off, k)), w));
+        }
     }
     V3 lms = v3(fmaf(0.8951f, xyz.x, fmaf(0.2664f, xyz.y, -0.1614f * xyz.z)),
                 fmaf(-0.7502f, xyz.x, fmaf(1.7135f, xyz.y, 0.0367f * xyz.z)),

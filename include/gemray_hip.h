@@ -175,6 +175,16 @@ gr_status gr_set_trace_mode(gr_ctx *ctx, int mode);
  * bit-identical for every value.  No reference counterpart. */
 gr_status gr_set_split_bounce(gr_ctx *ctx, uint32_t split_bounce);
 
+/* Hero tables: per hero wavelength, the comb's colour-matching values and map-spectrum
+ * lobes (one table per context, built at gr_open) and an absorbing material's dispersion
+ * indices and absorption spectra (one per scene), evaluated by the kernels' own functions,
+ * so results are bit-identical in every mode.  mode 0: every path evaluates them; 1
+ * (default): traces read the tables, and a scene's material table is built by its first
+ * trace of at least 2^26 tuples (a smaller one would not repay it); 2: as 1, with the
+ * material table built by any trace of an absorbing scene.  No reference counterpart (the
+ * reference evaluates them per path, spectral_transport.wgsl:672-1470). */
+gr_status gr_set_hero_tables(gr_ctx *ctx, int mode);
+
 /* GpuBackend::try_accumulate / GpuFrameRenderer::accumulate (gpu_backend.rs:160-193,
  * frame.rs:632-793): ADDS each pixel's sum over samples [sample_offset,
  * sample_offset + spp) into accum_xyz[3 * pixel + c].  accum_pixels must equal
