@@ -456,17 +456,11 @@ void free_scene(gr_scene_dev *s) {
     s->cert = nullptr;
 }
 
-// A material hero table costs about as much to fill as the lazy fills of kHeroEntries stone
-// paths (each evaluates the eight channels once); a trace of fewer tuples than this never
-// builds one.
-constexpr uint64_t kMatTableMinTuples = 1ull << 26;
-
 // Builds scene s's material hero table, stream-ordered before the traces that read it, when
 // the scene absorbs, the context reads hero tables and a trace of `tuples` tuples repays it.
 // Not having one (no memory) is not an error: the kernels then evaluate per path.
 gr_status ensure_mat_table(gr_ctx *ctx, const gr_scene_dev *s, uint64_t tuples) {
-    const int mode = ctx->hero_tables.load();
-    if (s->mat_tab || !s->sc.has_absorption || mode == 0 || (mode == 1 && tuples < kMatTableMinTuples)) return GR_OK;
+    if (s->mat_tab || !gr::wants_mat_table(s->sc, tuples, ctx->hero_tables.load())) return GR_OK;
     float *tab = nullptr;
     const size_t bytes = (size_t)gr::kHeroEntries * gr::mat_floats(s->sc.biaxial != 0) * sizeof(float);
     if (hipMalloc(&tab, bytes) != hipSuccess) {
@@ -689,6 +683,9 @@ gr_status tune(gr_ctx *ctx, const gr_scene_dev *s, uint32_t spp, gr::TuneConfig 
     const uint32_t n_pixels = s->sc.width * s->sc.height;
     const uint64_t want = std::max<uint64_t>(1, kTuneProbeTuples / n_pixels);
     const uint32_t probe_spp = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(spp, 1), want);
+    // The probes trace with the hero tables the tuned renders will read.
+    gr_status tst = ensure_mat_table(ctx, s, (uint64_t)n_pixels * spp);
+    if (tst != GR_OK) return tst;
     float *film = nullptr;
     GR_CHECK(hipMalloc(&film, (size_t)n_pixels * 3 * sizeof(float)));
     gr::TuneConfig start;

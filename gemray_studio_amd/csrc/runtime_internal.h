@@ -38,6 +38,15 @@ gr_status validate_scene(const gr_scene *s);
 void fill_dev_scene(const gr_scene *s, DevScene *d);
 // radical_inverse_base (optics/raytracer/sampling.rs:170-181), host evaluation.
 float h_radical_inverse(uint32_t n, uint32_t base);
+// A material hero table costs about as much to fill as the lazy fills of kHeroEntries stone
+// paths (each evaluates the eight channels once); a trace (or job) of fewer tuples than this
+// builds none in the default mode.
+constexpr uint64_t kMatTableMinTuples = 1ull << 26;
+// Whether a trace or job of `tuples` tuples of scene sc gets a material table in hero-table
+// mode `mode` (gr_set_hero_tables).
+inline bool wants_mat_table(const DevScene &sc, uint64_t tuples, int mode) {
+    return sc.has_absorption && (mode == 2 || (mode == 1 && tuples >= kMatTableMinTuples));
+}
 // Stops the job loop of ctx (if started), cancels what is left and frees its memory.
 void serve_shutdown(gr_ctx *ctx);
 

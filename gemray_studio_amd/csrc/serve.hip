@@ -103,6 +103,10 @@ struct Job {
     float4 *d_planes = nullptr;
     uint32_t *d_finishes = nullptr;
     float *d_map = nullptr;
+    // The material hero table of a large absorbing job (its own allocation, like a map), or
+    // null; decided at submission (mat_table) and counted in the footprint.
+    bool mat_table = false;
+    float *d_mat = nullptr;
     float2 *d_ri35 = nullptr;
     uint32_t *d_cert = nullptr;
     float *d_film = nullptr;
@@ -211,6 +215,7 @@ uint64_t job_footprint(const Job &j) {
     b += (uint64_t)j.total_spp * sizeof(float2);             // sample table
     b += j.n_pixels * sizeof(uint32_t);                      // miss certificates
     b += sizeof(DevScene) + GR_EV_COUNT * sizeof(unsigned long long);
+    if (j.mat_table) b += (uint64_t)kHeroEntries * mat_floats(j.host_sc.biaxial != 0) * sizeof(float);
     return b;
 }
 
@@ -225,6 +230,8 @@ void drop_env_map(ServeState &S, Job &j) {
 
 void free_job_device(ServeState &S, Job &j) {
     if (j.d_map) (void)hipFreeAsync(j.d_map, S.stream);
+    if (j.d_mat) (void)hipFreeAsync(j.d_mat, S.stream);
+    j.d_mat = nullptr;
     j.arena.reset();
     drop_env_map(S, j);
     j.d_scene = nullptr;
@@ -345,6 +352,18 @@ void admit_round(ServeState &S, const std::vector<std::shared_ptr<Job>> &jobs, s
             sc.map_rgb = j.env_map ? j.env_map->d : j.d_map;
             sc.finishes = j.d_finishes;
             sc.spec_tab = S.ctx->hero_tables.load() != 0 ? S.ctx->spec_tab : nullptr;
+            if (j.mat_table) {
+                // Filled on the control stream ahead of the job's first tick; without the
+                // memory the job evaluates per path (same bits).
+                const size_t bytes = (size_t)kHeroEntries * mat_floats(sc.biaxial != 0) * sizeof(float);
+                if (hipMallocAsync((void **)&j.d_mat, bytes, S.stream) != hipSuccess) {
+                    (void)hipGetLastError();
+                    j.d_mat = nullptr;
+                } else if (gr_launch_mat_table(sc, j.d_mat, S.stream) != hipSuccess) {
+                    gst = GR_ERR_DEVICE;
+                }
+                sc.mat_tab = j.d_mat;
+            }
             std::memcpy(host + o_scene, &sc, sizeof(DevScene));
             std::memcpy(host + o_planes, j.planes.data(), j.planes.size() * sizeof(float));
             if (!j.finishes.empty()) std::memcpy(host + o_fin, j.finishes.data(), j.finishes.size() * sizeof(uint32_t));
@@ -953,6 +972,7 @@ gr_status gr_job_submit(gr_ctx *ctx, const gr_job_desc *desc, uint64_t *job_id) 
         j->film.assign(j->n_pixels * 3, 0.0f);
         j->pending.resize(j->n_pixels * 3);
     }
+    j->mat_table = gr::wants_mat_table(j->host_sc, j->n_pixels * j->total_spp, ctx->hero_tables.load());
     j->footprint = gr::job_footprint(*j);
     j->t_submit = gr::Clock::now();
     std::lock_guard<std::mutex> g(S->mu);

@@ -75,8 +75,10 @@ def test_single_job_frame_identical_with_the_default_threshold(gpu):
 
 
 def test_job_loop_identical_with_and_without_tables(gpu):
+    """Mode 1 reads the spectral table; mode 2 also builds every absorbing job's material
+    table at admission (these jobs are under the default threshold)."""
     films = {}
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         gpu.set_hero_tables(mode)
         ids = [gpu.submit(sc, first, spp) for sc, first, spp in _mix()]
         out = []
@@ -88,5 +90,6 @@ def test_job_loop_identical_with_and_without_tables(gpu):
             out.append(f)
         films[mode] = out
     gpu.set_hero_tables(1)
-    for a, b in zip(films[0], films[1]):
-        assert np.array_equal(_bits(a), _bits(b))
+    for mode in (1, 2):
+        for a, b in zip(films[0], films[mode]):
+            assert np.array_equal(_bits(a), _bits(b)), f"mode {mode}"
