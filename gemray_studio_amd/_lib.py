@@ -124,7 +124,7 @@ class ServeMetrics(ctypes.Structure):
 class TuneChoice(ctypes.Structure):
     _fields_ = [("waves_per_eu", ctypes.c_uint32), ("split_bounce", ctypes.c_uint32), ("cont_div", ctypes.c_uint32),
                 ("trace_mode", ctypes.c_uint32), ("chunk_tuples", ctypes.c_uint64), ("probe_ms", ctypes.c_double),
-                ("probes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("probes", ctypes.c_uint32), ("park_lanes", ctypes.c_uint32)]
 
 
 # Every symbol include/gemray_hip.h declares.

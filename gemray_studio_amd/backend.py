@@ -294,7 +294,7 @@ class GpuBackend:
         with self._mu:
             st = self._L.gr_autotune(self._ctx, scene.handle, spp, ctypes.byref(c))
         check(st, "gr_autotune")
-        return {name: getattr(c, name) for name, _ in _lib.TuneChoice._fields_ if name != "reserved"}
+        return {name: getattr(c, name) for name, _ in _lib.TuneChoice._fields_}
 
     def set_autotune(self, enabled: bool) -> None:
         self._require()

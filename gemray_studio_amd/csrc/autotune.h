@@ -4,8 +4,8 @@
 // The reference fixes its dispatch shape (@workgroup_size(64), spectral_transport.wgsl:672)
 // and its chunk budget (CHUNK_BUDGET_BYTES, frame.rs:192).  Here the knobs that change speed
 // but never a result bit -- the trace kernels' occupancy variant, the path-length split
-// bounce, the continuation-record capacity, the launch size and the trace mode -- are picked
-// per scene class and size by timing probe renders.
+// bounce and early-parking threshold, the continuation-record capacity, the launch size and
+// the trace mode -- are picked per scene class and size by timing probe renders.
 #pragma once
 
 #include <stdint.h>
@@ -22,8 +22,9 @@ struct TuneConfig {
     uint32_t cont_div = 4;                // continuation records for 1 / cont_div of a launch's tuples
     uint32_t trace_mode = 0;              // 0 plain, 1 persistent wavefronts
     uint64_t chunk_tuples = 1ull << 25;   // tuples per launch
-    std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint64_t> key() const {
-        return std::make_tuple(waves_per_eu, split_bounce, cont_div, trace_mode, chunk_tuples);
+    uint32_t park_lanes = 16;             // early parking below this many live lanes (0: none)
+    std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint64_t, uint32_t> key() const {
+        return std::make_tuple(waves_per_eu, split_bounce, cont_div, trace_mode, chunk_tuples, park_lanes);
     }
     bool operator==(const TuneConfig &o) const { return key() == o.key(); }
 };
@@ -35,10 +36,11 @@ struct TuneSpace {
     std::vector<uint32_t> cont_div{4, 2, 1, 8};
     std::vector<uint64_t> chunk_tuples{1ull << 25, 1ull << 24, 1ull << 23};
     std::vector<uint32_t> trace_mode{0, 1};
+    std::vector<uint32_t> park_lanes{16, 0, 8, 24, 32};
 };
 
-// Coordinate descent from `start`: each knob in turn (occupancy, split bounce, continuation
-// capacity, launch size, trace mode) takes the value of its list whose probe is fastest with
+// Coordinate descent from `start`: each knob in turn (occupancy, split bounce, early-parking
+// threshold, continuation capacity, launch size, trace mode) takes the value of its list whose probe is fastest with
 // the other knobs at the current best, for up to `passes` passes or until a pass changes
 // nothing.  A value replaces the current one only when its probe is faster by more than the
 // fraction `margin`, so equal timings keep the earlier choice.  time(config) returns the
@@ -47,7 +49,7 @@ struct TuneSpace {
 template <class TimeFn>
 TuneConfig coordinate_descent(const TuneConfig &start, const TuneSpace &space, double margin, int passes,
                               TimeFn &&time, double *best_ms_out = nullptr, int *n_timed = nullptr) {
-    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint64_t>, double> memo;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint64_t, uint32_t>, double> memo;
     auto timed = [&](const TuneConfig &c) {
         auto it = memo.find(c.key());
         if (it != memo.end()) return it->second;
@@ -73,6 +75,7 @@ TuneConfig coordinate_descent(const TuneConfig &start, const TuneSpace &space, d
         };
         try_knob(&TuneConfig::waves_per_eu, space.waves_per_eu);
         try_knob(&TuneConfig::split_bounce, space.split_bounce);
+        try_knob(&TuneConfig::park_lanes, space.park_lanes);
         try_knob(&TuneConfig::cont_div, space.cont_div);
         try_knob(&TuneConfig::chunk_tuples, space.chunk_tuples);
         try_knob(&TuneConfig::trace_mode, space.trace_mode);

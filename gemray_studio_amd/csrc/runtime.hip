@@ -510,7 +510,7 @@ struct Workspace {
 gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, uint32_t n_pixels,
                        uint32_t sample_offset, uint32_t spp, float *film, float *xyz_host, uint32_t *info_host,
                        const Workspace &w, uint64_t chunk_pixels, bool wavefront, uint32_t split_bounce,
-                       uint32_t waves_per_eu) {
+                       uint32_t park_lanes, uint32_t waves_per_eu) {
     GR_CHECK(hipMemsetAsync(w.stats, 0, GR_EV_COUNT * sizeof(unsigned long long), ctx->stream));
     GR_CHECK(hipEventRecord(ctx->ev_begin, ctx->stream));
     const gr::DevScene sc = launch_scene(ctx, s);
@@ -553,6 +553,8 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
         args.rec_cap = (uint32_t)w.rec_cap;
         args.trace_done = wavefront ? e.k1 : nullptr;
         args.split_bounce = wavefront ? 0u : split_bounce;
+        args.park_min = gr::kParkMin;
+        args.park_lanes = park_lanes;
         args.cont = w.cont;
         args.cont_cap = w.cont_cap;
         args.cont_count = w.cont_count;
@@ -596,6 +598,7 @@ gr::TuneConfig context_config(gr_ctx *ctx) {
     gr::TuneConfig c;
     c.waves_per_eu = 0;
     c.split_bounce = ctx->split_bounce.load(std::memory_order_relaxed);
+    c.park_lanes = ctx->park_lanes.load(std::memory_order_relaxed);
     c.cont_div = 4;
     c.trace_mode = (uint32_t)ctx->trace_mode;
     c.chunk_tuples = ctx->chunk_tuples;
@@ -644,7 +647,7 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     w.ri35 = ctx->ri35;
     w.stats = ctx->stats;
     return trace_chunks(ctx, s, pixel_begin, n_pixels, sample_offset, spp, film, xyz_host, info_host, w,
-                        chunk_pixels, wavefront, split_bounce, c.waves_per_eu);
+                        chunk_pixels, wavefront, split_bounce, c.park_lanes, c.waves_per_eu);
 }
 
 // Probe launches of the autotuner: the scene's frame at as many samples as make about this
@@ -655,6 +658,7 @@ gr::TuneConfig from_choice(const gr_tune_choice &c) {
     gr::TuneConfig t;
     t.waves_per_eu = c.waves_per_eu;
     t.split_bounce = c.split_bounce;
+    t.park_lanes = c.park_lanes;
     t.cont_div = c.cont_div;
     t.trace_mode = c.trace_mode;
     t.chunk_tuples = c.chunk_tuples;
@@ -664,6 +668,7 @@ void to_choice(const gr::TuneConfig &t, double ms, int probes, gr_tune_choice *c
     std::memset(c, 0, sizeof(*c));
     c->waves_per_eu = t.waves_per_eu;
     c->split_bounce = t.split_bounce;
+    c->park_lanes = t.park_lanes;
     c->cont_div = t.cont_div;
     c->trace_mode = t.trace_mode;
     c->chunk_tuples = t.chunk_tuples;
@@ -671,7 +676,8 @@ void to_choice(const gr::TuneConfig &t, double ms, int probes, gr_tune_choice *c
     c->probes = (uint32_t)probes;
 }
 bool valid_choice(const gr_tune_choice &c) {
-    return c.waves_per_eu >= 2 && c.waves_per_eu <= 4 && c.split_bounce <= GR_MAX_BOUNCES && c.cont_div >= 1 &&
+    return c.waves_per_eu >= 2 && c.waves_per_eu <= 4 && c.split_bounce <= GR_MAX_BOUNCES && c.park_lanes <= 64 &&
+           c.cont_div >= 1 &&
            c.cont_div <= 64 && (c.trace_mode == GR_TRACE_PLAIN || c.trace_mode == GR_TRACE_WAVEFRONT) &&
            c.chunk_tuples >= 1 && c.chunk_tuples <= kMaxChunkTuples;
 }
@@ -691,6 +697,7 @@ gr_status tune(gr_ctx *ctx, const gr_scene_dev *s, uint32_t spp, gr::TuneConfig 
     gr::TuneConfig start;
     start.chunk_tuples = ctx->chunk_tuples;
     start.split_bounce = ctx->split_bounce.load(std::memory_order_relaxed);
+    start.park_lanes = ctx->park_lanes.load(std::memory_order_relaxed);
     gr_status st = GR_OK;
     auto time = [&](const gr::TuneConfig &c) {
         const gr_status r = run_trace(ctx, s, 0, n_pixels, 0, probe_spp, film, nullptr, nullptr, &c);
@@ -1141,7 +1148,8 @@ gr_status gr_render_noalloc(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sam
     w.stats = reinterpret_cast<unsigned long long *>(ctx->scratch + kScratchStats);
     GR_CHECK(gr_launch_ri35(sample_offset, spp, ri35, ctx->stream));
     return trace_chunks(ctx, scene, 0, n_pixels, sample_offset, spp, film->d, nullptr, nullptr, w, chunk_pixels,
-                        false, ctx->split_bounce.load(std::memory_order_relaxed), 0u);
+                        false, ctx->split_bounce.load(std::memory_order_relaxed),
+                        ctx->park_lanes.load(std::memory_order_relaxed), 0u);
 }
 
 gr_status gr_accumulate(gr_ctx *ctx, const gr_scene *scene, uint32_t sample_offset, uint32_t spp,

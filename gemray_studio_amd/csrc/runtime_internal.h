@@ -84,6 +84,9 @@ struct gr_ctx {
     // Plain mode: park paths running after this many segments.  Atomic: the job loop's
     // scheduler thread reads it without the context mutex, which a render holds throughout.
     std::atomic<uint32_t> split_bounce{6};
+    // Early parking: from kParkMin iterations on, a wave with fewer live lanes than this
+    // parks them (0: only at split_bounce).  Read like split_bounce.
+    std::atomic<uint32_t> park_lanes{16};
     float *cont = nullptr;      // continuation records, kContFields x cont_cap
     uint32_t cont_cap = 0;
     uint32_t *cont_count = nullptr;

@@ -486,7 +486,7 @@ std::vector<Piece> plan_tick(ServeState &S, std::vector<Job *> jobs, uint64_t bu
 // Queues one tick's pieces on slot T: the batched trace launches (one per material class,
 // each followed by its continuation launches when split_bounce > 0) and the film reduce,
 // bracketed by the slot's events.  Returns without waiting.
-gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_bounce) {
+gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_bounce, uint32_t park_lanes) {
     std::stable_sort(pieces.begin(), pieces.end(),
                      [](const Piece &a, const Piece &b) { return a.job->cls < b.job->cls; });
     uint64_t tuples = 0, cont_floats = 0;
@@ -525,6 +525,7 @@ gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_b
         if (split_bounce) {
             const uint64_t cap = (p.n_pixels * p.spp + 3) / 4;
             b.split_bounce = split_bounce;
+            b.park_lanes = park_lanes;
             b.cont_cap = (uint32_t)cap;
             b.cont[0] = T.cont + cont_off;
             b.cont[1] = T.cont + cont_off + cap * kContFields;
@@ -792,7 +793,7 @@ void loop(ServeState *Sp) {
         // Plan the next tick over the jobs that still have pixels to hand out.
         std::vector<Job *> schedulable;
         uint64_t tick_tuples = 0;
-        uint32_t split_bounce = 0;
+        uint32_t split_bounce = 0, park_lanes = 0;
         int policy = GR_SERVE_DEADLINE;
         {
             std::lock_guard<std::mutex> g(S.mu);
@@ -800,6 +801,7 @@ void loop(ServeState *Sp) {
                 if (!j->cancel_req && j->next_pixel < j->n_pixels) schedulable.push_back(j.get());
             tick_tuples = S.tick_tuples;
             split_bounce = S.ctx->split_bounce.load(std::memory_order_relaxed);
+            park_lanes = S.ctx->park_lanes.load(std::memory_order_relaxed);
             policy = S.policy;
         }
         TickSlot &next = S.slot[S.ticks_launched % kTickSlots];
@@ -810,7 +812,7 @@ void loop(ServeState *Sp) {
             // that tick completed hold no pixels still to hand out, so the plan skips them.
             if (next.busy) st = complete_tick(S, next, finished);
             if (st == GR_OK) {
-                st = launch_tick(next, plan_tick(S, schedulable, tick_tuples, policy), split_bounce);
+                st = launch_tick(next, plan_tick(S, schedulable, tick_tuples, policy), split_bounce, park_lanes);
                 if (st == GR_OK) S.ticks_launched++;
             }
         } else {

@@ -18,20 +18,20 @@ hipError_t launch_plain_i(const DevScene &sc, const TraceArgs &a, hipStream_t st
     }
     hipLaunchKernelGGL((trace_kernel<A, B, E, F, I, W>), grid, dim3(kBlock), 0, stream, sc, a.planes, a.ri35,
                        a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats,
-                       split ? a.split_bounce : 0u, a.cont, a.cont_cap, a.cont_count);
+                       split ? a.split_bounce : 0u, a.park_min, a.park_lanes, a.cont, a.cont_cap, a.cont_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !split) return e;
     // Level k (k = 1..kContLevels) continues the records of level k-1 and, except the
-    // last, parks again at (k + 1) * split_bounce; records alternate between the two
-    // halves of cont.
+    // last, parks again after at most split_bounce iterations (earlier by the park_lanes
+    // rule); records alternate between the two halves of cont.
     dim3 cgrid((a.cont_cap + kBlock - 1) / kBlock);
     for (int k = 1; k <= kContLevels; ++k) {
         float *in = a.cont + (size_t)((k - 1) & 1) * kContFields * a.cont_cap;
         float *out = a.cont + (size_t)(k & 1) * kContFields * a.cont_cap;
-        const uint32_t next_split = k < kContLevels ? (uint32_t)(k + 1) * a.split_bounce : 0u;
+        const uint32_t split_iters = k < kContLevels ? a.split_bounce : 0u;
         hipLaunchKernelGGL((cont_kernel<A, B, E, F, I, W>), cgrid, dim3(kBlock), 0, stream, sc, a.planes, in,
-                           a.cont_cap, a.cont_count + (k - 1), a.xyz, a.info, a.stats, next_split, out,
-                           a.cont_count + k);
+                           a.cont_cap, a.cont_count + (k - 1), a.xyz, a.info, a.stats, split_iters, a.park_min,
+                           a.park_lanes, out, a.cont_count + k);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -1124,6 +1124,19 @@ __device__ __forceinline__ bool park(const PathState &ps, uint32_t t, const floa
     return true;
 }
 
+// Whether a wave parks its live paths after `it` iterations of this launch: at the level's
+// cap of `cap` iterations (0: never, the last level), or earlier, from park_min iterations
+// on, once fewer than park_lanes of its lanes are still tracing (0: no early parking).  A
+// wave whose paths have mostly ended would otherwise run its few long paths with most lanes
+// idle; parked, they continue in fully populated waves of the next level.  Every live lane
+// of a wave has run the same iterations of the launch and the ballot counts them all, so the
+// decision is the wave's.
+__device__ __forceinline__ bool park_due(uint32_t it, uint32_t cap, uint32_t park_min, uint32_t park_lanes) {
+    if (cap == 0u) return false;
+    if (it >= cap) return true;
+    return park_lanes != 0u && it >= park_min && (uint32_t)__popcll(__ballot(1)) < park_lanes;
+}
+
 // XYZ (and, with kInfo, the per-path record) of a finished path into tuple t's slot.
 template <int kEnv, bool kInfo>
 __device__ __forceinline__ void write_result(const PathState &ps, const DevScene &sc, uint32_t t,
@@ -1207,19 +1220,20 @@ __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block
 //
 // Path-length split: a wave runs until its longest path ends, and stone paths are short
 // with a long tail (on the oval-brilliant workload 69 % end after 3 segments, but most
-// 64-sample waves hold one path that runs much longer).  With split_bounce > 0, a path
-// still running after split_bounce segments is parked as a continuation record (one
-// atomic per wave reserves the records) and cont_kernel continues the parked paths in
-// fully populated waves, parking again at each further multiple of split_bounce for
-// kContLevels levels.  Parking moves state, not arithmetic: every path's result is
-// bit-identical to an unsplit run.  When the records run out, the lane carries on.
+// 64-sample waves hold one path that runs much longer).  With split_bounce > 0, a wave's
+// paths still running after split_bounce segments -- or, with park_lanes > 0, after
+// park_min segments once fewer than park_lanes of its lanes are still tracing (park_due)
+// -- are parked as continuation records (one atomic per wave reserves the records) and
+// cont_kernel continues the parked paths in fully populated waves, parking again by the
+// same rule for kContLevels levels.  Parking moves state, not arithmetic: every path's
+// result is bit-identical to an unsplit run.  When the records run out, the lane carries on.
 template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo, int kW>
 __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
     const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp,
     uint32_t n_tuples, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
-    unsigned long long *__restrict__ stats, uint32_t split_bounce, float *__restrict__ cont, uint32_t cont_cap,
-    uint32_t *__restrict__ cont_count) {
+    unsigned long long *__restrict__ stats, uint32_t split_bounce, uint32_t park_min, uint32_t park_lanes,
+    float *__restrict__ cont, uint32_t cont_cap, uint32_t *__restrict__ cont_count) {
     __shared__ float4 planes[GR_MAX_PLANES];
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
@@ -1239,8 +1253,10 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
         PathState ps;
         init_path(ps, pixel_begin + local_pixel, sample_offset + s, ri35[s], cert[pixel_begin + local_pixel], sc);
         bool parked = false;
+        uint32_t it = 0u;
         while (!bounce_step<kAniso, kAbsorb, kFrost, kInfo>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == split_bounce && park<kAniso == 2, kInfo>(ps, t, lcache, cont, cont_cap, cont_count)) {
+            if (park_due(++it, split_bounce, park_min, park_lanes) &&
+                park<kAniso == 2, kInfo>(ps, t, lcache, cont, cont_cap, cont_count)) {
                 parked = true;
                 break;
             }
@@ -1251,15 +1267,15 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
 }
 
 // Continues the paths the previous launch parked: record r per lane, for r < *cont_count
-// (capped at cont_cap); workgroups past the count return at once.  With split_bounce > 0
-// the paths still running at that bounce are parked again, into next / next_count, for
-// the next level.
+// (capped at cont_cap); workgroups past the count return at once.  With split_iters > 0 a
+// wave parks its paths again, into next / next_count, for the next level, after split_iters
+// iterations of this launch or earlier by the park_lanes rule (park_due).
 template <int kAniso, bool kAbsorb, int kEnv, bool kFrost, bool kInfo, int kW>
 __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float *__restrict__ cont, uint32_t cont_cap,
     const uint32_t *__restrict__ cont_count, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
-    unsigned long long *__restrict__ stats, uint32_t split_bounce, float *__restrict__ next,
-    uint32_t *__restrict__ next_count) {
+    unsigned long long *__restrict__ stats, uint32_t split_iters, uint32_t park_min, uint32_t park_lanes,
+    float *__restrict__ next, uint32_t *__restrict__ next_count) {
     const uint32_t count = min(*cont_count, cont_cap);
     if (blockIdx.x * kBlock >= count) return;
     __shared__ float4 planes[GR_MAX_PLANES];
@@ -1279,8 +1295,10 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
         PathState ps;
         const uint32_t t = load_path<kAniso == 2, kInfo>(ps, lcache, cont, cont_cap, r);
         bool parked = false;
+        uint32_t it = 0u;
         while (!bounce_step<kAniso, kAbsorb, kFrost, kInfo>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == split_bounce && park<kAniso == 2, kInfo>(ps, t, lcache, next, cont_cap, next_count)) {
+            if (park_due(++it, split_iters, park_min, park_lanes) &&
+                park<kAniso == 2, kInfo>(ps, t, lcache, next, cont_cap, next_count)) {
                 parked = true;
                 break;
             }
@@ -1311,6 +1329,7 @@ struct BatchSliceView {
     uint32_t trace_block_begin;
     uint32_t reduce_block_begin;
     uint32_t split_bounce;
+    uint32_t park_lanes;
     uint32_t cont_cap;
     uint32_t cont_block_begin;
     GR_AS_GLOBAL float *cont[2];
@@ -1374,8 +1393,9 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
         init_path(ps, J.pixel_begin + local_pixel, J.sample_offset + s, make_float2(J.ri35[2 * s], J.ri35[2 * s + 1]),
                   J.cert[J.pixel_begin + local_pixel], sc);
         bool parked = false;
+        uint32_t it = 0u;
         while (!bounce_step<kAniso, kAbsorb, kFrost, false>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == J.split_bounce &&
+            if (park_due(++it, J.split_bounce, kParkMin, J.park_lanes) &&
                 park<kAniso == 2, false>(ps, t, lcache, (float *)J.cont[0], J.cont_cap, (uint32_t *)J.cont_count)) {
                 parked = true;
                 break;
@@ -1412,10 +1432,11 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
     if (r < count) {
         PathState ps;
         const uint32_t t = load_path<kAniso == 2, false>(ps, lcache, (const float *)J.cont[(level - 1) & 1], J.cont_cap, r);
-        const uint32_t next_split = level < kContLevels ? (uint32_t)(level + 1) * J.split_bounce : 0u;
+        const uint32_t split_iters = level < kContLevels ? J.split_bounce : 0u;
         bool parked = false;
+        uint32_t it = 0u;
         while (!bounce_step<kAniso, kAbsorb, kFrost, false>(ps, sc, planes, cam_side, lcache, ev)) {
-            if (ps.bounce == next_split &&
+            if (park_due(++it, split_iters, kParkMin, J.park_lanes) &&
                 park<kAniso == 2, false>(ps, t, lcache, (float *)J.cont[level & 1], J.cont_cap, (uint32_t *)(J.cont_count + level))) {
                 parked = true;
                 break;

@@ -301,11 +301,12 @@ typedef struct {
     uint64_t chunk_tuples;  /* tuples per launch                                               */
     double probe_ms;        /* gr_autotune: the chosen configuration's probe time              */
     uint32_t probes;        /* gr_autotune: configurations timed                               */
-    uint32_t reserved;
+    uint32_t park_lanes;    /* from 3 segments on, a wave with fewer live lanes than this parks
+                               them early (0: only at split_bounce); at most 64               */
 } gr_tune_choice;
 /* Times probe renders of the scene (its frame at up to spp samples, about 2^25 tuples) over
- * the configurations by coordinate descent (occupancy, split bounce, continuation capacity,
- * launch size, trace mode; a change must win by more than 1 %) and caches the fastest under
+ * the configurations by coordinate descent (occupancy, split bounce, early parking,
+ * continuation capacity, launch size, trace mode; a change must win by more than 1 %) and caches the fastest under
  * the scene's class (material and environment kernels), plane-count bucket and launch-size
  * bucket; writes it to *out. */
 gr_status gr_autotune(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t spp, gr_tune_choice *out);

@@ -46,7 +46,7 @@ MIRRORS = {
     "gr_options": _lib.Options, "gr_camera": _lib.Camera, "gr_material": _lib.MaterialStruct,
     "gr_env": _lib.Env, "gr_scene": _lib.Scene, "gr_stats": _lib.Stats,
     "gr_job_desc": _lib.JobDesc, "gr_job_stats": _lib.JobStats, "gr_serve_metrics": _lib.ServeMetrics,
-    "gr_metrics_pose": _lib.MetricsPose, "gr_gem_metrics": _lib.GemMetrics,
+    "gr_metrics_pose": _lib.MetricsPose, "gr_gem_metrics": _lib.GemMetrics, "gr_tune_choice": _lib.TuneChoice,
 }
 
 

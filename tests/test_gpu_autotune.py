@@ -1,5 +1,6 @@
 """The autotuner on the device (needs an MI355X): every configuration it can choose -- each
-occupancy variant, split bounce, continuation capacity, launch size and trace mode --
+occupancy variant, split bounce, early-parking threshold, continuation capacity, launch size
+and trace mode --
 renders a film bit-identical to the default configuration's, and so do the tuned choice and
 a render with autotuning on."""
 import numpy as np
@@ -19,10 +20,12 @@ def gpu():
     b.close()
 
 
-BASE = {"waves_per_eu": 3, "split_bounce": 6, "cont_div": 4, "trace_mode": 0, "chunk_tuples": 1 << 25}
+BASE = {"waves_per_eu": 3, "split_bounce": 6, "cont_div": 4, "trace_mode": 0, "chunk_tuples": 1 << 25,
+        "park_lanes": 16}
 VARIANTS = ([dict(BASE, waves_per_eu=w) for w in (2, 4)] + [dict(BASE, split_bounce=s) for s in (0, 3, 4, 5, 7, 8, 10)]
+            + [dict(BASE, park_lanes=p) for p in (0, 1, 8, 24, 32, 64)]
             + [dict(BASE, cont_div=d) for d in (1, 2, 8, 64)] + [dict(BASE, chunk_tuples=c) for c in (1 << 14, 1 << 17)]
-            + [dict(BASE, split_bounce=3, cont_div=1)]
+            + [dict(BASE, split_bounce=3, cont_div=1), dict(BASE, split_bounce=4, park_lanes=64, cont_div=64)]
             + [dict(BASE, trace_mode=1), dict(BASE, waves_per_eu=4, split_bounce=3, cont_div=8, chunk_tuples=1 << 15)])
 
 
