@@ -332,14 +332,20 @@ def preview_workload(gpu, env_map, args):
     }
 
 
-def config_workload(gpu, scene, cam_args, spp, steps, cpu_budget_s, cpu_spp, no_cpu, label):
+def config_workload(gpu, scene, cam_args, spp, steps, cpu_budget_s, cpu_spp, no_cpu, label, autotune=True):
     """One more configuration, timed like the single job (device-resident scene and film,
-    steps of spp samples per pixel), with the oracle on a bounded window of the same frame
-    (row windows from the frame centre, cpu_spp samples, about cpu_budget_s seconds)."""
+    steps of spp samples per pixel, the autotuner's choice for it picked before the timed
+    region), with the oracle on a bounded window of the same frame (row windows from the
+    frame centre, cpu_spp samples, about cpu_budget_s seconds)."""
     dscene = gpu.upload(scene)
     film = gpu.film(scene.width, scene.height)
+    tuned = None
     try:
-        gpu.render(dscene, 0, min(spp, 4), film)  # first-launch costs
+        if autotune:
+            tuned = gpu.autotune(dscene, spp)
+            gpu.set_autotune(True)
+        else:
+            gpu.render(dscene, 0, min(spp, 4), film)  # first-launch costs (the probes pay them)
         segs = paths = 0
         ms = 0.0
         t0 = time.perf_counter()
@@ -351,9 +357,11 @@ def config_workload(gpu, scene, cam_args, spp, steps, cpu_budget_s, cpu_spp, no_
             ms += st["total_ms"]
         wall = time.perf_counter() - t0
     finally:
+        gpu.set_autotune(False)
         film.free()
         dscene.free()
     out = {"workload": label, "width": scene.width, "height": scene.height, "spp_per_step": spp, "steps": steps,
+           "autotuned": tuned,
            "n_planes": int(scene.planes.shape[0]), "max_bounces": scene.max_bounces,
            "segments_per_s": segs / wall, "paths_per_s": paths / wall, "segments_per_path": segs / max(paths, 1),
            "ms_per_step": wall * 1e3 / steps, "device_ms_per_step": ms / steps}
@@ -414,6 +422,7 @@ def main():
     paths = 0
     trace_ms = 0.0
     finish_ms = 0.0
+    device_ms = 0.0
     launches = 0
     events = None
     t0 = time.perf_counter()
@@ -424,6 +433,7 @@ def main():
         paths += st["paths"]
         trace_ms += st["trace_kernel_ms"]
         finish_ms += st["finish_kernel_ms"]
+        device_ms += st["total_ms"]
         launches += st["launches"]
         if events is None:
             events = dict(st["events"])
@@ -442,9 +452,13 @@ def main():
     flops = op_counts.algorithmic_flops(*model, n_bands=n_bands)
     flops_uniform = op_counts.algorithmic_flops_uniform(*model, n_bands=n_bands)
     background = events["first_miss"]
-    # One launch = one chunk's trace kernel plus, in wavefront mode, its finish kernel
-    # (the escape radiance pass); the op-count model covers both, so both are charged.
-    avg_launch_s = (trace_ms + finish_ms) / 1e3 / launches
+    # One launch = one chunk's trace kernel plus its continuation levels (plain mode) or its
+    # finish kernel (wavefront mode); the op-count model covers them all.  Consecutive chunks
+    # alternate between two streams: a chunk's trace kernel starts when the previous chunk's
+    # has ended, and the previous chunk's continuation levels run beside it, so a launch's
+    # duration is the device time of the step (HIP events on the library's stream around all
+    # of its launches) per launch.
+    avg_launch_s = device_ms / 1e3 / launches
     achieved_tf = flops / launches / avg_launch_s / 1e12
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"{'trace_kernel' if args.trace_mode == 'plain' else 'trace_wave_kernel'}"
@@ -491,7 +505,9 @@ def main():
             "kernel": ("trace_kernel + its continuation launches (cont_kernel)" if args.trace_mode == "plain"
                        else "trace_wave_kernel + finish_kernel"),
             "avg_launch_ms": avg_launch_s * 1e3,
-            "avg_trace_kernel_ms": trace_ms / launches,
+            # Per chunk, the trace kernel's start to its last continuation level's end (these
+            # spans overlap the next chunk's trace kernel, so they sum to more than the step).
+            "avg_chunk_span_ms": trace_ms / launches,
             "avg_finish_kernel_ms": finish_ms / launches,
             "launches": launches,
             "algorithmic_flops_per_launch": flops / launches,
@@ -511,12 +527,12 @@ def main():
         out["cpu_configuration"] = config_workload(
             gpu, c1, c1_cam, 16, 5, args.cpu_budget / 2, 16, args.no_cpu,
             "the CPU tracer's configuration: Diamond on the parametric round brilliant (57 facets + 16 girdle "
-            "planes), uniform furnace environment, 256x256 x 16 spp, depth 16")
+            "planes), uniform furnace environment, 256x256 x 16 spp, depth 16", not args.no_autotune)
         c5, c5_cam = stress_scene()
         out["stress"] = config_workload(
             gpu, c5, c5_cam, 16, 1, args.cpu_budget / 2, 4, args.no_cpu,
             "stress: Abbe-20, n_e - n_o = 0.17 stone on the 233-plane step cut, 2048x1024 HDR map, 4096x4096 "
-            "x 16 spp per step, depth 64")
+            "x 16 spp per step, depth 64", not args.no_autotune)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, cam_args, 16, args.cpu_budget)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

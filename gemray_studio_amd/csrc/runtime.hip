@@ -379,15 +379,15 @@ gr_status ensure_rec(gr_ctx *ctx, uint64_t tuples) {
     return GR_OK;
 }
 
-// Two halves of continuation records for a quarter of a launch's tuples each (paths
-// beyond that carry on in the launch that would have parked them).
+// Two halves of continuation records for 1 / div of a launch's tuples each (paths beyond
+// that carry on in the launch that would have parked them), and the counters of two lanes.
 gr_status ensure_cont(gr_ctx *ctx, uint64_t tuples, uint32_t div) {
     const uint64_t cap = (tuples + div - 1) / div;
     if (cap <= ctx->cont_cap) return GR_OK;
     if (ctx->cont) GR_CHECK(hipFree(ctx->cont));
     ctx->cont = nullptr;
     ctx->cont_cap = 0;
-    if (!ctx->cont_count) GR_CHECK(hipMalloc(&ctx->cont_count, (gr::kContLevels + 1) * sizeof(uint32_t)));
+    if (!ctx->cont_count) GR_CHECK(hipMalloc(&ctx->cont_count, 2 * (gr::kContLevels + 1) * sizeof(uint32_t)));
     GR_CHECK(hipMalloc(&ctx->cont, 2 * cap * gr::kContFields * sizeof(float)));
     ctx->cont_cap = (uint32_t)cap;
     return GR_OK;
@@ -485,19 +485,26 @@ gr::DevScene launch_scene(gr_ctx *ctx, const gr_scene_dev *s) {
     return sc;
 }
 
-// The device buffers one traced call uses besides the scene and the film: per-tuple XYZ
-// staging (xyz_cap tuples), per-tuple info, escape records (wavefront mode), continuation
-// records (two halves of kContFields x cont_cap floats) and their counters, the call's
-// sample table (radical inverses of its samples) and the event counters.
+// The device buffers one traced call uses besides the scene and the film, per lane: per-tuple
+// XYZ staging, per-tuple info, escape records (wavefront mode, rec_cap per field),
+// continuation records (two halves of kContFields x cont_cap floats) and their counters;
+// and, shared, the call's sample table (radical inverses of its samples) and the event
+// counters.  With two lanes, consecutive chunks alternate between them on two streams, so a
+// chunk's continuation levels -- a few long paths in a mostly idle device -- overlap the
+// next chunk's trace; each lane reuses its buffers only after its previous chunk, in stream
+// order.  Chunks write disjoint pixels of the film and add into the counters atomically.
 struct Workspace {
-    float *xyz = nullptr;
+    struct Lane {
+        float *xyz = nullptr;
+        uint32_t *info = nullptr;
+        float *rec = nullptr;
+        float *cont = nullptr;
+        uint32_t *cont_count = nullptr;
+    } lane[2];
+    uint32_t n_lanes = 1;
     uint64_t xyz_cap = 0;
-    uint32_t *info = nullptr;
-    float *rec = nullptr;
     uint64_t rec_cap = 0;
-    float *cont = nullptr;
     uint32_t cont_cap = 0;
-    uint32_t *cont_count = nullptr;
     const float2 *ri35 = nullptr;
     unsigned long long *stats = nullptr;
 };
@@ -513,6 +520,9 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
                        uint32_t park_lanes, uint32_t waves_per_eu) {
     GR_CHECK(hipMemsetAsync(w.stats, 0, GR_EV_COUNT * sizeof(unsigned long long), ctx->stream));
     GR_CHECK(hipEventRecord(ctx->ev_begin, ctx->stream));
+    const uint32_t n_lanes = n_pixels > chunk_pixels ? w.n_lanes : 1u;
+    const hipStream_t lane_stream[2] = {ctx->stream, ctx->stream2};
+    if (n_lanes > 1) GR_CHECK(hipStreamWaitEvent(ctx->stream2, ctx->ev_begin, 0));
     const gr::DevScene sc = launch_scene(ctx, s);
     double kernel_ms = 0.0, finish_ms = 0.0;
     uint32_t queued = 0;  // chunks whose timing events have not been read
@@ -527,6 +537,7 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
         return GR_OK;
     };
     uint32_t chunk = 0;
+    hipEvent_t prev_trace_done = nullptr;
     for (uint64_t done = 0; done < n_pixels; done += chunk_pixels, ++chunk) {
         uint32_t np = (uint32_t)((n_pixels - done) < chunk_pixels ? (n_pixels - done) : chunk_pixels);
         uint32_t first = pixel_begin + (uint32_t)done;
@@ -537,6 +548,8 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
             --queued;
         }
         const gr_ctx::ChunkEvents &e = ctx->chunk_ev[chunk % gr_ctx::kChunkRing];
+        const Workspace::Lane &L = w.lane[chunk % n_lanes];
+        const hipStream_t stream = lane_stream[chunk % n_lanes];
         gr::TraceArgs args{};
         args.planes = s->planes;
         args.ri35 = w.ri35;
@@ -545,37 +558,47 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
         args.sample_offset = sample_offset;
         args.spp = spp;
         args.n_tuples = n_tuples;
-        args.xyz = w.xyz;
-        args.info = info_host ? w.info : nullptr;
+        args.xyz = L.xyz;
+        args.info = info_host ? L.info : nullptr;
         args.stats = w.stats;
         args.wavefront = wavefront;
-        args.rec = w.rec;
+        args.rec = L.rec;
         args.rec_cap = (uint32_t)w.rec_cap;
-        args.trace_done = wavefront ? e.k1 : nullptr;
+        // The trace kernels of consecutive chunks do not overlap (each starts when the previous
+        // one has ended); a chunk's continuation levels or finish kernel overlap the next
+        // chunk's trace kernel.
+        const hipEvent_t trace_done = wavefront ? e.k1 : e.kt;
+        args.trace_done = trace_done;
         args.split_bounce = wavefront ? 0u : split_bounce;
         args.park_min = gr::kParkMin;
         args.park_lanes = park_lanes;
-        args.cont = w.cont;
+        args.cont = L.cont;
         args.cont_cap = w.cont_cap;
-        args.cont_count = w.cont_count;
+        args.cont_count = L.cont_count;
         args.waves_per_eu = waves_per_eu;
-        GR_CHECK(hipEventRecord(e.k0, ctx->stream));
-        GR_CHECK(gr_launch_trace(sc, args, ctx->stream));
-        if (!wavefront) GR_CHECK(hipEventRecord(e.k1, ctx->stream));
-        GR_CHECK(hipEventRecord(e.k2, ctx->stream));
-        if (film) GR_CHECK(gr_launch_film_reduce(w.xyz, np, spp, first, film, ctx->stream));
+        if (n_lanes > 1 && prev_trace_done) GR_CHECK(hipStreamWaitEvent(stream, prev_trace_done, 0));
+        prev_trace_done = trace_done;
+        GR_CHECK(hipEventRecord(e.k0, stream));
+        GR_CHECK(gr_launch_trace(sc, args, stream));
+        if (!wavefront) GR_CHECK(hipEventRecord(e.k1, stream));
+        GR_CHECK(hipEventRecord(e.k2, stream));
+        if (film) GR_CHECK(gr_launch_film_reduce(L.xyz, np, spp, first, film, stream));
         if (xyz_host)
-            GR_CHECK(hipMemcpyAsync(xyz_host + 3 * done * spp, w.xyz, (size_t)n_tuples * 3 * sizeof(float),
-                                    hipMemcpyDeviceToHost, ctx->stream));
+            GR_CHECK(hipMemcpyAsync(xyz_host + 3 * done * spp, L.xyz, (size_t)n_tuples * 3 * sizeof(float),
+                                    hipMemcpyDeviceToHost, stream));
         if (info_host)
-            GR_CHECK(hipMemcpyAsync(info_host + 4 * done * spp, w.info, (size_t)n_tuples * 4 * sizeof(uint32_t),
-                                    hipMemcpyDeviceToHost, ctx->stream));
+            GR_CHECK(hipMemcpyAsync(info_host + 4 * done * spp, L.info, (size_t)n_tuples * 4 * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, stream));
         ++queued;
         ctx->last.launches++;
     }
     for (uint32_t k = chunk - queued; k < chunk; ++k) {
         gr_status st = read_slot(k);
         if (st != GR_OK) return st;
+    }
+    if (n_lanes > 1) {
+        GR_CHECK(hipEventRecord(ctx->ev_join, ctx->stream2));
+        GR_CHECK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     }
     GR_CHECK(hipEventRecord(ctx->ev_end, ctx->stream));
     GR_CHECK(hipStreamSynchronize(ctx->stream));
@@ -618,17 +641,19 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     uint64_t cap = chunk_pixels * spp;
     const bool wavefront = c.trace_mode == GR_TRACE_WAVEFRONT;
     const uint32_t split_bounce = c.split_bounce;
-    gr_status st = ensure_xyz(ctx, cap);
+    // Two lanes (Workspace) when the call has more than one chunk.
+    const uint32_t n_lanes = n_pixels > chunk_pixels ? 2u : 1u;
+    gr_status st = ensure_xyz(ctx, n_lanes * cap);
     if (st != GR_OK) return st;
     if (info_host) {
-        st = ensure_info(ctx, cap);
+        st = ensure_info(ctx, n_lanes * cap);
         if (st != GR_OK) return st;
     }
     if (wavefront) {
-        st = ensure_rec(ctx, cap);
+        st = ensure_rec(ctx, n_lanes * cap);
         if (st != GR_OK) return st;
     } else if (split_bounce) {
-        st = ensure_cont(ctx, cap, c.cont_div);
+        st = ensure_cont(ctx, n_lanes * cap, c.cont_div);
         if (st != GR_OK) return st;
     }
     st = ensure_ri35(ctx, sample_offset, spp);
@@ -636,14 +661,18 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
     st = ensure_mat_table(ctx, s, (uint64_t)n_pixels * spp);
     if (st != GR_OK) return st;
     Workspace w;
-    w.xyz = ctx->xyz;
-    w.xyz_cap = ctx->xyz_cap;
-    w.info = ctx->info;
-    w.rec = ctx->rec;
-    w.rec_cap = ctx->rec_cap;
-    w.cont = ctx->cont;
-    w.cont_cap = (uint32_t)std::min<uint64_t>(ctx->cont_cap, (cap + c.cont_div - 1) / c.cont_div);
-    w.cont_count = ctx->cont_count;
+    w.n_lanes = n_lanes;
+    w.xyz_cap = cap;
+    w.rec_cap = cap;
+    w.cont_cap = (uint32_t)std::min<uint64_t>(ctx->cont_cap / n_lanes, (cap + c.cont_div - 1) / c.cont_div);
+    for (uint32_t l = 0; l < n_lanes; ++l) {
+        Workspace::Lane &L = w.lane[l];
+        L.xyz = ctx->xyz + (size_t)l * cap * 3;
+        L.info = ctx->info ? ctx->info + (size_t)l * cap * 4 : nullptr;
+        L.rec = ctx->rec ? ctx->rec + (size_t)l * cap * 20 : nullptr;
+        L.cont = ctx->cont ? ctx->cont + (size_t)l * 2 * w.cont_cap * gr::kContFields : nullptr;
+        L.cont_count = ctx->cont_count ? ctx->cont_count + (size_t)l * (gr::kContLevels + 1) : nullptr;
+    }
     w.ri35 = ctx->ri35;
     w.stats = ctx->stats;
     return trace_chunks(ctx, s, pixel_begin, n_pixels, sample_offset, spp, film, xyz_host, info_host, w,
@@ -731,7 +760,7 @@ constexpr size_t kScratchBytes = kScratchCounts + align256((gr::kContLevels + 1)
 bool create_chunk_events(gr_ctx *ctx) {
     for (gr_ctx::ChunkEvents &c : ctx->chunk_ev)
         if (hipEventCreate(&c.k0) != hipSuccess || hipEventCreate(&c.k1) != hipSuccess ||
-            hipEventCreate(&c.k2) != hipSuccess)
+            hipEventCreate(&c.k2) != hipSuccess || hipEventCreateWithFlags(&c.kt, hipEventDisableTiming) != hipSuccess)
             return false;
     return true;
 }
@@ -770,7 +799,9 @@ gr_status gr_open(gr_ctx **out, const gr_options *opts) {
     if (hipSetDevice(dev) != hipSuccess ||
         hipGetSymbolAddress((void **)&ctx->scratch, HIP_SYMBOL(scratch)) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev_begin) != hipSuccess || hipEventCreate(&ctx->ev_end) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
         !create_chunk_events(ctx) ||
         hipMalloc(&ctx->stats, GR_EV_COUNT * sizeof(unsigned long long)) != hipSuccess) {
         gr_close(ctx);
@@ -795,6 +826,7 @@ void gr_close(gr_ctx *ctx) {
     gr::serve_shutdown(ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     if (ctx->xyz) (void)hipFree(ctx->xyz);
     if (ctx->info) (void)hipFree(ctx->info);
     if (ctx->rec) (void)hipFree(ctx->rec);
@@ -804,12 +836,13 @@ void gr_close(gr_ctx *ctx) {
     if (ctx->mx_buf) (void)hipFree(ctx->mx_buf);
     if (ctx->stats) (void)hipFree(ctx->stats);
     if (ctx->spec_tab) (void)hipFree(ctx->spec_tab);
-    for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end})
+    for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_join})
         if (e) (void)hipEventDestroy(e);
     for (gr_ctx::ChunkEvents &c : ctx->chunk_ev)
-        for (hipEvent_t e : {c.k0, c.k1, c.k2})
+        for (hipEvent_t e : {c.k0, c.k1, c.k2, c.kt})
             if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
 }
 
@@ -1137,12 +1170,12 @@ gr_status gr_render_noalloc(gr_ctx *ctx, const gr_scene_dev *scene, uint32_t sam
     if (spp == 0) return GR_OK;
     uint64_t chunk_pixels = kNoAllocTuples / spp;
     if (chunk_pixels > n_pixels) chunk_pixels = n_pixels;
-    Workspace w;
-    w.xyz = reinterpret_cast<float *>(ctx->scratch + kScratchXyz);
+    Workspace w;  // one lane: chunks run in order on the context's stream
+    w.lane[0].xyz = reinterpret_cast<float *>(ctx->scratch + kScratchXyz);
     w.xyz_cap = kNoAllocTuples;
-    w.cont = reinterpret_cast<float *>(ctx->scratch + kScratchCont);
+    w.lane[0].cont = reinterpret_cast<float *>(ctx->scratch + kScratchCont);
     w.cont_cap = kNoAllocContCap;
-    w.cont_count = reinterpret_cast<uint32_t *>(ctx->scratch + kScratchCounts);
+    w.lane[0].cont_count = reinterpret_cast<uint32_t *>(ctx->scratch + kScratchCounts);
     float2 *ri35 = reinterpret_cast<float2 *>(ctx->scratch + kScratchRi35);
     w.ri35 = ri35;
     w.stats = reinterpret_cast<unsigned long long *>(ctx->scratch + kScratchStats);

@@ -55,12 +55,17 @@ void serve_shutdown(gr_ctx *ctx);
 struct gr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-    // Timing events of the chunks in flight (trace start, trace end, finish end): a call
-    // queues up to kChunkRing chunks ahead of the one whose times it reads.
+    // The second lane of a multi-chunk trace (trace_chunks), joined back into stream at its
+    // end through ev_join.  Default priority, like stream.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_join = nullptr;
+    // Timing events of the chunks in flight (trace start, trace end, finish end), and kt, the
+    // end of a plain chunk's trace kernel before its continuation levels, which the next
+    // chunk's trace waits for on the other lane: a call queues up to kChunkRing chunks ahead
+    // of the one whose times it reads.
     static constexpr uint32_t kChunkRing = 4;
     struct ChunkEvents {
-        hipEvent_t k0 = nullptr, k1 = nullptr, k2 = nullptr;
+        hipEvent_t k0 = nullptr, k1 = nullptr, k2 = nullptr, kt = nullptr;
     } chunk_ev[kChunkRing];
     std::mutex mu;
     uint64_t chunk_tuples = 0;

@@ -20,7 +20,12 @@ hipError_t launch_plain_i(const DevScene &sc, const TraceArgs &a, hipStream_t st
                        a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats,
                        split ? a.split_bounce : 0u, a.park_min, a.park_lanes, a.cont, a.cont_cap, a.cont_count);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !split) return e;
+    if (e != hipSuccess) return e;
+    if (a.trace_done) {
+        e = hipEventRecord(a.trace_done, stream);
+        if (e != hipSuccess) return e;
+    }
+    if (!split) return hipSuccess;
     // Level k (k = 1..kContLevels) continues the records of level k-1 and, except the
     // last, parks again after at most split_bounce iterations (earlier by the park_lanes
     // rule); records alternate between the two halves of cont.

@@ -95,7 +95,8 @@ struct TraceArgs {
     bool wavefront;              // persistent wavefront kernel + finish pass
     float *rec;                  // device escape records (wavefront mode), 20 x rec_cap
     uint32_t rec_cap;
-    hipEvent_t trace_done;       // recorded between the trace and finish kernels, or null
+    hipEvent_t trace_done;       // recorded after the trace kernel (before the finish kernel or the
+                                 // continuation levels), or null
     // Plain mode: paths running after split_bounce segments are parked in cont (two halves
     // of kContFields x cont_cap floats, kContLevels + 1 counters at cont_count) and
     // continued by further launches; 0 = no split.
