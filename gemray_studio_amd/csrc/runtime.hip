@@ -679,8 +679,10 @@ gr_status run_trace(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin, ui
                         chunk_pixels, wavefront, split_bounce, c.park_lanes, c.waves_per_eu);
 }
 
-// Probe launches of the autotuner: the scene's frame at as many samples as make about this
-// many tuples (at most the call's own).
+// Probe launches of the autotuner: about this many tuples (at most the call's own) of the
+// frame's middle rows at the call's samples per pixel -- the densest band of a centred stone,
+// traced in the waves (samples of a pixel) the call will trace, so a configuration whose
+// continuation records run out in the frame's stone-filled chunks shows it in the probes.
 constexpr uint64_t kTuneProbeTuples = 1ull << 25;
 
 gr::TuneConfig from_choice(const gr_tune_choice &c) {
@@ -716,8 +718,9 @@ bool valid_choice(const gr_tune_choice &c) {
 // class and size.  ctx->mu is held.
 gr_status tune(gr_ctx *ctx, const gr_scene_dev *s, uint32_t spp, gr::TuneConfig *out, double *ms, int *probes) {
     const uint32_t n_pixels = s->sc.width * s->sc.height;
-    const uint64_t want = std::max<uint64_t>(1, kTuneProbeTuples / n_pixels);
-    const uint32_t probe_spp = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(spp, 1), want);
+    const uint32_t probe_spp = std::max<uint32_t>(spp, 1);
+    const uint32_t probe_pixels = (uint32_t)std::min<uint64_t>(n_pixels, std::max<uint64_t>(1, kTuneProbeTuples / probe_spp));
+    const uint32_t probe_begin = (n_pixels - probe_pixels) / 2;
     // The probes trace with the hero tables the tuned renders will read.
     gr_status tst = ensure_mat_table(ctx, s, (uint64_t)n_pixels * spp);
     if (tst != GR_OK) return tst;
@@ -729,7 +732,7 @@ gr_status tune(gr_ctx *ctx, const gr_scene_dev *s, uint32_t spp, gr::TuneConfig 
     start.park_lanes = ctx->park_lanes.load(std::memory_order_relaxed);
     gr_status st = GR_OK;
     auto time = [&](const gr::TuneConfig &c) {
-        const gr_status r = run_trace(ctx, s, 0, n_pixels, 0, probe_spp, film, nullptr, nullptr, &c);
+        const gr_status r = run_trace(ctx, s, probe_begin, probe_pixels, 0, probe_spp, film, nullptr, nullptr, &c);
         if (r != GR_OK) st = r;
         return r == GR_OK ? ctx->last.total_ms : 1e30;
     };

@@ -304,7 +304,7 @@ typedef struct {
     uint32_t park_lanes;    /* from 3 segments on, a wave with fewer live lanes than this parks
                                them early (0: only at split_bounce); at most 64               */
 } gr_tune_choice;
-/* Times probe renders of the scene (its frame at up to spp samples, about 2^25 tuples) over
+/* Times probe renders of the scene (its middle rows at spp samples, about 2^25 tuples) over
  * the configurations by coordinate descent (occupancy, split bounce, early parking,
  * continuation capacity, launch size, trace mode; a change must win by more than 1 %) and caches the fastest under
  * the scene's class (material and environment kernels), plane-count bucket and launch-size
