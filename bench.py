@@ -438,6 +438,7 @@ def main():
         trace_ms += st["trace_kernel_ms"]
         finish_ms += st["finish_kernel_ms"]
         device_ms += st["total_ms"]
+        tables = (st["spectral_table"], st["material_table"])
         launches += st["launches"]
         if events is None:
             events = dict(st["events"])
@@ -453,7 +454,8 @@ def main():
     m = scene.material
     model = (events, scene.planes.shape[0], paths, m.is_anisotropic, bool(m.o_bands or m.e_bands), False)
     n_bands = len(m.o_bands) + len(m.e_bands)
-    flops = op_counts.algorithmic_flops(*model, n_bands=n_bands)
+    flops = op_counts.algorithmic_flops(*model, n_bands=n_bands, spectral_table=tables[0],
+                                        material_table=tables[1])
     flops_uniform = op_counts.algorithmic_flops_uniform(*model, n_bands=n_bands)
     background = events["first_miss"]
     # One launch = one chunk's trace kernel plus its continuation levels (plain mode) or its
@@ -515,6 +517,10 @@ def main():
             "avg_finish_kernel_ms": finish_ms / launches,
             "launches": launches,
             "algorithmic_flops_per_launch": flops / launches,
+            # Hero tables the timed launches read (their lookups are charged no operations).
+            "spectral_table": tables[0], "material_table": tables[1],
+            # The device's exact per-class event counters, per launch (the model's inputs).
+            "events_per_launch": {k: v / launches for k, v in events.items()},
             # The same events charged as if no shortcut ran (every segment the full clip,
             # every path the full setup and finish): an upper bound on the credited work.
             "frac_if_every_segment_ran_the_full_clip": flops_uniform / launches / avg_launch_s / 1e12

@@ -77,7 +77,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("events", ctypes.c_uint64 * GR_EV_COUNT), ("paths", ctypes.c_uint64),
                 ("launches", ctypes.c_uint32), ("trace_kernel_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("n_planes", ctypes.c_uint32),
-                ("finish_kernel_ms", ctypes.c_double)]
+                ("finish_kernel_ms", ctypes.c_double), ("hero_tables", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class JobDesc(ctypes.Structure):

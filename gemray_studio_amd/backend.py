@@ -543,4 +543,6 @@ class GpuBackend:
             "finish_kernel_ms": float(s.finish_kernel_ms),
             "total_ms": float(s.total_ms),
             "n_planes": int(s.n_planes),
+            "spectral_table": bool(s.hero_tables & 1),
+            "material_table": bool(s.hero_tables & 2),
         }

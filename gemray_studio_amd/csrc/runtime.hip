@@ -608,6 +608,7 @@ gr_status trace_chunks(gr_ctx *ctx, const gr_scene_dev *s, uint32_t pixel_begin,
     GR_CHECK(hipMemcpy(ev, w.stats, sizeof(ev), hipMemcpyDeviceToHost));
     for (int i = 0; i < GR_EV_COUNT; ++i) ctx->last.events[i] = ev[i];
     ctx->last.paths = (uint64_t)n_pixels * spp;
+    ctx->last.hero_tables = (sc.spec_tab ? 1u : 0u) | (sc.mat_tab ? 2u : 0u);
     ctx->last.trace_kernel_ms = kernel_ms;
     ctx->last.finish_kernel_ms = finish_ms;
     ctx->last.total_ms = total;

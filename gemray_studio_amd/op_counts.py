@@ -36,6 +36,11 @@ bounce-loop iteration, transport.rs:781-929):
   finish    460: CMF x 8, spectral MIS, XYZ and white balance (color.rs:121-330), once per
              escaping path (a path that never escapes carries zero radiance)
   band_fill  48 per absorption band (8 channels x 6), once per path whose cache is filled
+
+With the hero tables (gr_set_hero_tables) a path looks up, instead of evaluating, the CMF
+values (344 of the finish's 460), the map's spectral lobes (8 x 20 of miss_map's 8 x 25; the
+weighted sum, 5 per channel, still runs), the dispersion indices and the band sums; a lookup
+is charged nothing, and the tables' one-off fills are not part of a launch.
 """
 OPS = {
     "clip_per_plane": 12,
@@ -61,14 +66,17 @@ OPS = {
     "dispersion": 120,
     "finish": 460,
     "band_fill": 48,
+    "finish_cmf": 344,
+    "miss_map_lobes": 160,
 }
 
 
 def algorithmic_flops(events: dict, n_planes: int, paths: int, anisotropic: bool,
-                      absorbing: bool, env_studio: bool, n_bands: int = 0) -> float:
+                      absorbing: bool, env_studio: bool, n_bands: int = 0, spectral_table: bool = False,
+                      material_table: bool = False) -> float:
     """FP32 operations of a launch from its exact per-class event counters.  Every path has
-    exactly one first segment (its camera ray); the certificate and exterior shortcuts and
-    the lazily filled caches are charged at the work that runs."""
+    exactly one first segment (its camera ray); the certificate and exterior shortcuts, the
+    lazily filled caches and the hero-table lookups are charged at the work that runs."""
     e = events
     hits = e["hit"]
     later = e["segment"] - paths
@@ -85,16 +93,25 @@ def algorithmic_flops(events: dict, n_planes: int, paths: int, anisotropic: bool
     flops += e["tir"] * OPS["tir"] + e["reflect"] * OPS["reflect"] + e["refract"] * OPS["refract"]
     flops += e["entry_aniso"] * OPS["entry"] + e["walkoff"] * OPS["walkoff"]
     flops += e["rr"] * OPS["rr"] + e["scatter"] * OPS["scatter"]
-    flops += e["miss"] * ((OPS["miss_studio"] if env_studio else OPS["miss_map"]) + OPS["finish"])
-    flops += paths * OPS["path_setup"] + (paths - e["first_miss"]) * OPS["dispersion"]
-    flops += e["absorption_fill"] * OPS["band_fill"] * n_bands
+    env_ops = OPS["miss_studio"] if env_studio else OPS["miss_map"]
+    finish = OPS["finish"]
+    if spectral_table:
+        finish -= OPS["finish_cmf"]
+        if not env_studio:
+            env_ops -= OPS["miss_map_lobes"]
+    flops += e["miss"] * (env_ops + finish)
+    flops += paths * OPS["path_setup"]
+    if not material_table:
+        flops += (paths - e["first_miss"]) * OPS["dispersion"]
+        flops += e["absorption_fill"] * OPS["band_fill"] * n_bands
     return float(flops)
 
 
 def algorithmic_flops_uniform(events: dict, n_planes: int, paths: int, anisotropic: bool,
                               absorbing: bool, env_studio: bool, n_bands: int = 0) -> float:
     """The same events charged as if every segment ran the full 12-per-plane clip and every
-    path paid the per-path setup, dispersion, finish and band sums (the shortcuts ignored)."""
+    path evaluated the per-path setup, dispersion, finish and band sums (the shortcuts and
+    tables ignored)."""
     e = dict(events)
     e["cert_miss"] = e["exterior_miss"] = e["first_miss"] = 0
     e["absorption_fill"] = paths if absorbing else 0

@@ -148,6 +148,8 @@ typedef struct {
     double total_ms;              /* wall time of the call on its stream                 */
     uint32_t n_planes;
     double finish_kernel_ms;      /* wavefront mode: sum of finish-kernel durations      */
+    uint32_t hero_tables;         /* tables the trace read: 1 spectral, 2 material         */
+    uint32_t reserved;
 } gr_stats;
 
 /* Trace kernels.  Both evaluate identical per-tuple results; they differ in scheduling.

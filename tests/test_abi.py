@@ -37,7 +37,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.MaterialStruct) == 4 * 93
     assert ctypes.sizeof(_lib.Camera) == 4 * 13
     assert ctypes.sizeof(_lib.Env) == 4 * 8 + 8 + 8
-    assert ctypes.sizeof(_lib.Stats) == 8 * 16 + 8 + 8 + 8 + 8 + 8 + 8
+    assert ctypes.sizeof(_lib.Stats) == 8 * 16 + 8 + 8 + 8 + 8 + 8 + 8 + 8
 
 
 # ctypes mirror of each C struct of the header, checked field by field against the C
