@@ -1,6 +1,7 @@
 // clip.h -- the convex facet-plane clip shared by the transport kernels and the
 // gemological metrics kernel: bit-identical to the reference's intersect_polyhedron /
-// intersect_polyhedron_soa over planes staged in LDS.
+// intersect_polyhedron_soa over planes staged in LDS.  The scans are host-callable as well,
+// so that a host build can check the screened scan against the plain one.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,7 +24,7 @@ namespace gr {
 // is kept and the scan runs on, to the same outcome.
 // kCam: the origin is the camera's, whose per-plane sides are staged in cam_side.
 template <bool kCam>
-__device__ __forceinline__ void clip_plane(float4 p, float side_cam, int i, V3 ro, V3 rd, float &t_near, int &near_idx,
+__host__ __device__ __forceinline__ void clip_plane(float4 p, float side_cam, int i, V3 ro, V3 rd, float &t_near, int &near_idx,
                                            float &t_far, int &far_idx, bool &outside) {
     const float denom = (p.x * rd.x + p.y * rd.y) + p.z * rd.z;
     const float side = kCam ? side_cam : p.w + ((p.x * ro.x + p.y * ro.y) + p.z * ro.z);
@@ -45,10 +46,10 @@ __device__ __forceinline__ void clip_plane(float4 p, float side_cam, int i, V3 r
     }
 }
 // Planes are read four at a time (the staged array is padded to a multiple of four with
-// planes 0.x + 0.y + 0.z - 1 <= 0, which are never steep, never outside and never win), so
-// one LDS wait covers four planes.
+// inert planes: zero normals with d = -1, never steep, never outside, never winning, or
+// copies of plane 0 -- see stage_planes), so one LDS wait covers four planes.
 template <bool kCam>
-__device__ __forceinline__ void clip_scan(const float4 *__restrict__ planes, const float *__restrict__ cam_side,
+__host__ __device__ __forceinline__ void clip_scan(const float4 *__restrict__ planes, const float *__restrict__ cam_side,
                                           uint32_t n, V3 ro, V3 rd, float &t_near, int &near_idx, float &t_far,
                                           int &far_idx, bool &outside) {
     const int n4 = (int)((n + 3u) & ~3u);
@@ -60,6 +61,63 @@ __device__ __forceinline__ void clip_scan(const float4 *__restrict__ planes, con
         clip_plane<kCam>(p1, cs.y, i + 1, ro, rd, t_near, near_idx, t_far, far_idx, outside);
         clip_plane<kCam>(p2, cs.z, i + 2, ro, rd, t_near, near_idx, t_far, far_idx, outside);
         clip_plane<kCam>(p3, cs.w, i + 3, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+    }
+}
+
+// The interior scan with a first-pass screen.  A plane can change the scan's outcome only if
+// it is near-parallel (|denom| <= 1e-7: the outside flag) or its exact screen
+// side + t_ext * denom is positive.  The first pass evaluates the two dot products in FMA
+// form (dq, sq: three operations each instead of five and six) and screens each plane with
+// them; only a plane the first pass cannot rule out goes through clip_plane, which
+// recomputes everything in the reference's operation order -- so the scan's result is
+// bit-identical to clip_scan.  Ruling out is rigorous:
+//   * the reference's denom and side are within gamma_3 sum|n_i d_i| and
+//     gamma_4 (|w| + sum|n_i o_i|) of the exact dot products, and dq and sq within gamma_3 of
+//     them, so |dq - denom| <= 6.01 u B |d|_inf and |sq - side| <= 7.01 u (W + B |o|_inf),
+//     where u = 2^-24, B bounds |n_x| + |n_y| + |n_z| and W bounds |w| over the staged planes;
+//   * with |dq| >= 1e-7 + md (md >= 6.01 u B |d|_inf) the plane is steep and dq has denom's
+//     sign, so the same extreme t_ext applies;
+//   * then fl(t_ext dq + sq) <= -M, with M >= (|sq - side| + |t_ext| |dq - denom| + u |S|),
+//     proves side + t_ext denom < 0: the exact screen fails.
+// The margins use 2^-20 = 16 u (twice the bound), and every comparison is written so that a
+// NaN or infinite operand fails the proof and takes the exact path.
+struct ClipMargins {
+    float base;  // 2^-20 (W + B |o|_inf)
+    float md;    // 2^-20 B |d|_inf
+    float amb;   // 1e-7 + md
+};
+__host__ __device__ __forceinline__ ClipMargins clip_margins(float w_max, float b_max, V3 ro, V3 rd) {
+    const float om = fmaxf(fmaxf(fabsf(ro.x), fabsf(ro.y)), fabsf(ro.z));
+    const float dm = fmaxf(fmaxf(fabsf(rd.x), fabsf(rd.y)), fabsf(rd.z));
+    ClipMargins m;
+    m.base = 0x1p-20f * fmaf(b_max, om, w_max);
+    m.md = 0x1p-20f * (b_max * dm);
+    m.amb = 1e-7f + m.md;
+    return m;
+}
+__host__ __device__ __forceinline__ void clip_plane_screened(float4 p, int i, V3 ro, V3 rd, const ClipMargins &m,
+                                                    float &t_near, int &near_idx, float &t_far, int &far_idx,
+                                                    bool &outside) {
+    const float dq = fmaf(p.z, rd.z, fmaf(p.y, rd.y, p.x * rd.x));
+    const float sq = fmaf(p.z, ro.z, fmaf(p.y, ro.y, fmaf(p.x, ro.x, p.w)));
+    const bool entering = dq < 0.0f;
+    const float t_ext = entering ? t_near : t_far;
+    const float m_ext = fmaf(fabsf(t_ext), m.md, m.base);  // M(t_ext)
+    const float screen = fmaf(t_ext, dq, sq);
+    const bool ruled_out = fabsf(dq) >= m.amb && screen <= -m_ext;
+    if (!ruled_out) clip_plane<false>(p, 0.0f, i, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+}
+__host__ __device__ __forceinline__ void clip_scan_screened(const float4 *__restrict__ planes, uint32_t n, float w_max,
+                                                   float b_max, V3 ro, V3 rd, float &t_near, int &near_idx,
+                                                   float &t_far, int &far_idx, bool &outside) {
+    const ClipMargins m = clip_margins(w_max, b_max, ro, rd);
+    const int n4 = (int)((n + 3u) & ~3u);
+    for (int i = 0; i < n4; i += 4) {
+        const float4 p0 = planes[i], p1 = planes[i + 1], p2 = planes[i + 2], p3 = planes[i + 3];
+        clip_plane_screened(p0, i, ro, rd, m, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane_screened(p1, i + 1, ro, rd, m, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane_screened(p2, i + 2, ro, rd, m, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane_screened(p3, i + 3, ro, rd, m, t_near, near_idx, t_far, far_idx, outside);
     }
 }
 

@@ -215,6 +215,16 @@ void fill_dev_scene(const gr_scene *s, DevScene *d) {
     d->width = s->width;
     d->height = s->height;
     d->n_planes = s->n_planes;
+    // Bounds for the interior clip's first-pass screen (clip_scan_screened), rounded up: W
+    // over |d|, B over |n_x| + |n_y| + |n_z| of the planes (the staged padding copies plane 0).
+    double w_max = 1e-20, b_max = 0.0;
+    for (uint32_t i = 0; i < s->n_planes; ++i) {
+        const float *p = s->planes + 4 * (size_t)i;
+        w_max = std::max(w_max, (double)std::fabs(p[3]));
+        b_max = std::max(b_max, (double)std::fabs(p[0]) + (double)std::fabs(p[1]) + (double)std::fabs(p[2]));
+    }
+    d->clip_w_max = std::nextafter((float)w_max, INFINITY);
+    d->clip_b_max = std::nextafter((float)b_max, INFINITY);
     d->max_bounces = s->max_bounces;
     const gr_material *m = s->material;
     d->disp_kind = m->disp_kind;

@@ -55,6 +55,10 @@ struct DevScene {
     // (mat_floats per entry; absorbing scenes only).
     const float *spec_tab;
     const float *mat_tab;
+    // Bounds over the facet planes for the interior clip's first-pass screen
+    // (clip_scan_screened, whose kernels pad with copies of plane 0): W >= max |d_i| and
+    // B >= max |n_x| + |n_y| + |n_z|.
+    float clip_w_max, clip_b_max;
 };
 
 // Hero tables.  Every per-path quantity that depends on the path only through its comb

@@ -398,6 +398,12 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         // that facet alone makes the scan's far extreme negative and the scan a miss.
         ev_inc(ev, GR_EV_EXTERIOR_MISS);
         outside = true;
+    } else if (kAbsorb) {
+        // The absorbing kernels take the screened scan (their planes are padded with copies
+        // of plane 0, which it rules out); in the others it measured slower, the register
+        // allocation of the longer scan costing more than the screen saves.
+        clip_scan_screened(planes, sc.n_planes, sc.clip_w_max, sc.clip_b_max, ro, rd, t_near, near_idx, t_far, far_idx,
+                           outside);
     } else {
         clip_scan<false>(planes, cam_side, sc.n_planes, ro, rd, t_near, near_idx, t_far, far_idx, outside);
     }
@@ -1186,14 +1192,20 @@ __device__ __forceinline__ void write_result(const PathState &ps, const DevScene
 constexpr uint32_t kWaveTuples = 64u * GR_WAVE_ROUNDS;
 
 // Facet planes into LDS, with the camera origin's signed distance to each
-// (p.w + n . origin, the clip's operations), padded to a multiple of four with inert
-// planes (normal 0, d = -1) for the scan's four-plane reads.
+// (p.w + n . origin, the clip's operations), padded to a multiple of four for the scan's
+// four-plane reads: with inert planes (normal 0, d = -1: never steep, never outside), or with
+// kCopyPad copies of plane 0.  A copy is inert too: it has plane 0's t, side and steepness,
+// and comes after it, so it never wins a strict extreme (equal t keeps the first index) and
+// sets the outside flag only where plane 0 has set it; unlike a zero-normal plane it is steep
+// wherever plane 0 is, so the interior scan's first pass can rule it out
+// (clip_scan_screened).
+template <bool kCopyPad = false>
 __device__ __forceinline__ void stage_planes(float4 *lds, float *cam_side, const float4 *__restrict__ g,
                                              const DevScene &sc) {
     const V3 o = v3(sc.cam_o[0], sc.cam_o[1], sc.cam_o[2]);
     const uint32_t n4 = (sc.n_planes + 3u) & ~3u;
     for (uint32_t i = threadIdx.x; i < n4; i += kBlock) {
-        const float4 p = i < sc.n_planes ? g[i] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        const float4 p = i < sc.n_planes ? g[i] : (kCopyPad ? g[0] : make_float4(0.0f, 0.0f, 0.0f, -1.0f));
         lds[i] = p;
         cam_side[i] = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
     }
@@ -1238,7 +1250,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes(planes, cam_side, planes_g, sc);
+    stage_planes<kAbsorb>(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1282,7 +1294,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes(planes, cam_side, planes_g, sc);
+    stage_planes<kAbsorb>(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1376,7 +1388,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kTraceBlocks)];
     const DevScene &sc = *(const DevScene *)J.scene;
-    stage_planes(planes, cam_side, (const float4 *)J.planes, sc);
+    stage_planes<kAbsorb>(planes, cam_side, (const float4 *)J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1421,7 +1433,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const DevScene &sc = *(const DevScene *)J.scene;
-    stage_planes(planes, cam_side, (const float4 *)J.planes, sc);
+    stage_planes<kAbsorb>(planes, cam_side, (const float4 *)J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1469,7 +1481,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes(planes, cam_side, planes_g, sc);
+    stage_planes<kAbsorb>(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
