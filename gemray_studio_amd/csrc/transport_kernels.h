@@ -362,6 +362,11 @@ __device__ __forceinline__ void channel_alphas(const PathState &ps, const DevSce
 // two path hashes of the per-tuple record (facet/event sequence and per-channel fates);
 // the kernels that write no record skip them, and with them the dead channels' refraction
 // indices, which only the fate hash reads.
+// The kernels whose interior clip takes the screened scan: absorbing, without the rare
+// features (frosted facets, volume scattering).
+template <bool kAbsorb, bool kFrost>
+constexpr bool kScreenedClip = kAbsorb && !kFrost;
+
 template <int kAniso, bool kAbsorb, bool kFrost, bool kInfo>
 __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, const float4 *__restrict__ planes,
                                             const float *__restrict__ cam_side, float *__restrict__ lcache,
@@ -398,10 +403,10 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         // that facet alone makes the scan's far extreme negative and the scan a miss.
         ev_inc(ev, GR_EV_EXTERIOR_MISS);
         outside = true;
-    } else if (kAbsorb) {
-        // The absorbing kernels take the screened scan (their planes are padded with copies
-        // of plane 0, which it rules out); in the others it measured slower, the register
-        // allocation of the longer scan costing more than the screen saves.
+    } else if (kScreenedClip<kAbsorb, kFrost>) {
+        // The screened scan (the planes padded with copies of plane 0, which it rules out);
+        // elsewhere it measured slower, the register allocation of the longer scan costing
+        // more than the screen saves.
         clip_scan_screened(planes, sc.n_planes, sc.clip_w_max, sc.clip_b_max, ro, rd, t_near, near_idx, t_far, far_idx,
                            outside);
     } else {
@@ -1250,7 +1255,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes<kAbsorb>(planes, cam_side, planes_g, sc);
+    stage_planes<kScreenedClip<kAbsorb, kFrost>>(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1294,7 +1299,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes<kAbsorb>(planes, cam_side, planes_g, sc);
+    stage_planes<kScreenedClip<kAbsorb, kFrost>>(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1388,7 +1393,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kTraceBlocks)];
     const DevScene &sc = *(const DevScene *)J.scene;
-    stage_planes<kAbsorb>(planes, cam_side, (const float4 *)J.planes, sc);
+    stage_planes<kScreenedClip<kAbsorb, kFrost>>(planes, cam_side, (const float4 *)J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1433,7 +1438,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const DevScene &sc = *(const DevScene *)J.scene;
-    stage_planes<kAbsorb>(planes, cam_side, (const float4 *)J.planes, sc);
+    stage_planes<kScreenedClip<kAbsorb, kFrost>>(planes, cam_side, (const float4 *)J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
@@ -1481,7 +1486,7 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
-    stage_planes<kAbsorb>(planes, cam_side, planes_g, sc);
+    stage_planes<kScreenedClip<kAbsorb, true>>(planes, cam_side, planes_g, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
