@@ -88,9 +88,38 @@ __device__ __forceinline__ Stokes scale(Stokes s, float k) {
     return Stokes{s.i * k, s.q * k, s.u * k, s.v * k};
 }
 __device__ __forceinline__ float intensity(const Stokes &s) { return fmaxf(s.i, 0.0f); }
+
+// The hardware square root, reciprocal square root and reciprocal (v_sqrt_f32, v_rsq_f32,
+// v_rcp_f32: within 1 ulp each) serve the quantities that only scale an absorption
+// coefficient -- never a direction, a clip t or a branch of the path, which keep the
+// correctly rounded sqrtf and division.  sqrt_hw scales an argument below 2^-64 by 2^64
+// first (exactly, and the root back by 2^-32), so the instruction never sees a denormal
+// or the small range where its error grows.
+__device__ __forceinline__ float sqrt_hw(float x) {  // x >= 0
+    const bool tiny = x < 0x1p-64f;
+    return __builtin_amdgcn_sqrtf(tiny ? x * 0x1p64f : x) * (tiny ? 0x1p-32f : 1.0f);
+}
+// (cos 2psi, sin 2psi) = (Q, U) / |(Q, U)| with the hardware reciprocal square root, on
+// (Q, U) scaled by the power of two that brings the larger magnitude into [0.5, 1) (exact),
+// so neither the squares nor the instruction meet an underflow or a denormal.  False when
+// Q = U = 0 (or NaN): psi = 0.
+__device__ __forceinline__ bool unit_qu(float q, float u, float &c2, float &s2) {
+    const float m = __uint_as_float(max(__float_as_uint(q) & 0x7FFFFFFFu, __float_as_uint(u) & 0x7FFFFFFFu));
+    if (!(m > 0.0f)) return false;
+    const int e = __builtin_amdgcn_frexp_expf(m);
+    const float qs = __builtin_amdgcn_ldexpf(q, -e), us = __builtin_amdgcn_ldexpf(u, -e);
+    const float inv_r = __builtin_amdgcn_rsqf(fmaf(qs, qs, us * us));
+    c2 = qs * inv_r;
+    s2 = us * inv_r;
+    return true;
+}
+
+// degree_of_polarization (polarization.rs:38-48): sqrt(Q^2 + U^2 + V^2) / I, clamped to
+// [0, 1], with the hardware square root and reciprocal (it only weights the absorption
+// coefficient; within 4 ulp of the reference's correctly rounded route).
 __device__ __forceinline__ float degree_of_polarization(const Stokes &s) {
     if (s.i <= 1e-7f) return 0.0f;
-    return clampf(sqrtf(fmaf(s.v, s.v, fmaf(s.u, s.u, s.q * s.q))) / s.i, 0.0f, 1.0f);
+    return clampf(sqrt_hw(fmaf(s.v, s.v, fmaf(s.u, s.u, s.q * s.q))) * __builtin_amdgcn_rcpf(s.i), 0.0f, 1.0f);
 }
 // f32::midpoint on x86_64 is computed in f64; for finite normal operands that equals
 // (a + b) * 0.5 in f32 (halving commutes with rounding).
@@ -284,7 +313,8 @@ __device__ __forceinline__ void frame_rotation_cs(V3 prev, V3 cur, V3 axis, floa
 // 2 cos psi sin psi = sin 2psi:
 //   e.A.e = sum_i w_i (S_i^2 + P_i^2) / 2 + cos 2psi sum_i w_i (S_i^2 - P_i^2) / 2
 //           + sin 2psi sum_i w_i S_i P_i,   w = (ao, ao, ae),
-// and cos 2psi = Q / r, sin 2psi = U / r with r = sqrt(Q^2 + U^2) (psi = 0 when r = 0).
+// and cos 2psi = Q / r, sin 2psi = U / r with r = sqrt(Q^2 + U^2) (psi = 0 when r = 0;
+// unit_qu).
 // The six channel-invariant sums are formed once per interior segment.
 struct QfInv {
     float h_xy, h_z, d_xy, d_z, x_xy, x_z;
@@ -303,10 +333,8 @@ __device__ __forceinline__ QfInv qf_invariants(V3 s_l, V3 p_l) {
 }
 __device__ __forceinline__ float qf_channel(const QfInv &v, float q, float u, float ao, float ae) {
     const float h = fmaf(ao, v.h_xy, ae * v.h_z);
-    const float r2 = fmaf(q, q, u * u);
-    if (!(r2 > 0.0f)) return h + fmaf(ao, v.d_xy, ae * v.d_z);
-    const float inv_r = 1.0f / sqrtf(r2);
-    const float c2 = q * inv_r, s2 = u * inv_r;
+    float c2, s2;
+    if (!unit_qu(q, u, c2, s2)) return h + fmaf(ao, v.d_xy, ae * v.d_z);
     const float d = fmaf(ao, v.d_xy, ae * v.d_z);
     const float x = fmaf(ao, v.x_xy, ae * v.x_z);
     return fmaf(c2, d, fmaf(s2, x, h));
@@ -333,10 +361,8 @@ __device__ __forceinline__ QfInv3 qf_invariants3(V3 s_l, V3 p_l) {
 __device__ __forceinline__ float qf_channel3(const QfInv3 &v, float q, float u, float ao, float ab, float ae) {
     const float h = fmaf(ao, v.h_x, fmaf(ab, v.h_y, ae * v.h_z));
     const float d = fmaf(ao, v.d_x, fmaf(ab, v.d_y, ae * v.d_z));
-    const float r2 = fmaf(q, q, u * u);
-    if (!(r2 > 0.0f)) return h + d;
-    const float inv_r = 1.0f / sqrtf(r2);
-    const float c2 = q * inv_r, s2 = u * inv_r;
+    float c2, s2;
+    if (!unit_qu(q, u, c2, s2)) return h + d;
     const float x = fmaf(ao, v.x_x, fmaf(ab, v.x_y, ae * v.x_z));
     return fmaf(c2, d, fmaf(s2, x, h));
 }

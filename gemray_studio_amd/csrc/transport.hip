@@ -207,6 +207,9 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__re
         case GR_PROBE_RGB_SPECTRAL:
             o[0] = rgb_to_spectral(v3(a[0], a[1], a[2]), a[3]);
             break;
+        case GR_PROBE_DOP:
+            o[0] = degree_of_polarization(Stokes{a[0], a[1], a[2], a[3]});
+            break;
         default:
             break;
     }

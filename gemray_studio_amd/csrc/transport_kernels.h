@@ -356,17 +356,17 @@ __device__ __forceinline__ void channel_alphas(const PathState &ps, const DevSce
     }
 }
 
+// The kernels whose interior clip takes the screened scan: absorbing, without the rare
+// features (frosted facets, volume scattering).
+template <bool kAbsorb, bool kFrost>
+constexpr bool kScreenedClip = kAbsorb && !kFrost;
+
 // One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
 // has ended (escape, Russian roulette, or the bounce cap).  kAniso: 0 isotropic, 1
 // uniaxial, 2 biaxial (an anisotropic stone with a biaxial indicatrix).  kInfo: keep the
 // two path hashes of the per-tuple record (facet/event sequence and per-channel fates);
 // the kernels that write no record skip them, and with them the dead channels' refraction
 // indices, which only the fate hash reads.
-// The kernels whose interior clip takes the screened scan: absorbing, without the rare
-// features (frosted facets, volume scattering).
-template <bool kAbsorb, bool kFrost>
-constexpr bool kScreenedClip = kAbsorb && !kFrost;
-
 template <int kAniso, bool kAbsorb, bool kFrost, bool kInfo>
 __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, const float4 *__restrict__ planes,
                                             const float *__restrict__ cam_side, float *__restrict__ lcache,

@@ -1927,6 +1927,9 @@ extern "C" float or_field_qform(float q, float u, const float s_l[3], const floa
     V3 e = add(mul(load3(s_l), tf_cos(psi)), mul(load3(p_l), tf_sin(psi)));
     return dot(v3(ao, ao, ae), mulv(e, e));
 }
+extern "C" float or_degree_of_polarization(float i, float q, float u, float v) {
+    return degree_of_polarization(Stokes{i, q, u, v});
+}
 extern "C" float or_walk_off_angle(float n_o, float n_e, float theta) {
     return walk_off_angle(n_o, n_e, theta);
 }

@@ -11,6 +11,10 @@ physical inputs and compared with the reference's route through the transcendent
   TIR retardation        rational cos/sin delta from tan(delta/2)  8e-7
   frame rotation         double angle of (cos psi, sin psi)        8e-7
   field quadratic form   cos/sin 2psi from (Q, U) / |(Q, U)|       8e-7 x max(ao, ae)
+                         (hardware reciprocal square root on (Q, U)
+                         scaled to a unit exponent; |Q|, |U| down to
+                         denormals)
+  degree of polarization hardware square root and reciprocal        5e-7 x the value
   lobe exponential       native exp (as the reference's shader)    2.4e-7 + |x| 1.2e-7 rel.
   CIE matching functions lobes on the native exp                   2e-6 of the largest
                                                                    component (the reference
@@ -107,12 +111,14 @@ def test_field_quadratic_form(gpu):
     s = unit(rng, N)
     p = np.cross(s, unit(rng, N))
     p = (p / np.linalg.norm(p, axis=1, keepdims=True)).astype(F)
-    mag = (10.0 ** rng.uniform(-6, 0, N)).astype(F)
+    mag = 10.0 ** rng.uniform(-6, 0, N)
+    mag[100:1100] = 10.0 ** rng.uniform(-44, -18, 1000)  # squares underflow; denormal Q, U
     ang = rng.uniform(0, 2 * np.pi, N)
     q = (mag * np.cos(ang)).astype(F)
     u = (mag * np.sin(ang)).astype(F)
     q[:100] = F(0.0)
     u[:100] = F(0.0)  # unpolarized in Q/U: psi = 0
+    u[1100:1200] = F(0.0)  # linear along +-Q only
     ao = rng.uniform(0.0, 1.0, N).astype(F)
     out = gpu.probe(_lib.GR_PROBE_FIELD_QFORM, np.concatenate([q[:, None], u[:, None], s, p, ao[:, None]], 1))[:, 0]
     ae = (F(1.0) - ao).astype(F)
@@ -120,6 +126,27 @@ def test_field_quadratic_form(gpu):
                                      float(ae[i])) for i in range(N)], F)
     scale = np.maximum(ao, ae)
     assert (np.abs(out - ref) / scale).max() <= 8e-7
+
+
+def test_degree_of_polarization(gpu):
+    """sqrt(Q^2 + U^2 + V^2) / I clamped to [0, 1] (polarization.rs:38-48), as the absorption
+    step evaluates it, against the oracle's correctly rounded route: within 5e-7 of the value
+    (4 ulp), over intensities from 1e-9 (below the 1e-7 cut, where both return 0) to 1e3,
+    degrees from 0 to 1.2 (clamped) and components down to denormals."""
+    rng = np.random.default_rng(12)
+    i = 10.0 ** rng.uniform(-9, 3, N)
+    dop = rng.uniform(0.0, 1.2, N)
+    dop[:500] = 10.0 ** rng.uniform(-40, -10, 500)
+    d = rng.normal(size=(N, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    quv = d * (i * dop)[:, None]
+    quv[500:600, 1:] = 0.0
+    x = np.concatenate([i[:, None], quv], 1).astype(F)
+    out = gpu.probe(_lib.GR_PROBE_DOP, x)[:, 0]
+    ref = np.array([L.or_degree_of_polarization(*map(float, x[k])) for k in range(N)], F)
+    assert np.all((out == 0) == (ref == 0))
+    assert np.all(np.abs(out.astype(np.float64) - ref) <= 5e-7 * ref.astype(np.float64) + 1e-38)
+    assert np.count_nonzero(ref == 1.0) > 1000 and np.count_nonzero((ref > 0) & (ref < 1)) > 10000
 
 
 def test_lobe_exponential(gpu):
