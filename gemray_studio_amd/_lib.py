@@ -143,7 +143,7 @@ GR_SERVE_EQUAL_SHARE, GR_SERVE_DEADLINE = 0, 1
 (GR_PROBE_NEFF, GR_PROBE_POYNTING, GR_PROBE_TIR_RETARDATION, GR_PROBE_FRAME_ROTATION, GR_PROBE_FIELD_QFORM,
  GR_PROBE_EXP, GR_PROBE_CMF, GR_PROBE_ACOS, GR_PROBE_BX_WAVE_INDICES, GR_PROBE_BX_EIGEN_SLOW, GR_PROBE_BX_EIGEN_FAST,
  GR_PROBE_BX_POYNTING_SLOW, GR_PROBE_BX_POYNTING_FAST, GR_PROBE_SAMPLE_DRAWS, GR_PROBE_SEED, GR_PROBE_STREAM_DRAW,
- GR_PROBE_RGB_SPECTRAL, GR_PROBE_DOP) = range(18)
+ GR_PROBE_RGB_SPECTRAL, GR_PROBE_DOP, GR_PROBE_FRESNEL_COMPANION) = range(19)
 
 _lib = None
 

@@ -99,6 +99,8 @@ __device__ __forceinline__ float sqrt_hw(float x) {  // x >= 0
     const bool tiny = x < 0x1p-64f;
     return __builtin_amdgcn_sqrtf(tiny ? x * 0x1p64f : x) * (tiny ? 0x1p-32f : 1.0f);
 }
+// a / b as a times the hardware reciprocal (within 2 ulp), for the same intensity-only uses.
+__device__ __forceinline__ float div_hw(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 // (cos 2psi, sin 2psi) = (Q, U) / |(Q, U)| with the hardware reciprocal square root, on
 // (Q, U) scaled by the power of two that brings the larger magnitude into [0.5, 1) (exact),
 // so neither the squares nor the instruction meet an underflow or a denormal.  False when
@@ -142,10 +144,11 @@ __device__ __forceinline__ Stokes apply_fresnel_reflection(Stokes s, float r_s, 
     float rp2 = r_p * r_p;
     return apply_abc(s, midpoint(rs2, rp2), 0.5f * (rs2 - rp2), r_s * r_p);
 }
+// The transmission factor (n2 cos_t) / (n1 cos_i) on the hardware reciprocal (div_hw).
 __device__ __forceinline__ Stokes apply_fresnel_transmission(Stokes s, float n1, float n2,
                                                              float cos_i, float cos_t,
                                                              float t_s, float t_p) {
-    float factor = (n2 * cos_t) / fmaxf(n1 * cos_i, 1e-6f);
+    float factor = div_hw(n2 * cos_t, fmaxf(n1 * cos_i, 1e-6f));
     float ts2 = t_s * t_s * factor;
     float tp2 = t_p * t_p * factor;
     return apply_abc(s, midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor);
@@ -414,13 +417,15 @@ __device__ __forceinline__ V3 arbitrary_perpendicular(V3 n) {
 
 // refraction.rs:45-50: delta = 2 atan(x) with x the tangent of the half retardation.
 // The retardation matrix (polarization.rs:141-147) needs only cos delta and sin delta,
-// which are the rational functions (1 - x^2) / (1 + x^2) and 2x / (1 + x^2).
+// which are the rational functions (1 - x^2) / (1 + x^2) and 2x / (1 + x^2); they only
+// rotate a channel's Stokes vector, so the square root and divisions are the hardware's
+// (sqrt_hw, div_hw).
 __device__ __forceinline__ void tir_retardation_cs(float n1k, float cos_i, float sin_i,
                                                    float *cos_d, float *sin_d) {
-    float x = (cos_i * sqrtf(fmaxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f))) /
-              fmaxf(n1k * sin_i * sin_i, 1e-6f);
+    float x = div_hw(cos_i * sqrt_hw(fmaxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f)),
+                     fmaxf(n1k * sin_i * sin_i, 1e-6f));
     float x2 = x * x;
-    float inv = 1.0f / (1.0f + x2);
+    float inv = __builtin_amdgcn_rcpf(1.0f + x2);
     *cos_d = (1.0f - x2) * inv;
     *sin_d = (2.0f * x) * inv;
 }
@@ -429,6 +434,14 @@ __device__ __forceinline__ void fresnel_rs_rp(float n1k, float n2k, float cos_i,
                                               float *r_s, float *r_p) {
     *r_s = fmaf(n2k, -cos_t, n1k * cos_i) / fmaf(n2k, cos_t, n1k * cos_i);
     *r_p = fmaf(n1k, -cos_t, n2k * cos_i) / fmaf(n1k, cos_t, n2k * cos_i);
+}
+// The same amplitudes on the hardware reciprocal, for the channels whose amplitudes only
+// scale their own Stokes vector and sampling weight (every channel but the hero's, whose
+// r_unpol decides the reflect/refract draw).
+__device__ __forceinline__ void fresnel_rs_rp_hw(float n1k, float n2k, float cos_i, float cos_t,
+                                                 float *r_s, float *r_p) {
+    *r_s = div_hw(fmaf(n2k, -cos_t, n1k * cos_i), fmaf(n2k, cos_t, n1k * cos_i));
+    *r_p = div_hw(fmaf(n1k, -cos_t, n2k * cos_i), fmaf(n1k, cos_t, n2k * cos_i));
 }
 
 }  // namespace gr

@@ -210,6 +210,16 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(int fn, const float *__re
         case GR_PROBE_DOP:
             o[0] = degree_of_polarization(Stokes{a[0], a[1], a[2], a[3]});
             break;
+        case GR_PROBE_FRESNEL_COMPANION: {
+            // a = (n1, n2, cos_i, cos_t): the refraction loop's companion arithmetic.
+            const float n1 = a[0], n2 = a[1], cos_i = a[2], cos_t = a[3];
+            fresnel_rs_rp_hw(n1, n2, cos_i, cos_t, &o[0], &o[1]);
+            const float num = 2.0f * n1 * cos_i;
+            const float t_s = div_hw(num, fmaf(n2, cos_t, n1 * cos_i));
+            const float t_p = div_hw(num, fmaf(n1, cos_t, n2 * cos_i));
+            o[2] = apply_fresnel_transmission(Stokes{1.0f, 0.0f, 0.0f, 0.0f}, n1, n2, cos_i, cos_t, t_s, t_p).i;
+            break;
+        }
         default:
             break;
     }

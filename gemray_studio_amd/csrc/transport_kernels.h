@@ -770,8 +770,9 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             } else {
                 float rs = r_s, rp = r_p;  // channel 0 (not past its critical angle: !tir)
                 if (k != 0) {
-                    float cos_t_k = sqrtf(fmaxf(1.0f - s2t_ch[k], 0.0f));
-                    fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
+                    // A companion's amplitudes only scale its own Stokes vector and weight.
+                    float cos_t_k = sqrt_hw(fmaxf(1.0f - s2t_ch[k], 0.0f));
+                    fresnel_rs_rp_hw(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
                 }
                 ps.pdf[k] *= clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
                 ps.st[k] = apply_fresnel_reflection(ps.st[k], rs, rp);
@@ -853,12 +854,14 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 }
             }
             if (alive) {
+                // The transmission and the weight's amplitudes only scale this channel's
+                // Stokes vector and weight (the draw used the hero's r_unpol above).
                 float num = 2.0f * n1k * cos_i;
-                float t_s = num / fmaf(n2k, cos_t_k, n1k * cos_i);
-                float t_p = num / fmaf(n1k, cos_t_k, n2k * cos_i);
+                float t_s = div_hw(num, fmaf(n2k, cos_t_k, n1k * cos_i));
+                float t_p = div_hw(num, fmaf(n1k, cos_t_k, n2k * cos_i));
                 ps.st[k] = scale(apply_fresnel_transmission(ps.st[k], n1k, n2k, cos_i, cos_t_k, t_s, t_p), inv_t);
                 float rs = r_s, rp = r_p;
-                if (k != 0 || !hero_rs) fresnel_rs_rp(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
+                if (k != 0 || !hero_rs) fresnel_rs_rp_hw(n1k, n2k, cos_i, cos_t_k, &rs, &rp);
                 ps.pdf[k] *= 1.0f - clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
             } else {
                 ps.st[k] = scale(ps.st[k], 0.0f);

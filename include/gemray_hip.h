@@ -247,13 +247,18 @@ gr_status gr_guide_buffers(gr_ctx *ctx, const gr_camera *camera, uint32_t width,
  *                                                                 renderer/env_map_spectrum.rs:60-82)
  *   GR_PROBE_DOP              in (I, Q, U, V)                    out (degree_of_polarization as the
  *                                                                 absorption step evaluates it,
- *                                                                 polarization.rs:38-48) */
+ *                                                                 polarization.rs:38-48)
+ *   GR_PROBE_FRESNEL_COMPANION in (n1, n2, cos_i, cos_t)         out (r_s, r_p, the transmission
+ *                                                                 Mueller element m00) as a
+ *                                                                 companion channel evaluates them
+ *                                                                 (refraction.rs:1049-1056,
+ *                                                                 polarization.rs:119-137) */
 enum { GR_PROBE_NEFF = 0, GR_PROBE_POYNTING = 1, GR_PROBE_TIR_RETARDATION = 2, GR_PROBE_FRAME_ROTATION = 3,
        GR_PROBE_FIELD_QFORM = 4, GR_PROBE_EXP = 5, GR_PROBE_CMF = 6, GR_PROBE_ACOS = 7,
        GR_PROBE_BX_WAVE_INDICES = 8, GR_PROBE_BX_EIGEN_SLOW = 9, GR_PROBE_BX_EIGEN_FAST = 10,
        GR_PROBE_BX_POYNTING_SLOW = 11, GR_PROBE_BX_POYNTING_FAST = 12, GR_PROBE_SAMPLE_DRAWS = 13,
        GR_PROBE_SEED = 14, GR_PROBE_STREAM_DRAW = 15, GR_PROBE_RGB_SPECTRAL = 16, GR_PROBE_DOP = 17,
-       GR_PROBE_COUNT = 18 };
+       GR_PROBE_FRESNEL_COMPANION = 18, GR_PROBE_COUNT = 19 };
 gr_status gr_probe(gr_ctx *ctx, int fn, const float *in, size_t n, float *out);
 
 /* Uploads a w x h RGB f32 lat-long map (EnvironmentMap::new, renderer/env_map.rs) to HBM;

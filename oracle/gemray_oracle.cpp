@@ -1927,6 +1927,15 @@ extern "C" float or_field_qform(float q, float u, const float s_l[3], const floa
     V3 e = add(mul(load3(s_l), tf_cos(psi)), mul(load3(p_l), tf_sin(psi)));
     return dot(v3(ao, ao, ae), mulv(e, e));
 }
+// The refraction channel's amplitudes and transmission Mueller element m00 on the
+// reference's operations (refraction.rs:1049-1056 and the t_s, t_p of :748-1011,
+// polarization.rs:119-137).
+extern "C" void or_fresnel_companion(float n1, float n2, float cos_i, float cos_t, float out[3]) {
+    fresnel_rs_rp(n1, n2, cos_i, cos_t, &out[0], &out[1]);
+    const float t_s = (2.0f * n1 * cos_i) / fma_(n2, cos_t, n1 * cos_i);
+    const float t_p = (2.0f * n1 * cos_i) / fma_(n1, cos_t, n2 * cos_i);
+    out[2] = fresnel_transmission(n1, n2, cos_i, cos_t, t_s, t_p).c[0];
+}
 extern "C" float or_degree_of_polarization(float i, float q, float u, float v) {
     return degree_of_polarization(Stokes{i, q, u, v});
 }

@@ -127,6 +127,7 @@ float or_neff_from_cos(float n_o, float n_e, float c);
 void or_frame_rotation_cs(const float prev[3], const float cur[3], const float axis[3], float out[2]);
 float or_field_qform(float q, float u, const float s_l[3], const float p_l[3], float ao, float ae);
 float or_degree_of_polarization(float i, float q, float u, float v);
+void or_fresnel_companion(float n1, float n2, float cos_i, float cos_t, float out[3]);
 void or_extraordinary_poynting_dir(const float w[3], const float c[3], float n_o, float n_e,
                                    float out[3]);
 float or_tir_phase_delta(float n1, float cos_i, float sin_i);

@@ -151,6 +151,8 @@ def _declare(L):
     L.or_field_qform.restype = ctypes.c_float
     L.or_degree_of_polarization.argtypes = [ctypes.c_float] * 4
     L.or_degree_of_polarization.restype = ctypes.c_float
+    L.or_fresnel_companion.argtypes = [ctypes.c_float] * 4 + [fp]
+    L.or_fresnel_companion.restype = None
     L.or_extraordinary_poynting_dir.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float, fp]
     L.or_tir_phase_delta.argtypes = [ctypes.c_float] * 3
     L.or_tir_phase_delta.restype = ctypes.c_float

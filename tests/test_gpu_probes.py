@@ -15,6 +15,8 @@ physical inputs and compared with the reference's route through the transcendent
                          scaled to a unit exponent; |Q|, |U| down to
                          denormals)
   degree of polarization hardware square root and reciprocal        5e-7 x the value
+  companion Fresnel      r_s, r_p, t_s, t_p and the transmission     3e-7 (r_s, r_p);
+                         factor on the hardware reciprocal           1e-6 x m00
   lobe exponential       native exp (as the reference's shader)    2.4e-7 + |x| 1.2e-7 rel.
   CIE matching functions lobes on the native exp                   2e-6 of the largest
                                                                    component (the reference
@@ -147,6 +149,30 @@ def test_degree_of_polarization(gpu):
     assert np.all((out == 0) == (ref == 0))
     assert np.all(np.abs(out.astype(np.float64) - ref) <= 5e-7 * ref.astype(np.float64) + 1e-38)
     assert np.count_nonzero(ref == 1.0) > 1000 and np.count_nonzero((ref > 0) & (ref < 1)) > 10000
+
+
+def test_companion_fresnel(gpu):
+    """A companion channel's Fresnel amplitudes and transmission Mueller element m00 on the
+    hardware reciprocal, against the reference's divisions (refraction.rs:1049-1056,
+    polarization.rs:119-137), entering and leaving stones of index 1.3-2.8, from normal to
+    grazing incidence."""
+    rng = np.random.default_rng(13)
+    n = rng.uniform(1.3, 2.8, N)
+    entering = rng.random(N) < 0.5
+    n1 = np.where(entering, 1.0, n)
+    n2 = np.where(entering, n, 1.0)
+    cos_i = np.concatenate([rng.uniform(1e-3, 1.0, N - 200), np.full(200, 1.0)])
+    s2 = (n1 / n2) ** 2 * (1.0 - cos_i ** 2)
+    keep = s2 < 1.0
+    x = np.stack([n1, n2, cos_i, np.sqrt(np.maximum(1.0 - s2, 0.0))], 1)[keep].astype(F)
+    out = gpu.probe(_lib.GR_PROBE_FRESNEL_COMPANION, x).astype(np.float64)
+    ref = np.zeros((len(x), 3), F)
+    for k in range(len(x)):
+        L.or_fresnel_companion(*map(float, x[k]), po.fptr(ref[k]))
+    ref = ref.astype(np.float64)
+    assert len(x) > 0.5 * N
+    assert np.abs(out[:, :2] - ref[:, :2]).max() <= 3e-7
+    assert np.all(np.abs(out[:, 2] - ref[:, 2]) <= 1e-6 * np.abs(ref[:, 2]))
 
 
 def test_lobe_exponential(gpu):
