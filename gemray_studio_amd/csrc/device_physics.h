@@ -101,6 +101,21 @@ __device__ __forceinline__ float sqrt_hw(float x) {  // x >= 0
 }
 // a / b as a times the hardware reciprocal (within 2 ulp), for the same intensity-only uses.
 __device__ __forceinline__ float div_hw(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// 1 / sqrt(x) for finite x > 0, scaled like sqrt_hw.
+__device__ __forceinline__ float rsqrt_hw(float x) {
+    const bool tiny = x < 0x1p-64f;
+    return __builtin_amdgcn_rsqf(tiny ? x * 0x1p64f : x) * (tiny ? 0x1p32f : 1.0f);
+}
+// normalize and normalize_or_zero for vectors that only orient a polarization frame or an
+// absorption axis: the same results to within 2 ulp per component.  normalize_or_zero_hw
+// returns the zero vector exactly where normalize_or_zero does (|a|^2 zero, infinite or
+// NaN).
+__device__ __forceinline__ V3 normalize_hw(V3 a) { return mul(a, rsqrt_hw(dot(a, a))); }
+__device__ __forceinline__ V3 normalize_or_zero_hw(V3 a) {
+    const float d = dot(a, a);
+    if (d > 0.0f && d < __builtin_huge_valf()) return mul(a, rsqrt_hw(d));
+    return v3(0.0f, 0.0f, 0.0f);
+}
 // (cos 2psi, sin 2psi) = (Q, U) / |(Q, U)| with the hardware reciprocal square root, on
 // (Q, U) scaled by the power of two that brings the larger magnitude into [0.5, 1) (exact),
 // so neither the squares nor the instruction meet an underflow or a denormal.  False when
@@ -398,15 +413,21 @@ __device__ __forceinline__ void stable_orthonormal_basis(V3 n, V3 *t_out, V3 *b_
     *t_out = t;
     *b_out = cross(n, t);
 }
-__device__ __forceinline__ V3 ordinary_eigen_polarization(V3 w, V3 c) {
-    V3 cr = cross(w, c);
-    if (len2(cr) > 1e-8f) return normalize(cr);
-    V3 t, b;
-    stable_orthonormal_basis(normalize_or_zero(w), &t, &b);
-    return t;
-}
-__device__ __forceinline__ V3 extraordinary_eigen_polarization(V3 w, V3 c) {
-    return normalize_or_zero(cross(w, ordinary_eigen_polarization(w, c)));
+// ordinary_eigen_polarization and extraordinary_eigen_polarization (birefringence.rs:101-118)
+// as the absorption step uses them: the ordinary mode along w x c (a stable perpendicular of
+// w when w is along the axis), the extraordinary one along w x e_o.  They only orient the
+// absorption tensor's frame, so the normalizations are the hardware's (normalize_hw).
+__device__ __forceinline__ void uniaxial_eigen_polarizations_hw(V3 w, V3 c, V3 *eo, V3 *ee) {
+    const V3 cr = cross(w, c);
+    V3 o;
+    if (len2(cr) > 1e-8f) {
+        o = normalize_hw(cr);
+    } else {
+        V3 b;
+        stable_orthonormal_basis(normalize_or_zero(w), &o, &b);
+    }
+    *eo = o;
+    *ee = normalize_or_zero_hw(cross(w, o));
 }
 
 // polarization.rs:172-200.

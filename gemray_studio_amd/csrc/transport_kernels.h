@@ -311,12 +311,12 @@ __device__ __forceinline__ void channel_alphas(const PathState &ps, const DevSce
         bx_eigen_polarizations(bx_of(LC_NO(0), sc.delta_beta_alpha, sc.birefringence_delta), fr, k, &ea, &eb);
     } else {
         const V3 c_axis = v3(sc.c_axis[0], sc.c_axis[1], sc.c_axis[2]);
-        ea = ordinary_eigen_polarization(k, c_axis);
-        eb = extraordinary_eigen_polarization(k, c_axis);
+        uniaxial_eigen_polarizations_hw(k, c_axis, &ea, &eb);
     }
-    V3 kk = normalize_or_zero(k);
+    // These directions only orient the absorption frame: hardware normalizations.
+    V3 kk = normalize_or_zero_hw(k);
     V3 s_raw = sub(s_axis, mul(kk, dot(kk, s_axis)));
-    V3 s_hat = len2(s_raw) > 1e-8f ? normalize(s_raw) : arbitrary_perpendicular(kk);
+    V3 s_hat = len2(s_raw) > 1e-8f ? normalize_hw(s_raw) : arbitrary_perpendicular(kk);
     V3 p_hat = cross(kk, s_hat);
     V3 ea_l = v3(dot(afr(sc, 0), ea), dot(afr(sc, 1), ea), dot(afr(sc, 2), ea));
     V3 eb_l = v3(dot(afr(sc, 0), eb), dot(afr(sc, 1), eb), dot(afr(sc, 2), eb));
@@ -549,7 +549,8 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
 
     // ---- Stokes frame rotation into the new plane of incidence
     //      (raytracer/absorption.rs:44-89) ----
-    V3 cpn = normalize_or_zero(cross(cur_k, normal));
+    // The plane-of-incidence normal only orients Stokes frames and absorption axes.
+    V3 cpn = normalize_or_zero_hw(cross(cur_k, normal));
     if (ps.have_prev && len2(cpn) > 1e-6f && len2(ps.prev_pn) > 1e-6f) {
         ev_inc(ev, GR_EV_ROTATE);
         float c2, s2;
