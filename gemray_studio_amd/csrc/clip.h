@@ -64,60 +64,73 @@ __host__ __device__ __forceinline__ void clip_scan(const float4 *__restrict__ pl
     }
 }
 
-// The interior scan with a first-pass screen.  A plane can change the scan's outcome only if
-// it is near-parallel (|denom| <= 1e-7: the outside flag) or its exact screen
-// side + t_ext * denom is positive.  The first pass evaluates the two dot products in FMA
-// form (dq, sq: three operations each instead of five and six) and screens each plane with
-// them; only a plane the first pass cannot rule out goes through clip_plane, which
-// recomputes everything in the reference's operation order -- so the scan's result is
-// bit-identical to clip_scan.  Ruling out is rigorous:
+// The interior scan with a first-pass screen.  The reference's step for a plane changes
+// something only if the plane is steep (|denom| > 1e-7) and its exact screen
+// S_ext = side + t_ext * denom is positive (t_ext = t_near for denom < 0, t_far for
+// denom > 0), or if it is not steep and side > 0 (the outside flag).  The first pass forms
+// the two dot products in FMA form (dq, sq: three operations each instead of five and six)
+// and rules the plane out when it proves both
+//   S_n = side + t_near denom < -1e-7 |t_near|   and   S_f = side + t_far denom < -1e-7 |t_far|.
+// That settles every case without knowing denom's sign or steepness: a steep plane's S_ext
+// is one of the two, so it is negative; a plane with |denom| <= 1e-7 has
+// side = S_n - t_near denom < -1e-7 |t_near| + |t_near| 1e-7 = 0.  Only a plane the first
+// pass cannot rule out goes through clip_plane, which recomputes everything in the
+// reference's operation order -- so the scan's result is bit-identical to clip_scan.  The
+// proof of S < -1e-7 |t| from fl(t dq + sq) < -M(t):
 //   * the reference's denom and side are within gamma_3 sum|n_i d_i| and
 //     gamma_4 (|w| + sum|n_i o_i|) of the exact dot products, and dq and sq within gamma_3 of
 //     them, so |dq - denom| <= 6.01 u B |d|_inf and |sq - side| <= 7.01 u (W + B |o|_inf),
 //     where u = 2^-24, B bounds |n_x| + |n_y| + |n_z| and W bounds |w| over the staged planes;
-//   * with |dq| >= 1e-7 + md (md >= 6.01 u B |d|_inf) the plane is steep and dq has denom's
-//     sign, so the same extreme t_ext applies;
-//   * then fl(t_ext dq + sq) <= -M, with M >= (|sq - side| + |t_ext| |dq - denom| + u |S|),
-//     proves side + t_ext denom < 0: the exact screen fails.
-// The margins use 2^-20 = 16 u (twice the bound), and every comparison is written so that a
-// NaN or infinite operand fails the proof and takes the exact path.
+//   * S <= fl(t dq + sq) + u |t dq + sq| + |t| |dq - denom| + |sq - side|, so
+//     M(t) = |t| (2^-20 B |d|_inf + 1e-7) + 2^-20 (W + B |o|_inf) -- the bounds taken at
+//     2^-20 = 16 u, twice over, which also covers the fma's own rounding and the margins'
+//     -- leaves S < -1e-7 |t|.
+// The margins of the two extremes are formed once and again only when clip_plane moves an
+// extreme.  The comparisons are strict, so that a NaN, or an overflow to -inf against an
+// infinite margin, fails the proof and takes the exact path.
 struct ClipMargins {
     float base;  // 2^-20 (W + B |o|_inf)
-    float md;    // 2^-20 B |d|_inf
-    float amb;   // 1e-7 + md
+    float md;    // 2^-20 B |d|_inf + 1e-7
 };
 __host__ __device__ __forceinline__ ClipMargins clip_margins(float w_max, float b_max, V3 ro, V3 rd) {
     const float om = fmaxf(fmaxf(fabsf(ro.x), fabsf(ro.y)), fabsf(ro.z));
     const float dm = fmaxf(fmaxf(fabsf(rd.x), fabsf(rd.y)), fabsf(rd.z));
     ClipMargins m;
     m.base = 0x1p-20f * fmaf(b_max, om, w_max);
-    m.md = 0x1p-20f * (b_max * dm);
-    m.amb = 1e-7f + m.md;
+    m.md = fmaf(0x1p-20f, b_max * dm, 1e-7f);
     return m;
 }
+// M(t) of the running extremes (after the scan starts and whenever clip_plane moves one).
+__host__ __device__ __forceinline__ void clip_extreme_margins(const ClipMargins &m, float t_near, float t_far,
+                                                               float &m_near, float &m_far) {
+    m_near = fmaf(fabsf(t_near), m.md, m.base);
+    m_far = fmaf(fabsf(t_far), m.md, m.base);
+}
 __host__ __device__ __forceinline__ void clip_plane_screened(float4 p, int i, V3 ro, V3 rd, const ClipMargins &m,
-                                                    float &t_near, int &near_idx, float &t_far, int &far_idx,
-                                                    bool &outside) {
+                                                             float &m_near, float &m_far, float &t_near,
+                                                             int &near_idx, float &t_far, int &far_idx,
+                                                             bool &outside) {
     const float dq = fmaf(p.z, rd.z, fmaf(p.y, rd.y, p.x * rd.x));
     const float sq = fmaf(p.z, ro.z, fmaf(p.y, ro.y, fmaf(p.x, ro.x, p.w)));
-    const bool entering = dq < 0.0f;
-    const float t_ext = entering ? t_near : t_far;
-    const float m_ext = fmaf(fabsf(t_ext), m.md, m.base);  // M(t_ext)
-    const float screen = fmaf(t_ext, dq, sq);
-    const bool ruled_out = fabsf(dq) >= m.amb && screen <= -m_ext;
-    if (!ruled_out) clip_plane<false>(p, 0.0f, i, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+    const bool ruled_out = fmaf(t_near, dq, sq) < -m_near && fmaf(t_far, dq, sq) < -m_far;
+    if (!ruled_out) {
+        clip_plane<false>(p, 0.0f, i, ro, rd, t_near, near_idx, t_far, far_idx, outside);
+        clip_extreme_margins(m, t_near, t_far, m_near, m_far);
+    }
 }
 __host__ __device__ __forceinline__ void clip_scan_screened(const float4 *__restrict__ planes, uint32_t n, float w_max,
                                                    float b_max, V3 ro, V3 rd, float &t_near, int &near_idx,
                                                    float &t_far, int &far_idx, bool &outside) {
     const ClipMargins m = clip_margins(w_max, b_max, ro, rd);
+    float m_near, m_far;
+    clip_extreme_margins(m, t_near, t_far, m_near, m_far);
     const int n4 = (int)((n + 3u) & ~3u);
     for (int i = 0; i < n4; i += 4) {
         const float4 p0 = planes[i], p1 = planes[i + 1], p2 = planes[i + 2], p3 = planes[i + 3];
-        clip_plane_screened(p0, i, ro, rd, m, t_near, near_idx, t_far, far_idx, outside);
-        clip_plane_screened(p1, i + 1, ro, rd, m, t_near, near_idx, t_far, far_idx, outside);
-        clip_plane_screened(p2, i + 2, ro, rd, m, t_near, near_idx, t_far, far_idx, outside);
-        clip_plane_screened(p3, i + 3, ro, rd, m, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane_screened(p0, i, ro, rd, m, m_near, m_far, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane_screened(p1, i + 1, ro, rd, m, m_near, m_far, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane_screened(p2, i + 2, ro, rd, m, m_near, m_far, t_near, near_idx, t_far, far_idx, outside);
+        clip_plane_screened(p3, i + 3, ro, rd, m, m_near, m_far, t_near, near_idx, t_far, far_idx, outside);
     }
 }
 

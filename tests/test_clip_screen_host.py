@@ -53,9 +53,10 @@ static Out screened(const std::vector<float4> &p, uint32_t n, float w, float b, 
 }
 static Out zero_margin(const std::vector<float4> &p, uint32_t n, V3 o, V3 d) {
     Out r = init();
-    const ClipMargins m{0.0f, 0.0f, 1e-7f};
+    const ClipMargins m{0.0f, 0.0f};
+    float mn = 0.0f, mf = 0.0f;
     for (uint32_t i = 0; i < ((n + 3u) & ~3u); ++i)
-        clip_plane_screened(p[i], (int)i, o, d, m, r.tn, r.ni, r.tf, r.fi, r.out);
+        clip_plane_screened(p[i], (int)i, o, d, m, mn, mf, r.tn, r.ni, r.tf, r.fi, r.out);
     return r;
 }
 int main(int argc, char **argv) {
