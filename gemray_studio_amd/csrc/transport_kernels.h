@@ -356,10 +356,10 @@ __device__ __forceinline__ void channel_alphas(const PathState &ps, const DevSce
     }
 }
 
-// The kernels whose interior clip takes the screened scan: absorbing, without the rare
-// features (frosted facets, volume scattering).
+// The kernels whose interior clip takes the screened scan: those without the rare features
+// (frosted facets, volume scattering), where it measured slower.
 template <bool kAbsorb, bool kFrost>
-constexpr bool kScreenedClip = kAbsorb && !kFrost;
+constexpr bool kScreenedClip = !kFrost;
 
 // One iteration of the bounce loop (transport.rs:781-929).  Returns true when the path
 // has ended (escape, Russian roulette, or the bounce cap).  kAniso: 0 isotropic, 1
@@ -405,8 +405,8 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
         outside = true;
     } else if (kScreenedClip<kAbsorb, kFrost>) {
         // The screened scan (the planes padded with copies of plane 0, which it rules out);
-        // elsewhere it measured slower, the register allocation of the longer scan costing
-        // more than the screen saves.
+        // in the rare-feature kernels it measured slower, the register allocation of the
+        // longer scan costing more than the screen saves.
         clip_scan_screened(planes, sc.n_planes, sc.clip_w_max, sc.clip_b_max, ro, rd, t_near, near_idx, t_far, far_idx,
                            outside);
     } else {
