@@ -112,6 +112,15 @@ __device__ __forceinline__ float comb_lambda(float hero_off, int k) {
     return 380.0f + (offset >= 400.0f ? offset - 400.0f : offset);
 }
 
+// The hero offset as the lazy fills read it: an identity DPP move (quad_perm 0,1,2,3, each
+// lane its own value) is convergent, so the compiler cannot hoist the fill's evaluation (a
+// pure function of the offset) out of the bounce loop into init_path, where its results
+// would stay live across the loop -- in the kernels without a material table that hoist
+// spilled the eight indices.  One VALU per fill.
+__device__ __forceinline__ float fill_offset(float h) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(h), 0xE4, 0xF, 0xF, false));
+}
+
 // The hero offset of entry e of a hero table (transport.h), and the entry of hero offset h:
 // kHeroEntries unless h is a key (every offset init_path forms is one; the check costs a
 // few instructions once per path and keeps any other value on the direct evaluation).
@@ -159,11 +168,6 @@ __device__ __forceinline__ float band_sum(float lk, const DevScene &sc) {
     return s;
 }
 
-// Per-lane event counters, two 16-bit counts per register: a lane counts at most
-// kWaveTuples paths x GR_MAX_BOUNCES segments (32768) before flush_events.
-constexpr int kEvWords = (GR_EV_COUNT + 1) / 2;
-__device__ __forceinline__ void ev_inc(uint32_t *ev, int i) { ev[i >> 1] += 1u << (16 * (i & 1)); }
-
 // The per-path dispersion and absorption caches (refraction.rs:127-174): 24 floats per
 // lane in LDS (32 in the kernels built for biaxial stones, whose trichroic tensor adds the
 // beta-axis absorption), channel-major so that a wave's accesses hit distinct banks.  They
@@ -174,6 +178,19 @@ constexpr int kBlock = 256;
 #define LC_AO(k) lcache[(8 + (k)) * kBlock + threadIdx.x]
 #define LC_AE(k) lcache[(16 + (k)) * kBlock + threadIdx.x]
 #define LC_AB(k) lcache[(24 + (k)) * kBlock + threadIdx.x]
+
+// Per-lane event counters, two 16-bit counts per word (a lane counts at most kWaveTuples
+// paths x GR_MAX_BOUNCES segments, 32768, before flush_events), in the lane's own LDS
+// words (word w at ev[w * kBlock], ev = the block's counter array + threadIdx.x): an event
+// is one LDS add without return.  Kept in registers, the eight words were live across the
+// bounce loop's divergent exits, and in the biaxial rare-feature kernel the compiler's copies
+// of them at a join read a register it leaves undefined on some incoming paths whenever the
+// kernel's register allocation changed (counts off, films unaffected).
+constexpr int kEvWords = (GR_EV_COUNT + 1) / 2;
+__device__ __forceinline__ void ev_inc(uint32_t *ev, int i) {
+    __hip_atomic_fetch_add(ev + (i >> 1) * kBlock, 1u << (16 * (i & 1)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 template <int kAniso>
 struct CacheFloats {
     static constexpr int value = (kAniso == 2 ? 32 : 24) * kBlock;
@@ -284,9 +301,10 @@ __device__ __forceinline__ void fill_absorption_cache(float hero_off, const DevS
         }
         return;
     }
+    const float h = fill_offset(hero_off);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const float lk = comb_lambda(hero_off, k);
+        const float lk = comb_lambda(h, k);
         const float so = band_sum<0>(lk, sc);
         LC_AO(k) = so;
         LC_AE(k) = band_sum<1>(lk, sc);
@@ -446,8 +464,9 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
 #pragma unroll
             for (int k = 0; k < 8; ++k) LC_NO(k) = m[k];
         } else {
+            const float h = fill_offset(ps.hero_off);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) LC_NO(k) = dispersion_index(comb_lambda(ps.hero_off, k), sc);
+            for (int k = 0; k < 8; ++k) LC_NO(k) = dispersion_index(comb_lambda(h, k), sc);
         }
         ps.have_n = true;
     }
@@ -1226,7 +1245,7 @@ __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block
                                              unsigned long long *stats) {
 #pragma unroll
     for (int i = 0; i < GR_EV_COUNT; ++i) {
-        uint32_t v = (ev[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+        uint32_t v = (ev[(i >> 1) * kBlock] >> (16 * (i & 1))) & 0xFFFFu;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if ((threadIdx.x & 63) == 0 && v) atomicAdd(&block_ev[i], v);
     }
@@ -1263,9 +1282,10 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
-    uint32_t ev[kEvWords];
+    __shared__ uint32_t lane_ev[kEvWords * kBlock];
+    uint32_t *const ev = lane_ev + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
+    for (int i = 0; i < kEvWords; ++i) ev[i * kBlock] = 0u;
 
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t < n_tuples) {
@@ -1307,9 +1327,10 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
-    uint32_t ev[kEvWords];
+    __shared__ uint32_t lane_ev[kEvWords * kBlock];
+    uint32_t *const ev = lane_ev + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
+    for (int i = 0; i < kEvWords; ++i) ev[i * kBlock] = 0u;
 
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r < count) {
@@ -1401,9 +1422,10 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
-    uint32_t ev[kEvWords];
+    __shared__ uint32_t lane_ev[kEvWords * kBlock];
+    uint32_t *const ev = lane_ev + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
+    for (int i = 0; i < kEvWords; ++i) ev[i * kBlock] = 0u;
 
     const uint32_t spp = J.spp;
     const uint32_t t = (blockIdx.x - J.trace_block_begin) * kBlock + threadIdx.x;
@@ -1446,9 +1468,10 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
-    uint32_t ev[kEvWords];
+    __shared__ uint32_t lane_ev[kEvWords * kBlock];
+    uint32_t *const ev = lane_ev + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
+    for (int i = 0; i < kEvWords; ++i) ev[i * kBlock] = 0u;
 
     if (r < count) {
         PathState ps;
@@ -1494,9 +1517,10 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
 
-    uint32_t ev[kEvWords];
+    __shared__ uint32_t lane_ev[kEvWords * kBlock];
+    uint32_t *const ev = lane_ev + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < kEvWords; ++i) ev[i] = 0u;
+    for (int i = 0; i < kEvWords; ++i) ev[i * kBlock] = 0u;
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
