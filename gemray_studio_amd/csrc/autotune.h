@@ -17,7 +17,7 @@
 namespace gr {
 
 struct TuneConfig {
-    uint32_t waves_per_eu = 3;            // occupancy variant of the plain trace kernels: 2, 3, 4
+    uint32_t waves_per_eu = 4;            // occupancy variant of the plain trace kernels: 2, 3, 4
     uint32_t split_bounce = 6;            // park paths running after this many segments (0: never)
     uint32_t cont_div = 4;                // continuation records for 1 / cont_div of a launch's tuples
     uint32_t trace_mode = 0;              // 0 plain, 1 persistent wavefronts
@@ -31,7 +31,7 @@ struct TuneConfig {
 
 // The values each knob is searched over, the current value first.
 struct TuneSpace {
-    std::vector<uint32_t> waves_per_eu{3, 2, 4};
+    std::vector<uint32_t> waves_per_eu{4, 3, 2};
     std::vector<uint32_t> split_bounce{6, 3, 4, 5, 7, 8, 10, 0};
     std::vector<uint32_t> cont_div{4, 2, 1, 8};
     std::vector<uint64_t> chunk_tuples{1ull << 25, 1ull << 24, 1ull << 23};

@@ -537,13 +537,14 @@ gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_b
     // Class launches: slices of one class are contiguous; block numbering restarts per class.
     struct ClassRange {
         int cls;
-        uint32_t first, count, blocks, cont_blocks;
+        uint32_t first, count, blocks, cont_blocks, max_planes;
     };
     std::vector<ClassRange> classes;
     for (size_t i = 0; i < sl.size(); ++i) {
         const int c = pieces[i].job->cls;
-        if (classes.empty() || classes.back().cls != c) classes.push_back(ClassRange{c, (uint32_t)i, 0, 0, 0});
+        if (classes.empty() || classes.back().cls != c) classes.push_back(ClassRange{c, (uint32_t)i, 0, 0, 0, 0});
         ClassRange &r = classes.back();
+        r.max_planes = std::max(r.max_planes, pieces[i].job->host_sc.n_planes);
         sl[i].trace_block_begin = r.blocks;
         r.blocks += (uint32_t)(((uint64_t)sl[i].n_pixels * sl[i].spp + kBlockTuples - 1) / kBlockTuples);
         sl[i].cont_block_begin = r.cont_blocks;
@@ -557,7 +558,7 @@ gr_status launch_tick(TickSlot &T, std::vector<Piece> &&pieces, uint32_t split_b
         SV_CHECK(hipMemsetAsync(T.cont_counts, 0, sl.size() * (kContLevels + 1) * sizeof(uint32_t), T.stream));
     for (const ClassRange &r : classes) {
         SV_CHECK(gr_launch_trace_batch(T.d_slices + r.first, r.count, r.blocks, split_bounce ? r.cont_blocks : 0u,
-                                       r.cls, T.stream));
+                                       r.max_planes, r.cls, T.stream));
         T.launches += split_bounce ? 1 + kContLevels : 1;
     }
     SV_CHECK(gr_launch_film_reduce_batch(T.d_slices, (uint32_t)sl.size(), reduce_blocks, T.stream));

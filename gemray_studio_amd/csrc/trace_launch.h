@@ -16,7 +16,8 @@ hipError_t launch_plain_i(const DevScene &sc, const TraceArgs &a, hipStream_t st
         hipError_t e = hipMemsetAsync(a.cont_count, 0, (kContLevels + 1) * sizeof(uint32_t), stream);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((trace_kernel<A, B, E, F, I, W>), grid, dim3(kBlock), 0, stream, sc, a.planes, a.ri35,
+    const size_t lds = plane_lds_bytes(sc.n_planes);
+    hipLaunchKernelGGL((trace_kernel<A, B, E, F, I, W>), grid, dim3(kBlock), lds, stream, sc, a.planes, a.ri35,
                        a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.xyz, a.info, a.stats,
                        split ? a.split_bounce : 0u, a.park_min, a.park_lanes, a.cont, a.cont_cap, a.cont_count);
     hipError_t e = hipGetLastError();
@@ -34,7 +35,7 @@ hipError_t launch_plain_i(const DevScene &sc, const TraceArgs &a, hipStream_t st
         float *in = a.cont + (size_t)((k - 1) & 1) * kContFields * a.cont_cap;
         float *out = a.cont + (size_t)(k & 1) * kContFields * a.cont_cap;
         const uint32_t split_iters = k < kContLevels ? a.split_bounce : 0u;
-        hipLaunchKernelGGL((cont_kernel<A, B, E, F, I, W>), cgrid, dim3(kBlock), 0, stream, sc, a.planes, in,
+        hipLaunchKernelGGL((cont_kernel<A, B, E, F, I, W>), cgrid, dim3(kBlock), lds, stream, sc, a.planes, in,
                            a.cont_cap, a.cont_count + (k - 1), a.xyz, a.info, a.stats, split_iters, a.park_min,
                            a.park_lanes, out, a.cont_count + k);
         e = hipGetLastError();
@@ -47,12 +48,13 @@ template <int A, bool B, int E>
 hipError_t launch_wave(const DevScene &sc, const TraceArgs &a, hipStream_t stream) {
     const uint32_t waves = (a.n_tuples + kWaveTuples - 1) / kWaveTuples;
     dim3 grid((waves + kBlock / 64 - 1) / (kBlock / 64));
+    const size_t lds = plane_lds_bytes(sc.n_planes);
     if (a.info)
-        hipLaunchKernelGGL((trace_wave_kernel<A, B, true>), grid, dim3(kBlock), 0, stream, sc, a.planes,
+        hipLaunchKernelGGL((trace_wave_kernel<A, B, true>), grid, dim3(kBlock), lds, stream, sc, a.planes,
                            a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec,
                            a.info, a.stats);
     else
-        hipLaunchKernelGGL((trace_wave_kernel<A, B, false>), grid, dim3(kBlock), 0, stream, sc, a.planes,
+        hipLaunchKernelGGL((trace_wave_kernel<A, B, false>), grid, dim3(kBlock), lds, stream, sc, a.planes,
                            a.ri35, a.cert, a.pixel_begin, a.sample_offset, a.spp, a.n_tuples, a.rec_cap, a.rec,
                            a.info, a.stats);
     hipError_t e = hipGetLastError();
@@ -75,17 +77,18 @@ hipError_t launch_wave(const DevScene &sc, const TraceArgs &a, hipStream_t strea
 
 template <int C>
 hipError_t launch_batch_class_impl(const BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
-                              uint32_t cont_blocks, hipStream_t stream) {
+                              uint32_t cont_blocks, uint32_t max_planes, hipStream_t stream) {
     constexpr int kA = (C & 16) ? 2 : (C & 1);
     constexpr bool kB = (C & 2) != 0, kF = (C & 8) != 0;
     constexpr int kE = (C & 4) ? 1 : 0;
     const auto *view = reinterpret_cast<const BatchSliceView *>(slices);
-    hipLaunchKernelGGL((trace_batch_kernel<kA, kB, kE, kF>), dim3(n_blocks), dim3(kBlock), 0, stream, view,
+    const size_t lds = plane_lds_bytes(max_planes);
+    hipLaunchKernelGGL((trace_batch_kernel<kA, kB, kE, kF>), dim3(n_blocks), dim3(kBlock), lds, stream, view,
                        n_slices);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || cont_blocks == 0) return e;
     for (int level = 1; level <= kContLevels; ++level) {
-        hipLaunchKernelGGL((cont_batch_kernel<kA, kB, kE, kF>), dim3(cont_blocks), dim3(kBlock), 0, stream,
+        hipLaunchKernelGGL((cont_batch_kernel<kA, kB, kE, kF>), dim3(cont_blocks), dim3(kBlock), lds, stream,
                            view, n_slices, level);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -102,7 +105,7 @@ hipError_t launch_trace_class(const DevScene &sc, const TraceArgs &a, hipStream_
     constexpr int kA = (C & 16) ? 2 : (C & 1);
     constexpr bool kB = (C & 2) != 0, kF = (C & 8) != 0;
     constexpr int kE = (C & 4) ? 1 : 0;
-    constexpr int kW = GR_WPE(kA);
+    constexpr int kW = GR_WPE(kA, kF);
     if (a.wavefront) return detail::launch_wave<kA, kB, kE>(sc, a, stream);
     // The per-path record kernels (parity tests) are built at the default occupancy only.
     if (a.info) return detail::launch_plain_i<kA, kB, kE, kF, true, kW>(sc, a, stream);
@@ -116,8 +119,8 @@ hipError_t launch_trace_class(const DevScene &sc, const TraceArgs &a, hipStream_
 
 template <int C>
 hipError_t launch_batch_class(const BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, uint32_t cont_blocks,
-                              hipStream_t stream) {
-    return detail::launch_batch_class_impl<C>(slices, n_slices, n_blocks, cont_blocks, stream);
+                              uint32_t max_planes, hipStream_t stream) {
+    return detail::launch_batch_class_impl<C>(slices, n_slices, n_blocks, cont_blocks, max_planes, stream);
 }
 
 }  // namespace gr
@@ -126,7 +129,8 @@ hipError_t launch_batch_class(const BatchSlice *slices, uint32_t n_slices, uint3
 // bit E (absorbing or not, rare features or not).
 #define GR_INSTANTIATE_TRACE_CLASS(C)                                                                      \
     template hipError_t gr::launch_trace_class<C>(const gr::DevScene &, const gr::TraceArgs &, hipStream_t); \
-    template hipError_t gr::launch_batch_class<C>(const gr::BatchSlice *, uint32_t, uint32_t, uint32_t, hipStream_t);
+    template hipError_t gr::launch_batch_class<C>(const gr::BatchSlice *, uint32_t, uint32_t, uint32_t, uint32_t, \
+                                                  hipStream_t);
 #define GR_INSTANTIATE_TRACE_CLASSES(A, E) \
     GR_INSTANTIATE_TRACE_CLASS((A) | (E))     \
     GR_INSTANTIATE_TRACE_CLASS((A) | (E) | 2) \

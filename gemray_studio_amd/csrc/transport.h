@@ -169,7 +169,7 @@ template <int C>
 hipError_t launch_trace_class(const DevScene &sc, const TraceArgs &a, hipStream_t stream);
 template <int C>
 hipError_t launch_batch_class(const BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks, uint32_t cont_blocks,
-                              hipStream_t stream);
+                              uint32_t max_planes, hipStream_t stream);
 
 }  // namespace gr
 
@@ -188,7 +188,7 @@ hipError_t gr_launch_cert_batch(const gr::CertSlice *slices, uint32_t n_slices, 
 // n_blocks workgroups, then, when the slices split paths, kContLevels continuation
 // launches of cont_blocks workgroups.
 hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
-                                 uint32_t cont_blocks, int cls, hipStream_t stream);
+                                 uint32_t cont_blocks, uint32_t max_planes, int cls, hipStream_t stream);
 // Adds each slice's per-pixel ordered sample sums into its job's film.
 hipError_t gr_launch_film_reduce_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
                                        hipStream_t stream);

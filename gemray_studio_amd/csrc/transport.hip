@@ -340,11 +340,11 @@ hipError_t gr_launch_trace(const gr::DevScene &sc, const gr::TraceArgs &a, hipSt
 }
 
 hipError_t gr_launch_trace_batch(const gr::BatchSlice *slices, uint32_t n_slices, uint32_t n_blocks,
-                                 uint32_t cont_blocks, int cls, hipStream_t stream) {
+                                 uint32_t cont_blocks, uint32_t max_planes, int cls, hipStream_t stream) {
     if (n_slices == 0 || n_blocks == 0) return hipSuccess;
     switch (cls) {
 #define GR_BATCH_CASE(C) \
-    case C: return gr::launch_batch_class<C>(slices, n_slices, n_blocks, cont_blocks, stream);
+    case C: return gr::launch_batch_class<C>(slices, n_slices, n_blocks, cont_blocks, max_planes, stream);
         GR_CLASS_CASES(GR_BATCH_CASE)
 #undef GR_BATCH_CASE
         default: return hipErrorInvalidValue;

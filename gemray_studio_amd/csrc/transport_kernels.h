@@ -1200,17 +1200,19 @@ __device__ __forceinline__ void write_result(const PathState &ps, const DevScene
 // Occupancy target of the trace kernels (waves per SIMD), which sets their register budget
 // (2: 256 VGPRs, 3: 168, 4: 128).  The plain trace and continuation kernels are built at 2,
 // 3 and 4 waves/SIMD and the autotuner picks one per scene class and plane count; the
-// default, chosen by measurement on the single-job workload, is 3 (25 % faster than 2 or 4
-// there), for the biaxial kernels too (8 % faster than 2 despite ~100 spilled VGPRs).  The
-// batched and persistent kernels take the default.
+// default, chosen by measurement, is 4 for the isotropic and uniaxial kernels (with the
+// event counters in LDS and the planes sized to the stone, they fit 128 VGPRs and four
+// workgroups' LDS per CU: 17 % faster than 3 on the single-job workload) and 3 for the
+// biaxial ones, whose 32-float lane caches leave LDS for three workgroups, and for the
+// rare-feature kernels, which spill at 128 VGPRs (7-15 % slower at 4).  The batched and
+// persistent kernels take the default.
 #ifndef GR_TRACE_WAVES_PER_EU
-#define GR_TRACE_WAVES_PER_EU 3
+#define GR_TRACE_WAVES_PER_EU 4
 #endif
-#ifndef GR_BIAX_WAVES_PER_EU
-#define GR_BIAX_WAVES_PER_EU 3
+#ifndef GR_HEAVY_WAVES_PER_EU
+#define GR_HEAVY_WAVES_PER_EU 3
 #endif
-#define GR_WPE(kAniso) ((kAniso) == 2 ? GR_BIAX_WAVES_PER_EU : GR_TRACE_WAVES_PER_EU)
-#define GR_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_WPE(kAniso), GR_WPE(kAniso))))
+#define GR_WPE(kAniso, kFrost) (((kAniso) == 2 || (kFrost)) ? GR_HEAVY_WAVES_PER_EU : GR_TRACE_WAVES_PER_EU)
 #define GR_OCCUPANCY(w) __attribute__((amdgpu_waves_per_eu((w), (w))))
 
 // Tuples per wavefront of trace_wave_kernel: 64 lanes x this many rounds.
@@ -1218,6 +1220,19 @@ __device__ __forceinline__ void write_result(const PathState &ps, const DevScene
 #define GR_WAVE_ROUNDS 8
 #endif
 constexpr uint32_t kWaveTuples = 64u * GR_WAVE_ROUNDS;
+
+// The trace kernels keep the facet planes and the camera origin's sides in the launch's
+// dynamic LDS, sized to its plane count (plane_lds_bytes; for a batched launch, its largest
+// stone's): 20 bytes per plane rather than a fixed GR_MAX_PLANES array's 20 KB, which beside
+// the biaxial lane caches and the event counters left room for two workgroups per CU.
+__host__ __device__ constexpr uint32_t plane_slots(uint32_t n) { return (n + 3u) & ~3u; }
+__host__ __device__ constexpr size_t plane_lds_bytes(uint32_t n) {
+    return (size_t)plane_slots(n) * (sizeof(float4) + sizeof(float));
+}
+extern __shared__ float4 gr_plane_lds[];
+#define GR_PLANE_LDS(n)                  \
+    float4 *const planes = gr_plane_lds; \
+    float *const cam_side = reinterpret_cast<float *>(gr_plane_lds + plane_slots(n))
 
 // Facet planes into LDS, with the camera origin's signed distance to each
 // (p.w + n . origin, the clip's operations), padded to a multiple of four for the scan's
@@ -1274,8 +1289,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void trace_kernel(
     uint32_t n_tuples, float *__restrict__ xyz_out, uint32_t *__restrict__ info_out,
     unsigned long long *__restrict__ stats, uint32_t split_bounce, uint32_t park_min, uint32_t park_lanes,
     float *__restrict__ cont, uint32_t cont_cap, uint32_t *__restrict__ cont_count) {
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
+    GR_PLANE_LDS(sc.n_planes);
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes<kScreenedClip<kAbsorb, kFrost>>(planes, cam_side, planes_g, sc);
@@ -1319,8 +1333,7 @@ __global__ __launch_bounds__(kBlock) GR_OCCUPANCY(kW) void cont_kernel(
     float *__restrict__ next, uint32_t *__restrict__ next_count) {
     const uint32_t count = min(*cont_count, cont_cap);
     if (blockIdx.x * kBlock >= count) return;
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
+    GR_PLANE_LDS(sc.n_planes);
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes<kScreenedClip<kAbsorb, kFrost>>(planes, cam_side, planes_g, sc);
@@ -1410,14 +1423,13 @@ __device__ __forceinline__ uint32_t find_slice(const BatchSliceView *__restrict_
 // per-lane work is the plain kernel's, so each tuple's result is bit-identical to a plain
 // launch of the same job.
 template <int kAniso, bool kAbsorb, int kEnv, bool kFrost>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
+__global__ __launch_bounds__(kBlock) GR_OCCUPANCY(GR_WPE(kAniso, kFrost)) void trace_batch_kernel(
     const BatchSliceView *__restrict__ slices, uint32_t n_slices) {
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kTraceBlocks)];
     const DevScene &sc = *(const DevScene *)J.scene;
+    GR_PLANE_LDS(sc.n_planes);
     stage_planes<kScreenedClip<kAbsorb, kFrost>>(planes, cam_side, (const float4 *)J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
@@ -1453,17 +1465,16 @@ __global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_batch_kernel(
 // belongs to one slice (binary search of cont_block_begin) and continues that slice's
 // records of the previous level, parking again for the next level except at the last.
 template <int kAniso, bool kAbsorb, int kEnv, bool kFrost>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void cont_batch_kernel(
+__global__ __launch_bounds__(kBlock) GR_OCCUPANCY(GR_WPE(kAniso, kFrost)) void cont_batch_kernel(
     const BatchSliceView *__restrict__ slices, uint32_t n_slices, int level) {
     const BatchSliceView &J = slices[find_slice(slices, n_slices, blockIdx.x, kContBlocks)];
     const uint32_t count = min(J.cont_count[level - 1], J.cont_cap);
     const uint32_t r = (blockIdx.x - J.cont_block_begin) * kBlock + threadIdx.x;
     if ((blockIdx.x - J.cont_block_begin) * kBlock >= count) return;
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     const DevScene &sc = *(const DevScene *)J.scene;
+    GR_PLANE_LDS(sc.n_planes);
     stage_planes<kScreenedClip<kAbsorb, kFrost>>(planes, cam_side, (const float4 *)J.planes, sc);
     if (threadIdx.x < GR_EV_COUNT) block_ev[threadIdx.x] = 0u;
     __syncthreads();
@@ -1505,12 +1516,11 @@ constexpr int kRecFields = 20;
 // max_bounces iterations, so the loop ends after at most kWaveTuples * max_bounces
 // iterations.
 template <int kAniso, bool kAbsorb, bool kInfo>
-__global__ __launch_bounds__(kBlock) GR_TRACE_OCCUPANCY void trace_wave_kernel(
+__global__ __launch_bounds__(kBlock) GR_OCCUPANCY(GR_WPE(kAniso, true)) void trace_wave_kernel(
     const DevScene sc, const float4 *__restrict__ planes_g, const float2 *__restrict__ ri35,
     const uint32_t *__restrict__ cert, uint32_t pixel_begin, uint32_t sample_offset, uint32_t spp, uint32_t n_tuples, uint32_t rec_cap,
     float *__restrict__ rec, uint32_t *__restrict__ info_out, unsigned long long *__restrict__ stats) {
-    __shared__ float4 planes[GR_MAX_PLANES];
-    __shared__ __align__(16) float cam_side[GR_MAX_PLANES];
+    GR_PLANE_LDS(sc.n_planes);
     __shared__ float lcache[CacheFloats<kAniso>::value];
     __shared__ uint32_t block_ev[GR_EV_COUNT];
     stage_planes<kScreenedClip<kAbsorb, true>>(planes, cam_side, planes_g, sc);
