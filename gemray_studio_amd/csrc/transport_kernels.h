@@ -947,9 +947,22 @@ template <int kEnv>
 __device__ __forceinline__ V3 finish_escaped(V3 dir, const float *inten, const float *pdf,
                                              float hero_off, const DevScene &sc) {
     float rad[8];
-    // The spectral table's entry for this hero offset (kHeroEntries: evaluate instead).
+    // The spectral table's entry for this hero offset (kHeroEntries: evaluate instead), read
+    // whole before the environment lookup: one round trip to memory, overlapping that work,
+    // instead of one for the lobes after it and one per live channel's CIE values.
     const uint32_t he = sc.spec_tab ? hero_entry(hero_off) : kHeroEntries;
-    const float *spec = sc.spec_tab + (size_t)(he < kHeroEntries ? he : 0u) * kSpecFloats;
+    float spec[kSpecFloats];
+    if (he < kHeroEntries) {
+        const float4 *e4 = reinterpret_cast<const float4 *>(sc.spec_tab + (size_t)he * kSpecFloats);
+#pragma unroll
+        for (int i = 0; i < kSpecFloats / 4; ++i) {
+            const float4 v = e4[i];
+            spec[4 * i] = v.x;
+            spec[4 * i + 1] = v.y;
+            spec[4 * i + 2] = v.z;
+            spec[4 * i + 3] = v.w;
+        }
+    }
     if (kEnv == 0) {
         // sample_studio_environment_with_rig (environment.rs:280-321): the direction-only
         // terms are formed once, the fma chain runs per channel.
