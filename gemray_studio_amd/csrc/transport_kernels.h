@@ -290,17 +290,9 @@ __device__ __forceinline__ bool camera_miss_certified(const float4 *__restrict__
 // With the scene's material table, the sums are read from the path's entry.
 template <bool kBiax>
 __device__ __forceinline__ void fill_absorption_cache(float hero_off, const DevScene &sc, float *__restrict__ lcache) {
-    const uint32_t e = sc.mat_tab ? hero_entry(hero_off) : kHeroEntries;
-    if (e < kHeroEntries) {
-        const float *m = sc.mat_tab + (size_t)e * mat_floats(kBiax);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            LC_AO(k) = m[8 + k];
-            LC_AE(k) = m[16 + k];
-            if (kBiax) LC_AB(k) = m[24 + k];
-        }
-        return;
-    }
+    // From the material table the sums were read with the dispersion indices at the path's
+    // first hit (every interior segment comes after it); they are evaluated otherwise.
+    if (sc.mat_tab && hero_entry(hero_off) < kHeroEntries) return;
     const float h = fill_offset(hero_off);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -457,12 +449,22 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
     }
     ev_inc(ev, GR_EV_HIT);
     if (!ps.have_n) {
-        // The comb's dispersion indices: the material table's entry, or evaluated.
+        // The comb's dispersion indices: the material table's entry, or evaluated.  From the
+        // table, an absorbing stone's band sums come in the same round trip (the rest of the
+        // entry), instead of at the path's first interior segment.
         const uint32_t e = sc.mat_tab ? hero_entry(ps.hero_off) : kHeroEntries;
         if (e < kHeroEntries) {
             const float *m = sc.mat_tab + (size_t)e * mat_floats(kBiax);
 #pragma unroll
             for (int k = 0; k < 8; ++k) LC_NO(k) = m[k];
+            if (kAbsorb) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    LC_AO(k) = m[8 + k];
+                    LC_AE(k) = m[16 + k];
+                    if (kBiax) LC_AB(k) = m[24 + k];
+                }
+            }
         } else {
             const float h = fill_offset(ps.hero_off);
 #pragma unroll
