@@ -406,7 +406,7 @@ class GpuBackend:
         self._require()
         check(self._L.gr_serve_set_tick_tuples(self._ctx, tuples), "gr_serve_set_tick_tuples")
 
-    def set_policy(self, policy: int, slack: float = 0.5) -> None:
+    def set_policy(self, policy: int, slack: float = 0.25) -> None:
         """How ticks are shared among running jobs: _lib.GR_SERVE_DEADLINE (deadline =
         submit + slack x estimated service) or _lib.GR_SERVE_EQUAL_SHARE."""
         self._require()

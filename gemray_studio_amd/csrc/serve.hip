@@ -43,7 +43,10 @@ using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kDefaultTickTuples = 1ull << 24;
 constexpr uint64_t kMinTickTuples = GR_MAX_SAMPLES_PER_CALL;  // one pixel always fits
-constexpr double kDefaultSlack = 0.5;
+// The deadline policy's default slack, measured on the served 40-job mix: 0.25 beats the
+// plain first-come-first-served path at both p50 and p99 (0.5 had the lower p50 but a p99
+// 8 % above it once the kernels' service rate rose).
+constexpr double kDefaultSlack = 0.25;
 constexpr double kDefaultRate = 2.0e6;  // tuples per ms before any tick has completed
 constexpr uint64_t kMaxTickTuples = 1ull << 28;
 constexpr uint32_t kBlockTuples = 256;  // workgroup size of the batched kernels

@@ -452,7 +452,7 @@ gr_status gr_metrics_reset(gr_ctx *ctx);
 gr_status gr_serve_set_tick_tuples(gr_ctx *ctx, uint64_t tuples);
 /* How a tick's tuples are shared among running jobs (no counterpart in the reference, which
  * serves one request at a time):
- *   GR_SERVE_DEADLINE (default, slack 0.5): in order of deadline = submit time + slack x the
+ *   GR_SERVE_DEADLINE (default, slack 0.25): in order of deadline = submit time + slack x the
  *     job's estimated service time (its tuples over the measured service rate).  Slack 0 is
  *     first-come-first-served (the best worst-case latency); larger slacks let short jobs
  *     overtake long ones (a lower median).  Deadlines are fixed at submission.
