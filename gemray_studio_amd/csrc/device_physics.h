@@ -436,19 +436,22 @@ __device__ __forceinline__ V3 arbitrary_perpendicular(V3 n) {
     return normalize_or_zero(sub(a, mul(n, dot(n, a))));
 }
 
-// refraction.rs:45-50: delta = 2 atan(x) with x the tangent of the half retardation.
-// The retardation matrix (polarization.rs:141-147) needs only cos delta and sin delta,
-// which are the rational functions (1 - x^2) / (1 + x^2) and 2x / (1 + x^2); they only
-// rotate a channel's Stokes vector, so the square root and divisions are the hardware's
-// (sqrt_hw, div_hw).
+// refraction.rs:45-50: delta = 2 atan(x) with x = cos_i sqrt(A) / B the tangent of the half
+// retardation, A = n1^2 sin_i^2 - 1, B = n1 sin_i^2.  The retardation matrix
+// (polarization.rs:141-147) needs only cos delta and sin delta, the rational functions
+// (1 - x^2) / (1 + x^2) and 2x / (1 + x^2) -- with numerator and denominator times B^2,
+// (B^2 - cos_i^2 A) / (B^2 + cos_i^2 A) and 2 cos_i sqrt(A) B / (B^2 + cos_i^2 A): one
+// reciprocal and one square root.  They only rotate a channel's Stokes vector, so both are
+// the hardware's (v_rcp_f32, sqrt_hw).
 __device__ __forceinline__ void tir_retardation_cs(float n1k, float cos_i, float sin_i,
                                                    float *cos_d, float *sin_d) {
-    float x = div_hw(cos_i * sqrt_hw(fmaxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f)),
-                     fmaxf(n1k * sin_i * sin_i, 1e-6f));
-    float x2 = x * x;
-    float inv = __builtin_amdgcn_rcpf(1.0f + x2);
-    *cos_d = (1.0f - x2) * inv;
-    *sin_d = (2.0f * x) * inv;
+    const float a = fmaxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f);
+    const float b = fmaxf(n1k * sin_i * sin_i, 1e-6f);
+    const float ca = (cos_i * cos_i) * a;
+    const float b2 = b * b;
+    const float inv = __builtin_amdgcn_rcpf(b2 + ca);
+    *cos_d = (b2 - ca) * inv;
+    *sin_d = ((2.0f * cos_i) * sqrt_hw(a)) * (b * inv);
 }
 
 __device__ __forceinline__ void fresnel_rs_rp(float n1k, float n2k, float cos_i, float cos_t,

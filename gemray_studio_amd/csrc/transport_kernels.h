@@ -780,7 +780,6 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
             ev_inc(ev, GR_EV_REFLECT);
             event = 2u;
         }
-        const float inv_r = tir ? 1.0f : 1.0f / r_unpol;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (!GR_LIVE(ps, k)) continue;
@@ -799,7 +798,14 @@ __device__ __forceinline__ bool bounce_step(PathState &ps, const DevScene &sc, c
                 ps.pdf[k] *= clampf(0.5f * fmaf(rp, rp, rs * rs), 1e-4f, 1.0f - 1e-4f);
                 ps.st[k] = apply_fresnel_reflection(ps.st[k], rs, rp);
             }
-            ps.st[k] = scale(ps.st[k], inv_r);
+        }
+        // The partial reflection divides every channel by the hero's r_unpol; TIR's factor is
+        // exactly 1.0, and skipping it leaves every bit as it is.
+        if (!tir) {
+            const float inv_r = 1.0f / r_unpol;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (GR_LIVE(ps, k)) ps.st[k] = scale(ps.st[k], inv_r);
         }
         new_k = sub(cur_k, mul(normal, 2.0f * dot(cur_k, normal)));
         new_s = new_k;
