@@ -98,6 +98,15 @@ def _cases(srb):
         Case("zircon-scattering-frosted-girdle", dataclasses.replace(M.by_name("Zircon"), scattering_sigma_s=1.0,
                                                                      scattering_g=0.0), srb,
              B.Studio("Daylight", 1.0, 0.4, 0.35), 24, 24, 16, finishes=frosted_girdle(74)),
+        # The largest stone the runtime accepts (1024 planes): the kernels stage the plane
+        # arrays in dynamic LDS sized to the scene (~20 KB here), on the isotropic, the
+        # absorbing uniaxial and the biaxial kernels.
+        Case("diamond-1024-planes", M.by_name("Diamond"), scenes.random_convex_stone(29, 1024),
+             B.Studio("RingLights", 1.0, 0.85, 0.95), 24, 24, 16),
+        Case("ruby-1024-planes-map", M.by_name("Ruby"), scenes.random_convex_stone(29, 1024), B.HdrMap(env_small),
+             24, 24, 16),
+        Case("alexandrite-1024-planes-map", M.by_name("Alexandrite"), scenes.random_convex_stone(31, 1024),
+             B.HdrMap(env_small), 24, 24, 16),
     ]
 
 
@@ -107,7 +116,8 @@ CASE_IDS = ["diamond-ringlights", "cz-daylight", "sapphire-ringlights", "ruby-in
             "sapphire-rounded-edges-map", "diamond-frosted-girdle", "zircon-frosted-girdle-map",
             "alexandrite-ringlights", "topaz-map", "tanzanite-oval-map", "alexandrite-frosted-girdle-map",
             "diamond-hazy-ringlights", "ruby-scattering-map", "tanzanite-scattering-oval-map",
-            "zircon-scattering-frosted-girdle"]
+            "zircon-scattering-frosted-girdle", "diamond-1024-planes", "ruby-1024-planes-map",
+            "alexandrite-1024-planes-map"]
 
 
 @pytest.mark.parametrize("idx", range(len(CASE_IDS)), ids=CASE_IDS)
@@ -408,8 +418,9 @@ def test_kernels_with_and_without_path_records_agree(gpu, srb, idx):
     assert np.array_equal(film.view(np.uint32), want.view(np.uint32)), case.name
 
 
-@pytest.mark.parametrize("idx", [0, 2, 8, 10, 17, 21], ids=["diamond", "sapphire", "pleochroic-oval", "stress",
-                                                            "tanzanite-oval", "ruby-scattering"])
+@pytest.mark.parametrize("idx", [0, 2, 8, 10, 17, 21, 23, 24, 25],
+                         ids=["diamond", "sapphire", "pleochroic-oval", "stress", "tanzanite-oval", "ruby-scattering",
+                              "diamond-1024-planes", "ruby-1024-planes-map", "alexandrite-1024-planes-map"])
 def test_plain_and_wavefront_kernels_agree_bit_for_bit(gpu, srb, idx):
     """The persistent-wavefront kernel only reschedules tuples: every tuple's XYZ, segment
     count, termination and path hashes equal the plain kernel's, and so does the film."""
@@ -431,8 +442,9 @@ def test_plain_and_wavefront_kernels_agree_bit_for_bit(gpu, srb, idx):
     assert out["plain"][3] == out["wavefront"][3]
 
 
-@pytest.mark.parametrize("idx", [0, 2, 8, 10, 17, 21], ids=["diamond", "sapphire", "pleochroic-oval", "stress",
-                                                            "tanzanite-oval", "ruby-scattering"])
+@pytest.mark.parametrize("idx", [0, 2, 8, 10, 17, 21, 24],
+                         ids=["diamond", "sapphire", "pleochroic-oval", "stress", "tanzanite-oval", "ruby-scattering",
+                              "ruby-1024-planes-map"])
 def test_path_length_split_is_bit_identical(gpu, srb, idx):
     """Parking paths at the split bounce and finishing them in a second launch moves state
     only: outputs, films and counters equal the unsplit run bit for bit."""

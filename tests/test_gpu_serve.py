@@ -102,6 +102,34 @@ def test_job_film_equals_accumulate_alone_and_in_the_mix(gpu):
         gpu.set_policy(_lib.GR_SERVE_DEADLINE)
 
 
+def test_largest_stone_shares_a_launch_with_small_ones(gpu):
+    """A 1024-plane job (the runtime's limit) in one class launch with 17- and 74-plane
+    jobs: the launch stages planes in dynamic LDS sized to its largest scene, and every
+    job's film equals its own accumulate bit for bit, alone and cut into small ticks."""
+    srb = po.standard_round_brilliant()
+    env = scenes.procedural_env(5, 128, 64)
+    cam = B.camera_new(0.35, 0.28, 5.0, 18.0)
+    ruby = M.by_name("Ruby")
+    jobs = [
+        (B.SceneRef(cam, 28, 20, scenes.random_convex_stone(29, 1024), ruby, 16, B.HdrMap(env)), 1, 8),
+        (B.SceneRef(cam, 20, 24, scenes.random_convex_stone(3, 17), ruby, 16, B.HdrMap(env)), 0, 12),
+        (B.SceneRef(cam, 24, 24, srb, ruby, 16, B.HdrMap(env)), 2, 6),
+        (B.SceneRef(cam, 16, 20, scenes.random_convex_stone(31, 1024), M.by_name("Alexandrite"), 16,
+                    B.HdrMap(env)), 0, 8),
+    ]
+    want = [_accumulate(gpu, *j) for j in jobs]
+    for w, a in zip(want, [_run_jobs(gpu, [j])[0] for j in jobs]):
+        assert np.array_equal(w.view(np.uint32), a.view(np.uint32))
+    gpu.set_tick_tuples(1 << 16)
+    try:
+        mixed = _run_jobs(gpu, jobs)
+    finally:
+        gpu.set_tick_tuples(1 << 24)
+    for w, m in zip(want, mixed):
+        assert np.array_equal(w.view(np.uint32), m.view(np.uint32))
+    assert all(np.isfinite(w).all() and w.max() > 0 for w in want)
+
+
 def test_job_counters_and_metrics(gpu):
     gpu.metrics_reset()
     jobs = _mix()[:3]
