@@ -1275,15 +1275,31 @@ __device__ __forceinline__ void stage_planes(float4 *lds, float *cam_side, const
     }
 }
 
-// Per-class counters: wave-level sum, one LDS atomic per wave, one global atomic per
-// counter per workgroup.
+// Per-class counters, once the workgroup's lanes have all finished: thread (w, g) =
+// (threadIdx.x / 32, threadIdx.x % 32) sums counter word w of lanes 8g..8g+7 packed, splits
+// the sum into its two 16-bit counts, sums those over the 32 groups with five shuffles and
+// adds them to the workgroup's counters with one LDS atomic per class; then one global
+// atomic per class per workgroup.  The packed sum cannot carry: the eight lanes belong to
+// one wave, and a class is counted at most once per segment, so a wave's count is at most
+// kWaveTuples x GR_MAX_BOUNCES (the persistent kernel's wave; one path per lane elsewhere).
+static_assert(kEvWords * 32 == kBlock, "one counter word per 32 threads");
+static_assert(kWaveTuples * GR_MAX_BOUNCES < 65536u, "a wave's counts fit 16 bits");
 __device__ __forceinline__ void flush_events(const uint32_t *ev, uint32_t *block_ev,
                                              unsigned long long *stats) {
+    __syncthreads();
+    const uint32_t w = threadIdx.x >> 5, g = threadIdx.x & 31u;
+    const uint32_t *src = ev - threadIdx.x + w * kBlock + 8u * g;
+    uint32_t packed = 0u;
 #pragma unroll
-    for (int i = 0; i < GR_EV_COUNT; ++i) {
-        uint32_t v = (ev[(i >> 1) * kBlock] >> (16 * (i & 1))) & 0xFFFFu;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&block_ev[i], v);
+    for (int j = 0; j < 8; ++j) packed += src[j];
+    uint32_t lo = packed & 0xFFFFu, hi = packed >> 16;
+    for (int off = 16; off > 0; off >>= 1) {
+        lo += __shfl_xor(lo, off, 64);
+        hi += __shfl_xor(hi, off, 64);
+    }
+    if (g == 0u) {
+        if (lo) atomicAdd(&block_ev[2 * w], lo);
+        if (hi && 2 * w + 1 < GR_EV_COUNT) atomicAdd(&block_ev[2 * w + 1], hi);
     }
     __syncthreads();
     if (threadIdx.x < GR_EV_COUNT && block_ev[threadIdx.x])
